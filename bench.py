@@ -1,0 +1,149 @@
+"""Benchmark: MLUPS of the lyotropic active-nematic step at 4096^2 on one MI355X.
+
+Workload (the large single-domain case): 4096 x 4096 periodic D2Q9, FP32 with
+shifted populations and compensated phi, npc = 2 (predictor + 2 correctors),
+zeta = 0.03, init seed 7 (random director, phi = 0.5 + U(-0.05, 0.05)).
+The same grid is also timed in FP64 (reference arithmetic).
+
+MLUPS = LX*LY*steps / seconds / 1e6; one step = predictor + npc correctors.
+The timed region is bracketed by HIP events on the library's stream; the
+fields are resident in HBM throughout.  roofline: the corrector sweep, whose
+algorithmic traffic is 36 values per node (read f, fn, Q, QN, phi terms; write
+f, Q, phi) = 144 B per node in FP32, against 8 TB/s.  cpu_baseline: the CPU
+oracle (the reference's algorithm and layout, OpenMP) on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PHYS = dict(GammaQ=0.1, GammaP=0.1, xi=0.7, tauNem=2.0, tauIso=2.0, rho=40.0, friction=0.0,
+            AA=0.01, CC=0.0, LL=0.04, KK=0.04, zeta=0.03, zetaI=0.0, conc=0.5)
+INIT = dict(config="random", seed=7, angle=90.0, noise=1.0, qamp=0.5, phinoise=0.05)
+HBM_PEAK = 8.0e12
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_corrector_fp32_4096.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0):
+    m = lyo.Lyotropic(L, L, npc=npc, precision=precision, **PHYS)
+    if chunk:
+        m.set_launch(chunk, 0)
+    m.configure(**INIT)
+    if warmup:
+        m.step(warmup)
+    m.synchronize()
+    total_ms, (pred_ms, corr_ms) = m.time_steps(steps, per_sweep=True)
+    fsum, _ = m.runtime_checks()
+    m.close()
+    return total_ms, pred_ms, corr_ms
+
+
+def cpu_baseline(L, steps, threads):
+    from oracle import pyoracle as po
+    o = po.Oracle(L, L, npc=2, nthreads=threads, **PHYS)
+    o.configure(**INIT)
+    o.step(1)  # warm-up (first touch of the work arrays)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return L * L * steps / dt / 1e6
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--npc", type=int, default=2)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the x-slab decomposition is design only)", file=sys.stderr)
+        sys.exit(2)
+
+    from mass_hd_amd import lyo
+    L, npc = args.size, args.npc
+    N = L * L
+
+    log("fp32 %dx%d npc=%d: %d warmup + %d timed steps" % (L, L, npc, args.warmup, args.steps))
+    t32, p32, c32 = gpu_run(lyo, L, lyo.FP32, npc, args.steps, args.warmup, args.chunk)
+    mlups32 = N * args.steps / (t32 / 1e3) / 1e6
+    log("fp32: %.3f ms/step, %.0f MLUPS, predictor %.3f ms, corrector %.3f ms"
+        % (t32 / args.steps, mlups32, p32, c32))
+
+    extra = {}
+    if not args.skip_fp64:
+        s64 = max(2, args.steps // 4)
+        log("fp64 %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
+        t64, p64, c64 = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, args.chunk)
+        mlups64 = N * s64 / (t64 / 1e3) / 1e6
+        log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
+        extra = {"fp64_mlups": round(mlups64, 1), "fp32_over_fp64": round(mlups32 / mlups64, 3),
+                 "fp64_kernel_set": "fused sweeps, reference expression order",
+                 "fp64_corrector_GBps": round(36 * 8 * N / (c64 / 1e3) / 1e9, 1)}
+
+    alg_bytes = 36 * 4 * N  # per corrector launch
+    achieved = alg_bytes / (c32 / 1e3)
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("size") == L and t.get("npc") == npc:
+            traffic = t.get("bytes_per_launch")
+
+    cpu = None
+    if not args.skip_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log("cpu baseline: oracle %dx%d npc=2, %d threads, 1 warmup + %d steps"
+            % (L, L, threads, args.cpu_steps))
+        v = cpu_baseline(L, args.cpu_steps, threads)
+        cpu = {"value": round(v, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+               "sample": "%dx%d FP64 npc=2, %d steps after 1 warm-up step, OpenMP" %
+                         (L, L, args.cpu_steps)}
+        log("cpu: %.2f MLUPS" % v)
+
+    line = {
+        "metric": "MLUPS at 4096^2 (FP32 plain, npc=2)",
+        "value": round(mlups32, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (random director field, seed 7)",
+        "config": {"workload": "lyotropic active nematic %dx%d periodic D2Q9, FP32 shifted "
+                               "populations, npc=%d, zeta=0.03" % (L, L, npc),
+                   "LX": L, "LY": L, "npc": npc},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": traffic,
+                     "kernel": "sweep_kernel<float,false,CORR> (corrector sweep)",
+                     "kernel_ms": round(c32, 4), "predictor_ms": round(p32, 4)},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

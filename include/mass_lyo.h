@@ -1,0 +1,178 @@
+/*
+ * mass_lyo.h -- C ABI of the MI355X (gfx950) lyotropic active-nematic step.
+ *
+ * The reference has no device boundary: its Lyotropic model runs the step in
+ * OpenMP loops behind the C++ Model plugin interface (src/models.hpp:30-116).
+ * This header is the thin layer between that interface and the HIP kernels.
+ * Each entry point names the reference member it stands in for.
+ *
+ * Conventions
+ *  - Node index k = y + LY*x (src/geometry.hpp:424-431); y is contiguous.
+ *  - Population order v = 0..8 with directions (0,0),(1,0),(-1,0),(0,1),
+ *    (0,-1),(1,1),(-1,-1),(-1,1),(1,-1) (src/geometry.hpp:17-27).
+ *  - Host buffers are FP64.  Populations cross the boundary as AoS, N*9
+ *    doubles, ff[9*k+v], the layout of the reference's LBField.
+ *  - Every call returns 0 on success or a MASS_LYO_E* code; no exception
+ *    crosses the boundary.  mass_lyo_last_error() gives the message.
+ *  - One handle is used by one host thread.  Device work is queued on the
+ *    handle's own HIP stream; calls that return data synchronise it.
+ *  - Only periodic boundaries (bc=0) exist on the device path.
+ */
+#ifndef MASS_LYO_H_
+#define MASS_LYO_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MASS_LYO_ABI_VERSION 1
+
+enum mass_lyo_status {
+  MASS_LYO_OK = 0,
+  MASS_LYO_EINVAL = 1,   /* bad argument or parameter */
+  MASS_LYO_EDEVICE = 2,  /* HIP runtime error / no usable gfx950 device */
+  MASS_LYO_ENAN = 3,     /* NaN found while downloading a field */
+  MASS_LYO_ECHECK = 4,   /* conservation check failed */
+  MASS_LYO_ENOMEM = 5
+};
+
+enum mass_lyo_precision {
+  MASS_LYO_FP64 = 0,  /* IEEE FP64, reference expression order, no FMA:
+                         bit-identical to the reference arithmetic */
+  MASS_LYO_FP32 = 1   /* FP32 with shifted populations f - w_i*rho and a
+                         compensated phi update */
+};
+
+/* Model parameters: the runcard options of Lyotropic::GetOptions
+ * (src/models/lyotropic.cpp:551-602), plus the grid size from the factory
+ * (src/models.hpp:190) and the precision of the device path. */
+typedef struct mass_lyo_params {
+  int32_t LX, LY;
+  int32_t npc;          /* corrector sweeps, >= 1 */
+  int32_t precision;    /* enum mass_lyo_precision */
+  int32_t conserve_phi; /* must be 0: plain Lyotropic (lyotropic.hpp:46) */
+  int32_t flags;        /* reserved, 0 */
+  double GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
+  double AA, CC, LL, KK, zeta, zetaI, conc;
+} mass_lyo_params;
+
+/* Initial configuration (Lyotropic::ConfigureAtNode, lyotropic.cpp:69-160). */
+enum mass_lyo_config {
+  MASS_LYO_CONFIG_CIRCLE = 0,
+  MASS_LYO_CONFIG_SQUARE = 1,
+  MASS_LYO_CONFIG_STRIPE = 2,
+  MASS_LYO_CONFIG_HALF = 3,
+  MASS_LYO_CONFIG_RANDOM = 4,  /* nematic everywhere; phi = conc +
+                                  U(-phinoise, phinoise), drawn after theta */
+  MASS_LYO_CONFIG_BOXATWALL = 5,
+  MASS_LYO_CONFIG_CUTTINGATWALL = 6,
+  MASS_LYO_CONFIG_CIRCLEATWALL = 7,
+  MASS_LYO_CONFIG_WETTEDWALL = 8
+};
+
+typedef struct mass_lyo_init {
+  int32_t config;     /* enum mass_lyo_config */
+  uint32_t seed;      /* mt19937 seed (src/random.cpp:43-46) */
+  double level, radius, angle_deg, noise;
+  double qamp;        /* Qxx = qamp*order*cos(2 theta); the reference is 1 */
+  double phinoise;    /* half width of the phi draw (config RANDOM) */
+} mass_lyo_init;
+
+/* Field ids for mass_lyo_get_field.  FF is N*9 AoS; all others N doubles.
+ * Derived fields (N, UX ... SIGMA_ACT_YX) are those of the last
+ * UpdateQuantities call of the last step (lyotropic.cpp:168-247), i.e. the
+ * start of the last corrector sweep, and are zero before the first step,
+ * as in the reference. */
+enum mass_lyo_field {
+  MASS_LYO_FF = 0,
+  MASS_LYO_QQXX = 1,
+  MASS_LYO_QQYX = 2,
+  MASS_LYO_PHI = 3,
+  MASS_LYO_N = 4,
+  MASS_LYO_UX = 5,
+  MASS_LYO_UY = 6,
+  MASS_LYO_SIGMAXX = 7,
+  MASS_LYO_SIGMAXY = 8,
+  MASS_LYO_SIGMAYX = 9,
+  MASS_LYO_SIGMAYY = 10,
+  MASS_LYO_SIGMA_BULK = 11,
+  MASS_LYO_SIGMA_ELASTIC_XX = 12,
+  MASS_LYO_SIGMA_ELASTIC_YX = 13,
+  MASS_LYO_SIGMA_PHASE_FIELD_XX = 14,
+  MASS_LYO_SIGMA_PHASE_FIELD_YX = 15,
+  MASS_LYO_SIGMA_ACTIVE_XX = 16,
+  MASS_LYO_SIGMA_ACTIVE_YX = 17,
+  MASS_LYO_MU = 18,
+  MASS_LYO_HHXX = 19,
+  MASS_LYO_HHYX = 20,
+  MASS_LYO_NUM_FIELDS = 21
+};
+
+typedef struct mass_lyo_s* mass_lyo_t;
+
+/* Replaces the Lyotropic(LX,LY,BC) factory + Lyotropic::Initialize
+ * (src/models.hpp:196-205, src/models/lyotropic.cpp:15-67): validates the
+ * parameters, selects the device, allocates all device fields and the
+ * handle's stream.  Fields start at zero (src/fields.hpp:29-33). */
+int mass_lyo_create(const mass_lyo_params* params, mass_lyo_t* out);
+
+/* Frees every device buffer, event and the stream. */
+void mass_lyo_destroy(mass_lyo_t h);
+
+/* Message of the last failed call on h (or of the last failed create when
+ * h is NULL).  Valid until the next call on the same handle. */
+const char* mass_lyo_last_error(mass_lyo_t h);
+
+/* Replaces Lyotropic::Configure (src/models/lyotropic.cpp:69-166): draws the
+ * initial fields on the host with mt19937 and the host libm, exactly as the
+ * reference does, and uploads them.  Returns the reference's check totals
+ * ftot (sum of f) and ptot (sum of phi) (lyotropic.cpp:158-159); either
+ * pointer may be NULL. */
+int mass_lyo_configure(mass_lyo_t h, const mass_lyo_init* init, double* ftot,
+                       double* ptot);
+
+/* Uploads an arbitrary state: ff (N*9 AoS), Qxx, Qyx, phi (N each).  Derived
+ * fields are reset to zero. */
+int mass_lyo_set_state(mass_lyo_t h, const double* ff, const double* Qxx,
+                       const double* Qyx, const double* phi);
+
+/* Replaces nsteps calls of Lyotropic::Step (src/models/lyotropic.cpp:500-525):
+ * one predictor sweep (UpdateQuantities, UpdateFields(true), Move) and npc
+ * corrector sweeps per step.  Asynchronous. */
+int mass_lyo_step(mass_lyo_t h, int32_t nsteps);
+
+/* Downloads one field as FP64 (the data of Lyotropic::serialize_frame,
+ * src/models/lyotropic.hpp:121-138).  Synchronises.  Returns MASS_LYO_ENAN if
+ * the field holds a NaN (the check of src/write.cpp:41). */
+int mass_lyo_get_field(mass_lyo_t h, int32_t field, double* out);
+
+/* FP64 device reductions used by Lyotropic::RuntimeChecks
+ * (src/models/lyotropic.cpp:527-549): sum of all populations and sum of phi.
+ * Fixed summation tree: the result depends only on the grid size.
+ * Synchronises. */
+int mass_lyo_sums(mass_lyo_t h, double* fsum, double* phisum);
+
+/* Waits for all queued device work of the handle. */
+int mass_lyo_synchronize(mass_lyo_t h);
+
+/* Timing: runs nsteps steps bracketed by HIP events on the handle's stream
+ * and returns the elapsed milliseconds.  When sweep_ms is not NULL, every
+ * sweep launch is also bracketed by its own event pair; sweep_ms[0] gets the
+ * mean predictor-sweep time and sweep_ms[1] the mean corrector-sweep time. */
+int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms,
+                        double* sweep_ms);
+
+/* Launch geometry of the sweep kernels (strip-march): rows per wave chunk
+ * and waves per block.  0 keeps the current value.  Results do not depend on
+ * it. */
+int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block);
+
+/* Version of this ABI (MASS_LYO_ABI_VERSION). */
+int mass_lyo_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,276 @@
+// lyo_capi.cpp -- the C ABI of include/mass_lyo.h.
+//
+// Owns the device, the stream, the per-run engine (FP64 or FP32) and the FP64
+// staging buffers used to move host AoS fields across the boundary.  No
+// exception crosses the ABI: every entry point returns a status code and keeps
+// the message in the handle.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mass_lyo.h"
+#include "lyo_config.hpp"
+#include "lyo_engine.hpp"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kMaxPart = 4096;
+
+}  // namespace
+
+struct mass_lyo_s {
+  mass_lyo_params prm{};
+  int N = 0, NP = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<lyo::Engine> eng;
+  double* d_stage = nullptr;  // N*9 doubles (AoS populations) + 3N
+  double* d_part = nullptr;   // 2*kMaxPart partial sums + 2 results
+  lyo::LaunchCfg launch;
+  std::string err;
+};
+
+static int fail(mass_lyo_t h, int code, const std::string& msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+static int hip_fail(mass_lyo_t h, hipError_t e, const char* what) {
+  return fail(h, MASS_LYO_EDEVICE,
+              std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+}
+
+static const double* param_block(const mass_lyo_params& p) { return &p.GammaQ; }
+
+extern "C" {
+
+int mass_lyo_abi_version(void) { return MASS_LYO_ABI_VERSION; }
+
+const char* mass_lyo_last_error(mass_lyo_t h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
+  if (!p || !out) return fail(nullptr, MASS_LYO_EINVAL, "null argument");
+  *out = nullptr;
+  if (p->LX < 2 || p->LY < 2)
+    return fail(nullptr, MASS_LYO_EINVAL, "LX and LY must be at least 2");
+  const long long N = (long long)p->LX * (long long)p->LY;
+  if (N * 9 >= (1LL << 31) - 4096)
+    return fail(nullptr, MASS_LYO_EINVAL, "grid too large (LX*LY*9 must stay below 2^31)");
+  if (p->npc < 1) return fail(nullptr, MASS_LYO_EINVAL, "npc must be at least 1");
+  if (p->precision != MASS_LYO_FP64 && p->precision != MASS_LYO_FP32)
+    return fail(nullptr, MASS_LYO_EINVAL, "precision must be MASS_LYO_FP64 or MASS_LYO_FP32");
+  if (p->conserve_phi)
+    return fail(nullptr, MASS_LYO_EINVAL,
+                "conserve_phi is not part of the plain lyotropic model");
+  const double* v = param_block(*p);
+  for (int i = 0; i < 14; ++i)
+    if (!std::isfinite(v[i])) return fail(nullptr, MASS_LYO_EINVAL, "non-finite parameter");
+  if (!(p->rho > 0)) return fail(nullptr, MASS_LYO_EINVAL, "rho must be positive");
+
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1)
+    return fail(nullptr, MASS_LYO_EDEVICE, "no HIP device available (this library runs on gfx950)");
+  int dev = 0;
+  e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDevice");
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(nullptr, MASS_LYO_EDEVICE,
+                std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+
+  auto h = std::make_unique<mass_lyo_s>();
+  h->prm = *p;
+  h->N = (int)N;
+  h->NP = (int)N;
+  h->device = dev;
+  e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipStreamCreate");
+  h->eng.reset(p->precision == MASS_LYO_FP64 ? lyo::make_engine_fp64() : lyo::make_engine_fp32());
+  e = h->eng->alloc(p->LX, p->LY, h->NP, p->npc);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_stage, sizeof(double) * 12 * (size_t)N);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_part, sizeof(double) * (2 * kMaxPart + 2));
+  if (e != hipSuccess) {
+    std::string msg = std::string("device allocation failed: ") + hipGetErrorString(e);
+    mass_lyo_destroy(h.release());
+    return fail(nullptr, MASS_LYO_ENOMEM, msg);
+  }
+  h->eng->set_params(v);
+  // default chunk length: enough waves to fill 256 CUs several times over
+  const int nstrips = (p->LY + lyo::kStrip - 1) / lyo::kStrip;
+  int xc = 64;
+  while (xc > 4 && (long long)nstrips * ((p->LX + xc - 1) / xc) < 8192) xc >>= 1;
+  h->launch.xc = xc;
+  *out = h.release();
+  return MASS_LYO_OK;
+}
+
+void mass_lyo_destroy(mass_lyo_t h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->eng) h->eng->release();
+  h->eng.reset();
+  if (h->d_stage) hipFree(h->d_stage);
+  if (h->d_part) hipFree(h->d_part);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+static int upload_state(mass_lyo_t h, const double* ff, const double* qxx, const double* qyx,
+                        const double* phi) {
+  const size_t N = (size_t)h->N;
+  double* d_ff = h->d_stage;
+  double* d_qxx = d_ff + 9 * N;
+  double* d_qyx = d_qxx + N;
+  double* d_phi = d_qyx + N;
+  hipError_t e = hipMemcpyAsync(d_ff, ff, sizeof(double) * 9 * N, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_qxx, qxx, sizeof(double) * N, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_qyx, qyx, sizeof(double) * N, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_phi, phi, sizeof(double) * N, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = h->eng->pack(d_ff, d_qxx, d_qyx, d_phi, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "state upload");
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_set_state(mass_lyo_t h, const double* ff, const double* qxx, const double* qyx,
+                       const double* phi) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!ff || !qxx || !qyx || !phi) return fail(h, MASS_LYO_EINVAL, "null field pointer");
+  return upload_state(h, ff, qxx, qyx, phi);
+}
+
+int mass_lyo_configure(mass_lyo_t h, const mass_lyo_init* init, double* ftot, double* ptot) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!init) return fail(h, MASS_LYO_EINVAL, "null init");
+  lyo::HostInit hi;
+  hi.LX = (unsigned)h->prm.LX;
+  hi.LY = (unsigned)h->prm.LY;
+  hi.config = init->config;
+  hi.seed = init->seed;
+  hi.level = init->level;
+  hi.radius = init->radius;
+  hi.angle_deg = init->angle_deg;
+  hi.noise = init->noise;
+  hi.qamp = init->qamp;
+  hi.phinoise = init->phinoise;
+  hi.conc = h->prm.conc;
+  hi.rho = h->prm.rho;
+  std::vector<double> ff, qxx, qyx, phi;
+  std::string msg;
+  if (lyo::configure_host(hi, ff, qxx, qyx, phi, ftot, ptot, msg) != 0)
+    return fail(h, MASS_LYO_EINVAL, msg);
+  return upload_state(h, ff.data(), qxx.data(), qyx.data(), phi.data());
+}
+
+int mass_lyo_step(mass_lyo_t h, int32_t nsteps) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (nsteps < 0) return fail(h, MASS_LYO_EINVAL, "negative step count");
+  hipError_t e = h->eng->steps(nsteps, h->launch, h->stream, nullptr);
+  if (e != hipSuccess) return hip_fail(h, e, "sweep launch");
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_synchronize(mass_lyo_t h) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "hipStreamSynchronize");
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_get_field(mass_lyo_t h, int32_t field, double* out) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!out) return fail(h, MASS_LYO_EINVAL, "null output");
+  if (field < 0 || field >= MASS_LYO_NUM_FIELDS) return fail(h, MASS_LYO_EINVAL, "unknown field id");
+  const size_t n = (field == MASS_LYO_FF ? 9 : 1) * (size_t)h->N;
+  hipError_t e = h->eng->field(field, h->d_stage, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, h->d_stage, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "field download");
+  for (size_t i = 0; i < n; ++i)
+    if (out[i] != out[i]) return fail(h, MASS_LYO_ENAN, "nan found while writing file.");
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_sums(mass_lyo_t h, double* fsum, double* phisum) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  double r[2] = {0, 0};
+  double* d_out = h->d_part + 2 * kMaxPart;
+  hipError_t e = h->eng->sums(h->d_part, kMaxPart, d_out, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(r, d_out, sizeof(r), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "sums");
+  if (fsum) *fsum = r[0];
+  if (phisum) *phisum = r[1];
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms, double* sweep_ms) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (nsteps < 1) return fail(h, MASS_LYO_EINVAL, "nsteps must be at least 1");
+  hipEvent_t t0, t1;
+  hipError_t e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  if (e != hipSuccess) return hip_fail(h, e, "hipEventCreate");
+  lyo::SweepTimer tm;
+  if (sweep_ms) {
+    const size_t pairs = (size_t)nsteps * (1 + (size_t)h->prm.npc);
+    tm.ev.resize(2 * pairs);
+    tm.kind.resize(pairs);
+    for (auto& ev : tm.ev) {
+      e = hipEventCreate(&ev);
+      if (e != hipSuccess) break;
+    }
+  }
+  if (e == hipSuccess) e = hipEventRecord(t0, h->stream);
+  if (e == hipSuccess) e = h->eng->steps(nsteps, h->launch, h->stream, sweep_ms ? &tm : nullptr);
+  if (e == hipSuccess) e = hipEventRecord(t1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(t1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+  if (e == hipSuccess && sweep_ms) {
+    double acc[2] = {0, 0};
+    int cnt[2] = {0, 0};
+    for (size_t i = 0; i < tm.used / 2 && e == hipSuccess; ++i) {
+      float m = 0;
+      e = hipEventElapsedTime(&m, tm.ev[2 * i], tm.ev[2 * i + 1]);
+      acc[tm.kind[i]] += m;
+      cnt[tm.kind[i]] += 1;
+    }
+    sweep_ms[0] = cnt[0] ? acc[0] / cnt[0] : 0;
+    sweep_ms[1] = cnt[1] ? acc[1] / cnt[1] : 0;
+  }
+  for (auto& ev : tm.ev)
+    if (ev) hipEventDestroy(ev);
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  if (e != hipSuccess) return hip_fail(h, e, "timed steps");
+  if (total_ms) *total_ms = ms;
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (chunk_rows < 0 || chunk_rows > 4096 || waves_per_block < 0 || waves_per_block > 4)
+    return fail(h, MASS_LYO_EINVAL, "chunk_rows must be in [1,4096], waves_per_block in [1,4]");
+  if (chunk_rows) h->launch.xc = chunk_rows;
+  if (waves_per_block) h->launch.wpb = waves_per_block;
+  return MASS_LYO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+// lyo_config.hpp -- host-side initial configuration.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace lyo {
+
+struct HostInit {
+  unsigned LX, LY;
+  int config;
+  uint32_t seed;
+  double level, radius, angle_deg, noise, qamp, phinoise;
+  double conc, rho;
+};
+
+// Fills ff (N*9 AoS), qxx, qyx, phi; returns the check totals ftot, ptot.
+int configure_host(const HostInit& in, std::vector<double>& ff, std::vector<double>& qxx,
+                   std::vector<double>& qyx, std::vector<double>& phi, double* ftot,
+                   double* ptot, std::string& err);
+
+}  // namespace lyo

@@ -1,0 +1,270 @@
+// lyo_physics.hpp -- per-node arithmetic of the lyotropic step, for gfx950.
+//
+// Two instantiations share this code:
+//   EXACT=true,  T=double : the reference's FP64 arithmetic in the reference's
+//                           expression order (true divisions, +0*stencil terms,
+//                           sequential sums).  Built with -ffp-contract=off so
+//                           every operation rounds exactly as the reference's
+//                           x86-64 build does.
+//   EXACT=false, T=float  : populations stored as deviations f' = f - w_i*rho
+//                           from the rest equilibrium, reciprocals instead of
+//                           divisions, FMA contraction allowed.
+//
+// Reference: src/models/lyotropic.cpp:168-341, src/lb.hpp:15-41,73-90,127-136.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lyo {
+
+template <class T>
+struct Params {
+  T GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
+  T AA, CC, LL, KK, zeta, zetaI, conc;
+};
+
+// Rest-equilibrium weights fe(u=0, n=1) of src/lb.hpp:24-41: these define the
+// shift of the FP32 populations (they are not the forcing weights w[]).
+__device__ __host__ constexpr double rest_weight(int v) {
+  return v == 0 ? 4. / 9. : (v < 5 ? 1. / 9. : 1. / 36.);
+}
+
+// ---------------------------------------------------------- stencils
+// src/lb.hpp:73-90: derivX/derivY = .5*(a[d1]-a[d2]) + 0*stencil
+template <class T, bool EXACT>
+__device__ __forceinline__ T dcen(T ap, T am) {
+  if constexpr (EXACT) {
+    return T(.5) * (ap - am) + T(0);
+  } else {
+    return T(.5) * (ap - am);
+  }
+}
+// src/lb.hpp:127-136: a[d1]+a[d2]+a[d3]+a[d4] - 4*a[d0] + 0*stencil, with
+// d1 = x+1, d2 = x-1, d3 = y+1, d4 = y-1.
+template <class T, bool EXACT>
+__device__ __forceinline__ T lap5(T c, T xp, T xm, T yp, T ym) {
+  if constexpr (EXACT) {
+    return xp + xm + yp + ym - T(4) * c + T(0);
+  } else {
+    return (xp + xm) + (yp + ym) - T(4) * c;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T fast_rcp(T x) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_amdgcn_rcpf(x);
+  } else {
+    return T(1) / x;
+  }
+}
+
+// Values of one node that its neighbours' UpdateFields reads
+// (ux, uy, ux_phi, uy_phi, MU, sigma** of lyotropic.cpp:224-239).
+template <class T>
+struct NbrVals {
+  T ux, uy, uxp, uyp, mu, sxx, syy, sxy, syx;
+};
+// Values UpdateFields reads at the node itself (lyotropic.cpp:255-266).
+template <class T>
+struct CenterVals {
+  T nn, dn, vx, vy, Hxx, Hyx, dxQxx, dyQxx, dxQyx, dyQyx;
+};
+// Diagnostic stress parts written into frames (lyotropic.cpp:240-246).
+template <class T>
+struct DiagVals {
+  T bulk, el_xx, el_yx, pf_xx, pf_yx, act_xx, act_yx;
+};
+
+// UpdateQuantitiesAtNode (src/models/lyotropic.cpp:168-247) for one node.
+// f: populations at the node (EXACT: f; else f').  q*/ph: 5-point stencil
+// values (centre, x+1, x-1, y+1, y-1) of Qxx, Qyx, phi.
+template <class T, bool EXACT, bool DIAG = false>
+__device__ __forceinline__ void update_quantities(
+    const T (&f)[9], T qxx_c, T qxx_xp, T qxx_xm, T qxx_yp, T qxx_ym, T qyx_c,
+    T qyx_xp, T qyx_xm, T qyx_yp, T qyx_ym, T p_c, T p_xp, T p_xm, T p_yp,
+    T p_ym, const Params<T>& P, NbrVals<T>& o, CenterVals<T>& c,
+    DiagVals<T>* dg = nullptr) {
+  const T Qxx = qxx_c, Qyx = qyx_c, p = p_c;
+  T nn, dn, vx, vy;
+  if constexpr (EXACT) {
+    nn = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    vx = (f[1] - f[2] + f[5] - f[6] - f[7] + f[8]) / nn;
+    vy = (f[3] - f[4] + f[5] - f[6] + f[7] - f[8]) / nn;
+    dn = T(0);
+  } else {
+    // n = rho + sum f' (the rest weights sum to one); momentum is unchanged
+    // by the shift because opposite directions carry equal weights.
+    dn = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+    nn = P.rho + dn;
+    const T inv = fast_rcp(nn);
+    vx = ((f[1] - f[2]) + (f[5] - f[6]) + (f[8] - f[7])) * inv;
+    vy = ((f[3] - f[4]) + (f[5] - f[6]) + (f[7] - f[8])) * inv;
+  }
+
+  const T del2Qxx = lap5<T, EXACT>(qxx_c, qxx_xp, qxx_xm, qxx_yp, qxx_ym);
+  const T dxQxx = dcen<T, EXACT>(qxx_xp, qxx_xm);
+  const T dyQxx = dcen<T, EXACT>(qxx_yp, qxx_ym);
+  const T del2Qyx = lap5<T, EXACT>(qyx_c, qyx_xp, qyx_xm, qyx_yp, qyx_ym);
+  const T dxQyx = dcen<T, EXACT>(qyx_xp, qyx_xm);
+  const T dyQyx = dcen<T, EXACT>(qyx_yp, qyx_ym);
+  const T del2p = lap5<T, EXACT>(p_c, p_xp, p_xm, p_yp, p_ym);
+  const T dxPhi = dcen<T, EXACT>(p_xp, p_xm);
+  const T dyPhi = dcen<T, EXACT>(p_yp, p_ym);
+
+  const T term = T(0) - Qxx * Qxx - Qyx * Qyx;
+  const T Hxx = P.CC * term * Qxx + P.LL * del2Qxx;
+  const T Hyx = P.CC * term * Qyx + P.LL * del2Qyx;
+  T mu;
+  if constexpr (EXACT) {
+    mu = P.AA * p * (T(1) - p) * (T(1) - T(2) * p) + T(0) * P.CC * term - P.KK * del2p;
+  } else {
+    mu = P.AA * p * (T(1) - p) * (T(1) - T(2) * p) - P.KK * del2p;
+  }
+
+  const T sigmaB = T(.5) * P.AA * p * p * (T(1) - p) * (T(1) - p) +
+                   T(.5) * P.CC * term * term - mu * p;
+  const T sigmaF = T(2) * P.xi * ((Qxx * Qxx - T(1)) * Hxx + Qxx * Qyx * Hyx) -
+                   P.zeta * Qxx * (T(1) - p) +
+                   T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) +
+                   P.LL * (dyQxx * dyQxx + dyQyx * dyQyx - dxQxx * dxQxx - dxQyx * dxQyx);
+  const T sigmaS = T(2) * P.xi * (Qyx * Qxx * Hxx + (Qyx * Qyx - T(1)) * Hyx) -
+                   P.zeta * Qyx * (T(1) - p) - P.KK * dxPhi * dyPhi -
+                   T(2) * P.LL * (dxQxx * dyQxx + dxQyx * dyQyx);
+  const T sigmaA = T(2) * (Qxx * Hyx - Qyx * Hxx);
+
+  o.ux = vx;
+  o.uy = vy;
+  o.uxp = vx * p;
+  o.uyp = vy * p;
+  o.mu = mu;
+  o.sxx = sigmaF + sigmaB + P.zetaI * (P.conc - p);
+  o.syy = -sigmaF + sigmaB + P.zetaI * (P.conc - p);
+  o.sxy = sigmaS + sigmaA;
+  o.syx = sigmaS - sigmaA;
+
+  c.nn = nn;
+  c.dn = dn;
+  c.vx = vx;
+  c.vy = vy;
+  c.Hxx = Hxx;
+  c.Hyx = Hyx;
+  c.dxQxx = dxQxx;
+  c.dyQxx = dyQxx;
+  c.dxQyx = dxQyx;
+  c.dyQyx = dyQyx;
+
+  if constexpr (DIAG) {
+    dg->bulk = sigmaB + P.zetaI * (P.conc - p);
+    dg->el_xx = sigmaF - T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) + P.zeta * Qxx * (T(1) - p);
+    dg->el_yx = sigmaS + P.KK * dxPhi * dyPhi + P.zeta * Qxx * (T(1) - p);
+    dg->pf_xx = T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi);
+    dg->pf_yx = -P.KK * dxPhi * dyPhi;
+    dg->act_xx = -P.zeta * Qxx * (T(1) - p);
+    dg->act_yx = -P.zeta * Qyx * (T(1) - p);
+  }
+}
+
+// Increments of UpdateFieldsAtNode (src/models/lyotropic.cpp:268-308): the
+// Beris-Edwards D, the Cahn-Hilliard Dp, the force and the collision operator
+// Omega_v = (fe_v - f_v)/tau + w_v (F . c_v).
+// Stencil inputs: x+1 / x-1 values of ux, uy, ux_phi, sigmaXX, sigmaYX, MU and
+// y+1 / y-1 values of ux, uy, uy_phi, sigmaXY, sigmaYY, MU.
+template <class T>
+struct UFStencil {
+  T mu_c;
+  T ux_xp, ux_xm, uy_xp, uy_xm, uxp_xp, uxp_xm, sxx_xp, sxx_xm, syx_xp, syx_xm, mu_xp, mu_xm;
+  T ux_yp, ux_ym, uy_yp, uy_ym, uyp_yp, uyp_ym, sxy_yp, sxy_ym, syy_yp, syy_ym, mu_yp, mu_ym;
+};
+
+template <class T, bool EXACT>
+__device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Qyx,
+                                              T p, const T (&f)[9],
+                                              const UFStencil<T>& s,
+                                              const Params<T>& P, T& Dxx,
+                                              T& Dyx, T& Dp, T (&om)[9]) {
+  const T nn = c.nn, vx = c.vx, vy = c.vy;
+  const T Hxx = c.Hxx, Hyx = c.Hyx;
+  const T dxQxx = c.dxQxx, dyQxx = c.dyQxx, dxQyx = c.dxQyx, dyQyx = c.dyQyx;
+
+  const T del2mu = lap5<T, EXACT>(s.mu_c, s.mu_xp, s.mu_xm, s.mu_yp, s.mu_ym);
+  const T dxux = dcen<T, EXACT>(s.ux_xp, s.ux_xm);
+  const T dyux = dcen<T, EXACT>(s.ux_yp, s.ux_ym);
+  const T dxuy = dcen<T, EXACT>(s.uy_xp, s.uy_xm);
+  const T dyuy = dcen<T, EXACT>(s.uy_yp, s.uy_ym);
+  const T dxSxx = dcen<T, EXACT>(s.sxx_xp, s.sxx_xm);
+  const T dySxy = dcen<T, EXACT>(s.sxy_yp, s.sxy_ym);
+  const T dxSyx = dcen<T, EXACT>(s.syx_xp, s.syx_xm);
+  const T dySyy = dcen<T, EXACT>(s.syy_yp, s.syy_ym);
+  const T pFlux = dcen<T, EXACT>(s.uxp_xp, s.uxp_xm) + dcen<T, EXACT>(s.uyp_yp, s.uyp_ym);
+
+  const T expansion = dxux + dyuy;
+  const T shear = T(.5) * (dxuy + dyux);
+  const T vorticity = T(.5) * (dxuy - dyux);
+  const T traceQL = Qxx * (dxux - dyuy) + T(2) * Qyx * shear;
+
+  Dxx = P.GammaQ * Hxx - vx * dxQxx - vy * dyQxx - T(2) * vorticity * Qyx +
+        P.xi * ((Qxx + T(1)) * (T(2) * dxux - traceQL) + T(2) * Qyx * shear - expansion);
+  Dyx = P.GammaQ * Hyx - vx * dxQyx - vy * dyQyx + T(2) * vorticity * Qxx +
+        P.xi * (Qyx * (expansion - traceQL) + T(2) * shear);
+  if constexpr (EXACT) {
+    // conserve_phi is false for plain Lyotropic: the correction term is 0
+    Dp = P.GammaP * del2mu - pFlux - T(0);
+  } else {
+    Dp = P.GammaP * del2mu - pFlux;
+  }
+
+  const T Fx = dxSxx + dySxy - P.friction * vx;
+  const T Fy = dxSyx + dySyy - P.friction * vy;
+  const T tau = P.tauNem * p + P.tauIso * (T(1) - p);
+
+  if constexpr (EXACT) {
+    // src/lb.hpp:24-41 in the reference order
+    T fe[9];
+    const T v2 = vx * vx + vy * vy;
+    fe[1] = nn * (1. / 9. + 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
+    fe[2] = nn * (1. / 9. - 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
+    fe[3] = nn * (1. / 9. + 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
+    fe[4] = nn * (1. / 9. - 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
+    fe[5] = nn * (1. / 36. + 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
+    fe[6] = nn * (1. / 36. - 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
+    fe[7] = nn * (1. / 36. - 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
+    fe[8] = nn * (1. / 36. + 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
+    fe[0] = nn - fe[1] - fe[2] - fe[3] - fe[4] - fe[5] - fe[6] - fe[7] - fe[8];
+    // w[v]*(Fx*xdir(v) + Fy*ydir(v)) with w = {0, 1/3 x4, 1/12 x4} (lb.hpp:15-16)
+    constexpr int cx[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
+    constexpr int cy[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+    constexpr double wf[9] = {0, 1 / 3., 1 / 3., 1 / 3., 1 / 3., 1 / 12., 1 / 12., 1 / 12., 1 / 12.};
+#pragma unroll
+    for (int v = 0; v < 9; ++v)
+      om[v] = (fe[v] - f[v]) / tau + wf[v] * (Fx * T(cx[v]) + Fy * T(cy[v]));
+  } else {
+    // shifted equilibrium fe'_v = fe_v - rest_weight(v)*rho
+    //   = dn*rest_weight(v) + nn*(u-dependent part of fe_v/nn)
+    const T it = fast_rcp(tau);
+    const T a = T(1. / 3.) * (vx * vx + vy * vy);
+    const T bx = T(.5) * vx * vx, by = T(.5) * vy * vy;
+    const T sx = T(1. / 3.) * vx, sy = T(1. / 3.) * vy;
+    const T v2q = T(1. / 12.) * (vx * vx + vy * vy);
+    const T xy = T(.25) * vx * vy;
+    const T pp = T(1. / 12.) * (vx + vy), pm = T(1. / 12.) * (vx - vy);
+    const T dn9 = c.dn * T(1. / 9.), dn36 = c.dn * T(1. / 36.);
+    T fe[9];
+    fe[1] = dn9 + nn * (a - by + sx);
+    fe[2] = dn9 + nn * (a - by - sx);
+    fe[3] = dn9 + nn * (a - bx + sy);
+    fe[4] = dn9 + nn * (a - bx - sy);
+    fe[5] = dn36 + nn * (v2q + xy + pp);
+    fe[6] = dn36 + nn * (v2q + xy - pp);
+    fe[7] = dn36 + nn * (v2q - xy - pm);
+    fe[8] = dn36 + nn * (v2q - xy + pm);
+    fe[0] = c.dn - (((fe[1] + fe[2]) + (fe[3] + fe[4])) + ((fe[5] + fe[6]) + (fe[7] + fe[8])));
+    const T F3x = T(1. / 3.) * Fx, F3y = T(1. / 3.) * Fy;
+    const T F12p = T(1. / 12.) * (Fx + Fy), F12m = T(1. / 12.) * (Fx - Fy);
+    const T force[9] = {T(0), F3x, -F3x, F3y, -F3y, F12p, -F12p, -F12m, F12m};
+#pragma unroll
+    for (int v = 0; v < 9; ++v) om[v] = (fe[v] - f[v]) * it + force[v];
+  }
+}
+
+}  // namespace lyo

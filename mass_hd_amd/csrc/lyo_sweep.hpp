@@ -1,0 +1,289 @@
+// lyo_sweep.hpp -- fused predictor / corrector sweep kernels (gfx950).
+//
+// One sweep = UpdateQuantities + UpdateFields (+ Move for the predictor) of
+// src/models/lyotropic.cpp:343-386,500-525, fused so that the derived fields
+// (u, mu, sigma, H, grad Q) never leave registers.
+//
+// Work decomposition ("strip march"): one 64-lane wavefront owns a strip of
+// 64 consecutive y columns (y is the contiguous index, k = y + LY*x) and
+// marches along x over a chunk of rows.  Lanes 0-1 and 62-63 are halo: the
+// second-order stencil dependency (UpdateFields reads u/mu/sigma at the four
+// neighbours, which themselves read Q/phi at their four neighbours) needs two
+// columns on each side, so each wave writes 60 interior columns.  Neighbours
+// in y come from the adjacent lanes through DPP wave shifts; neighbours in x
+// are the previous / next row kept in registers.  Every global access is a
+// contiguous 64-lane load or store of one SoA plane.
+#pragma once
+
+#include "lyo_physics.hpp"
+
+namespace lyo {
+
+constexpr int kWave = 64;
+constexpr int kHalo = 2;
+constexpr int kStrip = kWave - 2 * kHalo;
+
+// value held by lane-1 (the y-1 neighbour); lane 0 receives 0
+__device__ __forceinline__ float from_ym(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+// value held by lane+1 (the y+1 neighbour); lane 63 receives 0
+__device__ __forceinline__ float from_yp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_ym(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double from_yp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+struct Geo {
+  int LX, LY, N;
+  int NP;  // plane stride (>= N) of every SoA field
+  int nstrips, nchunks, xc;
+  int nwaves;
+  int xcd;  // remap block ids so each XCD gets a contiguous band of chunks
+};
+
+enum Stage { PRED = 0, CORR = 1, CORR_LAST = 2 };
+
+// Buffer roles per stage (planes of N values; populations are 9 planes).
+//  PRED      in : f_in, qxx_in, qyx_in, phi_in
+//            out: f_out (streamed f*), fn_out (streamed fn*), qxx_out/qyx_out
+//                 (Q+D), qnxx_out/qnyx_out (Q+D/2), EXACT: phi_out (phi+Dp),
+//                 phn_out (phi+Dp/2); FP32: phi_out = Dp
+//  CORR      in : f_in, q*_in, phi_in (+ FP32: dphi_in, stencil phi =
+//                 phi_in + dphi_in), fn_in, qn*_in, phn_in (FP32: predictor Dp)
+//            out: f_out, qxx_out, qyx_out, phi_out (FP32: increment of phi)
+//  CORR_LAST as CORR; FP32 also reads phic_in and writes phi_out, phic_out
+//            (compensated phi).  For EXACT, CORR_LAST == CORR.
+template <class T>
+struct SweepArgs {
+  const T* f_in;
+  const T* qxx_in;
+  const T* qyx_in;
+  const T* phi_in;
+  const T* dphi_in;
+  const T* fn_in;
+  const T* qnxx_in;
+  const T* qnyx_in;
+  const T* phn_in;
+  const T* phic_in;
+  T* f_out;
+  T* fn_out;
+  T* qxx_out;
+  T* qyx_out;
+  T* qnxx_out;
+  T* qnyx_out;
+  T* phi_out;
+  T* phn_out;
+  T* phic_out;
+};
+
+template <class T>
+struct QP {
+  T xx, yx, p;
+};
+
+template <class T, bool EXACT, int STAGE>
+__global__ void __launch_bounds__(256)
+sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
+  int b = blockIdx.x;
+  if (g.xcd) {
+    // blocks are dispatched round-robin over the 8 XCDs; give XCD i the
+    // logical blocks [i*nb8, (i+1)*nb8) so halo rows stay in one L2
+    const int nb8 = (int)gridDim.x >> 3;
+    if (b < (nb8 << 3)) b = (b & 7) * nb8 + (b >> 3);
+  }
+  const int wid = b * ((int)blockDim.x >> 6) + ((int)threadIdx.x >> 6);
+  if (wid >= g.nwaves) return;  // whole waves exit; no block barriers below
+  const int lane = threadIdx.x & 63;
+  const int strip = wid % g.nstrips;
+  const int chunk = wid / g.nstrips;
+  const int LX = g.LX, LY = g.LY, N = g.NP;
+
+  const int ycol = strip * kStrip - kHalo + lane;
+  int y = ycol % LY;
+  if (y < 0) y += LY;
+  const int yp1 = (y + 1 == LY) ? 0 : y + 1;
+  const int ym1 = (y == 0) ? LY - 1 : y - 1;
+  const bool active = lane >= kHalo && lane < kWave - kHalo && ycol < LY;
+  const int x0 = chunk * g.xc;
+  const int x1 = min(x0 + g.xc, LX);
+
+  auto roff = [&](int r) {
+    r %= LX;
+    if (r < 0) r += LX;
+    return r * LY;
+  };
+  auto load_qp = [&](int r) {
+    const int o = roff(r) + y;
+    QP<T> q;
+    q.xx = A.qxx_in[o];
+    q.yx = A.qyx_in[o];
+    T ph = A.phi_in[o];
+    if constexpr (!EXACT && STAGE != PRED) ph += A.dphi_in[o];
+    q.p = ph;
+    return q;
+  };
+  auto load_f = [&](int r, T(&f)[9]) {
+    const int o = roff(r) + y;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) f[v] = A.f_in[v * N + o];
+  };
+  auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
+                NbrVals<T>& nv, CenterVals<T>& cv) {
+    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, from_yp(q0.xx), from_ym(q0.xx),
+                                q0.yx, qp.yx, qm.yx, from_yp(q0.yx), from_ym(q0.yx),
+                                q0.p, qp.p, qm.p, from_yp(q0.p), from_ym(q0.p), P, nv,
+                                cv);
+  };
+
+  // prologue: UpdateQuantities on rows x0-1 and x0
+  NbrVals<T> um, u0;
+  CenterVals<T> c0;
+  T fc[9];
+  QP<T> qb, qc;
+  {
+    const QP<T> qa = load_qp(x0 - 2);
+    qb = load_qp(x0 - 1);
+    qc = load_qp(x0);
+    T ft[9];
+    CenterVals<T> ct;
+    load_f(x0 - 1, ft);
+    uq(qa, qb, qc, ft, um, ct);
+    const QP<T> qd = load_qp(x0 + 1);
+    load_f(x0, fc);
+    uq(qb, qc, qd, fc, u0, c0);
+    qb = qc;
+    qc = qd;
+  }
+  // invariant at loop top: qb = row x, qc = row x+1; um/u0 = rows x-1/x;
+  // c0, fc = centre data of row x
+  for (int x = x0; x < x1; ++x) {
+    const int ox = roff(x);
+    const int o = ox + y;
+    // centre inputs of the corrector, issued early
+    T fnc[9];
+    T qn_xx = T(0), qn_yx = T(0), phn = T(0);
+    if constexpr (STAGE != PRED) {
+#pragma unroll
+      for (int v = 0; v < 9; ++v) fnc[v] = A.fn_in[v * N + o];
+      qn_xx = A.qnxx_in[o];
+      qn_yx = A.qnyx_in[o];
+      phn = A.phn_in[o];
+    }
+    // UpdateQuantities on row x+1
+    const QP<T> qd = load_qp(x + 2);
+    T fnx[9];
+    load_f(x + 1, fnx);
+    NbrVals<T> un;
+    CenterVals<T> cn;
+    uq(qb, qc, qd, fnx, un, cn);
+
+    // UpdateFields on row x
+    UFStencil<T> s;
+    s.mu_c = u0.mu;
+    s.ux_xp = un.ux;
+    s.ux_xm = um.ux;
+    s.uy_xp = un.uy;
+    s.uy_xm = um.uy;
+    s.uxp_xp = un.uxp;
+    s.uxp_xm = um.uxp;
+    s.sxx_xp = un.sxx;
+    s.sxx_xm = um.sxx;
+    s.syx_xp = un.syx;
+    s.syx_xm = um.syx;
+    s.mu_xp = un.mu;
+    s.mu_xm = um.mu;
+    s.ux_yp = from_yp(u0.ux);
+    s.ux_ym = from_ym(u0.ux);
+    s.uy_yp = from_yp(u0.uy);
+    s.uy_ym = from_ym(u0.uy);
+    s.uyp_yp = from_yp(u0.uyp);
+    s.uyp_ym = from_ym(u0.uyp);
+    s.sxy_yp = from_yp(u0.sxy);
+    s.sxy_ym = from_ym(u0.sxy);
+    s.syy_yp = from_yp(u0.syy);
+    s.syy_ym = from_ym(u0.syy);
+    s.mu_yp = from_yp(u0.mu);
+    s.mu_ym = from_ym(u0.mu);
+    T Dxx, Dyx, Dp, om[9];
+    const T Qxx = qb.xx, Qyx = qb.yx, p = qb.p;
+    update_fields<T, EXACT>(c0, Qxx, Qyx, p, fc, s, P, Dxx, Dyx, Dp, om);
+
+    if (active) {
+      if constexpr (STAGE == PRED) {
+        A.qnxx_out[o] = Qxx + T(.5) * Dxx;
+        A.qnyx_out[o] = Qyx + T(.5) * Dyx;
+        A.qxx_out[o] = Qxx + Dxx;
+        A.qyx_out[o] = Qyx + Dyx;
+        if constexpr (EXACT) {
+          A.phn_out[o] = p + T(.5) * Dp;
+          A.phi_out[o] = p + Dp;
+        } else {
+          A.phi_out[o] = Dp;
+        }
+        // collide and push to (x+cx, y+cy) (Move, lyotropic.cpp:371-386)
+        const int rm = roff(x - 1), rp = roff(x + 1);
+        const int rowv[9] = {ox, rp, rm, ox, ox, rp, rm, rm, rp};
+        const int colv[9] = {y, y, y, yp1, ym1, yp1, ym1, yp1, ym1};
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+          const int d = v * N + rowv[v] + colv[v];
+          A.fn_out[d] = fc[v] + T(.5) * om[v];
+          A.f_out[d] = fc[v] + om[v];
+        }
+      } else {
+        A.qxx_out[o] = qn_xx + T(.5) * Dxx;
+        A.qyx_out[o] = qn_yx + T(.5) * Dyx;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) A.f_out[v * N + o] = fnc[v] + T(.5) * om[v];
+        if constexpr (EXACT) {
+          A.phi_out[o] = phn + T(.5) * Dp;
+        } else if constexpr (STAGE == CORR) {
+          A.phi_out[o] = T(.5) * (phn + Dp);
+        } else {
+          // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, so the
+          // rounding of the addition is carried to the next step
+          const T pold = A.phi_in[o];
+          const T yv = T(.5) * (phn + Dp) + A.phic_in[o];
+          const T t = pold + yv;
+          const T bp = t - pold;
+          const T ap = t - bp;
+          const T err = (pold - ap) + (yv - bp);
+          A.phi_out[o] = t;
+          A.phic_out[o] = err;
+        }
+      }
+    }
+    // rotate the rolling rows
+    um = u0;
+    u0 = un;
+    c0 = cn;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) fc[v] = fnx[v];
+    qb = qc;
+    qc = qd;
+  }
+}
+
+// Launch helper: waves_per_block waves of 64 lanes per block.
+template <class T, bool EXACT, int STAGE>
+inline hipError_t launch_sweep(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
+                               int waves_per_block, hipStream_t s) {
+  const int threads = 64 * waves_per_block;
+  const int blocks = (g.nwaves + waves_per_block - 1) / waves_per_block;
+  hipLaunchKernelGGL((sweep_kernel<T, EXACT, STAGE>), dim3(blocks), dim3(threads), 0, s, a,
+                     p, g);
+  return hipGetLastError();
+}
+
+}  // namespace lyo

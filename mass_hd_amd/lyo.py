@@ -1,0 +1,208 @@
+"""Python host binding of the C ABI in include/mass_lyo.h.
+
+Mirrors the reference's Model interface for the lyotropic model
+(src/models.hpp:30-116, src/models/lyotropic.{hpp,cpp}): ``Lyotropic``
+takes the runcard options of ``Lyotropic::GetOptions``
+(src/models/lyotropic.cpp:551-602), ``configure`` stands in for ``Configure``,
+``step`` for ``Step``, ``runtime_checks`` for ``RuntimeChecks`` and
+``frame`` for ``serialize_frame``.  Every call goes to the HIP library; there
+is no CPU path, and constructing a model without the built library or without
+a gfx950 device raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmass_lyo.so")
+
+FP64, FP32 = 0, 1
+
+CONFIGS = {
+    "circle": 0, "square": 1, "stripe": 2, "half": 3, "random": 4,
+    "boxatwall": 5, "cuttingatwall": 6, "circleatwall": 7, "wettedwall": 8,
+}
+
+FIELDS = {
+    "ff": 0, "QQxx": 1, "QQyx": 2, "phi": 3, "n": 4, "ux": 5, "uy": 6,
+    "sigmaXX": 7, "sigmaXY": 8, "sigmaYX": 9, "sigmaYY": 10,
+    "sigma_bulk": 11, "sigma_elastic_xx": 12, "sigma_elastic_yx": 13,
+    "sigma_phase_field_xx": 14, "sigma_phase_field_yx": 15,
+    "sigma_active_xx": 16, "sigma_active_yx": 17, "MU": 18, "HHxx": 19, "HHyx": 20,
+}
+
+# the frame of Lyotropic::serialize_frame (src/models/lyotropic.hpp:121-138)
+FRAME_FIELDS = ["ff", "QQxx", "QQyx", "phi", "sigmaXX", "sigmaXY", "sigmaYX", "sigmaYY",
+                "sigma_bulk", "sigma_elastic_xx", "sigma_elastic_yx",
+                "sigma_phase_field_xx", "sigma_phase_field_yx", "sigma_active_xx",
+                "sigma_active_yx"]
+
+PHYS_KEYS = ["GammaQ", "GammaP", "xi", "tauNem", "tauIso", "rho", "friction",
+             "AA", "CC", "LL", "KK", "zeta", "zetaI", "conc"]
+
+
+class Params(ctypes.Structure):
+    _fields_ = ([("LX", ctypes.c_int32), ("LY", ctypes.c_int32), ("npc", ctypes.c_int32),
+                 ("precision", ctypes.c_int32), ("conserve_phi", ctypes.c_int32),
+                 ("flags", ctypes.c_int32)] + [(k, ctypes.c_double) for k in PHYS_KEYS])
+
+
+class Init(ctypes.Structure):
+    _fields_ = [("config", ctypes.c_int32), ("seed", ctypes.c_uint32),
+                ("level", ctypes.c_double), ("radius", ctypes.c_double),
+                ("angle_deg", ctypes.c_double), ("noise", ctypes.c_double),
+                ("qamp", ctypes.c_double), ("phinoise", ctypes.c_double)]
+
+
+class MassLyoError(RuntimeError):
+    """A non-zero status of the C ABI; ``code`` is the mass_lyo_status."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def load_library():
+    """Loads libmass_lyo.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmass_lyo.so is not built (%s); run __graft_entry__.build()"
+                           % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    dp = P(ctypes.c_double)
+    h = ctypes.c_void_p
+    lib.mass_lyo_create.argtypes = [P(Params), P(h)]
+    lib.mass_lyo_destroy.argtypes = [h]
+    lib.mass_lyo_destroy.restype = None
+    lib.mass_lyo_last_error.argtypes = [h]
+    lib.mass_lyo_last_error.restype = ctypes.c_char_p
+    lib.mass_lyo_configure.argtypes = [h, P(Init), dp, dp]
+    lib.mass_lyo_set_state.argtypes = [h, dp, dp, dp, dp]
+    lib.mass_lyo_step.argtypes = [h, ctypes.c_int32]
+    lib.mass_lyo_get_field.argtypes = [h, ctypes.c_int32, dp]
+    lib.mass_lyo_sums.argtypes = [h, dp, dp]
+    lib.mass_lyo_synchronize.argtypes = [h]
+    lib.mass_lyo_time_steps.argtypes = [h, ctypes.c_int32, dp, dp]
+    lib.mass_lyo_set_launch.argtypes = [h, ctypes.c_int32, ctypes.c_int32]
+    lib.mass_lyo_abi_version.argtypes = []
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Lyotropic:
+    """One lyotropic run on the GPU (the reference's ``Lyotropic`` model)."""
+
+    def __init__(self, LX, LY, npc=1, precision=FP64, **phys):
+        self.lib = load_library()
+        if isinstance(precision, str):
+            precision = {"fp64": FP64, "fp32": FP32}[precision]
+        missing = [k for k in PHYS_KEYS if k not in phys]
+        if missing:
+            raise ValueError("missing model parameters: %s" % ", ".join(missing))
+        unknown = set(phys) - set(PHYS_KEYS)
+        if unknown:
+            raise ValueError("unknown model parameters: %s" % ", ".join(sorted(unknown)))
+        self.LX, self.LY, self.N = int(LX), int(LY), int(LX) * int(LY)
+        self.npc, self.precision = int(npc), precision
+        self.phys = dict(phys)
+        p = Params(LX=self.LX, LY=self.LY, npc=self.npc, precision=precision,
+                   conserve_phi=0, flags=0, **{k: float(phys[k]) for k in PHYS_KEYS})
+        h = ctypes.c_void_p()
+        rc = self.lib.mass_lyo_create(ctypes.byref(p), ctypes.byref(h))
+        if rc != 0:
+            raise MassLyoError(rc, self.lib.mass_lyo_last_error(None).decode())
+        self.h = h
+        self.ftot = 0.0
+        self.ptot = 0.0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MassLyoError(rc, self.lib.mass_lyo_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mass_lyo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- Lyotropic::Configure
+    def configure(self, config="random", seed=1, level=0.0, radius=0.0, angle=0.0, noise=1.0,
+                  qamp=1.0, phinoise=0.0):
+        cfg = CONFIGS[config] if isinstance(config, str) else int(config)
+        ini = Init(config=cfg, seed=seed, level=level, radius=radius, angle_deg=angle,
+                   noise=noise, qamp=qamp, phinoise=phinoise)
+        ft, pt = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.mass_lyo_configure(self.h, ctypes.byref(ini), ctypes.byref(ft),
+                                                ctypes.byref(pt)))
+        self.ftot, self.ptot = ft.value, pt.value
+        return self.ftot, self.ptot
+
+    def set_state(self, ff, Qxx, Qyx, phi):
+        ff = np.ascontiguousarray(ff, dtype=np.float64).reshape(self.N * 9)
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(self.N) for x in (Qxx, Qyx, phi)]
+        self._check(self.lib.mass_lyo_set_state(self.h, _dptr(ff), _dptr(a[0]), _dptr(a[1]),
+                                                _dptr(a[2])))
+        self.ftot = float(ff.sum())
+        self.ptot = float(a[2].sum())
+
+    # ---- Lyotropic::Step
+    def step(self, nsteps=1):
+        self._check(self.lib.mass_lyo_step(self.h, int(nsteps)))
+
+    def synchronize(self):
+        self._check(self.lib.mass_lyo_synchronize(self.h))
+
+    def get(self, name):
+        fid = FIELDS[name] if isinstance(name, str) else int(name)
+        n = self.N * (9 if fid == 0 else 1)
+        out = np.empty(n, dtype=np.float64)
+        self._check(self.lib.mass_lyo_get_field(self.h, fid, _dptr(out)))
+        return out.reshape(self.N, 9) if fid == 0 else out
+
+    # ---- Lyotropic::serialize_frame
+    def frame(self):
+        return {k: self.get(k) for k in FRAME_FIELDS}
+
+    def sums(self):
+        f, p = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.mass_lyo_sums(self.h, ctypes.byref(f), ctypes.byref(p)))
+        return f.value, p.value
+
+    # ---- Lyotropic::RuntimeChecks (src/models/lyotropic.cpp:527-549)
+    def runtime_checks(self):
+        fcheck, pcheck = self.sums()
+        if abs(self.ftot - fcheck) > 1:
+            raise MassLyoError(4, "f is not conserved (%g/%g)" % (self.ftot, fcheck))
+        return fcheck, pcheck
+
+    def time_steps(self, nsteps, per_sweep=True):
+        tot = ctypes.c_double()
+        sw = (ctypes.c_double * 2)()
+        self._check(self.lib.mass_lyo_time_steps(self.h, int(nsteps), ctypes.byref(tot),
+                                                 sw if per_sweep else None))
+        return tot.value, (sw[0], sw[1])
+
+    def set_launch(self, chunk_rows=0, waves_per_block=0):
+        self._check(self.lib.mass_lyo_set_launch(self.h, int(chunk_rows), int(waves_per_block)))

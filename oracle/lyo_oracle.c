@@ -1,0 +1,509 @@
+/*
+ * lyo_oracle.c -- CPU oracle for the lyotropic active-nematic time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU-baseline
+ * "port".  Nothing in the shipped library links or calls this file.
+ *
+ * Literal FP64 restatement of the reference, following it line by line:
+ *   periodic neighbour table        src/geometry.cpp:100-110, src/tools.hpp:24-28
+ *   equilibrium distribution        src/lb.hpp:24-41
+ *   derivX / derivY / laplacian     src/lb.hpp:73-90, 127-136
+ *   UpdateQuantitiesAtNode          src/models/lyotropic.cpp:168-247
+ *   UpdateFieldsAtNode              src/models/lyotropic.cpp:249-341
+ *   UpdateQuantities/Fields, Move   src/models/lyotropic.cpp:343-386
+ *   Step                            src/models/lyotropic.cpp:500-525
+ *   RuntimeChecks sums              src/models/lyotropic.cpp:527-549
+ *   ConfigureAtNode                 src/models/lyotropic.cpp:69-166
+ *   mt19937 + random_real           src/random.cpp:12,21-24,43-46 (libstdc++
+ *                                   generate_canonical<double,53>: two 32-bit
+ *                                   draws, (g1 + g2*2^32)/2^64, clamped < 1;
+ *                                   uniform_real: U*(b-a)+a)
+ * Expression order is kept exactly (no reassociation; build without FMA
+ * contraction) so that an IEEE-exact device path can match it bit for bit.
+ * Layout is the reference's: AoS populations ff[k][v], k = y + LY*x.
+ */
+#include "lyo_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* src/lb.hpp:15-16: forcing weights */
+static const double W[9] = {0, 1/3., 1/3., 1/3., 1/3.,
+                            1/12., 1/12., 1/12., 1/12.};
+/* src/lb.hpp:21 */
+static const double SB = 1/12., SD = 1/12.;
+/* src/geometry.hpp:17-27 */
+static const int XDIR[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
+static const int YDIR[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+
+struct lyo_oracle {
+  lyo_oracle_params p;
+  uint32_t N;
+  uint32_t* nb; /* N*9 */
+  double *ff, *fn, *ff_tmp, *fn_tmp; /* N*9 each */
+  double* s[LYO_O_NFIELDS];          /* scalar fields (index >= 2) */
+  double* phi_tmp;
+  double totalphi, countphi, ftot, ptot;
+};
+
+/* ---------------------------------------------------------------- helpers */
+
+/* src/tools.hpp:24-28 (the double overload is the one the periodic table
+ * reaches: long + unsigned arguments fail template deduction) */
+static double modu_d(double num, double div) {
+  if (num < 0) return div + fmod(num, div);
+  return fmod(num, div);
+}
+
+static inline const uint32_t* get_neighbours(const lyo_oracle* o, uint32_t k) {
+  /* std::vector::at bounds check of geometry.hpp:544-545 */
+  if (k >= o->N) {
+    fprintf(stderr, "lyo_oracle: neighbour index %u out of range\n", k);
+    abort();
+  }
+  return o->nb + 9 * (size_t)k;
+}
+
+/* src/lb.hpp:73-90,127-136: the active (non-commented) stencils */
+static inline double derivX(const double* a, const uint32_t* d, double st) {
+  return .5 * (a[d[1]] - a[d[2]]) + 0 * st;
+}
+static inline double derivY(const double* a, const uint32_t* d, double st) {
+  return .5 * (a[d[3]] - a[d[4]]) + 0 * st;
+}
+static inline double laplacian(const double* a, const uint32_t* d, double st) {
+  return a[d[1]] + a[d[2]] + a[d[3]] + a[d[4]] - 4 * a[d[0]] + 0 * st;
+}
+
+/* src/lb.hpp:24-41 */
+static inline void equilibrium(double vx, double vy, double nn, double* fe) {
+  const double v2 = vx * vx + vy * vy;
+  fe[1] = nn * (1. / 9. + 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
+  fe[2] = nn * (1. / 9. - 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
+  fe[3] = nn * (1. / 9. + 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
+  fe[4] = nn * (1. / 9. - 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
+  fe[5] = nn * (1. / 36. + 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
+  fe[6] = nn * (1. / 36. - 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
+  fe[7] = nn * (1. / 36. - 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
+  fe[8] = nn * (1. / 36. + 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
+  fe[0] = nn - fe[1] - fe[2] - fe[3] - fe[4] - fe[5] - fe[6] - fe[7] - fe[8];
+}
+
+void lyo_oracle_equilibrium(double vx, double vy, double nn, double* fe9) {
+  equilibrium(vx, vy, nn, fe9);
+}
+
+/* ------------------------------------------------------------ mt19937 RNG */
+/* The generator fixed by the C++ standard ([rand.predef]: mt19937). */
+typedef struct { uint32_t mt[624]; int idx; } mt_state;
+
+static void mt_seed(mt_state* s, uint32_t seed) {
+  s->mt[0] = seed;
+  for (int i = 1; i < 624; ++i)
+    s->mt[i] = 1812433253u * (s->mt[i - 1] ^ (s->mt[i - 1] >> 30)) + (uint32_t)i;
+  s->idx = 624;
+}
+
+static uint32_t mt_next(mt_state* s) {
+  if (s->idx >= 624) {
+    for (int i = 0; i < 624; ++i) {
+      uint32_t y = (s->mt[i] & 0x80000000u) | (s->mt[(i + 1) % 624] & 0x7fffffffu);
+      uint32_t v = s->mt[(i + 397) % 624] ^ (y >> 1);
+      if (y & 1u) v ^= 0x9908b0dfu;
+      s->mt[i] = v;
+    }
+    s->idx = 0;
+  }
+  uint32_t y = s->mt[s->idx++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+/* generate_canonical<double,53> over a 32-bit engine: k = 2 draws */
+static double mt_canonical(mt_state* s) {
+  double sum = 0, tmp = 1;
+  for (int k = 2; k != 0; --k) {
+    sum += (double)mt_next(s) * tmp;
+    tmp *= 4294967296.0;
+  }
+  double ret = sum / tmp;
+  if (ret >= 1.0) ret = nextafter(1.0, 0.0);
+  return ret;
+}
+
+/* uniform_real_distribution<>(lo, hi)(gen) */
+static double mt_real(mt_state* s, double lo, double hi) {
+  return (mt_canonical(s) * (hi - lo)) + lo;
+}
+
+void lyo_oracle_mt19937(uint32_t seed, int n, uint32_t* out) {
+  mt_state s;
+  mt_seed(&s, seed);
+  for (int i = 0; i < n; ++i) out[i] = mt_next(&s);
+}
+
+void lyo_oracle_random_real(uint32_t seed, int n, double lo, double hi, double* out) {
+  mt_state s;
+  mt_seed(&s, seed);
+  for (int i = 0; i < n; ++i) out[i] = mt_real(&s, lo, hi);
+}
+
+/* ------------------------------------------------------------- lifecycle */
+
+lyo_oracle* lyo_oracle_create(const lyo_oracle_params* p) {
+  if (p->LX < 2 || p->LY < 2) return NULL;
+  lyo_oracle* o = (lyo_oracle*)calloc(1, sizeof(lyo_oracle));
+  o->p = *p;
+  o->N = (uint32_t)p->LX * (uint32_t)p->LY;
+  const size_t N = o->N;
+  o->nb = (uint32_t*)malloc(sizeof(uint32_t) * 9 * N);
+  /* Field::SetSize value-initialises: every field starts at zero */
+  o->ff = (double*)calloc(9 * N, sizeof(double));
+  o->fn = (double*)calloc(9 * N, sizeof(double));
+  o->ff_tmp = (double*)calloc(9 * N, sizeof(double));
+  o->fn_tmp = (double*)calloc(9 * N, sizeof(double));
+  for (int f = 2; f < LYO_O_NFIELDS; ++f) o->s[f] = (double*)calloc(N, sizeof(double));
+  o->phi_tmp = (double*)calloc(N, sizeof(double));
+  /* geometry.cpp:100-110: k = y + LY*x, neighbour v at ((x+cx) mod LX, (y+cy) mod LY) */
+  for (uint32_t k = 0; k < o->N; ++k) {
+    const long x = k / (uint32_t)p->LY;
+    const long y = k % (uint32_t)p->LY;
+    for (int v = 0; v < 9; ++v) {
+      const unsigned xx = (unsigned)modu_d((double)(x + XDIR[v]), (double)(unsigned)p->LX);
+      const unsigned yy = (unsigned)modu_d((double)(y + YDIR[v]), (double)(unsigned)p->LY);
+      o->nb[9 * (size_t)k + v] = yy + (unsigned)p->LY * xx;
+    }
+  }
+  return o;
+}
+
+void lyo_oracle_destroy(lyo_oracle* o) {
+  if (!o) return;
+  free(o->nb);
+  free(o->ff); free(o->fn); free(o->ff_tmp); free(o->fn_tmp);
+  for (int f = 2; f < LYO_O_NFIELDS; ++f) free(o->s[f]);
+  free(o->phi_tmp);
+  free(o);
+}
+
+void lyo_oracle_neighbours(const lyo_oracle* o, uint32_t* out) {
+  memcpy(out, o->nb, sizeof(uint32_t) * 9 * (size_t)o->N);
+}
+
+#define F(id) (o->s[id])
+
+/* ------------------------------------------------------------- configure */
+
+static unsigned udiff(unsigned a, unsigned b) { return a > b ? a - b : b - a; }
+static unsigned uwrap(unsigned x, unsigned L) {
+  unsigned a = x % L, b = L - x % L;
+  return a < b ? a : b;
+}
+
+int lyo_oracle_configure(lyo_oracle* o, const lyo_oracle_init* in) {
+  const unsigned LX = (unsigned)o->p.LX, LY = (unsigned)o->p.LY;
+  mt_state rng;
+  mt_seed(&rng, in->seed);
+  const double angle = in->angle_deg * M_PI / 180.;
+  const double level = in->level, radius = in->radius;
+  o->totalphi = 0; o->ftot = 0; o->ptot = 0;
+  for (unsigned k = 0; k < o->N; ++k) {
+    double nematicOrder = 0;
+    const unsigned x = k / LY, y = k % LY;
+    switch (in->config) {
+      case 0: /* circle */
+        if (pow(udiff(x, LX / 2), 2) + pow(udiff(y, LY / 2), 2) <= radius * radius) nematicOrder = 1;
+        break;
+      case 1: /* square */
+        if (udiff(LY / 2, y) < level / 2 && udiff(LX / 2, x) < level / 2) nematicOrder = 1;
+        break;
+      case 2: /* stripe */
+        if (udiff(LY / 2, y) < level / 2) nematicOrder = 1;
+        break;
+      case 3: /* half */
+        if (y < level) nematicOrder = 1;
+        break;
+      case 4: /* random: uniform order, phi drawn per node */
+        nematicOrder = 1;
+        break;
+      case 5: /* boxatwall, bc=0 branch */
+      case 6: /* cuttingatwall, bc=0 branch */
+        if (y < level && udiff(LX / 2, x) < level) nematicOrder = 1;
+        break;
+      case 7: /* circleatwall, bc=0 branch */
+        if (pow(uwrap(udiff(x, LX / 2), LX), 2) + pow(udiff(y, 0 * LY), 2) <= radius * radius) nematicOrder = 1;
+        break;
+      case 8: /* wettedwall, bc=0 branch */
+        if (y < level) nematicOrder = 1;
+        break;
+      default:
+        return -1;
+    }
+    const double theta = angle + in->noise * M_PI * (mt_real(&rng, 0., 1.) - .5);
+    F(LYO_O_QQXX)[k] = in->qamp * nematicOrder * (cos(2 * theta));
+    F(LYO_O_QQYX)[k] = in->qamp * nematicOrder * (sin(2 * theta));
+    if (in->config == 4)
+      F(LYO_O_PHI)[k] = o->p.conc + mt_real(&rng, -in->phinoise, in->phinoise);
+    else
+      F(LYO_O_PHI)[k] = nematicOrder * o->p.conc;
+    o->totalphi += F(LYO_O_PHI)[k];
+    F(LYO_O_UX)[k] = F(LYO_O_UY)[k] = F(LYO_O_UX_PHI)[k] = F(LYO_O_UY_PHI)[k] = 0;
+    F(LYO_O_N)[k] = o->p.rho;
+    equilibrium(F(LYO_O_UX)[k], F(LYO_O_UY)[k], F(LYO_O_N)[k], o->ff + 9 * (size_t)k);
+    for (int v = 0; v < 9; ++v) o->ftot = o->ftot + o->ff[9 * (size_t)k + v];
+    o->ptot += F(LYO_O_PHI)[k];
+  }
+  return 0;
+}
+
+void lyo_oracle_set_state(lyo_oracle* o, const double* ff, const double* Qxx,
+                          const double* Qyx, const double* phi) {
+  const size_t N = o->N;
+  memcpy(o->ff, ff, sizeof(double) * 9 * N);
+  memcpy(F(LYO_O_QQXX), Qxx, sizeof(double) * N);
+  memcpy(F(LYO_O_QQYX), Qyx, sizeof(double) * N);
+  memcpy(F(LYO_O_PHI), phi, sizeof(double) * N);
+  o->ftot = 0; o->ptot = 0; o->totalphi = 0;
+  for (size_t k = 0; k < N; ++k) {
+    for (int v = 0; v < 9; ++v) o->ftot = o->ftot + ff[9 * k + v];
+    o->ptot += phi[k];
+    o->totalphi += phi[k];
+  }
+}
+
+int lyo_oracle_get(const lyo_oracle* o, int field, double* out) {
+  const size_t N = o->N;
+  if (field == LYO_O_FF) { memcpy(out, o->ff, sizeof(double) * 9 * N); return 0; }
+  if (field == LYO_O_FN) { memcpy(out, o->fn, sizeof(double) * 9 * N); return 0; }
+  if (field < 2 || field >= LYO_O_NFIELDS) return -1;
+  memcpy(out, o->s[field], sizeof(double) * N);
+  return 0;
+}
+
+int lyo_oracle_set_field(lyo_oracle* o, int field, const double* in) {
+  const size_t N = o->N;
+  if (field == LYO_O_FF) { memcpy(o->ff, in, sizeof(double) * 9 * N); return 0; }
+  if (field == LYO_O_FN) { memcpy(o->fn, in, sizeof(double) * 9 * N); return 0; }
+  if (field < 2 || field >= LYO_O_NFIELDS) return -1;
+  memcpy(o->s[field], in, sizeof(double) * N);
+  return 0;
+}
+
+/* -------------------------------------------------------------- the step */
+
+/* src/models/lyotropic.cpp:168-247 */
+static void update_quantities_at_node(lyo_oracle* o, unsigned k) {
+  const lyo_oracle_params* P = &o->p;
+  const uint32_t* d = get_neighbours(o, k);
+  const double* f = o->ff + 9 * (size_t)k;
+  const double* QQxx = F(LYO_O_QQXX);
+  const double* QQyx = F(LYO_O_QQYX);
+  const double* phi = F(LYO_O_PHI);
+  const double Qxx = QQxx[k];
+  const double Qyx = QQyx[k];
+  const double p = phi[k];
+
+  const double nn = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+  const double vx = (f[1] - f[2] + f[5] - f[6] - f[7] + f[8]) / nn;
+  const double vy = (f[3] - f[4] + f[5] - f[6] + f[7] - f[8]) / nn;
+
+  const double del2Qxx = laplacian(QQxx, d, SD);
+  const double dxQxx = derivX(QQxx, d, SB);
+  const double dyQxx = derivY(QQxx, d, SB);
+  const double del2Qyx = laplacian(QQyx, d, SD);
+  const double dxQyx = derivX(QQyx, d, SB);
+  const double dyQyx = derivY(QQyx, d, SB);
+  const double del2p = laplacian(phi, d, SD);
+  const double dxPhi = derivX(phi, d, SB);
+  const double dyPhi = derivY(phi, d, SB);
+
+  const double term = 0 - Qxx * Qxx - Qyx * Qyx;
+  const double Hxx = P->CC * term * Qxx + P->LL * del2Qxx;
+  const double Hyx = P->CC * term * Qyx + P->LL * del2Qyx;
+  const double mu = P->AA * p * (1 - p) * (1 - 2 * p) + 0 * P->CC * term - P->KK * del2p;
+
+  const double sigmaB = .5 * P->AA * p * p * (1 - p) * (1 - p)
+                        + .5 * P->CC * term * term - mu * p;
+  const double sigmaF = 2 * P->xi * ((Qxx * Qxx - 1) * Hxx + Qxx * Qyx * Hyx)
+                        - P->zeta * Qxx * (1 - p) + .5 * P->KK * (dyPhi * dyPhi - dxPhi * dxPhi)
+                        + P->LL * (dyQxx * dyQxx + dyQyx * dyQyx - dxQxx * dxQxx - dxQyx * dxQyx);
+  const double sigmaS = 2 * P->xi * (Qyx * Qxx * Hxx + (Qyx * Qyx - 1) * Hyx)
+                        - P->zeta * Qyx * (1 - p) - P->KK * dxPhi * dyPhi
+                        - 2 * P->LL * (dxQxx * dyQxx + dxQyx * dyQyx);
+  const double sigmaA = 2 * (Qxx * Hyx - Qyx * Hxx);
+
+  F(LYO_O_UX)[k] = vx;
+  F(LYO_O_UY)[k] = vy;
+  F(LYO_O_UX_PHI)[k] = vx * p;
+  F(LYO_O_UY_PHI)[k] = vy * p;
+  F(LYO_O_N)[k] = nn;
+  F(LYO_O_HHXX)[k] = Hxx;
+  F(LYO_O_HHYX)[k] = Hyx;
+  F(LYO_O_MU)[k] = mu;
+  F(LYO_O_DXQQXX)[k] = dxQxx;
+  F(LYO_O_DXQQYX)[k] = dxQyx;
+  F(LYO_O_DYQQXX)[k] = dyQxx;
+  F(LYO_O_DYQQYX)[k] = dyQyx;
+  F(LYO_O_SIGMAXX)[k] = sigmaF + sigmaB + P->zetaI * (P->conc - p);
+  F(LYO_O_SIGMAYY)[k] = -sigmaF + sigmaB + P->zetaI * (P->conc - p);
+  F(LYO_O_SIGMAXY)[k] = sigmaS + sigmaA;
+  F(LYO_O_SIGMAYX)[k] = sigmaS - sigmaA;
+  F(LYO_O_SIGMA_BULK)[k] = sigmaB + P->zetaI * (P->conc - p);
+  F(LYO_O_SIGMA_EL_XX)[k] = sigmaF - .5 * P->KK * (dyPhi * dyPhi - dxPhi * dxPhi) + P->zeta * Qxx * (1 - p);
+  F(LYO_O_SIGMA_EL_YX)[k] = sigmaS + P->KK * dxPhi * dyPhi + P->zeta * Qxx * (1 - p);
+  F(LYO_O_SIGMA_PF_XX)[k] = .5 * P->KK * (dyPhi * dyPhi - dxPhi * dxPhi);
+  F(LYO_O_SIGMA_PF_YX)[k] = -P->KK * dxPhi * dyPhi;
+  F(LYO_O_SIGMA_ACT_XX)[k] = -P->zeta * Qxx * (1 - p);
+  F(LYO_O_SIGMA_ACT_YX)[k] = -P->zeta * Qyx * (1 - p);
+}
+
+/* src/models/lyotropic.cpp:249-341 */
+static void update_fields_at_node(lyo_oracle* o, unsigned k, int first) {
+  const lyo_oracle_params* P = &o->p;
+  const uint32_t* d = get_neighbours(o, k);
+  double* QQxx = F(LYO_O_QQXX);
+  double* QQyx = F(LYO_O_QQYX);
+  double* phi = F(LYO_O_PHI);
+
+  const double nn = F(LYO_O_N)[k];
+  const double vx = F(LYO_O_UX)[k];
+  const double vy = F(LYO_O_UY)[k];
+  const double Qxx = QQxx[k];
+  const double Qyx = QQyx[k];
+  const double p = phi[k];
+  const double Hxx = F(LYO_O_HHXX)[k];
+  const double Hyx = F(LYO_O_HHYX)[k];
+  const double dxQxx = F(LYO_O_DXQQXX)[k];
+  const double dyQxx = F(LYO_O_DYQQXX)[k];
+  const double dxQyx = F(LYO_O_DXQQYX)[k];
+  const double dyQyx = F(LYO_O_DYQQYX)[k];
+
+  const double del2mu = laplacian(F(LYO_O_MU), d, SD);
+  const double dxux = derivX(F(LYO_O_UX), d, SB);
+  const double dyux = derivY(F(LYO_O_UX), d, SB);
+  const double dxuy = derivX(F(LYO_O_UY), d, SB);
+  const double dyuy = derivY(F(LYO_O_UY), d, SB);
+  const double dxSxx = derivX(F(LYO_O_SIGMAXX), d, SB);
+  const double dySxy = derivY(F(LYO_O_SIGMAXY), d, SB);
+  const double dxSyx = derivX(F(LYO_O_SIGMAYX), d, SB);
+  const double dySyy = derivY(F(LYO_O_SIGMAYY), d, SB);
+  const double pFlux = derivX(F(LYO_O_UX_PHI), d, SB) + derivY(F(LYO_O_UY_PHI), d, SB);
+
+  const double expansion = dxux + dyuy;
+  const double shear = .5 * (dxuy + dyux);
+  const double vorticity = .5 * (dxuy - dyux);
+  const double traceQL = Qxx * (dxux - dyuy) + 2 * Qyx * shear;
+
+  const double Dxx = P->GammaQ * Hxx - vx * dxQxx - vy * dyQxx - 2 * vorticity * Qyx
+                     + P->xi * ((Qxx + 1) * (2 * dxux - traceQL) + 2 * Qyx * shear - expansion);
+  const double Dyx = P->GammaQ * Hyx - vx * dxQyx - vy * dyQyx + 2 * vorticity * Qxx
+                     + P->xi * (Qyx * (expansion - traceQL) + 2 * shear);
+  const double Dp = P->GammaP * del2mu - pFlux
+                    - (P->conserve_phi ? (o->countphi - o->totalphi) / o->N : 0);
+
+  const double Fx = dxSxx + dySxy - P->friction * vx;
+  const double Fy = dxSyx + dySyy - P->friction * vy;
+
+  const double tau = P->tauNem * p + P->tauIso * (1 - p);
+  double fe[9];
+  equilibrium(vx, vy, nn, fe);
+
+  double* ffk = o->ff + 9 * (size_t)k;
+  double* fnk = o->fn + 9 * (size_t)k;
+  if (first) {
+    F(LYO_O_QNXX)[k] = QQxx[k] + .5 * Dxx;
+    F(LYO_O_QNYX)[k] = QQyx[k] + .5 * Dyx;
+    F(LYO_O_PHN)[k] = phi[k] + .5 * Dp;
+    QQxx[k] = QQxx[k] + Dxx;
+    QQyx[k] = QQyx[k] + Dyx;
+    o->phi_tmp[k] = phi[k] + Dp;
+    for (int v = 0; v < 9; ++v) {
+      fnk[v] = ffk[v] + .5 * ((fe[v] - ffk[v]) / tau + W[v] * (Fx * XDIR[v] + Fy * YDIR[v]));
+      ffk[v] = ffk[v] + ((fe[v] - ffk[v]) / tau + W[v] * (Fx * XDIR[v] + Fy * YDIR[v]));
+    }
+  } else {
+    QQxx[k] = F(LYO_O_QNXX)[k] + .5 * Dxx;
+    QQyx[k] = F(LYO_O_QNYX)[k] + .5 * Dyx;
+    o->phi_tmp[k] = F(LYO_O_PHN)[k] + .5 * Dp;
+    for (int v = 0; v < 9; ++v)
+      ffk[v] = fnk[v] + .5 * ((fe[v] - ffk[v]) / tau + W[v] * (Fx * XDIR[v] + Fy * YDIR[v]));
+  }
+}
+
+/* src/models/lyotropic.cpp:343-360 */
+void lyo_oracle_update_quantities(lyo_oracle* o) {
+  double sum = 0;
+  const int nt = o->p.nthreads;
+  const unsigned N = o->N;
+  (void)nt;
+#pragma omp parallel for reduction(+ : sum) num_threads(nt) if (nt)
+  for (unsigned k = 0; k < N; ++k) {
+    sum = sum + F(LYO_O_PHI)[k];
+    update_quantities_at_node(o, k);
+  }
+  o->countphi = sum;
+}
+
+/* src/models/lyotropic.cpp:362-369 */
+void lyo_oracle_update_fields(lyo_oracle* o, int first) {
+  const int nt = o->p.nthreads;
+  const unsigned N = o->N;
+  (void)nt;
+#pragma omp parallel for num_threads(nt) if (nt)
+  for (unsigned k = 0; k < N; ++k) update_fields_at_node(o, k, first);
+  double* t = o->s[LYO_O_PHI];
+  o->s[LYO_O_PHI] = o->phi_tmp;
+  o->phi_tmp = t;
+}
+
+/* src/models/lyotropic.cpp:371-386 (periodic grid: TotalSize == DomainSize) */
+void lyo_oracle_move(lyo_oracle* o) {
+  const int nt = o->p.nthreads;
+  const unsigned N = o->N;
+  (void)nt;
+#pragma omp parallel for num_threads(nt) if (nt)
+  for (unsigned k = 0; k < N; ++k) {
+    const uint32_t* d = get_neighbours(o, k);
+    for (int v = 0; v < 9; ++v) {
+      o->ff_tmp[9 * (size_t)d[v] + v] = o->ff[9 * (size_t)k + v];
+      o->fn_tmp[9 * (size_t)d[v] + v] = o->fn[9 * (size_t)k + v];
+    }
+  }
+  double* t = o->ff; o->ff = o->ff_tmp; o->ff_tmp = t;
+  t = o->fn; o->fn = o->fn_tmp; o->fn_tmp = t;
+}
+
+/* src/models/lyotropic.cpp:500-525 (bc=0: the boundary-condition calls are
+ * no-ops) */
+void lyo_oracle_step(lyo_oracle* o, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) {
+    lyo_oracle_update_quantities(o);
+    lyo_oracle_update_fields(o, 1);
+    lyo_oracle_move(o);
+    for (int n = 1; n <= o->p.npc; ++n) {
+      lyo_oracle_update_quantities(o);
+      lyo_oracle_update_fields(o, 0);
+    }
+  }
+}
+
+/* src/models/lyotropic.cpp:527-549 */
+void lyo_oracle_sums(const lyo_oracle* o, double* fsum, double* phisum,
+                     double* ftot, double* ptot, double* totalphi,
+                     double* countphi) {
+  double fcheck = 0, pcheck = 0;
+  for (unsigned k = 0; k < o->N; ++k)
+    for (int v = 0; v < 9; ++v) fcheck = fcheck + o->ff[9 * (size_t)k + v];
+  for (unsigned k = 0; k < o->N; ++k) pcheck += o->s[LYO_O_PHI][k];
+  if (fsum) *fsum = fcheck;
+  if (phisum) *phisum = pcheck;
+  if (ftot) *ftot = o->ftot;
+  if (ptot) *ptot = o->ptot;
+  if (totalphi) *totalphi = o->totalphi;
+  if (countphi) *countphi = o->countphi;
+}

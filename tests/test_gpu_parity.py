@@ -1,0 +1,158 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+FP64 is the reference arithmetic in the reference order, so it must match the
+oracle bit for bit (initial fields, every frame field after one step and after
+100 steps).  FP32 is checked against the FP64 oracle with stated tolerances.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mass_hd_amd import lyo
+from oracle import pyoracle as po
+from tests.conftest import CONFIG0_INIT, CONFIG0_PHYS
+
+pytestmark = pytest.mark.gpu
+
+DERIVED = ["n", "ux", "uy", "sigmaXX", "sigmaXY", "sigmaYX", "sigmaYY", "sigma_bulk",
+           "sigma_elastic_xx", "sigma_elastic_yx", "sigma_phase_field_xx",
+           "sigma_phase_field_yx", "sigma_active_xx", "sigma_active_yx", "MU", "HHxx", "HHyx"]
+STATE = ["ff", "QQxx", "QQyx", "phi"]
+
+
+def pair(LX, LY, npc=1, precision=lyo.FP64, phys=CONFIG0_PHYS, init=CONFIG0_INIT):
+    g = lyo.Lyotropic(LX, LY, npc=npc, precision=precision, **phys)
+    o = po.Oracle(LX, LY, npc=npc, **phys)
+    fg = g.configure(**init)
+    fo = o.configure(**init)
+    return g, o, fg, fo
+
+
+def assert_bitwise(g, o, names):
+    for n in names:
+        a, b = g.get(n), o.get(n)
+        assert np.array_equal(a, b), "%s differs: max |d| = %g" % (n, np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("LX,LY", [(128, 128), (37, 53), (2, 2), (3, 130), (64, 61)])
+def test_initial_fields_bit_exact(require_gpu, LX, LY):
+    g, o, fg, fo = pair(LX, LY)
+    assert fg == fo  # ftot, ptot exactly as the reference computes them
+    assert_bitwise(g, o, STATE)
+    # derived fields are zero before the first step (Field value-initialisation)
+    for n in ("sigmaXX", "ux", "MU"):
+        assert not np.any(g.get(n))
+
+
+@pytest.mark.parametrize("npc", [1, 2, 3])
+@pytest.mark.parametrize("LX,LY", [(128, 128), (45, 67), (5, 3)])
+def test_fp64_one_step_bit_exact(require_gpu, LX, LY, npc):
+    g, o, _, _ = pair(LX, LY, npc=npc)
+    g.step(1)
+    o.step(1)
+    assert_bitwise(g, o, STATE)
+    # frame fields come from the last UpdateQuantities (start of the last corrector)
+    assert_bitwise(g, o, DERIVED)
+
+
+def test_fp64_config0_100_steps(require_gpu):
+    """128^2, npc=1: bit-exact after 100 steps (the contract allows 1e-11)."""
+    g, o, _, _ = pair(128, 128, npc=1)
+    g.step(100)
+    o.step(100)
+    for n in STATE + ["sigmaXX", "sigmaXY", "ux"]:
+        a, b = g.get(n), o.get(n)
+        rel = np.abs(a - b).max() / np.abs(b).max()
+        assert rel <= 1e-11, (n, rel)
+        assert np.array_equal(a, b), (n, rel)
+
+
+def test_fp64_npc2_50_steps_ragged(require_gpu):
+    g, o, _, _ = pair(97, 131, npc=2)
+    g.step(50)
+    o.step(50)
+    assert_bitwise(g, o, STATE)
+
+
+@pytest.mark.parametrize("chunk,wpb", [(1, 1), (3, 2), (7, 4), (128, 4)])
+def test_launch_shape_does_not_change_results(require_gpu, chunk, wpb):
+    ref, _, _, _ = pair(70, 125, npc=2)
+    ref.step(3)
+    g, _, _, _ = pair(70, 125, npc=2)
+    g.set_launch(chunk, wpb)
+    g.step(3)
+    for n in STATE:
+        assert np.array_equal(g.get(n), ref.get(n))
+    g32, _, _, _ = pair(70, 125, npc=2, precision=lyo.FP32)
+    r32, _, _, _ = pair(70, 125, npc=2, precision=lyo.FP32)
+    g32.set_launch(chunk, wpb)
+    g32.step(3)
+    r32.step(3)
+    for n in STATE:
+        assert np.array_equal(g32.get(n), r32.get(n))
+
+
+def test_sums_match_oracle(require_gpu):
+    g, o, _, _ = pair(128, 96, npc=1)
+    g.step(10)
+    o.step(10)
+    fs, ps = g.sums()
+    s = o.sums()
+    # the device sums in a fixed tree; compare with the exactly rounded sum of
+    # the (bit-identical) fields, and with the reference's serial sum
+    exact_f = math.fsum(o.get("ff").reshape(-1))
+    exact_p = math.fsum(o.get("phi"))
+    assert fs == pytest.approx(exact_f, rel=1e-15)
+    assert ps == pytest.approx(exact_p, rel=1e-15)
+    assert fs == pytest.approx(s["fsum"], rel=1e-13)
+    assert ps == pytest.approx(s["phisum"], rel=1e-13)
+    # fixed tree: repeated calls agree exactly
+    assert g.sums() == (fs, ps)
+    g.runtime_checks()
+
+
+def test_fp32_initial_and_100_steps(require_gpu):
+    """FP32 (shifted populations, compensated phi) against the FP64 oracle.
+    Tolerances: initial state to FP32 rounding; after 100 steps populations
+    within 1e-6 relative to rho and Q/phi within 1e-5 absolute."""
+    LX, LY = 128, 128
+    g, o, _, _ = pair(LX, LY, npc=1, precision=lyo.FP32)
+    assert np.allclose(g.get("ff"), o.get("ff"), rtol=0, atol=40 * 1e-7)
+    assert np.allclose(g.get("phi"), o.get("phi"), rtol=0, atol=1e-15)  # phi + phic
+    g.step(100)
+    o.step(100)
+    assert np.abs(g.get("ff") - o.get("ff")).max() < 40 * 1e-6
+    for n in ("QQxx", "QQyx", "phi"):
+        assert np.abs(g.get(n) - o.get(n)).max() < 1e-5, n
+    u_g, u_o = g.get("ux"), o.get("ux")
+    assert np.abs(u_g - u_o).max() < 1e-3 * np.abs(u_o).max()
+
+
+def test_fp32_phi_conservation(require_gpu):
+    """The compensated update keeps sum(phi) to far better than FP32 rounding."""
+    g, _, _, pt = pair(256, 256, npc=2, precision=lyo.FP32)
+    p0 = g.sums()[1]
+    g.step(200)
+    p1 = g.sums()[1]
+    # uncompensated FP32 drifts ~ sqrt(N*steps)*ulp(0.5) ~ 1e-3; compensated << that
+    assert abs(p1 - p0) < 1e-6 * p0
+
+
+def test_errors_are_codes_not_crashes(require_gpu):
+    with pytest.raises(lyo.MassLyoError) as e:
+        lyo.Lyotropic(1, 10, **CONFIG0_PHYS)
+    assert e.value.code == 1
+    with pytest.raises(lyo.MassLyoError):
+        lyo.Lyotropic(10, 10, npc=0, **CONFIG0_PHYS)
+    g = lyo.Lyotropic(8, 8, **CONFIG0_PHYS)
+    N = 64
+    ff = np.tile(po.equilibrium(0, 0, 40.0), (N, 1))
+    phi = np.full(N, 0.5)
+    phi[5] = np.nan
+    g.set_state(ff, np.zeros(N), np.zeros(N), phi)
+    with pytest.raises(lyo.MassLyoError) as e:
+        g.get("phi")
+    assert e.value.code == 3 and "nan" in str(e.value)
+    with pytest.raises(lyo.MassLyoError):
+        g.configure(config=99)

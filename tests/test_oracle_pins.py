@@ -1,0 +1,356 @@
+"""Pins the CPU oracle (oracle/lyo_oracle.c) against facts fixed independently
+of it: the C++ standard's mt19937, libstdc++'s own uniform_real_distribution
+and libm, numpy's MT19937, equilibrium moments, exact stencil values,
+conservation laws, fixed points and closed-form single-mode responses.
+
+The reference ships no golden vectors for this path (its only test,
+src/test_serialization.cpp, has no assertions) and cannot be built here
+(every translation unit includes boost/program_options.hpp, which is absent),
+so these checks are what pins the restatement.  Each one is chosen so that a
+plausible transcription error (a dropped term, a wrong sign, a swapped
+neighbour or operand) fails it.
+"""
+import math
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.conftest import CONFIG0_INIT, CONFIG0_PHYS
+
+ZERO_PHYS = dict(GammaQ=0.0, GammaP=0.0, xi=0.0, tauNem=1.0, tauIso=1.0, rho=40.0,
+                 friction=0.0, AA=0.0, CC=0.0, LL=0.0, KK=0.0, zeta=0.0, zetaI=0.0, conc=0.0)
+
+C = np.array([(0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, -1), (-1, 1), (1, -1)])
+
+
+def phys(**kw):
+    p = dict(ZERO_PHYS)
+    p.update(kw)
+    return p
+
+
+def rest_state(LX, LY, rho=40.0, u=(0.0, 0.0)):
+    fe = po.equilibrium(u[0], u[1], rho)
+    return np.tile(fe, (LX * LY, 1))
+
+
+def grid(LX, LY):
+    x, y = np.meshgrid(np.arange(LX), np.arange(LY), indexing="ij")
+    return x.reshape(-1), y.reshape(-1)
+
+
+# ------------------------------------------------------------------ RNG (a10)
+
+def test_mt19937_standard_value():
+    # [rand.predef]: the 10000th output of a default-constructed mt19937
+    assert po.mt19937(5489, 10000)[-1] == 4123659995
+
+
+@pytest.mark.parametrize("seed", [1, 7, 1001, 4294967295])
+def test_mt19937_matches_numpy(seed):
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(seed)
+    assert np.array_equal(po.mt19937(seed, 2000), bg.random_raw(2000).astype(np.uint32))
+
+
+_CXX = r"""
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+int main(int argc, char** argv) {
+  const unsigned seed = std::strtoul(argv[1], 0, 10);
+  const int n = std::atoi(argv[2]);
+  std::mt19937 gen(seed);
+  const double angle = 90. * M_PI / 180., noise = 1., qamp = .5, conc = .5, pn = .05;
+  for (int k = 0; k < n; ++k) {
+    const double theta = angle + noise * M_PI * (std::uniform_real_distribution<>(0., 1.)(gen) - .5);
+    const double qxx = qamp * 1. * (std::cos(2 * theta));
+    const double qyx = qamp * 1. * (std::sin(2 * theta));
+    const double phi = conc + std::uniform_real_distribution<>(-pn, pn)(gen);
+    std::printf("%a %a %a\n", qxx, qyx, phi);
+  }
+  return 0;
+}
+"""
+
+
+def test_configure_matches_libstdcxx():
+    """The oracle's init draws equal libstdc++'s uniform_real_distribution and
+    the host libm, bit for bit (random config, two draws per node)."""
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "r.cpp"), os.path.join(d, "r")
+        open(src, "w").write(_CXX)
+        subprocess.check_call(["g++", "-O2", "-o", exe, src])
+        LX, LY = 13, 17
+        out = subprocess.check_output([exe, "1", str(LX * LY)], text=True).split()
+    ref = np.array([float.fromhex(t) for t in out]).reshape(-1, 3)
+    o = po.Oracle(LX, LY, **CONFIG0_PHYS)
+    o.configure(**CONFIG0_INIT)
+    assert np.array_equal(o.get("QQxx"), ref[:, 0])
+    assert np.array_equal(o.get("QQyx"), ref[:, 1])
+    assert np.array_equal(o.get("phi"), ref[:, 2])
+    # theta ~ U[0, pi): the nematic tensor has trace zero and |Q| = amplitude
+    assert np.allclose(np.hypot(ref[:, 0], ref[:, 1]), 0.5)
+    assert abs(ref[:, 2].mean() - 0.5) < 0.01 and np.all(np.abs(ref[:, 2] - 0.5) <= 0.05)
+
+
+def test_configure_geometric_configs():
+    LX, LY = 20, 16
+    o = po.Oracle(LX, LY, **dict(CONFIG0_PHYS, conc=0.7))
+    o.configure(config="circle", radius=4.0, angle=0.0, noise=0.0)
+    x, y = grid(LX, LY)
+    inside = (x - LX // 2) ** 2 + (y - LY // 2) ** 2 <= 16
+    assert np.array_equal(o.get("phi"), np.where(inside, 0.7, 0.0))
+    # angle 0, no noise: Q = (1, 0) inside
+    assert np.array_equal(o.get("QQxx"), np.where(inside, 1.0, 0.0))
+    o.configure(config="stripe", level=6.0, noise=0.0)
+    assert np.array_equal(o.get("phi") > 0, np.abs(LY // 2 - y) < 3)
+    ft, pt = o.configure(config="half", level=5.0, noise=0.0)
+    assert pt == pytest.approx(0.7 * LX * 5)
+    assert ft == pytest.approx(40.0 * LX * LY, rel=1e-12)
+
+
+# ------------------------------------------------------ neighbour table (a1)
+
+@pytest.mark.parametrize("LX,LY", [(2, 2), (3, 5), (7, 4), (16, 9)])
+def test_neighbour_table_brute_force(LX, LY):
+    o = po.Oracle(LX, LY, **ZERO_PHYS)
+    nb = o.neighbours()
+    x, y = grid(LX, LY)
+    for v, (cx, cy) in enumerate(C):
+        exp = ((y + cy) % LY) + LY * ((x + cx) % LX)
+        assert np.array_equal(nb[:, v], exp)
+        assert len(np.unique(nb[:, v])) == LX * LY  # each direction is a bijection
+
+
+# --------------------------------------------------------- equilibrium (a3)
+
+@pytest.mark.parametrize("u", [(0.0, 0.0), (0.03, -0.01), (-0.07, 0.05)])
+def test_equilibrium_moments(u):
+    n = 37.5
+    fe = po.equilibrium(u[0], u[1], n)
+    assert fe.sum() == pytest.approx(n, rel=1e-15)
+    m = (fe[:, None] * C).sum(0)
+    assert np.allclose(m, n * np.array(u), rtol=0, atol=1e-13)
+    P = np.einsum("v,va,vb->ab", fe, C, C)
+    assert np.allclose(P, n * (np.outer(u, u) + np.eye(2) / 3), rtol=0, atol=1e-12)
+
+
+def test_rest_equilibrium_weights():
+    fe = po.equilibrium(0.0, 0.0, 1.0)
+    assert np.allclose(fe, [4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, rtol=1e-15)
+
+
+# -------------------------------------------- stencils through UQ (a4, a5)
+
+def test_stencils_exact_on_polynomials():
+    """derivX/derivY are exact on linear fields and the Laplacian is exact on
+    quadratics (interior nodes, away from the periodic seam)."""
+    LX, LY = 12, 10
+    x, y = grid(LX, LY)
+    qxx = 0.01 * x + 0.002 * y * y          # d/dx = .01, lap = .004
+    qyx = -0.003 * y + 0.001 * x * x        # d/dy = -.003, lap = .002
+    phi = 0.2 + 0.004 * x - 0.001 * y       # linear
+    o = po.Oracle(LX, LY, **phys(LL=1.0))
+    o.set_state(rest_state(LX, LY).reshape(-1), qxx, qyx, phi)
+    o.update_quantities()
+    inner = (x > 0) & (x < LX - 1) & (y > 0) & (y < LY - 1)
+    assert np.allclose(o.get("dxQQxx")[inner], 0.01, atol=1e-15)
+    assert np.allclose(o.get("dyQQyx")[inner], -0.003, atol=1e-15)
+    assert np.allclose(o.get("dyQQxx")[inner], 0.004 * y[inner], atol=1e-15)
+    # LL = 1, CC = 0: H = laplacian(Q)
+    assert np.allclose(o.get("HHxx")[inner], 0.004, atol=1e-14)
+    assert np.allclose(o.get("HHyx")[inner], 0.002, atol=1e-14)
+
+
+def test_density_velocity_from_populations():
+    LX, LY = 4, 5
+    rng = np.random.default_rng(3)
+    ff = rest_state(LX, LY) + rng.uniform(-0.2, 0.2, (LX * LY, 9))
+    o = po.Oracle(LX, LY, **ZERO_PHYS)
+    o.set_state(ff.reshape(-1), np.zeros(LX * LY), np.zeros(LX * LY), np.zeros(LX * LY))
+    o.update_quantities()
+    n = ff.sum(1)
+    assert np.allclose(o.get("n"), n, rtol=1e-15)
+    assert np.allclose(o.get("ux"), (ff * C[:, 0]).sum(1) / n, atol=1e-15)
+    assert np.allclose(o.get("uy"), (ff * C[:, 1]).sum(1) / n, atol=1e-15)
+
+
+# ------------------------------------------------- closed-form responses
+
+def _mode(LX, LY, m=1):
+    x, y = grid(LX, LY)
+    k = 2 * math.pi * m / LX
+    return x, y, k
+
+
+def test_q_relaxation_closed_form():
+    """u = 0, CC = 0: the predictor gives QN = Q + (1/2) GammaQ LL lap(Q), and
+    the discrete Laplacian of sin(kx) is (2 cos k - 2) sin(kx)."""
+    LX, LY = 16, 6
+    x, y, k = _mode(LX, LY)
+    q = 0.3 * np.sin(k * x)
+    P = phys(GammaQ=0.7, LL=0.05, xi=0.4)
+    o = po.Oracle(LX, LY, **P)
+    o.set_state(rest_state(LX, LY).reshape(-1), q, 0.5 * q, np.zeros(LX * LY))
+    o.update_quantities()
+    o.update_fields(True)
+    lam = 2 * math.cos(k) - 2
+    assert np.allclose(o.get("QNxx"), q * (1 + 0.5 * 0.7 * 0.05 * lam), atol=1e-15)
+    assert np.allclose(o.get("QNyx"), 0.5 * q * (1 + 0.5 * 0.7 * 0.05 * lam), atol=1e-15)
+    assert np.allclose(o.get("QQxx"), q * (1 + 0.7 * 0.05 * lam), atol=1e-15)
+
+
+def test_cahn_hilliard_closed_form():
+    """u = 0, AA = 0: mu = -KK lap(phi) and Dp = GammaP lap(mu)."""
+    LX, LY = 6, 20
+    x, y = grid(LX, LY)
+    k = 2 * math.pi * 2 / LY
+    eps = 0.01
+    phi = 0.5 + eps * np.cos(k * y)
+    o = po.Oracle(LX, LY, **phys(GammaP=0.3, KK=0.2))
+    o.set_state(rest_state(LX, LY).reshape(-1), np.zeros(LX * LY), np.zeros(LX * LY), phi)
+    o.update_quantities()
+    lam = 2 * math.cos(k) - 2
+    assert np.allclose(o.get("MU"), -0.2 * lam * eps * np.cos(k * y), atol=1e-15)
+    o.update_fields(True)
+    dp = 0.3 * (-0.2) * lam * lam * eps * np.cos(k * y)
+    assert np.allclose(o.get("phn"), phi + 0.5 * dp, atol=1e-15)
+    assert np.allclose(o.get("phi"), phi + dp, atol=1e-15)
+
+
+def test_double_well_chemical_potential():
+    """Uniform phi: mu = AA p (1-p)(1-2p), sigma_bulk = AA p^2 (1-p)^2 / 2 - mu p."""
+    LX, LY = 3, 4
+    for p0 in (0.2, 0.5, 0.9):
+        o = po.Oracle(LX, LY, **phys(AA=0.3, KK=1.0))
+        o.set_state(rest_state(LX, LY).reshape(-1), np.zeros(12), np.zeros(12),
+                    np.full(12, p0))
+        o.update_quantities()
+        mu = 0.3 * p0 * (1 - p0) * (1 - 2 * p0)
+        assert np.allclose(o.get("MU"), mu, rtol=1e-14, atol=1e-17)
+        assert np.allclose(o.get("sigma_bulk"), 0.15 * p0 ** 2 * (1 - p0) ** 2 - mu * p0,
+                           rtol=1e-13, atol=1e-17)
+
+
+def test_active_force_closed_form():
+    """Active stress only: sigma = -zeta (1-phi) Q, so after the predictor
+    collision the momentum of each node grows by F = div sigma."""
+    LX, LY = 24, 5
+    x, y, k = _mode(LX, LY)
+    A, zeta, p0 = 0.4, 0.05, 0.25
+    qxx = A * np.sin(k * x)
+    o = po.Oracle(LX, LY, **phys(zeta=zeta, tauNem=1.7, tauIso=1.7))
+    ff0 = rest_state(LX, LY)
+    o.set_state(ff0.reshape(-1), qxx, np.zeros(LX * LY), np.full(LX * LY, p0))
+    o.update_quantities()
+    assert np.allclose(o.get("sigmaXX"), -zeta * (1 - p0) * qxx, atol=1e-16)
+    assert np.allclose(o.get("sigmaYY"), zeta * (1 - p0) * qxx, atol=1e-16)
+    o.update_fields(True)
+    ff1 = o.get("ff")
+    dmom = ((ff1 - ff0) * C[:, 0]).sum(1)
+    Fx = -zeta * (1 - p0) * A * math.sin(k) * np.cos(k * x)
+    assert np.allclose(dmom, Fx, atol=1e-14)
+    assert np.allclose(((ff1 - ff0) * C[:, 1]).sum(1), 0, atol=1e-14)
+    # mass is untouched by the collision
+    assert np.allclose(ff1.sum(1), ff0.sum(1), rtol=1e-15)
+
+
+def test_elastic_stress_linear_order():
+    """Small Q: sigma_xx = -2 xi H_xx + O(Q^2), sigma_xy - sigma_yx =
+    2 sigmaA = 4 (Qxx Hyx - Qyx Hxx)."""
+    LX, LY = 16, 3
+    x, y, k = _mode(LX, LY)
+    eps = 1e-4
+    q = eps * np.sin(k * x)
+    o = po.Oracle(LX, LY, **phys(xi=0.6, LL=0.1))
+    o.set_state(rest_state(LX, LY).reshape(-1), q, 0.3 * q, np.zeros(LX * LY))
+    o.update_quantities()
+    H = o.get("HHxx")
+    assert np.allclose(H, 0.1 * (2 * math.cos(k) - 2) * q, atol=1e-18)
+    assert np.allclose(o.get("sigmaXX"), -2 * 0.6 * H, atol=10 * eps * eps)  # O(Q^2) terms
+    Hyx = o.get("HHyx")
+    anti = o.get("sigmaXY") - o.get("sigmaYX")
+    assert np.allclose(anti, 4 * (q * Hyx - 0.3 * q * H), atol=1e-20)
+
+
+def test_flow_alignment_closed_form():
+    """Q = 0 in a shear wave u_x = U sin(k y): the Beris-Edwards term gives
+    D_yx = xi (dux/dy) and D_xx = 0 (dux/dx = duy/dy = 0)."""
+    LX, LY = 4, 16
+    x, y = grid(LX, LY)
+    k = 2 * math.pi / LY
+    U, xi = 0.01, 0.8
+    ff = np.stack([po.equilibrium(U * math.sin(k * yy), 0.0, 40.0) for yy in y])
+    o = po.Oracle(LX, LY, **phys(xi=xi))
+    o.set_state(ff.reshape(-1), np.zeros(LX * LY), np.zeros(LX * LY), np.zeros(LX * LY))
+    o.update_quantities()
+    assert np.allclose(o.get("ux"), U * np.sin(k * y), atol=1e-15)
+    o.update_fields(True)
+    dyux = U * math.sin(k) * np.cos(k * y)
+    assert np.allclose(o.get("QNyx"), 0.5 * xi * dyux, atol=1e-15)
+    assert np.allclose(o.get("QNxx"), 0.0, atol=1e-15)
+
+
+# ------------------------------------------------ streaming + invariants
+
+def test_move_is_the_streaming_permutation():
+    LX, LY = 5, 7
+    rng = np.random.default_rng(0)
+    o = po.Oracle(LX, LY, **ZERO_PHYS)
+    ff = rng.random((LX * LY, 9))
+    o.set_field("ff", ff.reshape(-1))
+    o.set_field("fn", (2 * ff).reshape(-1))
+    o.move()
+    got = o.get("ff").reshape(LX, LY, 9)
+    src = ff.reshape(LX, LY, 9)
+    for v, (cx, cy) in enumerate(C):
+        # ff_new(x, y)[v] = ff_old(x - cx, y - cy)[v]
+        assert np.array_equal(got[:, :, v], np.roll(src[:, :, v], (cx, cy), axis=(0, 1)))
+    assert np.array_equal(o.get("fn"), 2 * got.reshape(-1, 9))
+
+
+def test_uniform_state_is_a_fixed_point():
+    LX, LY = 8, 6
+    o = po.Oracle(LX, LY, **dict(CONFIG0_PHYS))
+    N = LX * LY
+    o.set_state(rest_state(LX, LY).reshape(-1), np.full(N, 0.3), np.full(N, -0.2),
+                np.full(N, 0.45))
+    o.step(5)
+    assert np.allclose(o.get("QQxx"), 0.3, rtol=1e-14)
+    assert np.allclose(o.get("phi"), 0.45, rtol=1e-14)
+    assert np.allclose(o.get("ff"), rest_state(LX, LY), rtol=1e-13)
+
+
+@pytest.mark.parametrize("npc", [1, 2])
+def test_mass_momentum_phi_conservation(npc):
+    LX, LY = 24, 20
+    o = po.Oracle(LX, LY, npc=npc, **CONFIG0_PHYS)
+    o.configure(**CONFIG0_INIT)
+    f0 = o.get("ff")
+    p0 = o.get("phi").sum()
+    o.step(20)
+    f1 = o.get("ff")
+    assert abs(f1.sum() - f0.sum()) < 1e-11 * f0.sum()
+    # total force is a sum of central differences over a periodic grid: zero
+    assert np.allclose((f1[:, :, None] * C[None]).sum((0, 1)), 0, atol=1e-9)
+    assert abs(o.get("phi").sum() - p0) < 1e-11 * p0
+    s = o.sums()
+    assert s["fsum"] == pytest.approx(f1.sum(), rel=1e-14)
+
+
+def test_threads_do_not_change_results():
+    LX, LY = 32, 24
+    a = po.Oracle(LX, LY, npc=2, nthreads=0, **CONFIG0_PHYS)
+    b = po.Oracle(LX, LY, npc=2, nthreads=4, **CONFIG0_PHYS)
+    a.configure(**CONFIG0_INIT)
+    b.configure(**CONFIG0_INIT)
+    a.step(5)
+    b.step(5)
+    for f in ("ff", "QQxx", "QQyx", "phi", "sigmaXY"):
+        assert np.array_equal(a.get(f), b.get(f))
