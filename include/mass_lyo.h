@@ -53,7 +53,8 @@ typedef struct mass_lyo_params {
   int32_t npc;          /* corrector sweeps, >= 1 */
   int32_t precision;    /* enum mass_lyo_precision */
   int32_t conserve_phi; /* must be 0: plain Lyotropic (lyotropic.hpp:46) */
-  int32_t flags;        /* reserved, 0 */
+  int32_t plane_pad;    /* extra elements between SoA planes (0 = none);
+                           results do not depend on it */
   double GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
   double AA, CC, LL, KK, zeta, zetaI, conc;
 } mass_lyo_params;
@@ -164,10 +165,11 @@ int mass_lyo_synchronize(mass_lyo_t h);
 int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms,
                         double* sweep_ms);
 
-/* Launch geometry of the sweep kernels (strip-march): rows per wave chunk
- * and waves per block.  0 keeps the current value.  Results do not depend on
- * it. */
-int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block);
+/* Launch geometry of the sweep kernels (strip-march): rows per wave chunk,
+ * waves per block (0 keeps the current value) and XCD-aware block order
+ * (1 on, 0 off, -1 keep).  Results do not depend on it. */
+int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block,
+                        int32_t xcd_order);
 
 /* Version of this ABI (MASS_LYO_ABI_VERSION). */
 int mass_lyo_abi_version(void);

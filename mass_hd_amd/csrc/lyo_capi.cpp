@@ -93,10 +93,16 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
     return fail(nullptr, MASS_LYO_EDEVICE,
                 std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
 
+  if (p->plane_pad < 0 || p->plane_pad > (1 << 20))
+    return fail(nullptr, MASS_LYO_EINVAL, "plane_pad must be in [0, 2^20]");
+  // planes start on 256-byte boundaries
+  const long long NP = ((N + p->plane_pad + 63) / 64) * 64;
+  if (NP * 9 >= (1LL << 31) - 4096)
+    return fail(nullptr, MASS_LYO_EINVAL, "grid too large with this plane_pad");
   auto h = std::make_unique<mass_lyo_s>();
   h->prm = *p;
   h->N = (int)N;
-  h->NP = (int)N;
+  h->NP = (int)NP;
   h->device = dev;
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(nullptr, e, "hipStreamCreate");
@@ -110,10 +116,12 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
     return fail(nullptr, MASS_LYO_ENOMEM, msg);
   }
   h->eng->set_params(v);
-  // default chunk length: enough waves to fill 256 CUs several times over
+  // default chunk length: the longest chunk (least prologue rework) that
+  // still gives ~32 waves per SIMD, so waiting waves always have a successor
+  // (measured at 4096^2: 8 rows beat 16/32/64)
   const int nstrips = (p->LY + lyo::kStrip - 1) / lyo::kStrip;
   int xc = 64;
-  while (xc > 4 && (long long)nstrips * ((p->LX + xc - 1) / xc) < 8192) xc >>= 1;
+  while (xc > 4 && (long long)nstrips * ((p->LX + xc - 1) / xc) < 32768) xc >>= 1;
   h->launch.xc = xc;
   *out = h.release();
   return MASS_LYO_OK;
@@ -121,12 +129,12 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
 
 void mass_lyo_destroy(mass_lyo_t h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->eng) h->eng->release();
   h->eng.reset();
-  if (h->d_stage) hipFree(h->d_stage);
-  if (h->d_part) hipFree(h->d_part);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->d_stage) (void)hipFree(h->d_stage);
+  if (h->d_part) (void)hipFree(h->d_part);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -256,20 +264,24 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms, double* 
     sweep_ms[1] = cnt[1] ? acc[1] / cnt[1] : 0;
   }
   for (auto& ev : tm.ev)
-    if (ev) hipEventDestroy(ev);
-  hipEventDestroy(t0);
-  hipEventDestroy(t1);
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
   if (e != hipSuccess) return hip_fail(h, e, "timed steps");
   if (total_ms) *total_ms = ms;
   return MASS_LYO_OK;
 }
 
-int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block) {
+int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block,
+                        int32_t xcd_order) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
-  if (chunk_rows < 0 || chunk_rows > 4096 || waves_per_block < 0 || waves_per_block > 4)
-    return fail(h, MASS_LYO_EINVAL, "chunk_rows must be in [1,4096], waves_per_block in [1,4]");
+  if (chunk_rows < 0 || chunk_rows > 4096 || waves_per_block < 0 || waves_per_block > 4 ||
+      xcd_order < -1 || xcd_order > 1)
+    return fail(h, MASS_LYO_EINVAL,
+                "chunk_rows must be in [1,4096], waves_per_block in [1,4], xcd_order in {-1,0,1}");
   if (chunk_rows) h->launch.xc = chunk_rows;
   if (waves_per_block) h->launch.wpb = waves_per_block;
+  if (xcd_order >= 0) h->launch.xcd = xcd_order;
   return MASS_LYO_OK;
 }
 

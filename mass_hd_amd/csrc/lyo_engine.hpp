@@ -80,10 +80,10 @@ inline hipError_t timed_launch(const SweepArgs<T>& a, const Params<T>& p, const 
     e1 = tm->ev[tm->used + 1];
     tm->kind[tm->used / 2] = (STAGE == PRED) ? 0 : 1;
     tm->used += 2;
-    hipEventRecord(e0, s);
+    (void)hipEventRecord(e0, s);
   }
   hipError_t err = launch_sweep<T, EXACT, STAGE>(a, p, g, L.wpb, s);
-  if (e1) hipEventRecord(e1, s);
+  if (e1) (void)hipEventRecord(e1, s);
   return err;
 }
 
@@ -298,7 +298,7 @@ class EngineT final : public Engine {
   }
 
   void release() override {
-    for (T* p : ptrs_) hipFree(p);
+    for (T* p : ptrs_) (void)hipFree(p);
     ptrs_.clear();
     bytes_ = 0;
     B_ = Bufs<T>();

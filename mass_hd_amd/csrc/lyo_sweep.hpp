@@ -92,6 +92,20 @@ struct QP {
   T xx, yx, p;
 };
 
+// Data one loop iteration (UpdateFields on row x) needs from memory: Q/phi of
+// row x+2 and populations of row x+1 for UpdateQuantities on row x+1, and for
+// the correctors the centre inputs fn, QN, phn of row x.  The loop loads the
+// next iteration's RowIn before computing the current one, so the loads of one
+// row overlap the arithmetic of the previous one.
+template <class T, int STAGE>
+struct RowIn {
+  QP<T> q;
+  T dp;  // FP32 correctors: increment added to q.p when the row is used
+  T f[9];
+  T fn[(STAGE == PRED) ? 1 : 9];
+  T qnxx, qnyx, phn;
+};
+
 template <class T, bool EXACT, int STAGE>
 __global__ void __launch_bounds__(256)
 sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
@@ -102,7 +116,9 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
     const int nb8 = (int)gridDim.x >> 3;
     if (b < (nb8 << 3)) b = (b & 7) * nb8 + (b >> 3);
   }
-  const int wid = b * ((int)blockDim.x >> 6) + ((int)threadIdx.x >> 6);
+  // wave-uniform: keeps the row arithmetic in scalar registers
+  const int wid = __builtin_amdgcn_readfirstlane(b * ((int)blockDim.x >> 6) +
+                                                 ((int)threadIdx.x >> 6));
   if (wid >= g.nwaves) return;  // whole waves exit; no block barriers below
   const int lane = threadIdx.x & 63;
   const int strip = wid % g.nstrips;
@@ -118,9 +134,12 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
   const int x0 = chunk * g.xc;
   const int x1 = min(x0 + g.xc, LX);
 
+  // row index r in [x0-2, x1+3] (LX >= 2) -> offset of row (r mod LX)
   auto roff = [&](int r) {
-    r %= LX;
-    if (r < 0) r += LX;
+    r = (r < 0) ? r + LX : r;
+    r = (r < 0) ? r + LX : r;
+    r = (r >= LX) ? r - LX : r;
+    r = (r >= LX) ? r - LX : r;
     return r * LY;
   };
   auto load_qp = [&](int r) {
@@ -138,6 +157,24 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
 #pragma unroll
     for (int v = 0; v < 9; ++v) f[v] = A.f_in[v * N + o];
   };
+  auto load_row = [&](int x, RowIn<T, STAGE>& R) {
+    {
+      const int o = roff(x + 2) + y;
+      R.q.xx = A.qxx_in[o];
+      R.q.yx = A.qyx_in[o];
+      R.q.p = A.phi_in[o];
+      if constexpr (!EXACT && STAGE != PRED) R.dp = A.dphi_in[o];
+    }
+    load_f(x + 1, R.f);
+    if constexpr (STAGE != PRED) {
+      const int o = roff(x) + y;
+#pragma unroll
+      for (int v = 0; v < 9; ++v) R.fn[v] = A.fn_in[v * N + o];
+      R.qnxx = A.qnxx_in[o];
+      R.qnyx = A.qnyx_in[o];
+      R.phn = A.phn_in[o];
+    }
+  };
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
                 NbrVals<T>& nv, CenterVals<T>& cv) {
     update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, from_yp(q0.xx), from_ym(q0.xx),
@@ -151,42 +188,38 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
   CenterVals<T> c0;
   T fc[9];
   QP<T> qb, qc;
+  RowIn<T, STAGE> cur;
   {
     const QP<T> qa = load_qp(x0 - 2);
     qb = load_qp(x0 - 1);
     qc = load_qp(x0);
-    T ft[9];
-    CenterVals<T> ct;
-    load_f(x0 - 1, ft);
-    uq(qa, qb, qc, ft, um, ct);
     const QP<T> qd = load_qp(x0 + 1);
+    T ft[9];
+    load_f(x0 - 1, ft);
     load_f(x0, fc);
+    load_row(x0, cur);
+    CenterVals<T> ct;
+    uq(qa, qb, qc, ft, um, ct);
     uq(qb, qc, qd, fc, u0, c0);
     qb = qc;
     qc = qd;
   }
   // invariant at loop top: qb = row x, qc = row x+1; um/u0 = rows x-1/x;
-  // c0, fc = centre data of row x
+  // c0, fc = centre data of row x; cur = loads of this iteration
   for (int x = x0; x < x1; ++x) {
+    // prefetch unconditionally (the wrapped row past the chunk is in bounds
+    // and unused): a conditional load would make the outstanding-load count
+    // path dependent and force a full wait at the top of the loop
+    RowIn<T, STAGE> nxt;
+    load_row(x + 1, nxt);
     const int ox = roff(x);
     const int o = ox + y;
-    // centre inputs of the corrector, issued early
-    T fnc[9];
-    T qn_xx = T(0), qn_yx = T(0), phn = T(0);
-    if constexpr (STAGE != PRED) {
-#pragma unroll
-      for (int v = 0; v < 9; ++v) fnc[v] = A.fn_in[v * N + o];
-      qn_xx = A.qnxx_in[o];
-      qn_yx = A.qnyx_in[o];
-      phn = A.phn_in[o];
-    }
+
     // UpdateQuantities on row x+1
-    const QP<T> qd = load_qp(x + 2);
-    T fnx[9];
-    load_f(x + 1, fnx);
+    if constexpr (!EXACT && STAGE != PRED) cur.q.p += cur.dp;
     NbrVals<T> un;
     CenterVals<T> cn;
-    uq(qb, qc, qd, fnx, un, cn);
+    uq(qb, qc, cur.q, cur.f, un, cn);
 
     // UpdateFields on row x
     UFStencil<T> s;
@@ -242,19 +275,19 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
           A.f_out[d] = fc[v] + om[v];
         }
       } else {
-        A.qxx_out[o] = qn_xx + T(.5) * Dxx;
-        A.qyx_out[o] = qn_yx + T(.5) * Dyx;
+        A.qxx_out[o] = cur.qnxx + T(.5) * Dxx;
+        A.qyx_out[o] = cur.qnyx + T(.5) * Dyx;
 #pragma unroll
-        for (int v = 0; v < 9; ++v) A.f_out[v * N + o] = fnc[v] + T(.5) * om[v];
+        for (int v = 0; v < 9; ++v) A.f_out[v * N + o] = cur.fn[v] + T(.5) * om[v];
         if constexpr (EXACT) {
-          A.phi_out[o] = phn + T(.5) * Dp;
+          A.phi_out[o] = cur.phn + T(.5) * Dp;
         } else if constexpr (STAGE == CORR) {
-          A.phi_out[o] = T(.5) * (phn + Dp);
+          A.phi_out[o] = T(.5) * (cur.phn + Dp);
         } else {
           // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, so the
           // rounding of the addition is carried to the next step
           const T pold = A.phi_in[o];
-          const T yv = T(.5) * (phn + Dp) + A.phic_in[o];
+          const T yv = T(.5) * (cur.phn + Dp) + A.phic_in[o];
           const T t = pold + yv;
           const T bp = t - pold;
           const T ap = t - bp;
@@ -269,9 +302,10 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
     u0 = un;
     c0 = cn;
 #pragma unroll
-    for (int v = 0; v < 9; ++v) fc[v] = fnx[v];
+    for (int v = 0; v < 9; ++v) fc[v] = cur.f[v];
     qb = qc;
-    qc = qd;
+    qc = cur.q;
+    cur = nxt;
   }
 }
 

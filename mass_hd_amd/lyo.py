@@ -45,7 +45,7 @@ PHYS_KEYS = ["GammaQ", "GammaP", "xi", "tauNem", "tauIso", "rho", "friction",
 class Params(ctypes.Structure):
     _fields_ = ([("LX", ctypes.c_int32), ("LY", ctypes.c_int32), ("npc", ctypes.c_int32),
                  ("precision", ctypes.c_int32), ("conserve_phi", ctypes.c_int32),
-                 ("flags", ctypes.c_int32)] + [(k, ctypes.c_double) for k in PHYS_KEYS])
+                 ("plane_pad", ctypes.c_int32)] + [(k, ctypes.c_double) for k in PHYS_KEYS])
 
 
 class Init(ctypes.Structure):
@@ -90,7 +90,7 @@ def load_library():
     lib.mass_lyo_sums.argtypes = [h, dp, dp]
     lib.mass_lyo_synchronize.argtypes = [h]
     lib.mass_lyo_time_steps.argtypes = [h, ctypes.c_int32, dp, dp]
-    lib.mass_lyo_set_launch.argtypes = [h, ctypes.c_int32, ctypes.c_int32]
+    lib.mass_lyo_set_launch.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.mass_lyo_abi_version.argtypes = []
     _lib = lib
     return lib
@@ -103,7 +103,7 @@ def _dptr(a):
 class Lyotropic:
     """One lyotropic run on the GPU (the reference's ``Lyotropic`` model)."""
 
-    def __init__(self, LX, LY, npc=1, precision=FP64, **phys):
+    def __init__(self, LX, LY, npc=1, precision=FP64, plane_pad=0, **phys):
         self.lib = load_library()
         if isinstance(precision, str):
             precision = {"fp64": FP64, "fp32": FP32}[precision]
@@ -117,7 +117,8 @@ class Lyotropic:
         self.npc, self.precision = int(npc), precision
         self.phys = dict(phys)
         p = Params(LX=self.LX, LY=self.LY, npc=self.npc, precision=precision,
-                   conserve_phi=0, flags=0, **{k: float(phys[k]) for k in PHYS_KEYS})
+                   conserve_phi=0, plane_pad=int(plane_pad),
+                   **{k: float(phys[k]) for k in PHYS_KEYS})
         h = ctypes.c_void_p()
         rc = self.lib.mass_lyo_create(ctypes.byref(p), ctypes.byref(h))
         if rc != 0:
@@ -204,5 +205,6 @@ class Lyotropic:
                                                  sw if per_sweep else None))
         return tot.value, (sw[0], sw[1])
 
-    def set_launch(self, chunk_rows=0, waves_per_block=0):
-        self._check(self.lib.mass_lyo_set_launch(self.h, int(chunk_rows), int(waves_per_block)))
+    def set_launch(self, chunk_rows=0, waves_per_block=0, xcd_order=-1):
+        self._check(self.lib.mass_lyo_set_launch(self.h, int(chunk_rows), int(waves_per_block),
+                                                 int(xcd_order)))

@@ -80,17 +80,29 @@ def test_launch_shape_does_not_change_results(require_gpu, chunk, wpb):
     ref, _, _, _ = pair(70, 125, npc=2)
     ref.step(3)
     g, _, _, _ = pair(70, 125, npc=2)
-    g.set_launch(chunk, wpb)
+    g.set_launch(chunk, wpb, chunk % 2)
     g.step(3)
     for n in STATE:
         assert np.array_equal(g.get(n), ref.get(n))
     g32, _, _, _ = pair(70, 125, npc=2, precision=lyo.FP32)
     r32, _, _, _ = pair(70, 125, npc=2, precision=lyo.FP32)
-    g32.set_launch(chunk, wpb)
+    g32.set_launch(chunk, wpb, chunk % 2)
     g32.step(3)
     r32.step(3)
     for n in STATE:
         assert np.array_equal(g32.get(n), r32.get(n))
+
+
+@pytest.mark.parametrize("precision", [lyo.FP64, lyo.FP32])
+def test_plane_padding_does_not_change_results(require_gpu, precision):
+    outs = []
+    for pad in (0, 4160):
+        g = lyo.Lyotropic(61, 83, npc=2, precision=precision, plane_pad=pad, **CONFIG0_PHYS)
+        g.configure(**CONFIG0_INIT)
+        g.step(4)
+        outs.append([g.get(n) for n in STATE + ["sigmaXY"]])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
 
 
 def test_sums_match_oracle(require_gpu):
@@ -103,8 +115,8 @@ def test_sums_match_oracle(require_gpu):
     # the (bit-identical) fields, and with the reference's serial sum
     exact_f = math.fsum(o.get("ff").reshape(-1))
     exact_p = math.fsum(o.get("phi"))
-    assert fs == pytest.approx(exact_f, rel=1e-15)
-    assert ps == pytest.approx(exact_p, rel=1e-15)
+    assert fs == pytest.approx(exact_f, rel=4e-15)
+    assert ps == pytest.approx(exact_p, rel=4e-15)
     assert fs == pytest.approx(s["fsum"], rel=1e-13)
     assert ps == pytest.approx(s["phisum"], rel=1e-13)
     # fixed tree: repeated calls agree exactly

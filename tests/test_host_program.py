@@ -1,0 +1,199 @@
+"""The C++ host program (mass_hd_amd/bin/mass_hip): option passes, runcard
+semantics, error paths and the JSON archive format.  These run on the CPU:
+every case here stops before the model needs the GPU, except the ones marked
+gpu at the end.
+"""
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "mass_hd_amd", "bin", "mass_hip")
+HOST = os.path.join(ROOT, "mass_hd_amd", "host")
+
+RUNCARD = """# plain lyotropic model, 32^2
+model = lyotropic
+LX = 32
+LY = 32
+nsteps = 20
+ninfo = 10
+seed = 1
+config = random
+angle = 90
+noise = 1
+Qamp = 0.5
+phinoise = 0.05
+tauNem = 2.0
+tauIso = 2.0
+GammaQ = 0.1
+GammaP = 0.1
+xi = 0.7
+zeta = 0.02
+LL = 0.04
+AA = 0.01
+KK = 0.04
+CC = 0
+zetaI = 0
+conc = 0.5
+friction = 0
+npc = 1
+"""
+
+
+def run(args, cwd=None):
+    r = subprocess.run([EXE] + args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       cwd=cwd, timeout=120)
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.fixture
+def card(tmp_path):
+    def make(text=RUNCARD):
+        p = tmp_path / "run.dat"
+        p.write_text(text)
+        return str(p)
+    return make
+
+
+def test_no_argument():
+    rc, out, err = run([])
+    assert rc == 1 and "no argument provided" in err
+
+
+def test_list_models():
+    rc, out, err = run(["--list"])
+    assert rc == 0 and "lyotropic" in out
+
+
+def test_help_shows_model_options():
+    rc, out, _ = run(["-h", "-m", "lyotropic"])
+    assert rc == 0
+    for key in ("GammaQ", "npc", "config", "precision", "--LX", "--nsteps"):
+        assert key in out
+
+
+def test_missing_runcard(tmp_path):
+    rc, _, err = run([str(tmp_path / "nope.dat")])
+    assert rc == 1 and "can not open runcard file" in err
+
+
+def test_unknown_model(card):
+    rc, _, err = run([card(RUNCARD.replace("lyotropic", "nematic"))])
+    assert rc == 1 and "can not be found" in err
+
+
+def test_unknown_key_rejected_in_second_pass(card):
+    rc, _, err = run([card(RUNCARD + "foo = 3\n")])
+    assert rc == 1 and "unrecognised option 'foo'" in err
+
+
+def test_duplicate_key_rejected(card):
+    rc, _, err = run([card(RUNCARD + "LX = 64\n")])
+    assert rc == 1 and "more than once" in err
+
+
+def test_bad_value(card):
+    rc, _, err = run([card(RUNCARD.replace("LX = 32", "LX = abc"))])
+    assert rc == 1 and "is invalid" in err
+
+
+def test_nsubsteps_rejected(card):
+    # lyotropic.cpp:64-66, raised before any device work
+    rc, _, err = run([card(RUNCARD + "nsubsteps = 2\n"), "--no-write"])
+    assert rc == 1 and "nsubsteps=1" in err
+
+
+def test_boundary_conditions_rejected(card):
+    rc, _, err = run([card(RUNCARD + "bc = 1\n"), "--no-write"])
+    assert rc == 1 and "periodic" in err
+
+
+def test_command_line_overrides_runcard(card):
+    # the command line is stored first, so it wins; bc on the command line
+    rc, _, err = run([card(), "--bc", "3", "--no-write"])
+    assert rc == 1 and "bc=3" in err
+    # and a runcard value alone is used
+    rc, _, err = run([card(RUNCARD + "bc = 4\n"), "--no-write"])
+    assert rc == 1 and "bc=4" in err
+
+
+def test_bad_precision(card):
+    rc, _, err = run([card(RUNCARD + "precision = fp16\n"), "--no-write"])
+    assert rc == 1 and "precision" in err
+
+
+_SER = r"""
+#include <array>
+#include <iostream>
+#include <string>
+#include <vector>
+#include "serialization.hpp"
+int main() {
+  unsigned u = 7; double d = 0.1; std::string s = "lyotropic";
+  std::vector<double> v = {1.0, 2.5, -3e-7, 123456789.0};
+  std::vector<std::array<double, 9>> a(2);
+  for (int i = 0; i < 9; ++i) { a[0][i] = i; a[1][i] = 40.0 / (i + 1); }
+  {
+    oarchive ar(std::cout, "frame", 1);
+    ar.add("u", u).add("d", d).add("s", s).add("v", v).add("ff", a);
+  }
+  return 0;
+}
+"""
+
+
+def test_archive_format(tmp_path):
+    """The archive writes the reference's JSON layout (id/version/data, each
+    entry a {type, value} group) with six-significant-digit numbers."""
+    src = tmp_path / "t.cpp"
+    src.write_text(_SER)
+    exe = str(tmp_path / "t")
+    subprocess.check_call(["g++", "-std=c++17", "-I" + HOST, "-o", exe, str(src)])
+    out = subprocess.check_output([exe], text=True)
+    assert out.startswith('{\n  "id" : "frame",\n  "version" : 1,\n  "data" : {\n    "u" : {\n'
+                          '      "type" : "unsigned",\n      "value" : 7\n    },')
+    j = json.loads(out)
+    assert j["id"] == "frame" and j["version"] == 1
+    d = j["data"]
+    assert d["u"] == {"type": "unsigned", "value": 7}
+    assert d["d"] == {"type": "double", "value": 0.1}
+    assert d["s"] == {"type": "string", "value": "lyotropic"}
+    assert d["v"]["type"] == "array(double)"
+    assert d["v"]["value"] == [1.0, 2.5, -3e-07, 123457000.0]  # %g, 6 digits
+    assert d["ff"]["type"] == "array(array(double))"
+    assert np.allclose(d["ff"]["value"][1], [40.0 / (i + 1) for i in range(9)], rtol=1e-5)
+    assert '"value" : [ 1, 2.5, -3e-07, 1.23457e+08 ]' in out
+
+
+# ------------------------------------------------------------------ on the GPU
+
+@pytest.mark.gpu
+def test_host_run_matches_oracle(card, tmp_path, require_gpu):
+    """End to end through the Model interface: runcard -> frames on disk; the
+    frame fields equal the oracle's to the archive's six digits."""
+    from oracle import pyoracle as po
+    outdir = tmp_path / "out"
+    rc, out, err = run([card(), "-o", str(outdir), "-f"])
+    assert rc == 0, err
+    assert "fcheck:" in out
+    frames = sorted(os.listdir(outdir))
+    assert frames == ["frame0.json", "frame10.json", "frame20.json", "parameters.json"]
+    params = json.load(open(outdir / "parameters.json"))["data"]
+    assert params["LX"]["value"] == 32 and params["model_name"]["value"] == "lyotropic"
+    assert params["init_config"]["value"] == "random"
+    o = po.Oracle(32, 32, npc=1, GammaQ=0.1, GammaP=0.1, xi=0.7, tauNem=2.0, tauIso=2.0,
+                  rho=40.0, friction=0.0, AA=0.01, CC=0.0, LL=0.04, KK=0.04, zeta=0.02,
+                  zetaI=0.0, conc=0.5)
+    o.configure(config="random", seed=1, angle=90, noise=1, qamp=0.5, phinoise=0.05)
+    f0 = json.load(open(outdir / "frame0.json"))["data"]
+    assert not np.any(f0["sigmaXX"]["value"])  # zero before the first step
+    assert np.allclose(f0["QQxx"]["value"], o.get("QQxx"), rtol=1e-5, atol=1e-6)
+    o.step(20)
+    f20 = json.load(open(outdir / "frame20.json"))["data"]
+    for k in ("QQxx", "QQyx", "phi", "sigmaXY", "sigma_active_xx"):
+        assert np.allclose(f20[k]["value"], o.get(k), rtol=1e-5, atol=1e-6), k
+    assert np.allclose(np.array(f20["ff"]["value"]), o.get("ff"), rtol=1e-5)
