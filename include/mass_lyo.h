@@ -155,6 +155,29 @@ int mass_lyo_get_field(mass_lyo_t h, int32_t field, double* out);
  * Synchronises. */
 int mass_lyo_sums(mass_lyo_t h, double* fsum, double* phisum);
 
+/* Run diagnostics of the current state, computed on the device with the
+ * definitions of the reference's post-processing (POST/massPy):
+ *   velocity u = sum_v c_v f_v / sum_v f_v (base_modules/flow.py:7-11);
+ *   defects: director from Q (nematic/nematicPy.py:4-15), winding number over
+ *   the 8-ring with each step folded into (-pi/2, pi/2] (base_modules/
+ *   defects.py:15-50), |charge| > 0.4, 4-connected same-sign regions with
+ *   periodic wrap counted once (defects.py:58-110, nematicPy.py:74-95). */
+#define MASS_LYO_HIST_BINS 64
+typedef struct mass_lyo_diag {
+  double u_rms;           /* sqrt(mean(ux^2 + uy^2)) */
+  double u_mean;          /* mean |u| */
+  double phi_mean, phi_min, phi_max;
+  double defect_density;  /* (defects_pos + defects_neg) / (LX*LY) */
+  int64_t defects_pos;    /* +1/2 defects */
+  int64_t defects_neg;    /* -1/2 defects */
+  /* [0]: phi < 0 or NaN; [1..64]: phi in [(i-1)/64, i/64); [65]: phi >= 1 */
+  uint64_t phi_hist[MASS_LYO_HIST_BINS + 2];
+} mass_lyo_diag;
+
+/* Fills *out; when charge is not NULL it also receives the charge field
+ * (N doubles).  Synchronises. */
+int mass_lyo_diagnostics(mass_lyo_t h, mass_lyo_diag* out, double* charge);
+
 /* Waits for all queued device work of the handle. */
 int mass_lyo_synchronize(mass_lyo_t h);
 
@@ -170,6 +193,38 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms,
  * (1 on, 0 off, -1 keep).  Results do not depend on it. */
 int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block,
                         int32_t xcd_order);
+
+/* ------------------------------------------------------------------ batches
+ * Many independent runs (own size and parameters; shared npc and precision)
+ * advanced together.  No reference counterpart: the reference runs one model
+ * per process; a batch is what a parameter sweep over its runcards becomes on
+ * one GPU.  Each sweep of a step is one persistent launch over the work items
+ * of all runs, and two consecutive steps are replayed from a HIP graph.  Runs
+ * are addressed by index 0..nruns-1; per-run calls mirror the single-run ABI. */
+typedef struct mass_lyo_batch_s* mass_lyo_batch_t;
+
+int mass_lyo_batch_create(const mass_lyo_params* runs, int32_t nruns, mass_lyo_batch_t* out);
+void mass_lyo_batch_destroy(mass_lyo_batch_t b);
+const char* mass_lyo_batch_last_error(mass_lyo_batch_t b);
+int mass_lyo_batch_size(mass_lyo_batch_t b);
+int mass_lyo_batch_configure(mass_lyo_batch_t b, int32_t run, const mass_lyo_init* init,
+                             double* ftot, double* ptot);
+int mass_lyo_batch_set_state(mass_lyo_batch_t b, int32_t run, const double* ff,
+                             const double* Qxx, const double* Qyx, const double* phi);
+/* nsteps steps of every run (asynchronous) */
+int mass_lyo_batch_step(mass_lyo_batch_t b, int32_t nsteps);
+int mass_lyo_batch_get_field(mass_lyo_batch_t b, int32_t run, int32_t field, double* out);
+int mass_lyo_batch_sums(mass_lyo_batch_t b, int32_t run, double* fsum, double* phisum);
+int mass_lyo_batch_diagnostics(mass_lyo_batch_t b, int32_t run, mass_lyo_diag* out,
+                               double* charge);
+/* chunk rows of the sweep work items and the persistent grid size in waves
+ * (0: one wave per item).  Results do not depend on them. */
+int mass_lyo_batch_set_launch(mass_lyo_batch_t b, int32_t chunk_rows, int32_t grid_waves);
+/* Times candidate launch shapes on the GPU and keeps the fastest.  It steps
+ * the zeroed fields of a new batch, so it must be called before any run is
+ * configured; the chosen shape is returned. */
+int mass_lyo_batch_autotune(mass_lyo_batch_t b, int32_t* chunk_rows, int32_t* grid_waves);
+int mass_lyo_batch_time_steps(mass_lyo_batch_t b, int32_t nsteps, double* total_ms);
 
 /* Version of this ABI (MASS_LYO_ABI_VERSION). */
 int mass_lyo_abi_version(void);

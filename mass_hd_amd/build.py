@@ -35,6 +35,7 @@ UNITS = [
     ("lyo_fp64.hip", ["-ffp-contract=off"]),
     ("lyo_fp32.hip", ["-ffp-contract=fast"]),
     ("lyo_capi.cpp", ["-ffp-contract=off", "-x", "hip"]),
+    ("lyo_batch.cpp", ["-ffp-contract=off", "-x", "hip"]),
 ]
 # Host units built with g++ and the reference's flags (-O3, no -march): the
 # initial fields go through libm cos/sin exactly as the reference's build

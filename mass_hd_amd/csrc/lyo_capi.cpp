@@ -15,27 +15,11 @@
 
 #include "../../include/mass_lyo.h"
 #include "lyo_config.hpp"
-#include "lyo_engine.hpp"
+#include "lyo_handle.hpp"
 
 namespace {
-
 thread_local std::string g_create_error;
-
-constexpr int kMaxPart = 4096;
-
 }  // namespace
-
-struct mass_lyo_s {
-  mass_lyo_params prm{};
-  int N = 0, NP = 0;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::unique_ptr<lyo::Engine> eng;
-  double* d_stage = nullptr;  // N*9 doubles (AoS populations) + 3N
-  double* d_part = nullptr;   // 2*kMaxPart partial sums + 2 results
-  lyo::LaunchCfg launch;
-  std::string err;
-};
 
 static int fail(mass_lyo_t h, int code, const std::string& msg) {
   if (h)
@@ -134,6 +118,7 @@ void mass_lyo_destroy(mass_lyo_t h) {
   h->eng.reset();
   if (h->d_stage) (void)hipFree(h->d_stage);
   if (h->d_part) (void)hipFree(h->d_part);
+  for (void* p : h->diag_allocs) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -152,6 +137,7 @@ static int upload_state(mass_lyo_t h, const double* ff, const double* qxx, const
   if (e == hipSuccess) e = h->eng->pack(d_ff, d_qxx, d_qyx, d_phi, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return hip_fail(h, e, "state upload");
+  h->has_state = true;
   return MASS_LYO_OK;
 }
 
@@ -225,6 +211,54 @@ int mass_lyo_sums(mass_lyo_t h, double* fsum, double* phisum) {
   if (e != hipSuccess) return hip_fail(h, e, "sums");
   if (fsum) *fsum = r[0];
   if (phisum) *phisum = r[1];
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_diagnostics(mass_lyo_t h, mass_lyo_diag* out, double* charge) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!out) return fail(h, MASS_LYO_EINVAL, "null output");
+  hipError_t e = hipSuccess;
+  lyo::DiagScratch& d = h->diag;
+  const size_t N = (size_t)h->N;
+  if (!d.part) {
+    auto get = [&](size_t bytes, void** p) {
+      if (e == hipSuccess) e = hipMalloc(p, bytes);
+      if (e == hipSuccess) h->diag_allocs.push_back(*p);
+    };
+    d.max_part = kMaxPart;
+    get(sizeof(double) * 5 * kMaxPart, (void**)&d.part);
+    get(sizeof(double) * 8, (void**)&d.out);
+    get(sizeof(unsigned long long) * (lyo::kHistBins + 2), (void**)&d.hist);
+    get(sizeof(unsigned long long) * 2, (void**)&d.cnt);
+    get(N, (void**)&d.sgn);
+    get(sizeof(int) * N, (void**)&d.label);
+    if (e != hipSuccess) {
+      for (void* p : h->diag_allocs) (void)hipFree(p);
+      h->diag_allocs.clear();
+      d = lyo::DiagScratch();
+      return hip_fail(h, e, "diagnostics allocation");
+    }
+  }
+  double* w_dev = charge ? h->d_stage : nullptr;
+  e = h->eng->diagnostics(d, w_dev, h->stream);
+  double r[5];
+  unsigned long long hist[lyo::kHistBins + 2], cnt[2];
+  if (e == hipSuccess) e = hipMemcpyAsync(r, d.out, sizeof(r), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(hist, d.hist, sizeof(hist), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(cnt, d.cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && charge)
+    e = hipMemcpyAsync(charge, w_dev, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "diagnostics");
+  out->u_rms = std::sqrt(r[0] / (double)N);
+  out->u_mean = r[1] / (double)N;
+  out->phi_mean = r[2] / (double)N;
+  out->phi_min = r[3];
+  out->phi_max = r[4];
+  out->defects_pos = (int64_t)cnt[0];
+  out->defects_neg = (int64_t)cnt[1];
+  out->defect_density = (double)(cnt[0] + cnt[1]) / (double)N;
+  for (int i = 0; i < lyo::kHistBins + 2; ++i) out->phi_hist[i] = hist[i];
   return MASS_LYO_OK;
 }
 

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "lyo_aux.hpp"
+#include "lyo_diag.hpp"
 #include "lyo_sweep.hpp"
 
 namespace lyo {
@@ -87,85 +88,105 @@ inline hipError_t timed_launch(const SweepArgs<T>& a, const Params<T>& p, const 
   return err;
 }
 
+// Buffer roles of sweep c of a step whose state is S[cur]: c = 0 is the
+// predictor, c = 1..npc the correctors (Lyotropic::Step,
+// src/models/lyotropic.cpp:500-525).
+template <class T, bool EXACT>
+SweepArgs<T> stage_args(const Bufs<T>& B, int npc, int c, int cur) {
+  const int nxt = cur ^ 1;
+  SweepArgs<T> a{};
+  if (c == 0) {
+    a.f_in = B.Sf[cur];
+    a.qxx_in = B.Sqxx[cur];
+    a.qyx_in = B.Sqyx[cur];
+    a.phi_in = B.Sphi[cur];
+    a.f_out = B.Pfs;
+    a.fn_out = B.Pfns;
+    a.qxx_out = B.Pqsxx;
+    a.qyx_out = B.Pqsyx;
+    a.qnxx_out = B.Pqnxx;
+    a.qnyx_out = B.Pqnyx;
+    a.phi_out = B.Pphi;
+    a.phn_out = B.Pphn;
+    return a;
+  }
+  if (c == 1) {
+    a.f_in = B.Pfs;
+    a.qxx_in = B.Pqsxx;
+    a.qyx_in = B.Pqsyx;
+    if constexpr (EXACT) {
+      a.phi_in = B.Pphi;
+    } else {
+      a.phi_in = B.Sphi[cur];
+      a.dphi_in = B.Pphi;
+    }
+  } else {
+    const int pv = (c - 2) & 1;
+    a.f_in = B.Cf[pv];
+    a.qxx_in = B.Cqxx[pv];
+    a.qyx_in = B.Cqyx[pv];
+    if constexpr (EXACT) {
+      a.phi_in = B.Cphi[pv];
+    } else {
+      a.phi_in = B.Sphi[cur];
+      a.dphi_in = B.Cphi[pv];
+    }
+  }
+  a.fn_in = B.Pfns;
+  a.qnxx_in = B.Pqnxx;
+  a.qnyx_in = B.Pqnyx;
+  a.phn_in = EXACT ? B.Pphn : B.Pphi;
+  if (c == npc) {
+    a.f_out = B.Sf[nxt];
+    a.qxx_out = B.Sqxx[nxt];
+    a.qyx_out = B.Sqyx[nxt];
+    a.phi_out = B.Sphi[nxt];
+    if constexpr (!EXACT) {
+      a.phic_in = B.Sphic[cur];
+      a.phic_out = B.Sphic[nxt];
+    }
+  } else {
+    const int ov = (c - 1) & 1;
+    a.f_out = B.Cf[ov];
+    a.qxx_out = B.Cqxx[ov];
+    a.qyx_out = B.Cqyx[ov];
+    a.phi_out = B.Cphi[ov];
+  }
+  return a;
+}
+
+// The last corrector's inputs are the inputs of the last UpdateQuantities:
+// the frame's derived fields are recomputed from them.
+template <class T>
+void note_last_uq(Bufs<T>& B, const SweepArgs<T>& a) {
+  B.have_uq = true;
+  B.uq_f = a.f_in;
+  B.uq_qxx = a.qxx_in;
+  B.uq_qyx = a.qyx_in;
+  B.uq_phi = a.phi_in;
+  B.uq_dphi = a.dphi_in;
+}
+
 template <class T, bool EXACT>
 hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int npc,
                      int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   const Geo g = make_geo(LX, LY, NP, L);
   for (int st = 0; st < nsteps; ++st) {
-    const int cur = B.cur, nxt = cur ^ 1;
-    {
-      SweepArgs<T> a{};
-      a.f_in = B.Sf[cur];
-      a.qxx_in = B.Sqxx[cur];
-      a.qyx_in = B.Sqyx[cur];
-      a.phi_in = B.Sphi[cur];
-      a.f_out = B.Pfs;
-      a.fn_out = B.Pfns;
-      a.qxx_out = B.Pqsxx;
-      a.qyx_out = B.Pqsyx;
-      a.qnxx_out = B.Pqnxx;
-      a.qnyx_out = B.Pqnyx;
-      a.phi_out = B.Pphi;
-      a.phn_out = B.Pphn;
-      hipError_t e = timed_launch<T, EXACT, PRED>(a, P, g, L, s, tm);
+    const int cur = B.cur;
+    for (int c = 0; c <= npc; ++c) {
+      const SweepArgs<T> a = stage_args<T, EXACT>(B, npc, c, cur);
+      hipError_t e;
+      if (c == 0) {
+        e = timed_launch<T, EXACT, PRED>(a, P, g, L, s, tm);
+      } else if (c < npc) {
+        e = timed_launch<T, EXACT, CORR>(a, P, g, L, s, tm);
+      } else {
+        note_last_uq(B, a);
+        e = timed_launch<T, EXACT, CORR_LAST>(a, P, g, L, s, tm);
+      }
       if (e != hipSuccess) return e;
     }
-    for (int c = 1; c <= npc; ++c) {
-      SweepArgs<T> a{};
-      if (c == 1) {
-        a.f_in = B.Pfs;
-        a.qxx_in = B.Pqsxx;
-        a.qyx_in = B.Pqsyx;
-        if constexpr (EXACT) {
-          a.phi_in = B.Pphi;
-        } else {
-          a.phi_in = B.Sphi[cur];
-          a.dphi_in = B.Pphi;
-        }
-      } else {
-        const int pv = (c - 2) & 1;
-        a.f_in = B.Cf[pv];
-        a.qxx_in = B.Cqxx[pv];
-        a.qyx_in = B.Cqyx[pv];
-        if constexpr (EXACT) {
-          a.phi_in = B.Cphi[pv];
-        } else {
-          a.phi_in = B.Sphi[cur];
-          a.dphi_in = B.Cphi[pv];
-        }
-      }
-      a.fn_in = B.Pfns;
-      a.qnxx_in = B.Pqnxx;
-      a.qnyx_in = B.Pqnyx;
-      a.phn_in = EXACT ? B.Pphn : B.Pphi;
-      const bool last = (c == npc);
-      if (last) {
-        a.f_out = B.Sf[nxt];
-        a.qxx_out = B.Sqxx[nxt];
-        a.qyx_out = B.Sqyx[nxt];
-        a.phi_out = B.Sphi[nxt];
-        if constexpr (!EXACT) {
-          a.phic_in = B.Sphic[cur];
-          a.phic_out = B.Sphic[nxt];
-        }
-        B.have_uq = true;
-        B.uq_f = a.f_in;
-        B.uq_qxx = a.qxx_in;
-        B.uq_qyx = a.qyx_in;
-        B.uq_phi = a.phi_in;
-        B.uq_dphi = a.dphi_in;
-      } else {
-        const int ov = (c - 1) & 1;
-        a.f_out = B.Cf[ov];
-        a.qxx_out = B.Cqxx[ov];
-        a.qyx_out = B.Cqyx[ov];
-        a.phi_out = B.Cphi[ov];
-      }
-      hipError_t e = last ? timed_launch<T, EXACT, CORR_LAST>(a, P, g, L, s, tm)
-                          : timed_launch<T, EXACT, CORR>(a, P, g, L, s, tm);
-      if (e != hipSuccess) return e;
-    }
-    B.cur = nxt;
+    B.cur = cur ^ 1;
   }
   return hipSuccess;
 }
@@ -234,6 +255,38 @@ hipError_t state_sums(const Bufs<T>& B, int N, int NP, double rho, double* d_par
   return hipGetLastError();
 }
 
+// Device scratch of the diagnostics (allocated on first use by the handle).
+struct DiagScratch {
+  double* part = nullptr;              // 5 * max_part
+  int max_part = 0;
+  double* out = nullptr;               // 5
+  unsigned long long* hist = nullptr;  // kHistBins + 2
+  unsigned long long* cnt = nullptr;   // 2
+  signed char* sgn = nullptr;          // N
+  int* label = nullptr;                // N
+};
+
+template <class T, bool EXACT>
+hipError_t run_diagnostics(const Bufs<T>& B, int LX, int LY, int NP, double rho, DiagScratch& d,
+                           double* w_out, hipStream_t s) {
+  const int N = LX * LY, cur = B.cur;
+  hipError_t e = hipMemsetAsync(d.hist, 0, sizeof(unsigned long long) * (kHistBins + 2), s);
+  if (e == hipSuccess) e = hipMemsetAsync(d.cnt, 0, sizeof(unsigned long long) * 2, s);
+  if (e != hipSuccess) return e;
+  int nparts = (N + 4095) / 4096;
+  if (nparts > d.max_part) nparts = d.max_part;
+  const int per = (N + nparts - 1) / nparts;
+  hipLaunchKernelGGL((diag_partial_kernel<T, !EXACT>), dim3(nparts), dim3(256), 0, s, B.Sf[cur],
+                     B.Sphi[cur], (const T*)B.Sphic[cur], N, NP, per, rho, d.part, d.hist);
+  hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(64), 0, s, d.part, nparts, d.out);
+  const int bs = 256, nb = (N + bs - 1) / bs;
+  hipLaunchKernelGGL((charge_kernel<T>), dim3(nb), dim3(bs), 0, s, B.Sqxx[cur], B.Sqyx[cur], LX,
+                     LY, w_out, d.sgn, d.label);
+  hipLaunchKernelGGL(ccl_union_kernel, dim3(nb), dim3(bs), 0, s, d.sgn, d.label, LX, LY);
+  hipLaunchKernelGGL(ccl_count_kernel, dim3(nb), dim3(bs), 0, s, d.sgn, d.label, N, d.cnt);
+  return hipGetLastError();
+}
+
 // Precision-independent interface of one run's device state; the two
 // implementations live in their own translation units so that the FP64 one
 // is compiled without FMA contraction.
@@ -248,7 +301,18 @@ class Engine {
                           const double* d_phi, hipStream_t s) = 0;
   virtual hipError_t field(int id, double* d_out, hipStream_t s) = 0;
   virtual hipError_t sums(double* d_part, int max_part, double* d_out, hipStream_t s) = 0;
+  virtual hipError_t diagnostics(DiagScratch& d, double* w_out, hipStream_t s) = 0;
   virtual size_t bytes() const = 0;
+  // batched stepping: the job of sweep c at state parity cur (host bytes of a
+  // SweepJob<T>), the bookkeeping after n batched steps, and the launch of a
+  // batched sweep over device jobs
+  virtual size_t job_bytes() const = 0;
+  virtual int job_items(const LaunchCfg& L) const = 0;
+  virtual void fill_job(int c, int cur, int first, const LaunchCfg& L, void* job) const = 0;
+  virtual void after_batch_steps(int nsteps) = 0;
+  virtual int parity() const = 0;
+  virtual hipError_t launch_batch(const void* jobs, int njobs, int nitems, int c, int grid_waves,
+                                  int wpb, hipStream_t s) const = 0;
 };
 
 template <class T, bool EXACT>
@@ -335,7 +399,48 @@ class EngineT final : public Engine {
   hipError_t sums(double* d_part, int max_part, double* d_out, hipStream_t s) override {
     return state_sums<T, EXACT>(B_, LX_ * LY_, NP_, rho_, d_part, max_part, d_out, s);
   }
+  hipError_t diagnostics(DiagScratch& d, double* w_out, hipStream_t s) override {
+    return run_diagnostics<T, EXACT>(B_, LX_, LY_, NP_, rho_, d, w_out, s);
+  }
   size_t bytes() const override { return bytes_; }
+
+  size_t job_bytes() const override { return sizeof(SweepJob<T>); }
+  int job_items(const LaunchCfg& L) const override {
+    return make_geo(LX_, LY_, NP_, L).nwaves;
+  }
+  void fill_job(int c, int cur, int first, const LaunchCfg& L, void* job) const override {
+    SweepJob<T> j;
+    j.A = stage_args<T, EXACT>(B_, npc_, c, cur);
+    j.P = P_;
+    j.g = make_geo(LX_, LY_, NP_, L);
+    j.g.xcd = 0;
+    j.first = first;
+    *static_cast<SweepJob<T>*>(job) = j;
+  }
+  void after_batch_steps(int nsteps) override {
+    if (nsteps <= 0) return;
+    const int last_cur = B_.cur ^ ((nsteps - 1) & 1);
+    note_last_uq(B_, stage_args<T, EXACT>(B_, npc_, npc_, last_cur));
+    B_.cur ^= (nsteps & 1);
+  }
+  int parity() const override { return B_.cur; }
+  hipError_t launch_batch(const void* jobs, int njobs, int nitems, int c, int grid_waves, int wpb,
+                          hipStream_t s) const override {
+    const SweepJob<T>* J = static_cast<const SweepJob<T>*>(jobs);
+    const int waves = grid_waves > 0 && grid_waves < nitems ? grid_waves : nitems;
+    const int blocks = (waves + wpb - 1) / wpb;
+    if (c == 0) {
+      hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, PRED>), dim3(blocks), dim3(64 * wpb), 0,
+                         s, J, njobs, nitems);
+    } else if (c < npc_) {
+      hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, CORR>), dim3(blocks), dim3(64 * wpb), 0,
+                         s, J, njobs, nitems);
+    } else {
+      hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, CORR_LAST>), dim3(blocks), dim3(64 * wpb),
+                         0, s, J, njobs, nitems);
+    }
+    return hipGetLastError();
+  }
 
  private:
   Bufs<T> B_;

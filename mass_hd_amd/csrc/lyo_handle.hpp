@@ -1,0 +1,28 @@
+// lyo_handle.hpp -- the run handle behind mass_lyo_t (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mass_lyo.h"
+#include "lyo_engine.hpp"
+
+constexpr int kMaxPart = 4096;
+
+struct mass_lyo_s {
+  mass_lyo_params prm{};
+  int N = 0, NP = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<lyo::Engine> eng;
+  double* d_stage = nullptr;  // N*9 doubles (AoS populations) + 3N
+  double* d_part = nullptr;   // 2*kMaxPart partial sums + 2 results
+  lyo::LaunchCfg launch;
+  lyo::DiagScratch diag;
+  std::vector<void*> diag_allocs;
+  bool has_state = false;     // configured or set_state called
+  std::string err;
+};

@@ -23,24 +23,25 @@ constexpr int kWave = 64;
 constexpr int kHalo = 2;
 constexpr int kStrip = kWave - 2 * kHalo;
 
-// value held by lane-1 (the y-1 neighbour); lane 0 receives 0
+// value held by lane-1 (the y-1 neighbour); lane 0 receives 0.  The DPP
+// wave shift usually folds into the instruction that consumes it.
 __device__ __forceinline__ float from_ym(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 // value held by lane+1 (the y+1 neighbour); lane 63 receives 0
 __device__ __forceinline__ float from_yp(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double from_ym(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x138, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 __device__ __forceinline__ double from_yp(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x130, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x130, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
@@ -106,30 +107,28 @@ struct RowIn {
   T qnxx, qnyx, phn;
 };
 
+#ifndef LYO_SWEEP_WAVES_PER_EU
+#define LYO_SWEEP_WAVES_PER_EU 3
+#endif
+
+// One work item: wave-uniform item index wid in [0, g.nwaves) = one strip of
+// 64 columns over one chunk of rows of one run.
 template <class T, bool EXACT, int STAGE>
-__global__ void __launch_bounds__(256)
-sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
-  int b = blockIdx.x;
-  if (g.xcd) {
-    // blocks are dispatched round-robin over the 8 XCDs; give XCD i the
-    // logical blocks [i*nb8, (i+1)*nb8) so halo rows stay in one L2
-    const int nb8 = (int)gridDim.x >> 3;
-    if (b < (nb8 << 3)) b = (b & 7) * nb8 + (b >> 3);
-  }
-  // wave-uniform: keeps the row arithmetic in scalar registers
-  const int wid = __builtin_amdgcn_readfirstlane(b * ((int)blockDim.x >> 6) +
-                                                 ((int)threadIdx.x >> 6));
-  if (wid >= g.nwaves) return;  // whole waves exit; no block barriers below
+__device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T>& P,
+                                           const Geo& g, const int wid) {
   const int lane = threadIdx.x & 63;
   const int strip = wid % g.nstrips;
   const int chunk = wid / g.nstrips;
   const int LX = g.LX, LY = g.LY, N = g.NP;
 
   const int ycol = strip * kStrip - kHalo + lane;
-  int y = ycol % LY;
-  if (y < 0) y += LY;
-  const int yp1 = (y + 1 == LY) ? 0 : y + 1;
-  const int ym1 = (y == 0) ? LY - 1 : y - 1;
+  int yi = ycol % LY;
+  if (yi < 0) yi += LY;
+  // lane offsets are unsigned 32-bit and row bases wave-uniform, so every
+  // access is a scalar base plus a 32-bit lane offset (no 64-bit lane math)
+  const unsigned y = (unsigned)yi;
+  const unsigned yp1 = (yi + 1 == LY) ? 0u : (unsigned)(yi + 1);
+  const unsigned ym1 = (yi == 0) ? (unsigned)(LY - 1) : (unsigned)(yi - 1);
   const bool active = lane >= kHalo && lane < kWave - kHalo && ycol < LY;
   const int x0 = chunk * g.xc;
   const int x1 = min(x0 + g.xc, LX);
@@ -143,36 +142,36 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
     return r * LY;
   };
   auto load_qp = [&](int r) {
-    const int o = roff(r) + y;
+    const int o = roff(r);
     QP<T> q;
-    q.xx = A.qxx_in[o];
-    q.yx = A.qyx_in[o];
-    T ph = A.phi_in[o];
-    if constexpr (!EXACT && STAGE != PRED) ph += A.dphi_in[o];
+    q.xx = (A.qxx_in + o)[y];
+    q.yx = (A.qyx_in + o)[y];
+    T ph = (A.phi_in + o)[y];
+    if constexpr (!EXACT && STAGE != PRED) ph += (A.dphi_in + o)[y];
     q.p = ph;
     return q;
   };
   auto load_f = [&](int r, T(&f)[9]) {
-    const int o = roff(r) + y;
+    const int o = roff(r);
 #pragma unroll
-    for (int v = 0; v < 9; ++v) f[v] = A.f_in[v * N + o];
+    for (int v = 0; v < 9; ++v) f[v] = (A.f_in + (v * N + o))[y];
   };
   auto load_row = [&](int x, RowIn<T, STAGE>& R) {
     {
-      const int o = roff(x + 2) + y;
-      R.q.xx = A.qxx_in[o];
-      R.q.yx = A.qyx_in[o];
-      R.q.p = A.phi_in[o];
-      if constexpr (!EXACT && STAGE != PRED) R.dp = A.dphi_in[o];
+      const int o = roff(x + 2);
+      R.q.xx = (A.qxx_in + o)[y];
+      R.q.yx = (A.qyx_in + o)[y];
+      R.q.p = (A.phi_in + o)[y];
+      if constexpr (!EXACT && STAGE != PRED) R.dp = (A.dphi_in + o)[y];
     }
     load_f(x + 1, R.f);
     if constexpr (STAGE != PRED) {
-      const int o = roff(x) + y;
+      const int o = roff(x);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) R.fn[v] = A.fn_in[v * N + o];
-      R.qnxx = A.qnxx_in[o];
-      R.qnyx = A.qnyx_in[o];
-      R.phn = A.phn_in[o];
+      for (int v = 0; v < 9; ++v) R.fn[v] = (A.fn_in + (v * N + o))[y];
+      R.qnxx = (A.qnxx_in + o)[y];
+      R.qnyx = (A.qnyx_in + o)[y];
+      R.phn = (A.phn_in + o)[y];
     }
   };
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
@@ -213,7 +212,6 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
     RowIn<T, STAGE> nxt;
     load_row(x + 1, nxt);
     const int ox = roff(x);
-    const int o = ox + y;
 
     // UpdateQuantities on row x+1
     if constexpr (!EXACT && STAGE != PRED) cur.q.p += cur.dp;
@@ -254,46 +252,46 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
 
     if (active) {
       if constexpr (STAGE == PRED) {
-        A.qnxx_out[o] = Qxx + T(.5) * Dxx;
-        A.qnyx_out[o] = Qyx + T(.5) * Dyx;
-        A.qxx_out[o] = Qxx + Dxx;
-        A.qyx_out[o] = Qyx + Dyx;
+        (A.qnxx_out + ox)[y] = Qxx + T(.5) * Dxx;
+        (A.qnyx_out + ox)[y] = Qyx + T(.5) * Dyx;
+        (A.qxx_out + ox)[y] = Qxx + Dxx;
+        (A.qyx_out + ox)[y] = Qyx + Dyx;
         if constexpr (EXACT) {
-          A.phn_out[o] = p + T(.5) * Dp;
-          A.phi_out[o] = p + Dp;
+          (A.phn_out + ox)[y] = p + T(.5) * Dp;
+          (A.phi_out + ox)[y] = p + Dp;
         } else {
-          A.phi_out[o] = Dp;
+          (A.phi_out + ox)[y] = Dp;
         }
         // collide and push to (x+cx, y+cy) (Move, lyotropic.cpp:371-386)
         const int rm = roff(x - 1), rp = roff(x + 1);
         const int rowv[9] = {ox, rp, rm, ox, ox, rp, rm, rm, rp};
-        const int colv[9] = {y, y, y, yp1, ym1, yp1, ym1, yp1, ym1};
+        const unsigned colv[9] = {y, y, y, yp1, ym1, yp1, ym1, yp1, ym1};
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
-          const int d = v * N + rowv[v] + colv[v];
-          A.fn_out[d] = fc[v] + T(.5) * om[v];
-          A.f_out[d] = fc[v] + om[v];
+          const int d = v * N + rowv[v];
+          (A.fn_out + d)[colv[v]] = fc[v] + T(.5) * om[v];
+          (A.f_out + d)[colv[v]] = fc[v] + om[v];
         }
       } else {
-        A.qxx_out[o] = cur.qnxx + T(.5) * Dxx;
-        A.qyx_out[o] = cur.qnyx + T(.5) * Dyx;
+        (A.qxx_out + ox)[y] = cur.qnxx + T(.5) * Dxx;
+        (A.qyx_out + ox)[y] = cur.qnyx + T(.5) * Dyx;
 #pragma unroll
-        for (int v = 0; v < 9; ++v) A.f_out[v * N + o] = cur.fn[v] + T(.5) * om[v];
+        for (int v = 0; v < 9; ++v) (A.f_out + (v * N + ox))[y] = cur.fn[v] + T(.5) * om[v];
         if constexpr (EXACT) {
-          A.phi_out[o] = cur.phn + T(.5) * Dp;
+          (A.phi_out + ox)[y] = cur.phn + T(.5) * Dp;
         } else if constexpr (STAGE == CORR) {
-          A.phi_out[o] = T(.5) * (cur.phn + Dp);
+          (A.phi_out + ox)[y] = T(.5) * (cur.phn + Dp);
         } else {
           // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, so the
           // rounding of the addition is carried to the next step
-          const T pold = A.phi_in[o];
-          const T yv = T(.5) * (cur.phn + Dp) + A.phic_in[o];
+          const T pold = (A.phi_in + ox)[y];
+          const T yv = T(.5) * (cur.phn + Dp) + (A.phic_in + ox)[y];
           const T t = pold + yv;
           const T bp = t - pold;
           const T ap = t - bp;
           const T err = (pold - ap) + (yv - bp);
-          A.phi_out[o] = t;
-          A.phic_out[o] = err;
+          (A.phi_out + ox)[y] = t;
+          (A.phic_out + ox)[y] = err;
         }
       }
     }
@@ -306,6 +304,56 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
     qb = qc;
     qc = cur.q;
     cur = nxt;
+  }
+}
+
+template <class T, bool EXACT, int STAGE>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? LYO_SWEEP_WAVES_PER_EU : 1)))
+sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
+  int b = blockIdx.x;
+  if (g.xcd) {
+    // blocks are dispatched round-robin over the 8 XCDs; give XCD i the
+    // logical blocks [i*nb8, (i+1)*nb8) so halo rows stay in one L2
+    const int nb8 = (int)gridDim.x >> 3;
+    if (b < (nb8 << 3)) b = (b & 7) * nb8 + (b >> 3);
+  }
+  // wave-uniform: keeps the row arithmetic in scalar registers
+  const int wid = __builtin_amdgcn_readfirstlane(b * ((int)blockDim.x >> 6) +
+                                                 ((int)threadIdx.x >> 6));
+  if (wid >= g.nwaves) return;  // whole waves exit; no block barriers below
+  sweep_item<T, EXACT, STAGE>(A, P, g, wid);
+}
+
+// Batched sweeps: many independent runs of any size in one launch.  Job j
+// covers the work items [first, first + g.nwaves) of the global item list.
+template <class T>
+struct SweepJob {
+  SweepArgs<T> A;
+  Params<T> P;
+  Geo g;
+  int first;
+};
+
+// Persistent grid: each wave walks the global item list with a stride of the
+// whole grid, finds the job of its item (binary search over the jobs' first
+// items, scalar loads) and runs it.  Every wave reaches the end of the list.
+template <class T, bool EXACT, int STAGE>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? LYO_SWEEP_WAVES_PER_EU : 1)))
+sweep_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const int nitems) {
+  const int nw = (int)(gridDim.x * (blockDim.x >> 6));
+  int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x >> 6) +
+                                            ((int)threadIdx.x >> 6));
+  for (; item < nitems; item += nw) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].first <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const SweepJob<T>& J = jobs[lo];
+    sweep_item<T, EXACT, STAGE>(J.A, J.P, J.g, item - J.first);
   }
 }
 

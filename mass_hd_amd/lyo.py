@@ -55,6 +55,17 @@ class Init(ctypes.Structure):
                 ("qamp", ctypes.c_double), ("phinoise", ctypes.c_double)]
 
 
+HIST_BINS = 64
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [("u_rms", ctypes.c_double), ("u_mean", ctypes.c_double),
+                ("phi_mean", ctypes.c_double), ("phi_min", ctypes.c_double),
+                ("phi_max", ctypes.c_double), ("defect_density", ctypes.c_double),
+                ("defects_pos", ctypes.c_int64), ("defects_neg", ctypes.c_int64),
+                ("phi_hist", ctypes.c_uint64 * (HIST_BINS + 2))]
+
+
 class MassLyoError(RuntimeError):
     """A non-zero status of the C ABI; ``code`` is the mass_lyo_status."""
 
@@ -92,6 +103,23 @@ def load_library():
     lib.mass_lyo_time_steps.argtypes = [h, ctypes.c_int32, dp, dp]
     lib.mass_lyo_set_launch.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.mass_lyo_abi_version.argtypes = []
+    lib.mass_lyo_diagnostics.argtypes = [h, P(Diag), dp]
+    i32 = ctypes.c_int32
+    lib.mass_lyo_batch_create.argtypes = [P(Params), i32, P(h)]
+    lib.mass_lyo_batch_destroy.argtypes = [h]
+    lib.mass_lyo_batch_destroy.restype = None
+    lib.mass_lyo_batch_last_error.argtypes = [h]
+    lib.mass_lyo_batch_last_error.restype = ctypes.c_char_p
+    lib.mass_lyo_batch_size.argtypes = [h]
+    lib.mass_lyo_batch_configure.argtypes = [h, i32, P(Init), dp, dp]
+    lib.mass_lyo_batch_set_state.argtypes = [h, i32, dp, dp, dp, dp]
+    lib.mass_lyo_batch_step.argtypes = [h, i32]
+    lib.mass_lyo_batch_get_field.argtypes = [h, i32, i32, dp]
+    lib.mass_lyo_batch_sums.argtypes = [h, i32, dp, dp]
+    lib.mass_lyo_batch_diagnostics.argtypes = [h, i32, P(Diag), dp]
+    lib.mass_lyo_batch_set_launch.argtypes = [h, i32, i32]
+    lib.mass_lyo_batch_autotune.argtypes = [h, P(i32), P(i32)]
+    lib.mass_lyo_batch_time_steps.argtypes = [h, i32, dp]
     _lib = lib
     return lib
 
@@ -198,6 +226,19 @@ class Lyotropic:
             raise MassLyoError(4, "f is not conserved (%g/%g)" % (self.ftot, fcheck))
         return fcheck, pcheck
 
+    def diagnostics(self, charge=False):
+        """u_rms, phi statistics/histogram and +-1/2 defect counts of the
+        current state; with charge=True also the winding-number field."""
+        d = Diag()
+        w = np.empty(self.N) if charge else None
+        self._check(self.lib.mass_lyo_diagnostics(self.h, ctypes.byref(d),
+                                                  _dptr(w) if charge else None))
+        out = {k: getattr(d, k) for k, _ in Diag._fields_ if k != "phi_hist"}
+        out["phi_hist"] = np.array(d.phi_hist[:], dtype=np.int64)
+        if charge:
+            out["charge"] = w
+        return out
+
     def time_steps(self, nsteps, per_sweep=True):
         tot = ctypes.c_double()
         sw = (ctypes.c_double * 2)()
@@ -208,3 +249,97 @@ class Lyotropic:
     def set_launch(self, chunk_rows=0, waves_per_block=0, xcd_order=-1):
         self._check(self.lib.mass_lyo_set_launch(self.h, int(chunk_rows), int(waves_per_block),
                                                  int(xcd_order)))
+
+
+class Batch:
+    """Independent runs advanced together (persistent batched sweeps + HIP
+    graph).  ``runs`` is a list of dicts with LX, LY and the model parameters;
+    npc and precision are shared."""
+
+    def __init__(self, runs, npc=2, precision=FP32):
+        self.lib = load_library()
+        if isinstance(precision, str):
+            precision = {"fp64": FP64, "fp32": FP32}[precision]
+        arr = (Params * len(runs))()
+        self.sizes = []
+        for i, r in enumerate(runs):
+            phys = {k: float(r[k]) for k in PHYS_KEYS}
+            arr[i] = Params(LX=int(r["LX"]), LY=int(r["LY"]), npc=int(npc), precision=precision,
+                            conserve_phi=0, plane_pad=0, **phys)
+            self.sizes.append((int(r["LX"]), int(r["LY"])))
+        h = ctypes.c_void_p()
+        rc = self.lib.mass_lyo_batch_create(arr, len(runs), ctypes.byref(h))
+        if rc != 0:
+            raise MassLyoError(rc, self.lib.mass_lyo_batch_last_error(None).decode())
+        self.h = h
+        self.npc, self.precision = npc, precision
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MassLyoError(rc, self.lib.mass_lyo_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mass_lyo_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def configure(self, run, config="random", seed=1, level=0.0, radius=0.0, angle=0.0,
+                  noise=1.0, qamp=1.0, phinoise=0.0):
+        cfg = CONFIGS[config] if isinstance(config, str) else int(config)
+        ini = Init(config=cfg, seed=seed, level=level, radius=radius, angle_deg=angle,
+                   noise=noise, qamp=qamp, phinoise=phinoise)
+        ft, pt = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.mass_lyo_batch_configure(self.h, run, ctypes.byref(ini),
+                                                      ctypes.byref(ft), ctypes.byref(pt)))
+        return ft.value, pt.value
+
+    def set_state(self, run, ff, Qxx, Qyx, phi):
+        LX, LY = self.sizes[run]
+        ff = np.ascontiguousarray(ff, dtype=np.float64).reshape(LX * LY * 9)
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(LX * LY) for x in (Qxx, Qyx, phi)]
+        self._check(self.lib.mass_lyo_batch_set_state(self.h, run, _dptr(ff), _dptr(a[0]),
+                                                      _dptr(a[1]), _dptr(a[2])))
+
+    def step(self, nsteps=1):
+        self._check(self.lib.mass_lyo_batch_step(self.h, int(nsteps)))
+
+    def get(self, run, name):
+        LX, LY = self.sizes[run]
+        fid = FIELDS[name] if isinstance(name, str) else int(name)
+        out = np.empty(LX * LY * (9 if fid == 0 else 1))
+        self._check(self.lib.mass_lyo_batch_get_field(self.h, run, fid, _dptr(out)))
+        return out.reshape(LX * LY, 9) if fid == 0 else out
+
+    def sums(self, run):
+        f, p = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.mass_lyo_batch_sums(self.h, run, ctypes.byref(f), ctypes.byref(p)))
+        return f.value, p.value
+
+    def diagnostics(self, run):
+        d = Diag()
+        self._check(self.lib.mass_lyo_batch_diagnostics(self.h, run, ctypes.byref(d), None))
+        out = {k: getattr(d, k) for k, _ in Diag._fields_ if k != "phi_hist"}
+        out["phi_hist"] = np.array(d.phi_hist[:], dtype=np.int64)
+        return out
+
+    def set_launch(self, chunk_rows=0, grid_waves=0):
+        self._check(self.lib.mass_lyo_batch_set_launch(self.h, int(chunk_rows), int(grid_waves)))
+
+    def autotune(self):
+        c, g = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.mass_lyo_batch_autotune(self.h, ctypes.byref(c), ctypes.byref(g)))
+        return c.value, g.value
+
+    def time_steps(self, nsteps):
+        t = ctypes.c_double()
+        self._check(self.lib.mass_lyo_batch_time_steps(self.h, int(nsteps), ctypes.byref(t)))
+        return t.value

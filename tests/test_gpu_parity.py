@@ -168,3 +168,16 @@ def test_errors_are_codes_not_crashes(require_gpu):
     assert e.value.code == 3 and "nan" in str(e.value)
     with pytest.raises(lyo.MassLyoError):
         g.configure(config=99)
+
+
+def test_precision_report_small(require_gpu):
+    """FP32 against FP64 over 400 steps at 128^2, npc=2: per-field relative
+    errors stay small and the compensated phi drifts less than FP64 rounding
+    allows for an uncompensated FP32 sum."""
+    from mass_hd_amd.reports import precision_report
+    r = precision_report(size=128, steps=400, npc=2, chunks=2)
+    last = r["series"][-1]
+    assert last["ff"]["max_rel"] < 1e-5
+    assert last["QQxx"]["max_rel"] < 1e-3 and last["phi"]["max_rel"] < 1e-3
+    assert abs(last["phi_drift_fp32"]) < 1e-9
+    assert abs(last["phi_drift_fp64"]) < 1e-12
