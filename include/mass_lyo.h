@@ -226,6 +226,61 @@ int mass_lyo_batch_set_launch(mass_lyo_batch_t b, int32_t chunk_rows, int32_t gr
 int mass_lyo_batch_autotune(mass_lyo_batch_t b, int32_t* chunk_rows, int32_t* grid_waves);
 int mass_lyo_batch_time_steps(mass_lyo_batch_t b, int32_t nsteps, double* total_ms);
 
+/* ------------------------------------------------- surrogate corrector
+ * A small convolutional network that stands in for the corrector sweeps after
+ * the first.  No reference counterpart (the reference always runs npc
+ * correctors, lyotropic.cpp:517-524).  Four periodic 3x3 convolutions
+ * 8 -> 32 -> 32 -> 32 -> 8 channels, tanh-GELU, BF16 weights and activations,
+ * FP32 accumulation on MFMA.  Inputs per node: ux, uy, Qxx, Qyx, phi, mu and
+ * the force div(sigma) (x, y) of the one-corrector state; outputs: the
+ * increments of ux, uy, Qxx, Qyx, phi that the remaining correctors add.
+ * Parameters: for each layer W[9][cout][cin] (tap t = 3*(dx+1) + (dy+1)) then
+ * b[cout], FP32. */
+typedef struct mass_lyo_net_s* mass_lyo_net_t;
+
+int mass_lyo_net_create(uint32_t seed, mass_lyo_net_t* out);
+void mass_lyo_net_destroy(mass_lyo_net_t n);
+const char* mass_lyo_net_last_error(mass_lyo_net_t n);
+int mass_lyo_net_num_params(mass_lyo_net_t n);
+int mass_lyo_net_get_params(mass_lyo_net_t n, float* out);
+int mass_lyo_net_set_params(mass_lyo_net_t n, const float* in);
+/* normalisation: mean[8], 1/std[8] of the inputs, scale[8], 1/scale[8] of
+ * the outputs */
+int mass_lyo_net_normalization(mass_lyo_net_t n, float* out32);
+/* network on host data already in network units: x, y are [LX*LY][8] FP32
+ * (node k = y + LY*x), LY a multiple of 16 */
+int mass_lyo_net_forward_host(mass_lyo_net_t n, const float* x, int32_t LX, int32_t LY, float* y);
+/* loss sum((out - y)^2)/(2N) and its gradient wrt the parameters */
+int mass_lyo_net_grad_host(mass_lyo_net_t n, const float* x, const float* y, int32_t LX,
+                           int32_t LY, double* loss, float* grads);
+/* appends one training frame taken from run h, whose last step must be a
+ * plain step with npc >= 2: the inputs of its one-corrector state and the
+ * increments its remaining correctors made */
+int mass_lyo_net_collect(mass_lyo_net_t n, mass_lyo_t h);
+int mass_lyo_net_dataset_size(mass_lyo_net_t n);
+/* Adam on the GPU over the collected frames (one frame per iteration) */
+int mass_lyo_net_train(mass_lyo_net_t n, int32_t iters, float lr, double* loss_first,
+                       double* loss_last);
+
+typedef struct mass_lyo_surrogate_stats {
+  int64_t steps;            /* steps taken since enabling */
+  int64_t surrogate_steps;  /* predictor + 1 corrector + network */
+  int64_t plain_steps;      /* predictor + npc correctors */
+  int64_t judged;           /* plain steps that also measured the error */
+  int64_t fallbacks;        /* judgements above tolerance */
+  double last_error;        /* ||u_sur - u_plain|| / ||u_plain|| of the last judgement */
+  double max_error;
+} mass_lyo_surrogate_stats;
+
+/* From now on mass_lyo_step on h uses the network (h needs npc >= 2 and LY a
+ * multiple of 16; the network must be trained).  Every judge_every-th step is
+ * a plain step that also measures the velocity error of the surrogate; an
+ * error above tol makes the next judge_every - 1 steps plain.  The network
+ * must outlive its use by h. */
+int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_every, double tol);
+int mass_lyo_surrogate_disable(mass_lyo_t h);
+int mass_lyo_surrogate_get_stats(mass_lyo_t h, mass_lyo_surrogate_stats* out);
+
 /* Version of this ABI (MASS_LYO_ABI_VERSION). */
 int mass_lyo_abi_version(void);
 

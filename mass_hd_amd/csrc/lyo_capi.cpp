@@ -16,6 +16,7 @@
 #include "../../include/mass_lyo.h"
 #include "lyo_config.hpp"
 #include "lyo_handle.hpp"
+#include "lyo_net_internal.hpp"
 
 namespace {
 thread_local std::string g_create_error;
@@ -119,6 +120,8 @@ void mass_lyo_destroy(mass_lyo_t h) {
   if (h->d_stage) (void)hipFree(h->d_stage);
   if (h->d_part) (void)hipFree(h->d_part);
   for (void* p : h->diag_allocs) (void)hipFree(p);
+  for (void* p : {(void*)h->sur.feat, (void*)h->sur.delta, (void*)h->sur.part})
+    if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -171,11 +174,106 @@ int mass_lyo_configure(mass_lyo_t h, const mass_lyo_init* init, double* ftot, do
   return upload_state(h, ff.data(), qxx.data(), qyx.data(), phi.data());
 }
 
+// One step in surrogate mode: a judged plain step every judge_every steps
+// (plain step, network on its one-corrector state, velocity error against the
+// plain result), plain steps while a failed judgement is being served, and
+// otherwise predictor + one corrector + the network's increment.
+static int surrogate_step(mass_lyo_t h) {
+  auto& S = h->sur;
+  const int LX = h->prm.LX, LY = h->prm.LY;
+  hipError_t e = hipSuccess;
+  const bool judge = (S.count % S.judge_every) == 0;
+  if (judge) {
+    e = h->eng->steps(1, h->launch, h->stream, nullptr);
+    if (e == hipSuccess) e = h->eng->net_features(1, S.feat, h->stream);
+    if (e == hipSuccess) e = lyo::net_predict(S.net, S.feat, LX, LY, S.delta, h->stream);
+    const int nparts = 256;
+    if (e == hipSuccess) e = h->eng->net_judge(S.delta, S.part, nparts, h->stream);
+    double part[2 * 256];
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(part, S.part, sizeof(part), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "judged step");
+    double num = 0, den = 0;
+    for (int i = 0; i < nparts; ++i) {
+      num += part[2 * i];
+      den += part[2 * i + 1];
+    }
+    const double err = den > 0 ? std::sqrt(num / den) : 0.0;
+    S.stats.judged++;
+    S.stats.plain_steps++;
+    S.stats.last_error = err;
+    if (err > S.stats.max_error) S.stats.max_error = err;
+    if (!(err <= S.tol)) {
+      S.stats.fallbacks++;
+      S.plain_left = S.judge_every - 1;
+    } else {
+      S.plain_left = 0;
+    }
+  } else if (S.plain_left > 0) {
+    e = h->eng->steps(1, h->launch, h->stream, nullptr);
+    S.plain_left--;
+    S.stats.plain_steps++;
+  } else {
+    e = h->eng->steps_with(1, 1, h->launch, h->stream);
+    if (e == hipSuccess) e = h->eng->net_features(0, S.feat, h->stream);
+    if (e == hipSuccess) e = lyo::net_predict(S.net, S.feat, LX, LY, S.delta, h->stream);
+    if (e == hipSuccess) e = h->eng->net_apply(S.delta, h->stream);
+    S.stats.surrogate_steps++;
+  }
+  if (e != hipSuccess) return hip_fail(h, e, "surrogate step");
+  S.count++;
+  S.stats.steps++;
+  return MASS_LYO_OK;
+}
+
 int mass_lyo_step(mass_lyo_t h, int32_t nsteps) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (nsteps < 0) return fail(h, MASS_LYO_EINVAL, "negative step count");
+  if (h->sur.net) {
+    for (int i = 0; i < nsteps; ++i) {
+      const int rc = surrogate_step(h);
+      if (rc) return rc;
+    }
+    return MASS_LYO_OK;
+  }
   hipError_t e = h->eng->steps(nsteps, h->launch, h->stream, nullptr);
   if (e != hipSuccess) return hip_fail(h, e, "sweep launch");
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_every, double tol) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!net || !lyo::net_ready(net))
+    return fail(h, MASS_LYO_EINVAL, "the network is not trained (no normalisation yet)");
+  if (h->prm.npc < 2) return fail(h, MASS_LYO_EINVAL, "the surrogate replaces correctors 2..npc: npc must be >= 2");
+  if (h->prm.LY % 16) return fail(h, MASS_LYO_EINVAL, "the surrogate needs LY to be a multiple of 16");
+  if (judge_every < 1 || !(tol > 0)) return fail(h, MASS_LYO_EINVAL, "judge_every >= 1 and tol > 0");
+  auto& S = h->sur;
+  hipError_t e = hipSuccess;
+  const size_t N = (size_t)h->N;
+  if (!S.feat) e = hipMalloc((void**)&S.feat, N * 8 * sizeof(float));
+  if (e == hipSuccess && !S.delta) e = hipMalloc((void**)&S.delta, N * 8 * sizeof(float));
+  if (e == hipSuccess && !S.part) e = hipMalloc((void**)&S.part, 2 * 256 * sizeof(double));
+  if (e != hipSuccess) return hip_fail(h, e, "surrogate buffers");
+  S.net = net;
+  S.judge_every = judge_every;
+  S.tol = tol;
+  S.count = 0;
+  S.plain_left = 0;
+  S.stats = mass_lyo_surrogate_stats{};
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_surrogate_disable(mass_lyo_t h) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  h->sur.net = nullptr;
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_surrogate_get_stats(mass_lyo_t h, mass_lyo_surrogate_stats* out) {
+  if (!h || !out) return fail(h, MASS_LYO_EINVAL, "null argument");
+  *out = h->sur.stats;
   return MASS_LYO_OK;
 }
 

@@ -12,6 +12,7 @@
 
 #include "lyo_aux.hpp"
 #include "lyo_diag.hpp"
+#include "lyo_net_engine.hpp"
 #include "lyo_sweep.hpp"
 
 namespace lyo {
@@ -313,6 +314,15 @@ class Engine {
   virtual int parity() const = 0;
   virtual hipError_t launch_batch(const void* jobs, int njobs, int nitems, int c, int grid_waves,
                                   int wpb, hipStream_t s) const = 0;
+  // surrogate corrector hooks (lyo_net_engine.hpp).  src 0: current state;
+  // src 1: the one-corrector state of the last plain step (needs npc >= 2)
+  virtual hipError_t net_features(int src, float* feat, hipStream_t s) const = 0;
+  virtual hipError_t net_targets(float* tgt, hipStream_t s) const = 0;
+  virtual hipError_t net_apply(const float* delta, hipStream_t s) = 0;
+  virtual hipError_t net_judge(const float* delta, double* part, int nparts, hipStream_t s) const = 0;
+  // nsteps steps with npc_run correctors (npc_run <= npc of the engine)
+  virtual hipError_t steps_with(int npc_run, int nsteps, const LaunchCfg& L, hipStream_t s) = 0;
+  virtual int npc() const = 0;
 };
 
 template <class T, bool EXACT>
@@ -424,6 +434,63 @@ class EngineT final : public Engine {
     B_.cur ^= (nsteps & 1);
   }
   int parity() const override { return B_.cur; }
+
+  hipError_t net_features(int src, float* feat, hipStream_t s) const override {
+    const int N = LX_ * LY_, cur = B_.cur;
+    const int bs = 128, nb = (N + bs - 1) / bs;
+    const T *f, *qxx, *qyx, *phi, *dphi = nullptr, *phic = nullptr;
+    if (src == 0) {
+      f = B_.Sf[cur];
+      qxx = B_.Sqxx[cur];
+      qyx = B_.Sqyx[cur];
+      phi = B_.Sphi[cur];
+      phic = B_.Sphic[cur];
+    } else {
+      if (npc_ < 2) return hipErrorInvalidValue;
+      f = B_.Cf[0];
+      qxx = B_.Cqxx[0];
+      qyx = B_.Cqyx[0];
+      if (EXACT) {
+        phi = B_.Cphi[0];
+      } else {
+        phi = B_.Sphi[cur ^ 1];
+        dphi = B_.Cphi[0];
+      }
+    }
+    hipLaunchKernelGGL((features_kernel<T, EXACT>), dim3(nb), dim3(bs), 0, s, f, qxx, qyx, phi,
+                       dphi, phic, P_, LX_, LY_, NP_, feat);
+    return hipGetLastError();
+  }
+  hipError_t net_targets(float* tgt, hipStream_t s) const override {
+    if (npc_ < 2) return hipErrorInvalidValue;
+    const int N = LX_ * LY_, cur = B_.cur;
+    const int bs = 256, nb = (N + bs - 1) / bs;
+    hipLaunchKernelGGL((targets_kernel<T, !EXACT>), dim3(nb), dim3(bs), 0, s, B_.Sf[cur],
+                       B_.Sqxx[cur], B_.Sqyx[cur], B_.Sphi[cur], (const T*)B_.Sphic[cur], B_.Cf[0],
+                       B_.Cqxx[0], B_.Cqyx[0], EXACT ? B_.Cphi[0] : B_.Sphi[cur ^ 1],
+                       EXACT ? (const T*)nullptr : (const T*)B_.Cphi[0], N, NP_, rho_, tgt);
+    return hipGetLastError();
+  }
+  hipError_t net_apply(const float* delta, hipStream_t s) override {
+    const int N = LX_ * LY_, cur = B_.cur;
+    const int bs = 256, nb = (N + bs - 1) / bs;
+    hipLaunchKernelGGL((apply_increment_kernel<T, !EXACT>), dim3(nb), dim3(bs), 0, s, B_.Sf[cur],
+                       B_.Sqxx[cur], B_.Sqyx[cur], B_.Sphi[cur], B_.Sphic[cur], delta, N, NP_,
+                       rho_);
+    return hipGetLastError();
+  }
+  hipError_t net_judge(const float* delta, double* part, int nparts,
+                       hipStream_t s) const override {
+    if (npc_ < 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((judge_kernel<T, !EXACT>), dim3(nparts), dim3(256), 0, s, B_.Cf[0],
+                       B_.Sf[B_.cur], delta, LX_ * LY_, NP_, rho_, part);
+    return hipGetLastError();
+  }
+  hipError_t steps_with(int npc_run, int nsteps, const LaunchCfg& L, hipStream_t s) override {
+    if (npc_run < 1 || npc_run > npc_) return hipErrorInvalidValue;
+    return run_steps<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_run, nsteps, L, s, nullptr);
+  }
+  int npc() const override { return npc_; }
   hipError_t launch_batch(const void* jobs, int njobs, int nitems, int c, int grid_waves, int wpb,
                           hipStream_t s) const override {
     const SweepJob<T>* J = static_cast<const SweepJob<T>*>(jobs);

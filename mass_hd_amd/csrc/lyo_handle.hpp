@@ -24,5 +24,17 @@ struct mass_lyo_s {
   lyo::DiagScratch diag;
   std::vector<void*> diag_allocs;
   bool has_state = false;     // configured or set_state called
+  // surrogate corrector
+  struct Surrogate {
+    mass_lyo_net_t net = nullptr;
+    int judge_every = 50;
+    double tol = 1e-3;
+    long long count = 0;      // steps taken in surrogate mode
+    int plain_left = 0;       // plain steps still owed after a failed judgement
+    float* feat = nullptr;    // [N][8]
+    float* delta = nullptr;   // [N][8]
+    double* part = nullptr;   // judge partial sums
+    mass_lyo_surrogate_stats stats{};
+  } sur;
   std::string err;
 };

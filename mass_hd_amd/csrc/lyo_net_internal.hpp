@@ -1,0 +1,14 @@
+// lyo_net_internal.hpp -- network entry points used by the run handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/mass_lyo.h"
+
+namespace lyo {
+// delta (FP32 [N][8], physical units) = network(features FP32 [N][8]) on s
+hipError_t net_predict(mass_lyo_net_t n, const float* feat, int LX, int LY, float* delta,
+                       hipStream_t s);
+// trained (normalisation known)
+bool net_ready(mass_lyo_net_t n);
+}  // namespace lyo

@@ -343,3 +343,151 @@ class Batch:
         t = ctypes.c_double()
         self._check(self.lib.mass_lyo_batch_time_steps(self.h, int(nsteps), ctypes.byref(t)))
         return t.value
+
+
+class SurrogateStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("surrogate_steps", ctypes.c_int64),
+                ("plain_steps", ctypes.c_int64), ("judged", ctypes.c_int64),
+                ("fallbacks", ctypes.c_int64), ("last_error", ctypes.c_double),
+                ("max_error", ctypes.c_double)]
+
+
+def _net_api(lib):
+    if getattr(lib, "_net_api", False):
+        return lib
+    P = ctypes.POINTER
+    h = ctypes.c_void_p
+    fp = P(ctypes.c_float)
+    dp = P(ctypes.c_double)
+    i32 = ctypes.c_int32
+    lib.mass_lyo_net_create.argtypes = [ctypes.c_uint32, P(h)]
+    lib.mass_lyo_net_destroy.argtypes = [h]
+    lib.mass_lyo_net_destroy.restype = None
+    lib.mass_lyo_net_last_error.argtypes = [h]
+    lib.mass_lyo_net_last_error.restype = ctypes.c_char_p
+    lib.mass_lyo_net_num_params.argtypes = [h]
+    lib.mass_lyo_net_get_params.argtypes = [h, fp]
+    lib.mass_lyo_net_set_params.argtypes = [h, fp]
+    lib.mass_lyo_net_normalization.argtypes = [h, fp]
+    lib.mass_lyo_net_forward_host.argtypes = [h, fp, i32, i32, fp]
+    lib.mass_lyo_net_grad_host.argtypes = [h, fp, fp, i32, i32, dp, fp]
+    lib.mass_lyo_net_collect.argtypes = [h, h]
+    lib.mass_lyo_net_dataset_size.argtypes = [h]
+    lib.mass_lyo_net_train.argtypes = [h, i32, ctypes.c_float, dp, dp]
+    lib.mass_lyo_surrogate_enable.argtypes = [h, h, i32, ctypes.c_double]
+    lib.mass_lyo_surrogate_disable.argtypes = [h]
+    lib.mass_lyo_surrogate_get_stats.argtypes = [h, P(SurrogateStats)]
+    lib._net_api = True
+    return lib
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+NET_CHANNELS = [8, 32, 32, 32, 8]
+
+
+class Net:
+    """The surrogate corrector network (include/mass_lyo.h, mass_lyo_net_*)."""
+
+    def __init__(self, seed=0):
+        self.lib = _net_api(load_library())
+        h = ctypes.c_void_p()
+        rc = self.lib.mass_lyo_net_create(seed, ctypes.byref(h))
+        if rc != 0:
+            raise MassLyoError(rc, self.lib.mass_lyo_net_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MassLyoError(rc, self.lib.mass_lyo_net_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mass_lyo_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self):
+        return self.lib.mass_lyo_net_num_params(self.h)
+
+    def get_params(self):
+        out = np.empty(self.num_params, dtype=np.float32)
+        self._check(self.lib.mass_lyo_net_get_params(self.h, _fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.lib.mass_lyo_net_set_params(self.h, _fptr(p)))
+
+    def layers(self, flat=None):
+        """[(W[9, cout, cin], b[cout])] views of a flat parameter vector."""
+        flat = self.get_params() if flat is None else flat
+        out, o = [], 0
+        for l in range(4):
+            ci, co = NET_CHANNELS[l], NET_CHANNELS[l + 1]
+            W = flat[o:o + 9 * co * ci].reshape(9, co, ci)
+            o += 9 * co * ci
+            b = flat[o:o + co]
+            o += co
+            out.append((W, b))
+        return out
+
+    def normalization(self):
+        out = np.empty(32, dtype=np.float32)
+        self._check(self.lib.mass_lyo_net_normalization(self.h, _fptr(out)))
+        return out
+
+    def forward(self, x):
+        LX, LY, _ = x.shape
+        xi = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((LX, LY, 8), dtype=np.float32)
+        self._check(self.lib.mass_lyo_net_forward_host(self.h, _fptr(xi), LX, LY, _fptr(y)))
+        return y
+
+    def grad(self, x, y):
+        LX, LY, _ = x.shape
+        xi = np.ascontiguousarray(x, dtype=np.float32)
+        yi = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(self.num_params, dtype=np.float32)
+        L = ctypes.c_double()
+        self._check(self.lib.mass_lyo_net_grad_host(self.h, _fptr(xi), _fptr(yi), LX, LY,
+                                                     ctypes.byref(L), _fptr(g)))
+        return L.value, g
+
+    def collect(self, model):
+        self._check(self.lib.mass_lyo_net_collect(self.h, model.h))
+
+    @property
+    def dataset_size(self):
+        return self.lib.mass_lyo_net_dataset_size(self.h)
+
+    def train(self, iters, lr=1e-3):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.mass_lyo_net_train(self.h, int(iters), float(lr), ctypes.byref(a),
+                                                ctypes.byref(b)))
+        return a.value, b.value
+
+
+def enable_surrogate(model, net, judge_every=50, tol=1e-3):
+    lib = _net_api(model.lib)
+    model._check(lib.mass_lyo_surrogate_enable(model.h, net.h, int(judge_every), float(tol)))
+
+
+def disable_surrogate(model):
+    lib = _net_api(model.lib)
+    model._check(lib.mass_lyo_surrogate_disable(model.h))
+
+
+def surrogate_stats(model):
+    lib = _net_api(model.lib)
+    s = SurrogateStats()
+    model._check(lib.mass_lyo_surrogate_get_stats(model.h, ctypes.byref(s)))
+    return {k: getattr(s, k) for k, _ in SurrogateStats._fields_}
