@@ -58,8 +58,42 @@ def cpu_baseline(L, steps, threads):
     return L * L * steps / dt / 1e6
 
 
+def calibrate(gib=1, reps=10):
+    """Plain device-to-device copies of a known size through the HIP runtime:
+    the practical HBM ceiling, and the reference workload for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = gib << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        hip.hipMemset(p, 0, ctypes.c_size_t(n))
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # warm-up
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b, a, ctypes.c_size_t(n), 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    gbps = 2.0 * n * reps / (ms.value / 1e3) / 1e9
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration": "device-to-device copy", "bytes_read_per_copy": n,
+                      "bytes_written_per_copy": n, "copies": reps + 1,
+                      "copy_GBps_read_plus_write": round(gbps, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate", action="store_true",
+                    help="time plain 1 GiB device copies (HBM ceiling, PMC scale)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
@@ -71,6 +105,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the x-slab decomposition is design only)", file=sys.stderr)

@@ -225,6 +225,11 @@ int mass_lyo_batch_set_launch(mass_lyo_batch_t b, int32_t chunk_rows, int32_t gr
  * configured; the chosen shape is returned. */
 int mass_lyo_batch_autotune(mass_lyo_batch_t b, int32_t* chunk_rows, int32_t* grid_waves);
 int mass_lyo_batch_time_steps(mass_lyo_batch_t b, int32_t nsteps, double* total_ms);
+/* The run handle of member `run` (owned by the batch; valid until the batch
+ * is destroyed), after waiting for the batch's queued steps.  Stepping it on
+ * its own desynchronises it from the batch and is not allowed; reading,
+ * diagnostics and collecting training frames are. */
+mass_lyo_t mass_lyo_batch_run(mass_lyo_batch_t b, int32_t run);
 
 /* ------------------------------------------------- surrogate corrector
  * A small convolutional network that stands in for the corrector sweeps after

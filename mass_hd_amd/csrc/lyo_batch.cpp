@@ -269,6 +269,12 @@ int mass_lyo_batch_time_steps(mass_lyo_batch_t b, int32_t nsteps, double* total_
   return MASS_LYO_OK;
 }
 
+mass_lyo_t mass_lyo_batch_run(mass_lyo_batch_t b, int32_t run) {
+  if (!b || run < 0 || run >= (int)b->runs.size()) return nullptr;
+  if (sync_batch(b) != MASS_LYO_OK) return nullptr;
+  return b->runs[run];
+}
+
 int mass_lyo_batch_autotune(mass_lyo_batch_t b, int32_t* chunk_rows, int32_t* grid_waves) {
   if (!b) return bfail(nullptr, MASS_LYO_EINVAL, "null batch");
   for (auto* h : b->runs)

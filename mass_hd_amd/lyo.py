@@ -491,3 +491,19 @@ def surrogate_stats(model):
     s = SurrogateStats()
     model._check(lib.mass_lyo_surrogate_get_stats(model.h, ctypes.byref(s)))
     return {k: getattr(s, k) for k, _ in SurrogateStats._fields_}
+
+
+class RunRef:
+    """Borrowed handle of one member of a Batch (reading, diagnostics,
+    collecting training frames)."""
+
+    def __init__(self, batch, i):
+        lib = batch.lib
+        lib.mass_lyo_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        lib.mass_lyo_batch_run.restype = ctypes.c_void_p
+        h = lib.mass_lyo_batch_run(batch.h, int(i))
+        if not h:
+            raise MassLyoError(1, "no run %d in the batch" % i)
+        self.h = ctypes.c_void_p(h)
+        self.lib = lib
+        self.batch = batch  # keeps the batch alive

@@ -1,0 +1,162 @@
+"""The benchmark workloads of the lyotropic step on one MI355X.
+
+  config0  128^2 FP64, npc=1, 2000 steps (seed 1; reference init contract)
+  config1  1024^2, npc=2, 10^4 steps: FP64 vs FP32 error report
+  config2  64 independent runs (L in {128,192,256,384,512}, zeta log-uniform in
+           [1e-3, 5e-2], xi in {0.5, 0.7, 1.0}), FP32, npc=2, one batch
+  config3  surrogate corrector at 512^2 (trained on config2's 512^2 members)
+  config4  4096^2 FP32 plain npc=2 vs surrogate; defects and u_rms against
+           the FP64 plain run
+
+    python -m mass_hd_amd.workloads config2 --steps 5000 --out r.json
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import lyo
+from .reports import CONFIG0_INIT, CONFIG0_PHYS, precision_report
+
+SIZES = [128, 192, 256, 384, 512]
+XIS = [0.5, 0.7, 1.0]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def config2_runs(n=64, sampler_seed=2024, base_seed=1000):
+    """The sweep: sizes, activities and alignment parameters per run."""
+    rng = np.random.default_rng(sampler_seed)
+    runs = []
+    for i in range(n):
+        L = int(rng.choice(SIZES))
+        zeta = float(math.exp(rng.uniform(math.log(1e-3), math.log(5e-2))))
+        xi = float(rng.choice(XIS))
+        runs.append(dict(CONFIG0_PHYS, LX=L, LY=L, zeta=zeta, xi=xi, seed=base_seed + i))
+    return runs
+
+
+def make_batch(runs, npc=2, precision="fp32", autotune=True):
+    b = lyo.Batch([{k: v for k, v in r.items() if k != "seed"} for r in runs], npc=npc,
+                  precision=precision)
+    tuned = b.autotune() if autotune else None
+    for i, r in enumerate(runs):
+        b.configure(i, **dict(CONFIG0_INIT, seed=r["seed"]))
+    return b, tuned
+
+
+def config2(steps=5000, n=64, sampler_seed=2024, base_seed=1000, chunks=5):
+    runs = config2_runs(n, sampler_seed, base_seed)
+    nodes = sum(r["LX"] * r["LY"] for r in runs)
+    b, tuned = make_batch(runs)
+    log("config2: %d runs, %d nodes, autotuned launch %s" % (n, nodes, tuned))
+    t_ms = 0.0
+    done = 0
+    for c in range(chunks):
+        k = steps * (c + 1) // chunks - done
+        t_ms += b.time_steps(k)
+        done += k
+        log("  %d steps, %.1f MLUPS" % (done, nodes * done / (t_ms / 1e3) / 1e6))
+    diag = [b.diagnostics(i) for i in range(n)]
+    out = {"runs": len(runs), "nodes": nodes, "steps": steps, "ms": t_ms,
+           "mlups": nodes * steps / (t_ms / 1e3) / 1e6, "autotuned": tuned,
+           "sampler_seed": sampler_seed, "base_seed": base_seed,
+           "u_rms": [d["u_rms"] for d in diag],
+           "defects": [d["defects_pos"] + d["defects_neg"] for d in diag],
+           "fsum_ok": all(abs(b.sums(i)[0] - 40.0 * r["LX"] * r["LY"]) < 1.0
+                          for i, r in enumerate(runs))}
+    b.close()
+    return out
+
+
+def train_surrogate(frames_per_run=8, spacing=50, warm=200, iters=2000, lr=2e-3, n=64,
+                    sampler_seed=2024, base_seed=1000, seed=0):
+    """Trains the surrogate on frames of the 512^2 members of the config2
+    sweep (their own plain npc=2 steps)."""
+    runs = [r for r in config2_runs(n, sampler_seed, base_seed) if r["LX"] == 512]
+    net = lyo.Net(seed=seed)
+    b, _ = make_batch(runs, autotune=False)
+    b.step(warm)
+    for _ in range(frames_per_run):
+        b.step(spacing)
+        for i in range(len(runs)):
+            net.collect(lyo.RunRef(b, i))
+    pairs = net.dataset_size * 512 * 512
+    log("surrogate: %d frames (%d predictor->corrector pairs) from %d runs" %
+        (net.dataset_size, pairs, len(runs)))
+    t0 = time.time()
+    l0, l1 = net.train(iters, lr)
+    log("  trained %d iterations in %.1fs: loss %.4g -> %.4g" % (iters, time.time() - t0, l0, l1))
+    b.close()
+    return net, {"frames": net.dataset_size, "pairs": pairs, "runs": len(runs), "iters": iters,
+                 "loss_first": l0, "loss_last": l1}
+
+
+def compare(L, net, steps, zeta=None, seed=7, judge_every=50, tol=1e-3, every=None,
+            fp64_steps=0):
+    """Plain npc=2 vs surrogate (and FP64 plain over fp64_steps) from one
+    initial state; time series of u_rms and defect density."""
+    phys = dict(CONFIG0_PHYS)
+    if zeta is not None:
+        phys["zeta"] = zeta
+    every = every or max(1, steps // 20)
+    series = {}
+    timing = {}
+    for mode in (["fp64"] if fp64_steps else []) + ["plain", "surrogate"]:
+        prec = "fp64" if mode == "fp64" else "fp32"
+        m = lyo.Lyotropic(L, L, npc=2, precision=prec, **phys)
+        m.configure(**dict(CONFIG0_INIT, seed=seed))
+        if mode == "surrogate":
+            lyo.enable_surrogate(m, net, judge_every, tol)
+        n = fp64_steps if mode == "fp64" else steps
+        rows, t_ms, done = [], 0.0, 0
+        while done < n:
+            k = min(every, n - done)
+            t0 = time.time()
+            m.step(k)
+            m.synchronize()
+            t_ms += (time.time() - t0) * 1e3
+            done += k
+            d = m.diagnostics()
+            rows.append((done, d["u_rms"], d["defect_density"]))
+        timing[mode] = {"steps": n, "ms": t_ms, "mlups": L * L * n / (t_ms / 1e3) / 1e6}
+        if mode == "surrogate":
+            timing[mode]["stats"] = lyo.surrogate_stats(m)
+        series[mode] = rows
+        log("%s %d^2: %d steps, %.0f MLUPS, final u_rms %.4g, defect density %.4g" %
+            (mode, L, n, timing[mode]["mlups"], rows[-1][1], rows[-1][2]))
+        m.close()
+    return {"L": L, "steps": steps, "series": series, "timing": timing}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["config1", "config2", "config3", "config4"])
+    ap.add_argument("--steps", type=int, default=0)
+    ap.add_argument("--iters", type=int, default=2000)
+    ap.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+    if a.what == "config1":
+        r = precision_report(1024, a.steps or 10000, 2, 10, progress=log)
+    elif a.what == "config2":
+        r = config2(a.steps or 5000)
+    else:
+        net, tr = train_surrogate(iters=a.iters)
+        if a.what == "config3":
+            r = {"training": tr, "run": compare(512, net, a.steps or 2000)}
+        else:
+            r = {"training": tr, "run": compare(4096, net, a.steps or 100000, zeta=0.03, seed=7,
+                                                every=1000, fp64_steps=10000)}
+    txt = json.dumps(r, indent=1)
+    if a.out:
+        open(a.out, "w").write(txt)
+    print(txt)
+
+
+if __name__ == "__main__":
+    main()

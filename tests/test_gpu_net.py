@@ -45,27 +45,32 @@ def test_gradients_match_oracle(require_gpu):
     cos = float(g @ flat / (np.linalg.norm(g) * np.linalg.norm(flat)))
     assert cos > 0.998, cos
     assert np.linalg.norm(g - flat) / np.linalg.norm(flat) < 6e-2
-    # per layer, too
     o = 0
-    for (W, b) in gr:
+    for (W, b) in gr:  # per layer, too
         n = W.size + b.size
-        seg = g[o:o + n]
         ref = np.concatenate([W.reshape(-1), b])
-        assert np.linalg.norm(seg - ref) / np.linalg.norm(ref) < 8e-2
+        assert np.linalg.norm(g[o:o + n] - ref) / np.linalg.norm(ref) < 8e-2
         o += n
 
 
-def _model(L, npc, seed, precision="fp32"):
-    m = lyo.Lyotropic(L, L, npc=npc, precision=precision, **CONFIG0_PHYS)
+def _model(L, npc, seed):
+    m = lyo.Lyotropic(L, L, npc=npc, precision="fp32", **CONFIG0_PHYS)
     m.configure(**dict(CONFIG0_INIT, seed=seed))
     return m
+
+
+def _u(m):
+    """velocity of the populations (not the frame's derived ux/uy, which are
+    the inputs of the last corrector)"""
+    from oracle import diag_oracle as dg
+    vx, vy = dg.velocity(m.get("ff"), m.LX, m.LY)
+    return np.concatenate([vx.reshape(-1), vy.reshape(-1)])
 
 
 def test_training_reduces_loss_and_beats_no_network(require_gpu):
     L = 64
     net = lyo.Net(seed=11)
-    # training frames from plain npc=2 runs
-    for seed in (21, 22):
+    for seed in (21, 22):  # training frames from plain npc=2 runs
         m = _model(L, 2, seed)
         for _ in range(4):
             m.step(25)
@@ -75,45 +80,27 @@ def test_training_reduces_loss_and_beats_no_network(require_gpu):
     l0, l1 = net.train(300, lr=2e-3)
     assert l1 < 0.5 * l0, (l0, l1)
 
-    # gain: from one state, one step of (a) npc=2 (reference), (b) npc=1,
-    # (c) npc=1 + network
-    def one_step(mode):
-        m = _model(L, 2, 31)
-        m.step(40)
-        if mode == "plain":
-            m.step(1)
-        elif mode == "surrogate":
-            lyo.enable_surrogate(m, net, judge_every=1000000, tol=1.0)
-            m.step(1)  # step 0 of surrogate mode is judged (plain): skip it
-        u = m.get("ux"), m.get("uy")
-        m.close()
-        return np.concatenate(u)
-    # surrogate mode's first step is a judged plain step, so compare on the
-    # step after: build the states explicitly
-    m_ref = _model(L, 2, 31)
-    m_ref.step(41)
-    u_ref = np.concatenate([m_ref.get("ux"), m_ref.get("uy")])
-    m_sur = _model(L, 2, 31)
-    m_sur.step(40)
-    lyo.enable_surrogate(m_sur, net, judge_every=1000000, tol=1.0)
-    m_sur.step(1)   # judged plain step (count 0)
-    m_sur2 = _model(L, 2, 31)
-    m_sur2.step(39)
-    lyo.enable_surrogate(m_sur2, net, judge_every=1000000, tol=1.0)
-    m_sur2.step(2)  # judged plain step, then one surrogate step
-    s = lyo.surrogate_stats(m_sur2)
+    # From one state S (a run after 40 steps), one more step three ways:
+    # (a) plain npc=2 -- the reference; (b) predictor + one corrector with no
+    # network; (c) predictor + one corrector + the network.  Surrogate mode
+    # judges its first step, so (c) enables it one step earlier.
+    ref = _model(L, 2, 31)
+    ref.step(41)
+    u_ref = _u(ref)
+    base = _model(L, 2, 31)
+    base.step(40)
+    one = lyo.Lyotropic(L, L, npc=1, precision="fp32", **CONFIG0_PHYS)
+    one.set_state(base.get("ff"), base.get("QQxx"), base.get("QQyx"), base.get("phi"))
+    one.step(1)
+    sur = _model(L, 2, 31)
+    sur.step(39)
+    lyo.enable_surrogate(sur, net, judge_every=1000000, tol=1.0)
+    sur.step(2)  # a judged plain step, then one surrogate step
+    s = lyo.surrogate_stats(sur)
     assert s["surrogate_steps"] == 1 and s["judged"] == 1
-    u_sur = np.concatenate([m_sur2.get("ux"), m_sur2.get("uy")])
-    m_one = _model(L, 2, 31)
-    m_one.step(40)
-    # npc=1 step on the same state: a fresh npc=1 run stepped to the same point
-    m_np1 = lyo.Lyotropic(L, L, npc=1, precision="fp32", **CONFIG0_PHYS)
-    m_np1.set_state(m_one.get("ff"), m_one.get("QQxx"), m_one.get("QQyx"), m_one.get("phi"))
-    m_np1.step(1)
-    u_np1 = np.concatenate([m_np1.get("ux"), m_np1.get("uy")])
-    e_np1 = np.linalg.norm(u_np1 - u_ref) / np.linalg.norm(u_ref)
-    e_sur = np.linalg.norm(u_sur - u_ref) / np.linalg.norm(u_ref)
-    assert e_sur < 0.7 * e_np1, (e_sur, e_np1)
+    e_one = np.linalg.norm(_u(one) - u_ref) / np.linalg.norm(u_ref)
+    e_sur = np.linalg.norm(_u(sur) - u_ref) / np.linalg.norm(u_ref)
+    assert e_sur < 0.7 * e_one, (e_sur, e_one)
 
 
 def test_judge_falls_back_to_plain_steps(require_gpu):
@@ -135,3 +122,13 @@ def test_judge_falls_back_to_plain_steps(require_gpu):
     b.step(13)
     for k in ("ff", "QQxx", "phi"):
         assert np.array_equal(a.get(k), b.get(k))
+
+
+def test_surrogate_rejects_bad_setups(require_gpu):
+    net = lyo.Net(seed=1)
+    m = _model(32, 2, 1)
+    with pytest.raises(lyo.MassLyoError):
+        lyo.enable_surrogate(m, net)  # untrained: no normalisation
+    one = _model(32, 1, 1)
+    with pytest.raises(lyo.MassLyoError):
+        net.collect(one)  # npc=1 has no one-corrector/two-corrector pair
