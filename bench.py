@@ -5,12 +5,17 @@ shifted populations and compensated phi, npc = 2 (predictor + 2 correctors),
 zeta = 0.03, init seed 7 (random director, phi = 0.5 + U(-0.05, 0.05)).
 The same grid is also timed in FP64 (reference arithmetic).
 
-MLUPS = LX*LY*steps / seconds / 1e6; one step = predictor + npc correctors.
-The timed region is bracketed by HIP events on the library's stream; the
-fields are resident in HBM throughout.  roofline: the corrector sweep, whose
-algorithmic traffic is 36 values per node (read f, fn, Q, QN, phi terms; write
-f, Q, phi) = 144 B per node in FP32, against 8 TB/s.  cpu_baseline: the CPU
-oracle (the reference's algorithm and layout, OpenMP) on the host cores.
+MLUPS = LX*LY*steps / seconds / 1e6; one step = predictor + npc correctors
+(the last two correctors run as one fused kernel).  The timed region is
+bracketed by HIP events on the library's stream; the fields are resident in
+HBM throughout.  roofline: the kernel with the longest launch, priced by its
+algorithmic traffic per node (every plane it touches read or written once):
+  predictor  read f 9, Q 2, phi 1; write f* 9, fn* 9, Q+D 2, Q+D/2 2, Dp 1
+             = 35 values = 140 B/node (FP32)
+  fused pair read f* 9, Q* 2, phi 1, Dp 1, fn* 9, QN 2, phic 1; write f 9,
+             Q 2, phi 1, phic 1 = 38 values = 152 B/node (FP32)
+against 8 TB/s.  cpu_baseline: the CPU oracle (the reference's algorithm and
+layout, OpenMP) on the host cores.
 """
 import argparse
 import json
@@ -25,25 +30,31 @@ PHYS = dict(GammaQ=0.1, GammaP=0.1, xi=0.7, tauNem=2.0, tauIso=2.0, rho=40.0, fr
             AA=0.01, CC=0.0, LL=0.04, KK=0.04, zeta=0.03, zetaI=0.0, conc=0.5)
 INIT = dict(config="random", seed=7, angle=90.0, noise=1.0, qamp=0.5, phinoise=0.05)
 HBM_PEAK = 8.0e12
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_corrector_fp32_4096.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fp32_4096_npc2.json")
+# algorithmic values per node per launch (x4 bytes in FP32)
+VALUES = {"predictor": 35, "corrector": 37, "pair": 38}
+KERNEL = {"predictor": "sweep_kernel<float, false, 0>",
+          "corrector": "sweep_kernel<float, false, 1>",
+          "pair": "pair_kernel<float, false>"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0):
+def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-1):
     m = lyo.Lyotropic(L, L, npc=npc, precision=precision, **PHYS)
     if chunk:
         m.set_launch(chunk, 0)
+    m.set_fusion(fuse, pair_chunk)
     m.configure(**INIT)
     if warmup:
         m.step(warmup)
     m.synchronize()
-    total_ms, (pred_ms, corr_ms) = m.time_steps(steps, per_sweep=True)
-    fsum, _ = m.runtime_checks()
+    total_ms, sweeps = m.time_steps(steps, per_sweep=True)
+    m.runtime_checks()
     m.close()
-    return total_ms, pred_ms, corr_ms
+    return total_ms, dict(zip(("predictor", "corrector", "pair"), sweeps))
 
 
 def cpu_baseline(L, steps, threads):
@@ -100,6 +111,8 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--npc", type=int, default=2)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--pair-chunk", type=int, default=0)
+    ap.add_argument("--no-fuse", action="store_true", help="separate corrector sweeps")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -117,31 +130,34 @@ def main():
     L, npc = args.size, args.npc
     N = L * L
 
+    fuse = 0 if args.no_fuse else -1
     log("fp32 %dx%d npc=%d: %d warmup + %d timed steps" % (L, L, npc, args.warmup, args.steps))
-    t32, p32, c32 = gpu_run(lyo, L, lyo.FP32, npc, args.steps, args.warmup, args.chunk)
+    t32, k32 = gpu_run(lyo, L, lyo.FP32, npc, args.steps, args.warmup, args.chunk,
+                       args.pair_chunk, fuse)
     mlups32 = N * args.steps / (t32 / 1e3) / 1e6
-    log("fp32: %.3f ms/step, %.0f MLUPS, predictor %.3f ms, corrector %.3f ms"
-        % (t32 / args.steps, mlups32, p32, c32))
+    log("fp32: %.3f ms/step, %.0f MLUPS, mean launch ms: %s"
+        % (t32 / args.steps, mlups32, {k: round(v, 4) for k, v in k32.items() if v}))
 
     extra = {}
     if not args.skip_fp64:
         s64 = max(2, args.steps // 4)
         log("fp64 %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
-        t64, p64, c64 = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, args.chunk)
+        t64, k64 = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, args.chunk, 0, fuse)
         mlups64 = N * s64 / (t64 / 1e3) / 1e6
         log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
         extra = {"fp64_mlups": round(mlups64, 1), "fp32_over_fp64": round(mlups32 / mlups64, 3),
-                 "fp64_kernel_set": "fused sweeps, reference expression order",
-                 "fp64_corrector_GBps": round(36 * 8 * N / (c64 / 1e3) / 1e9, 1)}
+                 "fp64_kernel_set": "strip-march sweeps, reference expression order",
+                 "fp64_launch_ms": {k: round(v, 4) for k, v in k64.items() if v}}
 
-    alg_bytes = 36 * 4 * N  # per corrector launch
-    achieved = alg_bytes / (c32 / 1e3)
+    dom = max((k for k in k32 if k32[k] > 0), key=lambda k: k32[k])
+    alg_bytes = VALUES[dom] * 4 * N
+    achieved = alg_bytes / (k32[dom] / 1e3)
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        if t.get("size") == L and t.get("npc") == npc:
-            traffic = t.get("bytes_per_launch")
+        if t.get("size") == L and t.get("npc") == npc and dom in t.get("kernels", {}):
+            traffic = t["kernels"][dom].get("bytes_per_launch")
 
     cpu = None
     if not args.skip_cpu:
@@ -174,8 +190,9 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": traffic,
-                     "kernel": "sweep_kernel<float,false,CORR> (corrector sweep)",
-                     "kernel_ms": round(c32, 4), "predictor_ms": round(p32, 4)},
+                     "kernel": KERNEL[dom], "kernel_ms": round(k32[dom], 4),
+                     "alg_bytes_per_launch": alg_bytes,
+                     "launch_ms": {k: round(v, 4) for k, v in k32.items() if v}},
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -183,8 +183,9 @@ int mass_lyo_synchronize(mass_lyo_t h);
 
 /* Timing: runs nsteps steps bracketed by HIP events on the handle's stream
  * and returns the elapsed milliseconds.  When sweep_ms is not NULL, every
- * sweep launch is also bracketed by its own event pair; sweep_ms[0] gets the
- * mean predictor-sweep time and sweep_ms[1] the mean corrector-sweep time. */
+ * sweep launch is also bracketed by its own event pair; sweep_ms (3 doubles)
+ * gets the mean time of a predictor sweep, of a separate corrector sweep and
+ * of a fused last-corrector-pair launch (0 where there was none). */
 int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms,
                         double* sweep_ms);
 
@@ -193,6 +194,15 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms,
  * (1 on, 0 off, -1 keep).  Results do not depend on it. */
 int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block,
                         int32_t xcd_order);
+
+/* Fusion of the last two corrector sweeps (npc >= 2) into one kernel that
+ * keeps the first one's outputs in registers: 1 on (the default), 0 off
+ * (separate sweeps), -1 keep; pair_chunk_rows: rows per wave chunk of the
+ * fused kernel (0 keeps).  Results are bitwise the same either way; derived
+ * fields of a frame are recomputed on demand after a fused step.  No
+ * reference counterpart (Lyotropic::Step, src/models/lyotropic.cpp:500-525,
+ * always runs its correctors one after another). */
+int mass_lyo_set_fusion(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chunk_rows);
 
 /* ------------------------------------------------------------------ batches
  * Many independent runs (own size and parameters; shared npc and precision)

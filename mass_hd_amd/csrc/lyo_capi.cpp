@@ -108,6 +108,12 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
   int xc = 64;
   while (xc > 4 && (long long)nstrips * ((p->LX + xc - 1) / xc) < 32768) xc >>= 1;
   h->launch.xc = xc;
+  // the fused pair holds two stages' rows (2 waves per SIMD): ~16 waves per
+  // SIMD (measured at 4096^2: 16 rows ~ 64 rows > 32 > 8)
+  const int nstrips2 = (p->LY + lyo::kPairStrip - 1) / lyo::kPairStrip;
+  int xc2 = 64;
+  while (xc2 > 4 && (long long)nstrips2 * ((p->LX + xc2 - 1) / xc2) < 16384) xc2 >>= 1;
+  h->launch.xc2 = xc2;
   *out = h.release();
   return MASS_LYO_OK;
 }
@@ -384,8 +390,8 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms, double* 
   float ms = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
   if (e == hipSuccess && sweep_ms) {
-    double acc[2] = {0, 0};
-    int cnt[2] = {0, 0};
+    double acc[3] = {0, 0, 0};
+    int cnt[3] = {0, 0, 0};
     for (size_t i = 0; i < tm.used / 2 && e == hipSuccess; ++i) {
       float m = 0;
       e = hipEventElapsedTime(&m, tm.ev[2 * i], tm.ev[2 * i + 1]);
@@ -394,6 +400,7 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms, double* 
     }
     sweep_ms[0] = cnt[0] ? acc[0] / cnt[0] : 0;
     sweep_ms[1] = cnt[1] ? acc[1] / cnt[1] : 0;
+    sweep_ms[2] = cnt[2] ? acc[2] / cnt[2] : 0;
   }
   for (auto& ev : tm.ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -414,6 +421,15 @@ int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_bloc
   if (chunk_rows) h->launch.xc = chunk_rows;
   if (waves_per_block) h->launch.wpb = waves_per_block;
   if (xcd_order >= 0) h->launch.xcd = xcd_order;
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_set_fusion(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chunk_rows) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (fuse_last_pair < -1 || fuse_last_pair > 1 || pair_chunk_rows < 0 || pair_chunk_rows > 4096)
+    return fail(h, MASS_LYO_EINVAL, "fuse_last_pair in {-1,0,1}, pair_chunk_rows in [0,4096]");
+  if (fuse_last_pair >= 0) h->launch.fuse = fuse_last_pair;
+  if (pair_chunk_rows) h->launch.xc2 = pair_chunk_rows;
   return MASS_LYO_OK;
 }
 

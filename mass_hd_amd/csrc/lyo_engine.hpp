@@ -13,6 +13,7 @@
 #include "lyo_aux.hpp"
 #include "lyo_diag.hpp"
 #include "lyo_net_engine.hpp"
+#include "lyo_pair.hpp"
 #include "lyo_sweep.hpp"
 
 namespace lyo {
@@ -44,30 +45,38 @@ struct Bufs {
   const T* uq_qyx = nullptr;
   const T* uq_phi = nullptr;
   const T* uq_dphi = nullptr;
+  // >= 0: the last step ran its last two correctors fused, so the outputs of
+  // corrector npc-1 (the inputs of the last UpdateQuantities) were never
+  // stored; they are recomputed on demand from the predictor outputs and the
+  // old state S[lazy_cur], which stay intact until the next step
+  int lazy_cur = -1;
 };
 
 struct LaunchCfg {
   int xc = 32;   // rows per wave chunk
   int wpb = 4;   // waves per block
   int xcd = 1;   // XCD-aware block order
+  int fuse = 1;  // last two correctors in one kernel (npc >= 2)
+  int xc2 = 16;  // rows per wave chunk of the fused pair
 };
 
 // Optional per-sweep event timing.
 struct SweepTimer {
   std::vector<hipEvent_t> ev;  // pairs
-  std::vector<int> kind;       // 0 predictor, 1 corrector
+  std::vector<int> kind;       // 0 predictor, 1 corrector, 2 fused pair
   size_t used = 0;
 };
 
-inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L) {
+inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, bool pair = false) {
   Geo g;
   g.LX = LX;
   g.LY = LY;
   g.N = LX * LY;
   g.NP = NP;
-  g.nstrips = (LY + kStrip - 1) / kStrip;
-  g.xc = L.xc;
-  g.nchunks = (LX + L.xc - 1) / L.xc;
+  const int strip = pair ? kPairStrip : kStrip;
+  g.nstrips = (LY + strip - 1) / strip;
+  g.xc = pair ? L.xc2 : L.xc;
+  g.nchunks = (LX + g.xc - 1) / g.xc;
   g.nwaves = g.nstrips * g.nchunks;
   g.xcd = L.xcd;
   return g;
@@ -85,6 +94,22 @@ inline hipError_t timed_launch(const SweepArgs<T>& a, const Params<T>& p, const 
     (void)hipEventRecord(e0, s);
   }
   hipError_t err = launch_sweep<T, EXACT, STAGE>(a, p, g, L.wpb, s);
+  if (e1) (void)hipEventRecord(e1, s);
+  return err;
+}
+
+template <class T, bool EXACT>
+inline hipError_t timed_pair(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
+                             const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (tm && tm->used + 2 <= tm->ev.size()) {
+    e0 = tm->ev[tm->used];
+    e1 = tm->ev[tm->used + 1];
+    tm->kind[tm->used / 2] = 2;
+    tm->used += 2;
+    (void)hipEventRecord(e0, s);
+  }
+  hipError_t err = launch_pair<T, EXACT>(a, p, g, L.wpb, s);
   if (e1) (void)hipEventRecord(e1, s);
   return err;
 }
@@ -168,15 +193,40 @@ void note_last_uq(Bufs<T>& B, const SweepArgs<T>& a) {
   B.uq_dphi = a.dphi_in;
 }
 
+// Arguments of the fused last pair: the inputs of corrector npc-1 and the
+// outputs (and, FP32, compensation input) of corrector npc.
+template <class T, bool EXACT>
+SweepArgs<T> pair_args(const Bufs<T>& B, int npc, int cur) {
+  SweepArgs<T> a = stage_args<T, EXACT>(B, npc, npc - 1, cur);
+  const SweepArgs<T> b = stage_args<T, EXACT>(B, npc, npc, cur);
+  a.f_out = b.f_out;
+  a.qxx_out = b.qxx_out;
+  a.qyx_out = b.qyx_out;
+  a.phi_out = b.phi_out;
+  a.phic_in = b.phic_in;
+  a.phic_out = b.phic_out;
+  return a;
+}
+
 template <class T, bool EXACT>
 hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int npc,
                      int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   const Geo g = make_geo(LX, LY, NP, L);
+  const bool fuse = L.fuse && npc >= 2;
+  const Geo g2 = make_geo(LX, LY, NP, L, true);
   for (int st = 0; st < nsteps; ++st) {
     const int cur = B.cur;
+    B.lazy_cur = -1;
     for (int c = 0; c <= npc; ++c) {
-      const SweepArgs<T> a = stage_args<T, EXACT>(B, npc, c, cur);
       hipError_t e;
+      if (fuse && c == npc - 1) {
+        e = timed_pair<T, EXACT>(pair_args<T, EXACT>(B, npc, cur), P, g2, L, s, tm);
+        if (e != hipSuccess) return e;
+        B.have_uq = false;
+        B.lazy_cur = cur;
+        break;
+      }
+      const SweepArgs<T> a = stage_args<T, EXACT>(B, npc, c, cur);
       if (c == 0) {
         e = timed_launch<T, EXACT, PRED>(a, P, g, L, s, tm);
       } else if (c < npc) {
@@ -192,6 +242,23 @@ hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int
   return hipSuccess;
 }
 
+// Recomputes the outputs of corrector npc-1 of the last step when that step
+// ran its last pair fused: the separate sweep on the same inputs, so the
+// inputs of the last UpdateQuantities exist again.
+template <class T, bool EXACT>
+hipError_t materialize_uq(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int npc,
+                          hipStream_t s) {
+  if (B.lazy_cur < 0) return hipSuccess;
+  const int cur = B.lazy_cur;
+  const LaunchCfg L;
+  const hipError_t e = launch_sweep<T, EXACT, CORR>(stage_args<T, EXACT>(B, npc, npc - 1, cur), P,
+                                                    make_geo(LX, LY, NP, L), L.wpb, s);
+  if (e != hipSuccess) return e;
+  note_last_uq(B, stage_args<T, EXACT>(B, npc, npc, cur));
+  B.lazy_cur = -1;
+  return hipSuccess;
+}
+
 template <class T, bool EXACT>
 hipError_t pack_state(Bufs<T>& B, const double* d_ff, const double* d_qxx, const double* d_qyx,
                       const double* d_phi, int N, int NP, double rho, hipStream_t s) {
@@ -201,6 +268,7 @@ hipError_t pack_state(Bufs<T>& B, const double* d_ff, const double* d_qxx, const
                      d_phi, B.Sf[cur], B.Sqxx[cur], B.Sqyx[cur], B.Sphi[cur], B.Sphic[cur], N, NP,
                      rho);
   B.have_uq = false;
+  B.lazy_cur = -1;
   return hipGetLastError();
 }
 
@@ -316,10 +384,10 @@ class Engine {
                                   int wpb, hipStream_t s) const = 0;
   // surrogate corrector hooks (lyo_net_engine.hpp).  src 0: current state;
   // src 1: the one-corrector state of the last plain step (needs npc >= 2)
-  virtual hipError_t net_features(int src, float* feat, hipStream_t s) const = 0;
-  virtual hipError_t net_targets(float* tgt, hipStream_t s) const = 0;
+  virtual hipError_t net_features(int src, float* feat, hipStream_t s) = 0;
+  virtual hipError_t net_targets(float* tgt, hipStream_t s) = 0;
   virtual hipError_t net_apply(const float* delta, hipStream_t s) = 0;
-  virtual hipError_t net_judge(const float* delta, double* part, int nparts, hipStream_t s) const = 0;
+  virtual hipError_t net_judge(const float* delta, double* part, int nparts, hipStream_t s) = 0;
   // nsteps steps with npc_run correctors (npc_run <= npc of the engine)
   virtual hipError_t steps_with(int npc_run, int nsteps, const LaunchCfg& L, hipStream_t s) = 0;
   virtual int npc() const = 0;
@@ -404,6 +472,10 @@ class EngineT final : public Engine {
     return pack_state<T, EXACT>(B_, d_ff, d_qxx, d_qyx, d_phi, LX_ * LY_, NP_, rho_, s);
   }
   hipError_t field(int id, double* d_out, hipStream_t s) override {
+    if (id > 3) {
+      const hipError_t e = materialize_uq<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_, s);
+      if (e != hipSuccess) return e;
+    }
     return field_to_f64<T, EXACT>(B_, P_, LX_, LY_, NP_, id, rho_, d_out, s);
   }
   hipError_t sums(double* d_part, int max_part, double* d_out, hipStream_t s) override {
@@ -431,11 +503,16 @@ class EngineT final : public Engine {
     if (nsteps <= 0) return;
     const int last_cur = B_.cur ^ ((nsteps - 1) & 1);
     note_last_uq(B_, stage_args<T, EXACT>(B_, npc_, npc_, last_cur));
+    B_.lazy_cur = -1;
     B_.cur ^= (nsteps & 1);
   }
   int parity() const override { return B_.cur; }
 
-  hipError_t net_features(int src, float* feat, hipStream_t s) const override {
+  hipError_t net_features(int src, float* feat, hipStream_t s) override {
+    if (src != 0) {
+      const hipError_t e = materialize_uq<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_, s);
+      if (e != hipSuccess) return e;
+    }
     const int N = LX_ * LY_, cur = B_.cur;
     const int bs = 128, nb = (N + bs - 1) / bs;
     const T *f, *qxx, *qyx, *phi, *dphi = nullptr, *phic = nullptr;
@@ -461,8 +538,10 @@ class EngineT final : public Engine {
                        dphi, phic, P_, LX_, LY_, NP_, feat);
     return hipGetLastError();
   }
-  hipError_t net_targets(float* tgt, hipStream_t s) const override {
+  hipError_t net_targets(float* tgt, hipStream_t s) override {
     if (npc_ < 2) return hipErrorInvalidValue;
+    const hipError_t e = materialize_uq<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_, s);
+    if (e != hipSuccess) return e;
     const int N = LX_ * LY_, cur = B_.cur;
     const int bs = 256, nb = (N + bs - 1) / bs;
     hipLaunchKernelGGL((targets_kernel<T, !EXACT>), dim3(nb), dim3(bs), 0, s, B_.Sf[cur],
@@ -480,8 +559,10 @@ class EngineT final : public Engine {
     return hipGetLastError();
   }
   hipError_t net_judge(const float* delta, double* part, int nparts,
-                       hipStream_t s) const override {
+                       hipStream_t s) override {
     if (npc_ < 2) return hipErrorInvalidValue;
+    const hipError_t e = materialize_uq<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((judge_kernel<T, !EXACT>), dim3(nparts), dim3(256), 0, s, B_.Cf[0],
                        B_.Sf[B_.cur], delta, LX_ * LY_, NP_, rho_, part);
     return hipGetLastError();

@@ -1,0 +1,273 @@
+// lyo_pair.hpp -- the last two corrector sweeps of a step fused into one
+// kernel (gfx950).
+//
+// Lyotropic::Step (src/models/lyotropic.cpp:500-525) runs npc corrector
+// sweeps after the predictor; each corrector reads the state the previous one
+// wrote.  For the last pair (A = corrector npc-1, B = corrector npc) this
+// kernel keeps A's outputs (f, Q, phi of one row) in registers and hands them
+// to B in the same wave, so they never travel through HBM: per node the pair
+// reads the predictor outputs and the old state once and writes the new state
+// once.
+//
+// Both stages march along x in one wave.  Stage A is the strip march of
+// lyo_sweep.hpp and produces row x of A's outputs; stage B then runs
+// UpdateQuantities on row x-1 and UpdateFields on row x-2 from A's rows
+// x-2..x.  B needs A's outputs two columns to each side, so the halo is four
+// lanes per side and each wave writes 56 interior columns.  Every stage runs
+// the same device functions, in the same order, as the separate sweeps, so the
+// results are bitwise those of the two separate sweeps.
+#pragma once
+
+#include "lyo_sweep.hpp"
+
+namespace lyo {
+
+constexpr int kPairHalo = 4;
+constexpr int kPairStrip = kWave - 2 * kPairHalo;
+
+// Centre inputs of one corrector at one row: fn, QN and phn (predictor
+// half-step values, or, FP32, the predictor's increment of phi).
+template <class T>
+struct CenIn {
+  T fn[9];
+  T qnxx, qnyx, phn;
+};
+
+// Loads of one pair iteration x: Q and phi of row x+2 and f of row x+1 for
+// A's UpdateQuantities, the centre inputs of row x (A uses them at x, B two
+// iterations later), and, FP32, old phi and its compensation of row x-2 for
+// B's final update.
+template <class T, bool EXACT>
+struct PairIn {
+  QP<T> q;
+  T dp;
+  T f[9];
+  CenIn<T> c;
+  T pold, phic;
+};
+
+#ifndef LYO_PAIR_WAVES_PER_EU
+#define LYO_PAIR_WAVES_PER_EU 2
+#endif
+
+// A's inputs are those of the separate sweep npc-1 (f_in, q*_in, phi_in,
+// dphi_in, fn_in, qn*_in, phn_in); B's outputs those of sweep npc (f_out,
+// q*_out, phi_out, FP32 also phic_in -> phic_out).
+template <class T, bool EXACT>
+__device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>& P,
+                                          const Geo& g, const int wid) {
+  const int lane = threadIdx.x & 63;
+  const int strip = wid % g.nstrips;
+  const int chunk = wid / g.nstrips;
+  const int LX = g.LX, LY = g.LY, N = g.NP;
+
+  const int ycol = strip * kPairStrip - kPairHalo + lane;
+  int yi = ycol % LY;
+  if (yi < 0) yi += LY;
+  const unsigned y = (unsigned)yi;
+  const bool active = lane >= kPairHalo && lane < kWave - kPairHalo && ycol < LY;
+  const int x0 = chunk * g.xc;
+  const int x1 = min(x0 + g.xc, LX);
+  const int xs = x0 - 2;  // first row of A
+
+  // rows run from x0-4 to x1+4; any LX >= 2 (tiny grids wrap several times)
+  auto roff = [&](int r) {
+    while (r < 0) r += LX;
+    while (r >= LX) r -= LX;
+    return r * LY;
+  };
+  auto load_qp = [&](int r) {
+    const int o = roff(r);
+    QP<T> q;
+    q.xx = (A.qxx_in + o)[y];
+    q.yx = (A.qyx_in + o)[y];
+    T ph = (A.phi_in + o)[y];
+    if constexpr (!EXACT) ph += (A.dphi_in + o)[y];
+    q.p = ph;
+    return q;
+  };
+  auto load_f = [&](int r, T(&f)[9]) {
+    const int o = roff(r);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) f[v] = (A.f_in + (v * N + o))[y];
+  };
+  auto load_row = [&](int x, PairIn<T, EXACT>& R) {
+    {
+      const int o = roff(x + 2);
+      R.q.xx = (A.qxx_in + o)[y];
+      R.q.yx = (A.qyx_in + o)[y];
+      R.q.p = (A.phi_in + o)[y];
+      if constexpr (!EXACT) R.dp = (A.dphi_in + o)[y];
+    }
+    load_f(x + 1, R.f);
+    {
+      const int o = roff(x);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) R.c.fn[v] = (A.fn_in + (v * N + o))[y];
+      R.c.qnxx = (A.qnxx_in + o)[y];
+      R.c.qnyx = (A.qnyx_in + o)[y];
+      R.c.phn = (A.phn_in + o)[y];
+    }
+    if constexpr (!EXACT) {
+      const int o = roff(x - 2);
+      R.pold = (A.phi_in + o)[y];
+      R.phic = (A.phic_in + o)[y];
+    }
+  };
+  auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
+                NbrVals<T>& nv, CenterVals<T>& cv) {
+    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, from_yp(q0.xx), from_ym(q0.xx),
+                                q0.yx, qp.yx, qm.yx, from_yp(q0.yx), from_ym(q0.yx),
+                                q0.p, qp.p, qm.p, from_yp(q0.p), from_ym(q0.p), P, nv,
+                                cv);
+  };
+
+  // stage A rolling state (as in sweep_item); rb, rc: old phi of rows x, x+1
+  NbrVals<T> um, u0;
+  CenterVals<T> c0;
+  T fc[9];
+  QP<T> qb, qc;
+  T rb = T(0), rc = T(0);
+  PairIn<T, EXACT> cur;
+  {
+    const QP<T> qa = load_qp(xs - 2);
+    qb = load_qp(xs - 1);
+    qc = load_qp(xs);
+    const QP<T> qd = load_qp(xs + 1);
+    if constexpr (!EXACT) {
+      rb = (A.phi_in + roff(xs))[y];
+      rc = (A.phi_in + roff(xs + 1))[y];
+    }
+    T ft[9];
+    load_f(xs - 1, ft);
+    load_f(xs, fc);
+    load_row(xs, cur);
+    CenterVals<T> ct;
+    uq(qa, qb, qc, ft, um, ct);
+    uq(qb, qc, qd, fc, u0, c0);
+    qb = qc;
+    qc = qd;
+  }
+  // stage B rolling state: A's outputs of rows x-2 (bqb, bfc), x-1 (bqc,
+  // bfp); UpdateQuantities of rows x-3 (bum), x-2 (bu0); centre inputs of
+  // rows x-2 (cm2), x-1 (cm1)
+  // (zeroed: the first two iterations run B on them before A has filled them,
+  // and those results are never stored)
+  NbrVals<T> bum{}, bu0{};
+  CenterVals<T> bc0{};
+  T bfc[9] = {}, bfp[9] = {};
+  QP<T> bqb{}, bqc{};
+  CenIn<T> cm1{}, cm2{};
+
+  for (int x = xs; x <= x1 + 1; ++x) {
+    PairIn<T, EXACT> nxt;
+    load_row(x + 1, nxt);
+
+    // ---- stage A: UpdateQuantities on row x+1, UpdateFields on row x
+    T rn = cur.q.p;
+    if constexpr (!EXACT) cur.q.p += cur.dp;
+    NbrVals<T> un;
+    CenterVals<T> cn;
+    uq(qb, qc, cur.q, cur.f, un, cn);
+    QP<T> qa_out;
+    T fa[9];
+    {
+      const UFStencil<T> s = uf_stencil(um, u0, un);
+      T Dxx, Dyx, Dp, om[9];
+      update_fields<T, EXACT>(c0, qb.xx, qb.yx, qb.p, fc, s, P, Dxx, Dyx, Dp, om);
+      qa_out.xx = cur.c.qnxx + T(.5) * Dxx;
+      qa_out.yx = cur.c.qnyx + T(.5) * Dyx;
+#pragma unroll
+      for (int v = 0; v < 9; ++v) fa[v] = cur.c.fn[v] + T(.5) * om[v];
+      if constexpr (EXACT) {
+        qa_out.p = cur.c.phn + T(.5) * Dp;
+      } else {
+        // the separate sweep stores the increment, the next one adds it to
+        // old phi: the same two roundings here
+        const T inc = T(.5) * (cur.c.phn + Dp);
+        qa_out.p = rb + inc;
+      }
+    }
+    um = u0;
+    u0 = un;
+    c0 = cn;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) fc[v] = cur.f[v];
+    qb = qc;
+    qc = cur.q;
+    rb = rc;
+    rc = rn;
+
+    // ---- stage B: UpdateQuantities on row x-1, UpdateFields on row x-2
+    if (x >= x0) {  // wave-uniform
+      NbrVals<T> bun;
+      CenterVals<T> bcn;
+      uq(bqb, bqc, qa_out, bfp, bun, bcn);
+      if (x >= x0 + 2) {
+        const UFStencil<T> s = uf_stencil(bum, bu0, bun);
+        T Dxx, Dyx, Dp, om[9];
+        update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
+        if (active) {
+          const int ox = roff(x - 2);
+          (A.qxx_out + ox)[y] = cm2.qnxx + T(.5) * Dxx;
+          (A.qyx_out + ox)[y] = cm2.qnyx + T(.5) * Dyx;
+#pragma unroll
+          for (int v = 0; v < 9; ++v) (A.f_out + (v * N + ox))[y] = cm2.fn[v] + T(.5) * om[v];
+          if constexpr (EXACT) {
+            (A.phi_out + ox)[y] = cm2.phn + T(.5) * Dp;
+          } else {
+            // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, as the
+            // separate last corrector
+            const T pold = cur.pold;
+            const T yv = T(.5) * (cm2.phn + Dp) + cur.phic;
+            const T t = pold + yv;
+            const T bp = t - pold;
+            const T ap = t - bp;
+            const T err = (pold - ap) + (yv - bp);
+            (A.phi_out + ox)[y] = t;
+            (A.phic_out + ox)[y] = err;
+          }
+        }
+      }
+      bum = bu0;
+      bu0 = bun;
+      bc0 = bcn;
+    }
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      bfc[v] = bfp[v];
+      bfp[v] = fa[v];
+    }
+    bqb = bqc;
+    bqc = qa_out;
+    cm2 = cm1;
+    cm1 = cur.c;
+    cur = nxt;
+  }
+}
+
+template <class T, bool EXACT>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? LYO_PAIR_WAVES_PER_EU : 1)))
+pair_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
+  int b = blockIdx.x;
+  if (g.xcd) {
+    const int nb8 = (int)gridDim.x >> 3;
+    if (b < (nb8 << 3)) b = (b & 7) * nb8 + (b >> 3);
+  }
+  const int wid = __builtin_amdgcn_readfirstlane(b * ((int)blockDim.x >> 6) +
+                                                 ((int)threadIdx.x >> 6));
+  if (wid >= g.nwaves) return;
+  pair_item<T, EXACT>(A, P, g, wid);
+}
+
+template <class T, bool EXACT>
+inline hipError_t launch_pair(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
+                              int waves_per_block, hipStream_t s) {
+  const int threads = 64 * waves_per_block;
+  const int blocks = (g.nwaves + waves_per_block - 1) / waves_per_block;
+  hipLaunchKernelGGL((pair_kernel<T, EXACT>), dim3(blocks), dim3(threads), 0, s, a, p, g);
+  return hipGetLastError();
+}
+
+}  // namespace lyo

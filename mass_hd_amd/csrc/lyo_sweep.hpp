@@ -111,6 +111,41 @@ struct RowIn {
 #define LYO_SWEEP_WAVES_PER_EU 3
 #endif
 
+// Neighbour values UpdateFields needs at the centre row: x neighbours from the
+// rows before (um) and after (un), y neighbours from the adjacent lanes of the
+// centre row (u0).
+template <class T>
+__device__ __forceinline__ UFStencil<T> uf_stencil(const NbrVals<T>& um, const NbrVals<T>& u0,
+                                                   const NbrVals<T>& un) {
+  UFStencil<T> s;
+  s.mu_c = u0.mu;
+  s.ux_xp = un.ux;
+  s.ux_xm = um.ux;
+  s.uy_xp = un.uy;
+  s.uy_xm = um.uy;
+  s.uxp_xp = un.uxp;
+  s.uxp_xm = um.uxp;
+  s.sxx_xp = un.sxx;
+  s.sxx_xm = um.sxx;
+  s.syx_xp = un.syx;
+  s.syx_xm = um.syx;
+  s.mu_xp = un.mu;
+  s.mu_xm = um.mu;
+  s.ux_yp = from_yp(u0.ux);
+  s.ux_ym = from_ym(u0.ux);
+  s.uy_yp = from_yp(u0.uy);
+  s.uy_ym = from_ym(u0.uy);
+  s.uyp_yp = from_yp(u0.uyp);
+  s.uyp_ym = from_ym(u0.uyp);
+  s.sxy_yp = from_yp(u0.sxy);
+  s.sxy_ym = from_ym(u0.sxy);
+  s.syy_yp = from_yp(u0.syy);
+  s.syy_ym = from_ym(u0.syy);
+  s.mu_yp = from_yp(u0.mu);
+  s.mu_ym = from_ym(u0.mu);
+  return s;
+}
+
 // One work item: wave-uniform item index wid in [0, g.nwaves) = one strip of
 // 64 columns over one chunk of rows of one run.
 template <class T, bool EXACT, int STAGE>
@@ -220,32 +255,7 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
     uq(qb, qc, cur.q, cur.f, un, cn);
 
     // UpdateFields on row x
-    UFStencil<T> s;
-    s.mu_c = u0.mu;
-    s.ux_xp = un.ux;
-    s.ux_xm = um.ux;
-    s.uy_xp = un.uy;
-    s.uy_xm = um.uy;
-    s.uxp_xp = un.uxp;
-    s.uxp_xm = um.uxp;
-    s.sxx_xp = un.sxx;
-    s.sxx_xm = um.sxx;
-    s.syx_xp = un.syx;
-    s.syx_xm = um.syx;
-    s.mu_xp = un.mu;
-    s.mu_xm = um.mu;
-    s.ux_yp = from_yp(u0.ux);
-    s.ux_ym = from_ym(u0.ux);
-    s.uy_yp = from_yp(u0.uy);
-    s.uy_ym = from_ym(u0.uy);
-    s.uyp_yp = from_yp(u0.uyp);
-    s.uyp_ym = from_ym(u0.uyp);
-    s.sxy_yp = from_yp(u0.sxy);
-    s.sxy_ym = from_ym(u0.sxy);
-    s.syy_yp = from_yp(u0.syy);
-    s.syy_ym = from_ym(u0.syy);
-    s.mu_yp = from_yp(u0.mu);
-    s.mu_ym = from_ym(u0.mu);
+    const UFStencil<T> s = uf_stencil(um, u0, un);
     T Dxx, Dyx, Dp, om[9];
     const T Qxx = qb.xx, Qyx = qb.yx, p = qb.p;
     update_fields<T, EXACT>(c0, Qxx, Qyx, p, fc, s, P, Dxx, Dyx, Dp, om);

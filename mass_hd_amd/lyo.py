@@ -102,6 +102,7 @@ def load_library():
     lib.mass_lyo_synchronize.argtypes = [h]
     lib.mass_lyo_time_steps.argtypes = [h, ctypes.c_int32, dp, dp]
     lib.mass_lyo_set_launch.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    lib.mass_lyo_set_fusion.argtypes = [h, ctypes.c_int32, ctypes.c_int32]
     lib.mass_lyo_abi_version.argtypes = []
     lib.mass_lyo_diagnostics.argtypes = [h, P(Diag), dp]
     i32 = ctypes.c_int32
@@ -241,14 +242,20 @@ class Lyotropic:
 
     def time_steps(self, nsteps, per_sweep=True):
         tot = ctypes.c_double()
-        sw = (ctypes.c_double * 2)()
+        sw = (ctypes.c_double * 3)()
         self._check(self.lib.mass_lyo_time_steps(self.h, int(nsteps), ctypes.byref(tot),
                                                  sw if per_sweep else None))
-        return tot.value, (sw[0], sw[1])
+        return tot.value, (sw[0], sw[1], sw[2])
 
     def set_launch(self, chunk_rows=0, waves_per_block=0, xcd_order=-1):
         self._check(self.lib.mass_lyo_set_launch(self.h, int(chunk_rows), int(waves_per_block),
                                                  int(xcd_order)))
+
+    def set_fusion(self, fuse_last_pair=-1, pair_chunk_rows=0):
+        """1: the last two correctors run as one kernel (default); 0: separate
+        sweeps.  Results are bitwise identical."""
+        self._check(self.lib.mass_lyo_set_fusion(self.h, int(fuse_last_pair),
+                                                 int(pair_chunk_rows)))
 
 
 class Batch:

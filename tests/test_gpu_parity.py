@@ -94,6 +94,40 @@ def test_launch_shape_does_not_change_results(require_gpu, chunk, wpb):
 
 
 @pytest.mark.parametrize("precision", [lyo.FP64, lyo.FP32])
+@pytest.mark.parametrize("npc", [2, 3])
+@pytest.mark.parametrize("LX,LY,xc2", [(128, 128, 32), (45, 67, 7), (5, 3, 1), (2, 60, 4),
+                                       (70, 113, 64)])
+def test_fused_pair_equals_separate_sweeps(require_gpu, precision, npc, LX, LY, xc2):
+    """The last two correctors in one kernel give the separate sweeps' results
+    bit for bit, for the state and for the frame's derived fields (which the
+    fused path recomputes on demand); asking for derived fields mid-run does
+    not disturb the run."""
+    runs = []
+    for fuse in (0, 1):
+        g = lyo.Lyotropic(LX, LY, npc=npc, precision=precision, **CONFIG0_PHYS)
+        g.set_fusion(fuse, xc2)
+        g.configure(**CONFIG0_INIT)
+        g.step(4)
+        mid = [g.get(n) for n in ("ux", "sigmaXY", "MU")]
+        g.step(3)
+        runs.append((g, mid))
+    (a, ma), (b, mb) = runs
+    for x, y in zip(ma, mb):
+        assert np.array_equal(x, y)
+    for n in STATE + DERIVED:
+        x, y = a.get(n), b.get(n)
+        assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
+
+
+def test_fused_pair_timing_kinds(require_gpu):
+    for npc, kinds in ((1, (1, 1, 0)), (2, (1, 0, 1)), (3, (1, 1, 1))):
+        g = lyo.Lyotropic(64, 64, npc=npc, precision=lyo.FP32, **CONFIG0_PHYS)
+        g.configure(**CONFIG0_INIT)
+        _, ms = g.time_steps(2)
+        assert tuple(int(m > 0) for m in ms) == kinds, (npc, ms)
+
+
+@pytest.mark.parametrize("precision", [lyo.FP64, lyo.FP32])
 def test_plane_padding_does_not_change_results(require_gpu, precision):
     outs = []
     for pad in (0, 4160):
