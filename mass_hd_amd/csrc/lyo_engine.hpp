@@ -82,7 +82,7 @@ inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, bool pair = fals
   return g;
 }
 
-template <class T, bool EXACT, int STAGE>
+template <class T, bool EXACT, int STAGE, bool HB>
 inline hipError_t timed_launch(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                                const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -93,12 +93,12 @@ inline hipError_t timed_launch(const SweepArgs<T>& a, const Params<T>& p, const 
     tm->used += 2;
     (void)hipEventRecord(e0, s);
   }
-  hipError_t err = launch_sweep<T, EXACT, STAGE>(a, p, g, L.wpb, s);
+  hipError_t err = launch_sweep<T, EXACT, STAGE, HB>(a, p, g, L.wpb, s);
   if (e1) (void)hipEventRecord(e1, s);
   return err;
 }
 
-template <class T, bool EXACT>
+template <class T, bool EXACT, bool HB>
 inline hipError_t timed_pair(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                              const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -109,7 +109,7 @@ inline hipError_t timed_pair(const SweepArgs<T>& a, const Params<T>& p, const Ge
     tm->used += 2;
     (void)hipEventRecord(e0, s);
   }
-  hipError_t err = launch_pair<T, EXACT>(a, p, g, L.wpb, s);
+  hipError_t err = launch_pair<T, EXACT, HB>(a, p, g, L.wpb, s);
   if (e1) (void)hipEventRecord(e1, s);
   return err;
 }
@@ -162,6 +162,9 @@ SweepArgs<T> stage_args(const Bufs<T>& B, int npc, int c, int cur) {
   a.qnxx_in = B.Pqnxx;
   a.qnyx_in = B.Pqnyx;
   a.phn_in = EXACT ? B.Pphn : B.Pphi;
+  a.fold_in = B.Sf[cur];
+  a.qoxx_in = B.Sqxx[cur];
+  a.qoyx_in = B.Sqyx[cur];
   if (c == npc) {
     a.f_out = B.Sf[nxt];
     a.qxx_out = B.Sqxx[nxt];
@@ -208,19 +211,45 @@ SweepArgs<T> pair_args(const Bufs<T>& B, int npc, int cur) {
   return a;
 }
 
+// Which FP32 launches use the half-step shortcut (SweepArgs, flag HB): the
+// first corrector rebuilds fn and QN from its own inputs; the predictor stores
+// them only for a corrector c >= 2 that runs as a separate sweep (or as stage A
+// of a fused pair); the fused pair rebuilds them when its stage A is corrector 1.
+inline bool pred_stores_half(int npc, bool fuse) { return npc >= 3 || (npc == 2 && !fuse); }
+
+template <class T, bool EXACT>
+hipError_t launch_corrector(const SweepArgs<T>& a, const Params<T>& P, const Geo& g,
+                            const LaunchCfg& L, hipStream_t s, SweepTimer* tm, int c, bool last) {
+  if constexpr (!EXACT) {
+    if (c == 1) {
+      return last ? timed_launch<T, EXACT, CORR_LAST, true>(a, P, g, L, s, tm)
+                  : timed_launch<T, EXACT, CORR, true>(a, P, g, L, s, tm);
+    }
+  }
+  return last ? timed_launch<T, EXACT, CORR_LAST, false>(a, P, g, L, s, tm)
+              : timed_launch<T, EXACT, CORR, false>(a, P, g, L, s, tm);
+}
+
 template <class T, bool EXACT>
 hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int npc,
                      int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   const Geo g = make_geo(LX, LY, NP, L);
   const bool fuse = L.fuse && npc >= 2;
   const Geo g2 = make_geo(LX, LY, NP, L, true);
+  const bool store_half = !EXACT && pred_stores_half(npc, fuse);  // EXACT always stores
   for (int st = 0; st < nsteps; ++st) {
     const int cur = B.cur;
     B.lazy_cur = -1;
     for (int c = 0; c <= npc; ++c) {
       hipError_t e;
       if (fuse && c == npc - 1) {
-        e = timed_pair<T, EXACT>(pair_args<T, EXACT>(B, npc, cur), P, g2, L, s, tm);
+        const SweepArgs<T> a = pair_args<T, EXACT>(B, npc, cur);
+        if constexpr (EXACT) {
+          e = timed_pair<T, EXACT, false>(a, P, g2, L, s, tm);
+        } else {
+          e = (npc == 2) ? timed_pair<T, EXACT, true>(a, P, g2, L, s, tm)
+                         : timed_pair<T, EXACT, false>(a, P, g2, L, s, tm);
+        }
         if (e != hipSuccess) return e;
         B.have_uq = false;
         B.lazy_cur = cur;
@@ -228,12 +257,15 @@ hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int
       }
       const SweepArgs<T> a = stage_args<T, EXACT>(B, npc, c, cur);
       if (c == 0) {
-        e = timed_launch<T, EXACT, PRED>(a, P, g, L, s, tm);
-      } else if (c < npc) {
-        e = timed_launch<T, EXACT, CORR>(a, P, g, L, s, tm);
+        if constexpr (EXACT) {
+          e = timed_launch<T, EXACT, PRED, false>(a, P, g, L, s, tm);
+        } else {
+          e = store_half ? timed_launch<T, EXACT, PRED, true>(a, P, g, L, s, tm)
+                         : timed_launch<T, EXACT, PRED, false>(a, P, g, L, s, tm);
+        }
       } else {
-        note_last_uq(B, a);
-        e = timed_launch<T, EXACT, CORR_LAST>(a, P, g, L, s, tm);
+        if (c == npc) note_last_uq(B, a);
+        e = launch_corrector<T, EXACT>(a, P, g, L, s, tm, c, c == npc);
       }
       if (e != hipSuccess) return e;
     }
@@ -251,8 +283,9 @@ hipError_t materialize_uq(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP
   if (B.lazy_cur < 0) return hipSuccess;
   const int cur = B.lazy_cur;
   const LaunchCfg L;
-  const hipError_t e = launch_sweep<T, EXACT, CORR>(stage_args<T, EXACT>(B, npc, npc - 1, cur), P,
-                                                    make_geo(LX, LY, NP, L), L.wpb, s);
+  const hipError_t e = launch_corrector<T, EXACT>(stage_args<T, EXACT>(B, npc, npc - 1, cur), P,
+                                                  make_geo(LX, LY, NP, L), L, s, nullptr,
+                                                  npc - 1, false);
   if (e != hipSuccess) return e;
   note_last_uq(B, stage_args<T, EXACT>(B, npc, npc, cur));
   B.lazy_cur = -1;
@@ -422,11 +455,13 @@ class EngineT final : public Engine {
       if (!EXACT && e == hipSuccess) e = get(n1, &B_.Sphic[i]);
     }
     if (e == hipSuccess) e = get(n9, &B_.Pfs);
-    if (e == hipSuccess) e = get(n9, &B_.Pfns);
+    // fn* and Q+D/2: FP32 needs them only for correctors c >= 2
+    const bool half = EXACT || npc >= 2;
+    if (half && e == hipSuccess) e = get(n9, &B_.Pfns);
     if (e == hipSuccess) e = get(n1, &B_.Pqsxx);
     if (e == hipSuccess) e = get(n1, &B_.Pqsyx);
-    if (e == hipSuccess) e = get(n1, &B_.Pqnxx);
-    if (e == hipSuccess) e = get(n1, &B_.Pqnyx);
+    if (half && e == hipSuccess) e = get(n1, &B_.Pqnxx);
+    if (half && e == hipSuccess) e = get(n1, &B_.Pqnyx);
     if (e == hipSuccess) e = get(n1, &B_.Pphi);
     if (EXACT && e == hipSuccess) e = get(n1, &B_.Pphn);
     const int nc = npc >= 3 ? 2 : (npc == 2 ? 1 : 0);
@@ -577,15 +612,29 @@ class EngineT final : public Engine {
     const SweepJob<T>* J = static_cast<const SweepJob<T>*>(jobs);
     const int waves = grid_waves > 0 && grid_waves < nitems ? grid_waves : nitems;
     const int blocks = (waves + wpb - 1) / wpb;
+    // batched steps run every corrector as its own sweep
+    const dim3 grid(blocks), block(64 * wpb);
+    constexpr bool HB = !EXACT;  // EXACT: the flag is ignored, one instance
     if (c == 0) {
-      hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, PRED>), dim3(blocks), dim3(64 * wpb), 0,
-                         s, J, njobs, nitems);
+      if (HB && pred_stores_half(npc_, false))
+        hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, PRED, HB>), grid, block, 0, s, J, njobs,
+                           nitems);
+      else
+        hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, PRED, false>), grid, block, 0, s, J,
+                           njobs, nitems);
+    } else if (c == 1) {
+      if (c < npc_)
+        hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, CORR, HB>), grid, block, 0, s, J, njobs,
+                           nitems);
+      else
+        hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, CORR_LAST, HB>), grid, block, 0, s, J,
+                           njobs, nitems);
     } else if (c < npc_) {
-      hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, CORR>), dim3(blocks), dim3(64 * wpb), 0,
-                         s, J, njobs, nitems);
+      hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, CORR, false>), grid, block, 0, s, J, njobs,
+                         nitems);
     } else {
-      hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, CORR_LAST>), dim3(blocks), dim3(64 * wpb),
-                         0, s, J, njobs, nitems);
+      hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, CORR_LAST, false>), grid, block, 0, s, J,
+                         njobs, nitems);
     }
     return hipGetLastError();
   }

@@ -26,7 +26,9 @@ constexpr int kPairHalo = 4;
 constexpr int kPairStrip = kWave - 2 * kPairHalo;
 
 // Centre inputs of one corrector at one row: fn, QN and phn (predictor
-// half-step values, or, FP32, the predictor's increment of phi).
+// half-step values, or, FP32, the predictor's increment of phi).  As loaded by
+// a rebuilding FP32 stage (HB) fn and QN hold the old populations at the
+// upstream nodes and old Q.
 template <class T>
 struct CenIn {
   T fn[9];
@@ -51,11 +53,13 @@ struct PairIn {
 #endif
 
 // A's inputs are those of the separate sweep npc-1 (f_in, q*_in, phi_in,
-// dphi_in, fn_in, qn*_in, phn_in); B's outputs those of sweep npc (f_out,
-// q*_out, phi_out, FP32 also phic_in -> phic_out).
-template <class T, bool EXACT>
+// dphi_in, fn_in, qn*_in or, HB, fold_in, qo*_in, phn_in); B's outputs those
+// of sweep npc (f_out, q*_out, phi_out, FP32 also phic_in -> phic_out).  With
+// HB (FP32, npc = 2) A rebuilds the half-step values and hands them to B.
+template <class T, bool EXACT, bool HB>
 __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>& P,
                                           const Geo& g, const int wid) {
+  constexpr bool kRebuild = !EXACT && HB;
   const int lane = threadIdx.x & 63;
   const int strip = wid % g.nstrips;
   const int chunk = wid / g.nstrips;
@@ -65,6 +69,8 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
   int yi = ycol % LY;
   if (yi < 0) yi += LY;
   const unsigned y = (unsigned)yi;
+  const unsigned yp1 = (yi + 1 == LY) ? 0u : (unsigned)(yi + 1);
+  const unsigned ym1 = (yi == 0) ? (unsigned)(LY - 1) : (unsigned)(yi - 1);
   const bool active = lane >= kPairHalo && lane < kWave - kPairHalo && ycol < LY;
   const int x0 = chunk * g.xc;
   const int x1 = min(x0 + g.xc, LX);
@@ -100,7 +106,16 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
       if constexpr (!EXACT) R.dp = (A.dphi_in + o)[y];
     }
     load_f(x + 1, R.f);
-    {
+    if constexpr (kRebuild) {
+      const int o = roff(x), om = roff(x - 1), op = roff(x + 1);
+      const int rowv[9] = {o, om, op, o, o, om, op, op, om};
+      const unsigned colv[9] = {y, y, y, ym1, yp1, ym1, yp1, ym1, yp1};
+#pragma unroll
+      for (int v = 0; v < 9; ++v) R.c.fn[v] = (A.fold_in + (v * N + rowv[v]))[colv[v]];
+      R.c.qnxx = (A.qoxx_in + o)[y];
+      R.c.qnyx = (A.qoyx_in + o)[y];
+      R.c.phn = (A.phn_in + o)[y];
+    } else {
       const int o = roff(x);
 #pragma unroll
       for (int v = 0; v < 9; ++v) R.c.fn[v] = (A.fn_in + (v * N + o))[y];
@@ -171,20 +186,30 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
     uq(qb, qc, cur.q, cur.f, un, cn);
     QP<T> qa_out;
     T fa[9];
+    CenIn<T> cen;  // half-step values of row x (B uses them at x+2)
+    if constexpr (kRebuild) {
+#pragma unroll
+      for (int v = 0; v < 9; ++v) cen.fn[v] = half_of(fc[v], cur.c.fn[v]);
+      cen.qnxx = half_of(qb.xx, cur.c.qnxx);
+      cen.qnyx = half_of(qb.yx, cur.c.qnyx);
+      cen.phn = cur.c.phn;
+    } else {
+      cen = cur.c;
+    }
     {
       const UFStencil<T> s = uf_stencil(um, u0, un);
       T Dxx, Dyx, Dp, om[9];
       update_fields<T, EXACT>(c0, qb.xx, qb.yx, qb.p, fc, s, P, Dxx, Dyx, Dp, om);
-      qa_out.xx = cur.c.qnxx + T(.5) * Dxx;
-      qa_out.yx = cur.c.qnyx + T(.5) * Dyx;
+      qa_out.xx = cen.qnxx + T(.5) * Dxx;
+      qa_out.yx = cen.qnyx + T(.5) * Dyx;
 #pragma unroll
-      for (int v = 0; v < 9; ++v) fa[v] = cur.c.fn[v] + T(.5) * om[v];
+      for (int v = 0; v < 9; ++v) fa[v] = cen.fn[v] + T(.5) * om[v];
       if constexpr (EXACT) {
-        qa_out.p = cur.c.phn + T(.5) * Dp;
+        qa_out.p = cen.phn + T(.5) * Dp;
       } else {
         // the separate sweep stores the increment, the next one adds it to
         // old phi: the same two roundings here
-        const T inc = T(.5) * (cur.c.phn + Dp);
+        const T inc = T(.5) * (cen.phn + Dp);
         qa_out.p = rb + inc;
       }
     }
@@ -241,12 +266,12 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
     bqb = bqc;
     bqc = qa_out;
     cm2 = cm1;
-    cm1 = cur.c;
+    cm1 = cen;
     cur = nxt;
   }
 }
 
-template <class T, bool EXACT>
+template <class T, bool EXACT, bool HB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? LYO_PAIR_WAVES_PER_EU : 1)))
 pair_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
@@ -258,15 +283,15 @@ pair_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
   const int wid = __builtin_amdgcn_readfirstlane(b * ((int)blockDim.x >> 6) +
                                                  ((int)threadIdx.x >> 6));
   if (wid >= g.nwaves) return;
-  pair_item<T, EXACT>(A, P, g, wid);
+  pair_item<T, EXACT, HB>(A, P, g, wid);
 }
 
-template <class T, bool EXACT>
+template <class T, bool EXACT, bool HB>
 inline hipError_t launch_pair(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                               int waves_per_block, hipStream_t s) {
   const int threads = 64 * waves_per_block;
   const int blocks = (g.nwaves + waves_per_block - 1) / waves_per_block;
-  hipLaunchKernelGGL((pair_kernel<T, EXACT>), dim3(blocks), dim3(threads), 0, s, a, p, g);
+  hipLaunchKernelGGL((pair_kernel<T, EXACT, HB>), dim3(blocks), dim3(threads), 0, s, a, p, g);
   return hipGetLastError();
 }
 

@@ -65,6 +65,13 @@ enum Stage { PRED = 0, CORR = 1, CORR_LAST = 2 };
 //            out: f_out, qxx_out, qyx_out, phi_out (FP32: increment of phi)
 //  CORR_LAST as CORR; FP32 also reads phic_in and writes phi_out, phic_out
 //            (compensated phi).  For EXACT, CORR_LAST == CORR.
+// Half-step values (template flag HB, FP32 only).  fn* and Q+D/2 follow from
+// the full predictor step and its source, fn = (f* + f_src)/2 with f_src the
+// old population at the upstream node, QN = (Q* + Q)/2.  With HB the first
+// corrector rebuilds them from its own inputs (f*, Q*) and the old state
+// (fold_in, qo*_in) instead of loading fn_in/qn*_in, and the predictor stores
+// them only when a later corrector loads them (HB on PRED).  EXACT always
+// stores and loads them in the reference's form (f + om/2, Q + D/2).
 template <class T>
 struct SweepArgs {
   const T* f_in;
@@ -77,6 +84,9 @@ struct SweepArgs {
   const T* qnyx_in;
   const T* phn_in;
   const T* phic_in;
+  const T* fold_in;  // FP32 HB correctors: old populations, Q
+  const T* qoxx_in;
+  const T* qoyx_in;
   T* f_out;
   T* fn_out;
   T* qxx_out;
@@ -95,9 +105,11 @@ struct QP {
 
 // Data one loop iteration (UpdateFields on row x) needs from memory: Q/phi of
 // row x+2 and populations of row x+1 for UpdateQuantities on row x+1, and for
-// the correctors the centre inputs fn, QN, phn of row x.  The loop loads the
-// next iteration's RowIn before computing the current one, so the loads of one
-// row overlap the arithmetic of the previous one.
+// the correctors the centre inputs fn, QN, phn of row x (rebuilding FP32
+// correctors: the old populations at the upstream nodes and old Q instead of
+// fn, QN).  The loop loads the next iteration's RowIn before computing the
+// current one, so the loads of one row overlap the arithmetic of the previous
+// one.
 template <class T, int STAGE>
 struct RowIn {
   QP<T> q;
@@ -106,6 +118,13 @@ struct RowIn {
   T fn[(STAGE == PRED) ? 1 : 9];
   T qnxx, qnyx, phn;
 };
+
+// FP32 half-step value from a full step and its source (see SweepArgs); one
+// expression wherever it is formed, so stored and rebuilt values agree bitwise
+template <class T>
+__device__ __forceinline__ T half_of(T full, T src) {
+  return T(.5) * (full + src);
+}
 
 #ifndef LYO_SWEEP_WAVES_PER_EU
 #define LYO_SWEEP_WAVES_PER_EU 3
@@ -148,9 +167,11 @@ __device__ __forceinline__ UFStencil<T> uf_stencil(const NbrVals<T>& um, const N
 
 // One work item: wave-uniform item index wid in [0, g.nwaves) = one strip of
 // 64 columns over one chunk of rows of one run.
-template <class T, bool EXACT, int STAGE>
+template <class T, bool EXACT, int STAGE, bool HB>
 __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T>& P,
                                            const Geo& g, const int wid) {
+  constexpr bool kStoreHalf = STAGE == PRED && (EXACT || HB);
+  constexpr bool kRebuild = STAGE != PRED && !EXACT && HB;
   const int lane = threadIdx.x & 63;
   const int strip = wid % g.nstrips;
   const int chunk = wid / g.nstrips;
@@ -200,7 +221,17 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
       if constexpr (!EXACT && STAGE != PRED) R.dp = (A.dphi_in + o)[y];
     }
     load_f(x + 1, R.f);
-    if constexpr (STAGE != PRED) {
+    if constexpr (kRebuild) {
+      // old populations at the upstream node x - c_v, old Q at x
+      const int o = roff(x), om = roff(x - 1), op = roff(x + 1);
+      const int rowv[9] = {o, om, op, o, o, om, op, op, om};
+      const unsigned colv[9] = {y, y, y, ym1, yp1, ym1, yp1, ym1, yp1};
+#pragma unroll
+      for (int v = 0; v < 9; ++v) R.fn[v] = (A.fold_in + (v * N + rowv[v]))[colv[v]];
+      R.qnxx = (A.qoxx_in + o)[y];
+      R.qnyx = (A.qoyx_in + o)[y];
+      R.phn = (A.phn_in + o)[y];
+    } else if constexpr (STAGE != PRED) {
       const int o = roff(x);
 #pragma unroll
       for (int v = 0; v < 9; ++v) R.fn[v] = (A.fn_in + (v * N + o))[y];
@@ -262,14 +293,19 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
 
     if (active) {
       if constexpr (STAGE == PRED) {
-        (A.qnxx_out + ox)[y] = Qxx + T(.5) * Dxx;
-        (A.qnyx_out + ox)[y] = Qyx + T(.5) * Dyx;
-        (A.qxx_out + ox)[y] = Qxx + Dxx;
-        (A.qyx_out + ox)[y] = Qyx + Dyx;
+        const T Qsx = Qxx + Dxx, Qsy = Qyx + Dyx;
+        (A.qxx_out + ox)[y] = Qsx;
+        (A.qyx_out + ox)[y] = Qsy;
         if constexpr (EXACT) {
+          (A.qnxx_out + ox)[y] = Qxx + T(.5) * Dxx;
+          (A.qnyx_out + ox)[y] = Qyx + T(.5) * Dyx;
           (A.phn_out + ox)[y] = p + T(.5) * Dp;
           (A.phi_out + ox)[y] = p + Dp;
         } else {
+          if constexpr (kStoreHalf) {
+            (A.qnxx_out + ox)[y] = half_of(Qsx, Qxx);
+            (A.qnyx_out + ox)[y] = half_of(Qsy, Qyx);
+          }
           (A.phi_out + ox)[y] = Dp;
         }
         // collide and push to (x+cx, y+cy) (Move, lyotropic.cpp:371-386)
@@ -279,14 +315,32 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
           const int d = v * N + rowv[v];
-          (A.fn_out + d)[colv[v]] = fc[v] + T(.5) * om[v];
-          (A.f_out + d)[colv[v]] = fc[v] + om[v];
+          const T fs = fc[v] + om[v];
+          if constexpr (EXACT) {
+            (A.fn_out + d)[colv[v]] = fc[v] + T(.5) * om[v];
+          } else if constexpr (kStoreHalf) {
+            (A.fn_out + d)[colv[v]] = half_of(fs, fc[v]);
+          }
+          (A.f_out + d)[colv[v]] = fs;
         }
       } else {
-        (A.qxx_out + ox)[y] = cur.qnxx + T(.5) * Dxx;
-        (A.qyx_out + ox)[y] = cur.qnyx + T(.5) * Dyx;
+        // centre half-step values fn, QN of row x
+        T fnv[9], qnx, qny;
+        if constexpr (kRebuild) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) (A.f_out + (v * N + ox))[y] = cur.fn[v] + T(.5) * om[v];
+          for (int v = 0; v < 9; ++v) fnv[v] = half_of(fc[v], cur.fn[v]);
+          qnx = half_of(Qxx, cur.qnxx);
+          qny = half_of(Qyx, cur.qnyx);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) fnv[v] = cur.fn[v];
+          qnx = cur.qnxx;
+          qny = cur.qnyx;
+        }
+        (A.qxx_out + ox)[y] = qnx + T(.5) * Dxx;
+        (A.qyx_out + ox)[y] = qny + T(.5) * Dyx;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) (A.f_out + (v * N + ox))[y] = fnv[v] + T(.5) * om[v];
         if constexpr (EXACT) {
           (A.phi_out + ox)[y] = cur.phn + T(.5) * Dp;
         } else if constexpr (STAGE == CORR) {
@@ -317,7 +371,7 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
   }
 }
 
-template <class T, bool EXACT, int STAGE>
+template <class T, bool EXACT, int STAGE, bool HB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? LYO_SWEEP_WAVES_PER_EU : 1)))
 sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
@@ -332,7 +386,7 @@ sweep_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
   const int wid = __builtin_amdgcn_readfirstlane(b * ((int)blockDim.x >> 6) +
                                                  ((int)threadIdx.x >> 6));
   if (wid >= g.nwaves) return;  // whole waves exit; no block barriers below
-  sweep_item<T, EXACT, STAGE>(A, P, g, wid);
+  sweep_item<T, EXACT, STAGE, HB>(A, P, g, wid);
 }
 
 // Batched sweeps: many independent runs of any size in one launch.  Job j
@@ -348,7 +402,7 @@ struct SweepJob {
 // Persistent grid: each wave walks the global item list with a stride of the
 // whole grid, finds the job of its item (binary search over the jobs' first
 // items, scalar loads) and runs it.  Every wave reaches the end of the list.
-template <class T, bool EXACT, int STAGE>
+template <class T, bool EXACT, int STAGE, bool HB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? LYO_SWEEP_WAVES_PER_EU : 1)))
 sweep_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const int nitems) {
@@ -363,17 +417,17 @@ sweep_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const 
       else hi = mid - 1;
     }
     const SweepJob<T>& J = jobs[lo];
-    sweep_item<T, EXACT, STAGE>(J.A, J.P, J.g, item - J.first);
+    sweep_item<T, EXACT, STAGE, HB>(J.A, J.P, J.g, item - J.first);
   }
 }
 
 // Launch helper: waves_per_block waves of 64 lanes per block.
-template <class T, bool EXACT, int STAGE>
+template <class T, bool EXACT, int STAGE, bool HB>
 inline hipError_t launch_sweep(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                                int waves_per_block, hipStream_t s) {
   const int threads = 64 * waves_per_block;
   const int blocks = (g.nwaves + waves_per_block - 1) / waves_per_block;
-  hipLaunchKernelGGL((sweep_kernel<T, EXACT, STAGE>), dim3(blocks), dim3(threads), 0, s, a,
+  hipLaunchKernelGGL((sweep_kernel<T, EXACT, STAGE, HB>), dim3(blocks), dim3(threads), 0, s, a,
                      p, g);
   return hipGetLastError();
 }
