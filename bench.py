@@ -42,11 +42,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-1):
+def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-1, alt=-1):
     m = lyo.Lyotropic(L, L, npc=npc, precision=precision, **PHYS)
     if chunk:
         m.set_launch(chunk, 0)
-    m.set_fusion(fuse, pair_chunk)
+    m.set_schedule(fuse, pair_chunk, alt)
     m.configure(**INIT)
     if warmup:
         m.step(warmup)
@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--pair-chunk", type=int, default=0)
     ap.add_argument("--no-fuse", action="store_true", help="separate corrector sweeps")
+    ap.add_argument("--alt", type=int, default=-1,
+                    help="march order: 0 every chunk upwards, 1 alternating (default)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -131,9 +133,10 @@ def main():
     N = L * L
 
     fuse = 0 if args.no_fuse else -1
+    alt = args.alt
     log("fp32 %dx%d npc=%d: %d warmup + %d timed steps" % (L, L, npc, args.warmup, args.steps))
     t32, k32 = gpu_run(lyo, L, lyo.FP32, npc, args.steps, args.warmup, args.chunk,
-                       args.pair_chunk, fuse)
+                       args.pair_chunk, fuse, alt)
     mlups32 = N * args.steps / (t32 / 1e3) / 1e6
     log("fp32: %.3f ms/step, %.0f MLUPS, mean launch ms: %s"
         % (t32 / args.steps, mlups32, {k: round(v, 4) for k, v in k32.items() if v}))
@@ -142,7 +145,7 @@ def main():
     if not args.skip_fp64:
         s64 = max(2, args.steps // 4)
         log("fp64 %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
-        t64, k64 = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, args.chunk, 0, fuse)
+        t64, k64 = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, args.chunk, 0, fuse, alt)
         mlups64 = N * s64 / (t64 / 1e3) / 1e6
         log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
         extra = {"fp64_mlups": round(mlups64, 1), "fp32_over_fp64": round(mlups32 / mlups64, 3),

@@ -195,14 +195,17 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms,
 int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_block,
                         int32_t xcd_order);
 
-/* Fusion of the last two corrector sweeps (npc >= 2) into one kernel that
- * keeps the first one's outputs in registers: 1 on (the default), 0 off
- * (separate sweeps), -1 keep; pair_chunk_rows: rows per wave chunk of the
- * fused kernel (0 keeps).  Results are bitwise the same either way; derived
- * fields of a frame are recomputed on demand after a fused step.  No
- * reference counterpart (Lyotropic::Step, src/models/lyotropic.cpp:500-525,
- * always runs its correctors one after another). */
-int mass_lyo_set_fusion(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chunk_rows);
+/* Sweep schedule.  fuse_last_pair: the last two corrector sweeps (npc >= 2,
+ * LX >= 8) in one kernel that keeps the first one's outputs in registers, 1
+ * on (the default) or 0 off (separate sweeps); pair_chunk_rows: rows per
+ * wave chunk of that kernel (0 keeps); alternate: neighbouring chunks march
+ * in opposite directions so that they reach their shared rows together, 1 on
+ * (the default) or 0 off (every chunk upwards).  -1 keeps a setting.  Results are bitwise the
+ * same either way; derived fields of a frame are recomputed on demand after a
+ * fused step.  No reference counterpart (Lyotropic::Step,
+ * src/models/lyotropic.cpp:500-525, runs its correctors one after another). */
+int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chunk_rows,
+                          int32_t alternate);
 
 /* ------------------------------------------------------------------ batches
  * Many independent runs (own size and parameters; shared npc and precision)

@@ -424,12 +424,16 @@ int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_bloc
   return MASS_LYO_OK;
 }
 
-int mass_lyo_set_fusion(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chunk_rows) {
+int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chunk_rows,
+                          int32_t alternate) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
-  if (fuse_last_pair < -1 || fuse_last_pair > 1 || pair_chunk_rows < 0 || pair_chunk_rows > 4096)
-    return fail(h, MASS_LYO_EINVAL, "fuse_last_pair in {-1,0,1}, pair_chunk_rows in [0,4096]");
+  if (fuse_last_pair < -1 || fuse_last_pair > 1 || pair_chunk_rows < 0 ||
+      pair_chunk_rows > 4096 || alternate < -1 || alternate > 1)
+    return fail(h, MASS_LYO_EINVAL,
+                "fuse_last_pair and alternate in {-1,0,1}, pair_chunk_rows in [0,4096]");
   if (fuse_last_pair >= 0) h->launch.fuse = fuse_last_pair;
   if (pair_chunk_rows) h->launch.xc2 = pair_chunk_rows;
+  if (alternate >= 0) h->launch.alt = alternate;
   return MASS_LYO_OK;
 }
 

@@ -58,6 +58,7 @@ struct LaunchCfg {
   int xcd = 1;   // XCD-aware block order
   int fuse = 1;  // last two correctors in one kernel (npc >= 2)
   int xc2 = 16;  // rows per wave chunk of the fused pair
+  int alt = 1;   // alternate the march direction between neighbouring chunks
 };
 
 // Optional per-sweep event timing.
@@ -79,6 +80,7 @@ inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, bool pair = fals
   g.nchunks = (LX + g.xc - 1) / g.xc;
   g.nwaves = g.nstrips * g.nchunks;
   g.xcd = L.xcd;
+  g.alt = L.alt;
   return g;
 }
 
@@ -234,7 +236,7 @@ template <class T, bool EXACT>
 hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int npc,
                      int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   const Geo g = make_geo(LX, LY, NP, L);
-  const bool fuse = L.fuse && npc >= 2;
+  const bool fuse = L.fuse && npc >= 2 && LX >= 8;  // the pair kernel wraps rows once
   const Geo g2 = make_geo(LX, LY, NP, L, true);
   const bool store_half = !EXACT && pred_stores_half(npc, fuse);  // EXACT always stores
   for (int st = 0; st < nsteps; ++st) {

@@ -74,12 +74,12 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
   const bool active = lane >= kPairHalo && lane < kWave - kPairHalo && ycol < LY;
   const int x0 = chunk * g.xc;
   const int x1 = min(x0 + g.xc, LX);
-  const int xs = x0 - 2;  // first row of A
 
-  // rows run from x0-4 to x1+4; any LX >= 2 (tiny grids wrap several times)
+  // rows run from x0-4 to x1+4; the fused pair is used for LX >= 8, so one
+  // wrap in either direction suffices
   auto roff = [&](int r) {
-    while (r < 0) r += LX;
-    while (r >= LX) r -= LX;
+    r = (r < 0) ? r + LX : r;
+    r = (r >= LX) ? r - LX : r;
     return r * LY;
   };
   auto load_qp = [&](int r) {
@@ -97,15 +97,16 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
 #pragma unroll
     for (int v = 0; v < 9; ++v) f[v] = (A.f_in + (v * N + o))[y];
   };
-  auto load_row = [&](int x, PairIn<T, EXACT>& R) {
+  // loads of the iteration on row x of a march in direction d
+  auto load_row = [&](int x, int d, PairIn<T, EXACT>& R) {
     {
-      const int o = roff(x + 2);
+      const int o = roff(x + 2 * d);
       R.q.xx = (A.qxx_in + o)[y];
       R.q.yx = (A.qyx_in + o)[y];
       R.q.p = (A.phi_in + o)[y];
       if constexpr (!EXACT) R.dp = (A.dphi_in + o)[y];
     }
-    load_f(x + 1, R.f);
+    load_f(x + d, R.f);
     if constexpr (kRebuild) {
       const int o = roff(x), om = roff(x - 1), op = roff(x + 1);
       const int rowv[9] = {o, om, op, o, o, om, op, op, om};
@@ -124,7 +125,7 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
       R.c.phn = (A.phn_in + o)[y];
     }
     if constexpr (!EXACT) {
-      const int o = roff(x - 2);
+      const int o = roff(x - 2 * d);
       R.pold = (A.phi_in + o)[y];
       R.phic = (A.phic_in + o)[y];
     }
@@ -137,137 +138,164 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
                                 cv);
   };
 
-  // stage A rolling state (as in sweep_item); rb, rc: old phi of rows x, x+1
-  NbrVals<T> um, u0;
-  CenterVals<T> c0;
-  T fc[9];
-  QP<T> qb, qc;
-  T rb = T(0), rc = T(0);
-  PairIn<T, EXACT> cur;
-  {
-    const QP<T> qa = load_qp(xs - 2);
-    qb = load_qp(xs - 1);
-    qc = load_qp(xs);
-    const QP<T> qd = load_qp(xs + 1);
-    if constexpr (!EXACT) {
-      rb = (A.phi_in + roff(xs))[y];
-      rc = (A.phi_in + roff(xs + 1))[y];
-    }
-    T ft[9];
-    load_f(xs - 1, ft);
-    load_f(xs, fc);
-    load_row(xs, cur);
-    CenterVals<T> ct;
-    uq(qa, qb, qc, ft, um, ct);
-    uq(qb, qc, qd, fc, u0, c0);
-    qb = qc;
-    qc = qd;
-  }
-  // stage B rolling state: A's outputs of rows x-2 (bqb, bfc), x-1 (bqc,
-  // bfp); UpdateQuantities of rows x-3 (bum), x-2 (bu0); centre inputs of
-  // rows x-2 (cm2), x-1 (cm1)
-  // (zeroed: the first two iterations run B on them before A has filled them,
-  // and those results are never stored)
-  NbrVals<T> bum{}, bu0{};
-  CenterVals<T> bc0{};
-  T bfc[9] = {}, bfp[9] = {};
-  QP<T> bqb{}, bqc{};
-  CenIn<T> cm1{}, cm2{};
-
-  for (int x = xs; x <= x1 + 1; ++x) {
-    PairIn<T, EXACT> nxt;
-    load_row(x + 1, nxt);
-
-    // ---- stage A: UpdateQuantities on row x+1, UpdateFields on row x
-    T rn = cur.q.p;
-    if constexpr (!EXACT) cur.q.p += cur.dp;
-    NbrVals<T> un;
-    CenterVals<T> cn;
-    uq(qb, qc, cur.q, cur.f, un, cn);
-    QP<T> qa_out;
-    T fa[9];
-    CenIn<T> cen;  // half-step values of row x (B uses them at x+2)
-    if constexpr (kRebuild) {
-#pragma unroll
-      for (int v = 0; v < 9; ++v) cen.fn[v] = half_of(fc[v], cur.c.fn[v]);
-      cen.qnxx = half_of(qb.xx, cur.c.qnxx);
-      cen.qnyx = half_of(qb.yx, cur.c.qnyx);
-      cen.phn = cur.c.phn;
-    } else {
-      cen = cur.c;
-    }
-    {
-      const UFStencil<T> s = uf_stencil(um, u0, un);
-      T Dxx, Dyx, Dp, om[9];
-      update_fields<T, EXACT>(c0, qb.xx, qb.yx, qb.p, fc, s, P, Dxx, Dyx, Dp, om);
-      qa_out.xx = cen.qnxx + T(.5) * Dxx;
-      qa_out.yx = cen.qnyx + T(.5) * Dyx;
-#pragma unroll
-      for (int v = 0; v < 9; ++v) fa[v] = cen.fn[v] + T(.5) * om[v];
-      if constexpr (EXACT) {
-        qa_out.p = cen.phn + T(.5) * Dp;
+  // The chunk is marched upwards (D = +1) or downwards (D = -1), alternating
+  // with the chunk parity as in sweep_item.  "prev"/"next" are rows x-D/x+D.
+  auto march = [&](auto dir) {
+    constexpr int D = decltype(dir)::value;
+    auto uqd = [&](const QP<T>& qprev, const QP<T>& q0, const QP<T>& qnext, const T(&f)[9],
+                   NbrVals<T>& nv, CenterVals<T>& cv) {
+      if constexpr (D > 0) {
+        uq(qprev, q0, qnext, f, nv, cv);
       } else {
-        // the separate sweep stores the increment, the next one adds it to
-        // old phi: the same two roundings here
-        const T inc = T(.5) * (cen.phn + Dp);
-        qa_out.p = rb + inc;
+        uq(qnext, q0, qprev, f, nv, cv);
       }
-    }
-    um = u0;
-    u0 = un;
-    c0 = cn;
-#pragma unroll
-    for (int v = 0; v < 9; ++v) fc[v] = cur.f[v];
-    qb = qc;
-    qc = cur.q;
-    rb = rc;
-    rc = rn;
+    };
+    auto ufs = [&](const NbrVals<T>& prev, const NbrVals<T>& u0, const NbrVals<T>& next) {
+      return (D > 0) ? uf_stencil(prev, u0, next) : uf_stencil(next, u0, prev);
+    };
+    // stage A visits rows xs, xs+D, ...: two rows before the chunk, the
+    // chunk, two rows after it; stage B trails A by two rows
+    const int xs = ((D > 0) ? x0 : x1 - 1) - 2 * D;
+    const int nA = x1 - x0 + 4;
 
-    // ---- stage B: UpdateQuantities on row x-1, UpdateFields on row x-2
-    if (x >= x0) {  // wave-uniform
-      NbrVals<T> bun;
-      CenterVals<T> bcn;
-      uq(bqb, bqc, qa_out, bfp, bun, bcn);
-      if (x >= x0 + 2) {
-        const UFStencil<T> s = uf_stencil(bum, bu0, bun);
-        T Dxx, Dyx, Dp, om[9];
-        update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
-        if (active) {
-          const int ox = roff(x - 2);
-          (A.qxx_out + ox)[y] = cm2.qnxx + T(.5) * Dxx;
-          (A.qyx_out + ox)[y] = cm2.qnyx + T(.5) * Dyx;
+    // stage A rolling state (as in sweep_item); rb, rc: old phi of rows x, x+D
+    NbrVals<T> um, u0;
+    CenterVals<T> c0;
+    T fc[9];
+    QP<T> qb, qc;
+    T rb = T(0), rc = T(0);
+    PairIn<T, EXACT> cur;
+    {
+      const QP<T> qa = load_qp(xs - 2 * D);
+      qb = load_qp(xs - D);
+      qc = load_qp(xs);
+      const QP<T> qd = load_qp(xs + D);
+      if constexpr (!EXACT) {
+        rb = (A.phi_in + roff(xs))[y];
+        rc = (A.phi_in + roff(xs + D))[y];
+      }
+      T ft[9];
+      load_f(xs - D, ft);
+      load_f(xs, fc);
+      load_row(xs, D, cur);
+      CenterVals<T> ct;
+      uqd(qa, qb, qc, ft, um, ct);
+      uqd(qb, qc, qd, fc, u0, c0);
+      qb = qc;
+      qc = qd;
+    }
+    // stage B rolling state: A's outputs of rows x-2D (bqb, bfc), x-D (bqc,
+    // bfp); UpdateQuantities of rows x-3D (bum), x-2D (bu0); half-step values
+    // of rows x-2D (cm2), x-D (cm1).  Zeroed: the first two iterations run
+    // nothing of B, the next two only its UpdateQuantities, so no stored value
+    // depends on them.
+    NbrVals<T> bum{}, bu0{};
+    CenterVals<T> bc0{};
+    T bfc[9] = {}, bfp[9] = {};
+    QP<T> bqb{}, bqc{};
+    CenIn<T> cm1{}, cm2{};
+
+    for (int i = 0, x = xs; i < nA; ++i, x += D) {
+      PairIn<T, EXACT> nxt;
+      load_row(x + D, D, nxt);
+
+      // ---- stage A: UpdateQuantities on row x+D, UpdateFields on row x
+      T rn = cur.q.p;
+      if constexpr (!EXACT) cur.q.p += cur.dp;
+      NbrVals<T> un;
+      CenterVals<T> cn;
+      uqd(qb, qc, cur.q, cur.f, un, cn);
+      QP<T> qa_out;
+      T fa[9];
+      CenIn<T> cen;  // half-step values of row x (B uses them two rows later)
+      if constexpr (kRebuild) {
 #pragma unroll
-          for (int v = 0; v < 9; ++v) (A.f_out + (v * N + ox))[y] = cm2.fn[v] + T(.5) * om[v];
-          if constexpr (EXACT) {
-            (A.phi_out + ox)[y] = cm2.phn + T(.5) * Dp;
-          } else {
-            // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, as the
-            // separate last corrector
-            const T pold = cur.pold;
-            const T yv = T(.5) * (cm2.phn + Dp) + cur.phic;
-            const T t = pold + yv;
-            const T bp = t - pold;
-            const T ap = t - bp;
-            const T err = (pold - ap) + (yv - bp);
-            (A.phi_out + ox)[y] = t;
-            (A.phic_out + ox)[y] = err;
-          }
+        for (int v = 0; v < 9; ++v) cen.fn[v] = half_of(fc[v], cur.c.fn[v]);
+        cen.qnxx = half_of(qb.xx, cur.c.qnxx);
+        cen.qnyx = half_of(qb.yx, cur.c.qnyx);
+        cen.phn = cur.c.phn;
+      } else {
+        cen = cur.c;
+      }
+      {
+        const UFStencil<T> s = ufs(um, u0, un);
+        T Dxx, Dyx, Dp, om[9];
+        update_fields<T, EXACT>(c0, qb.xx, qb.yx, qb.p, fc, s, P, Dxx, Dyx, Dp, om);
+        qa_out.xx = cen.qnxx + T(.5) * Dxx;
+        qa_out.yx = cen.qnyx + T(.5) * Dyx;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) fa[v] = cen.fn[v] + T(.5) * om[v];
+        if constexpr (EXACT) {
+          qa_out.p = cen.phn + T(.5) * Dp;
+        } else {
+          // the separate sweep stores the increment, the next one adds it to
+          // old phi: the same two roundings here
+          const T inc = T(.5) * (cen.phn + Dp);
+          qa_out.p = rb + inc;
         }
       }
-      bum = bu0;
-      bu0 = bun;
-      bc0 = bcn;
-    }
+      um = u0;
+      u0 = un;
+      c0 = cn;
 #pragma unroll
-    for (int v = 0; v < 9; ++v) {
-      bfc[v] = bfp[v];
-      bfp[v] = fa[v];
+      for (int v = 0; v < 9; ++v) fc[v] = cur.f[v];
+      qb = qc;
+      qc = cur.q;
+      rb = rc;
+      rc = rn;
+
+      // ---- stage B: UpdateQuantities on row x-D, UpdateFields on row x-2D
+      if (i >= 2) {  // wave-uniform
+        NbrVals<T> bun;
+        CenterVals<T> bcn;
+        uqd(bqb, bqc, qa_out, bfp, bun, bcn);
+        if (i >= 4) {
+          const UFStencil<T> s = ufs(bum, bu0, bun);
+          T Dxx, Dyx, Dp, om[9];
+          update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
+          if (active) {
+            const int ox = roff(x - 2 * D);
+            (A.qxx_out + ox)[y] = cm2.qnxx + T(.5) * Dxx;
+            (A.qyx_out + ox)[y] = cm2.qnyx + T(.5) * Dyx;
+#pragma unroll
+            for (int v = 0; v < 9; ++v)
+              (A.f_out + (v * N + ox))[y] = cm2.fn[v] + T(.5) * om[v];
+            if constexpr (EXACT) {
+              (A.phi_out + ox)[y] = cm2.phn + T(.5) * Dp;
+            } else {
+              // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, as
+              // the separate last corrector
+              const T pold = cur.pold;
+              const T yv = T(.5) * (cm2.phn + Dp) + cur.phic;
+              const T t = pold + yv;
+              const T bp = t - pold;
+              const T ap = t - bp;
+              const T err = (pold - ap) + (yv - bp);
+              (A.phi_out + ox)[y] = t;
+              (A.phic_out + ox)[y] = err;
+            }
+          }
+        }
+        bum = bu0;
+        bu0 = bun;
+        bc0 = bcn;
+      }
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        bfc[v] = bfp[v];
+        bfp[v] = fa[v];
+      }
+      bqb = bqc;
+      bqc = qa_out;
+      cm2 = cm1;
+      cm1 = cen;
+      cur = nxt;
     }
-    bqb = bqc;
-    bqc = qa_out;
-    cm2 = cm1;
-    cm1 = cen;
-    cur = nxt;
+  };
+  if (g.alt && !(chunk & 1)) {
+    march(std::integral_constant<int, -1>{});
+  } else {
+    march(std::integral_constant<int, 1>{});
   }
 }
 

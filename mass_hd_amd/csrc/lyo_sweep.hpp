@@ -15,6 +15,8 @@
 // contiguous 64-lane load or store of one SoA plane.
 #pragma once
 
+#include <type_traits>
+
 #include "lyo_physics.hpp"
 
 namespace lyo {
@@ -51,6 +53,7 @@ struct Geo {
   int nstrips, nchunks, xc;
   int nwaves;
   int xcd;  // remap block ids so each XCD gets a contiguous band of chunks
+  int alt;  // 1: even chunks march downwards, odd upwards; 0: all upwards
 };
 
 enum Stage { PRED = 0, CORR = 1, CORR_LAST = 2 };
@@ -212,15 +215,16 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
 #pragma unroll
     for (int v = 0; v < 9; ++v) f[v] = (A.f_in + (v * N + o))[y];
   };
-  auto load_row = [&](int x, RowIn<T, STAGE>& R) {
+  // loads of the iteration on row x of a march in direction d (+1 or -1)
+  auto load_row = [&](int x, int d, RowIn<T, STAGE>& R) {
     {
-      const int o = roff(x + 2);
+      const int o = roff(x + 2 * d);
       R.q.xx = (A.qxx_in + o)[y];
       R.q.yx = (A.qyx_in + o)[y];
       R.q.p = (A.phi_in + o)[y];
       if constexpr (!EXACT && STAGE != PRED) R.dp = (A.dphi_in + o)[y];
     }
-    load_f(x + 1, R.f);
+    load_f(x + d, R.f);
     if constexpr (kRebuild) {
       // old populations at the upstream node x - c_v, old Q at x
       const int o = roff(x), om = roff(x - 1), op = roff(x + 1);
@@ -248,45 +252,62 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
                                 cv);
   };
 
-  // prologue: UpdateQuantities on rows x0-1 and x0
-  NbrVals<T> um, u0;
-  CenterVals<T> c0;
-  T fc[9];
-  QP<T> qb, qc;
-  RowIn<T, STAGE> cur;
-  {
-    const QP<T> qa = load_qp(x0 - 2);
-    qb = load_qp(x0 - 1);
-    qc = load_qp(x0);
-    const QP<T> qd = load_qp(x0 + 1);
-    T ft[9];
-    load_f(x0 - 1, ft);
-    load_f(x0, fc);
-    load_row(x0, cur);
-    CenterVals<T> ct;
-    uq(qa, qb, qc, ft, um, ct);
-    uq(qb, qc, qd, fc, u0, c0);
-    qb = qc;
-    qc = qd;
-  }
-  // invariant at loop top: qb = row x, qc = row x+1; um/u0 = rows x-1/x;
-  // c0, fc = centre data of row x; cur = loads of this iteration
-  for (int x = x0; x < x1; ++x) {
+  // The chunk is marched upwards (D = +1, rows x0 .. x1-1) or downwards
+  // (D = -1).  Alternating the direction with the chunk parity makes two
+  // neighbouring chunks reach their shared boundary rows at the same time
+  // (both at their start or both at their end), so the rows one of them
+  // loads are still in L2 for the other.  "prev"/"next" below are the rows
+  // x-D / x+D; the stencils take the rows x-1 / x+1, swapped for D = -1.
+  auto march = [&](auto dir) {
+    constexpr int D = decltype(dir)::value;
+    auto uqd = [&](const QP<T>& qprev, const QP<T>& q0, const QP<T>& qnext, const T(&f)[9],
+                   NbrVals<T>& nv, CenterVals<T>& cv) {
+      if constexpr (D > 0) {
+        uq(qprev, q0, qnext, f, nv, cv);
+      } else {
+        uq(qnext, q0, qprev, f, nv, cv);
+      }
+    };
+    const int xb = (D > 0) ? x0 : x1 - 1;
+    // prologue: UpdateQuantities on rows xb-D and xb
+    NbrVals<T> um, u0;
+    CenterVals<T> c0;
+    T fc[9];
+    QP<T> qb, qc;
+    RowIn<T, STAGE> cur;
+    {
+      const QP<T> qa = load_qp(xb - 2 * D);
+      qb = load_qp(xb - D);
+      qc = load_qp(xb);
+      const QP<T> qd = load_qp(xb + D);
+      T ft[9];
+      load_f(xb - D, ft);
+      load_f(xb, fc);
+      load_row(xb, D, cur);
+      CenterVals<T> ct;
+      uqd(qa, qb, qc, ft, um, ct);
+      uqd(qb, qc, qd, fc, u0, c0);
+      qb = qc;
+      qc = qd;
+    }
+    // invariant at loop top: qb = row x, qc = row x+D; um/u0 = rows x-D/x;
+    // c0, fc = centre data of row x; cur = loads of this iteration
+    for (int i = 0, x = xb; i < x1 - x0; ++i, x += D) {
     // prefetch unconditionally (the wrapped row past the chunk is in bounds
     // and unused): a conditional load would make the outstanding-load count
     // path dependent and force a full wait at the top of the loop
     RowIn<T, STAGE> nxt;
-    load_row(x + 1, nxt);
+    load_row(x + D, D, nxt);
     const int ox = roff(x);
 
-    // UpdateQuantities on row x+1
+    // UpdateQuantities on row x+D
     if constexpr (!EXACT && STAGE != PRED) cur.q.p += cur.dp;
     NbrVals<T> un;
     CenterVals<T> cn;
-    uq(qb, qc, cur.q, cur.f, un, cn);
+    uqd(qb, qc, cur.q, cur.f, un, cn);
 
     // UpdateFields on row x
-    const UFStencil<T> s = uf_stencil(um, u0, un);
+    const UFStencil<T> s = (D > 0) ? uf_stencil(um, u0, un) : uf_stencil(un, u0, um);
     T Dxx, Dyx, Dp, om[9];
     const T Qxx = qb.xx, Qyx = qb.yx, p = qb.p;
     update_fields<T, EXACT>(c0, Qxx, Qyx, p, fc, s, P, Dxx, Dyx, Dp, om);
@@ -368,6 +389,12 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
     qb = qc;
     qc = cur.q;
     cur = nxt;
+    }
+  };
+  if (g.alt && !(chunk & 1)) {
+    march(std::integral_constant<int, -1>{});
+  } else {
+    march(std::integral_constant<int, 1>{});
   }
 }
 

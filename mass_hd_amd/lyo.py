@@ -102,7 +102,7 @@ def load_library():
     lib.mass_lyo_synchronize.argtypes = [h]
     lib.mass_lyo_time_steps.argtypes = [h, ctypes.c_int32, dp, dp]
     lib.mass_lyo_set_launch.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
-    lib.mass_lyo_set_fusion.argtypes = [h, ctypes.c_int32, ctypes.c_int32]
+    lib.mass_lyo_set_schedule.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.mass_lyo_abi_version.argtypes = []
     lib.mass_lyo_diagnostics.argtypes = [h, P(Diag), dp]
     i32 = ctypes.c_int32
@@ -251,11 +251,14 @@ class Lyotropic:
         self._check(self.lib.mass_lyo_set_launch(self.h, int(chunk_rows), int(waves_per_block),
                                                  int(xcd_order)))
 
-    def set_fusion(self, fuse_last_pair=-1, pair_chunk_rows=0):
-        """1: the last two correctors run as one kernel (default); 0: separate
-        sweeps.  Results are bitwise identical."""
-        self._check(self.lib.mass_lyo_set_fusion(self.h, int(fuse_last_pair),
-                                                 int(pair_chunk_rows)))
+    def set_schedule(self, fuse_last_pair=-1, pair_chunk_rows=0, alternate=-1):
+        """fuse_last_pair 1: the last two correctors run as one kernel
+        (default), 0: separate sweeps; pair_chunk_rows: rows per wave chunk of
+        the fused kernel (0 keeps); alternate 1: neighbouring chunks march in
+        opposite directions (default), 0: all upwards.  -1 keeps a setting.
+        Results are bitwise identical either way."""
+        self._check(self.lib.mass_lyo_set_schedule(self.h, int(fuse_last_pair),
+                                                   int(pair_chunk_rows), int(alternate)))
 
 
 class Batch:

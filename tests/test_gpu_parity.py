@@ -103,20 +103,23 @@ def test_fused_pair_equals_separate_sweeps(require_gpu, precision, npc, LX, LY, 
     fused path recomputes on demand); asking for derived fields mid-run does
     not disturb the run."""
     runs = []
-    for fuse in (0, 1):
+    # (fused, march order): separate sweeps, all chunks upwards or alternating;
+    # fused, alternating
+    for fuse, alt in ((0, 0), (0, 1), (1, 1)):
         g = lyo.Lyotropic(LX, LY, npc=npc, precision=precision, **CONFIG0_PHYS)
-        g.set_fusion(fuse, xc2)
+        g.set_schedule(fuse, xc2, alt)
         g.configure(**CONFIG0_INIT)
         g.step(4)
         mid = [g.get(n) for n in ("ux", "sigmaXY", "MU")]
         g.step(3)
         runs.append((g, mid))
-    (a, ma), (b, mb) = runs
-    for x, y in zip(ma, mb):
-        assert np.array_equal(x, y)
-    for n in STATE + DERIVED:
-        x, y = a.get(n), b.get(n)
-        assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
+    a, ma = runs[0]
+    for b, mb in runs[1:]:
+        for x, y in zip(ma, mb):
+            assert np.array_equal(x, y)
+        for n in STATE + DERIVED:
+            x, y = a.get(n), b.get(n)
+            assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
 
 
 def test_fused_pair_timing_kinds(require_gpu):
