@@ -68,9 +68,9 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
   const int ycol = strip * kPairStrip - kPairHalo + lane;
   int yi = ycol % LY;
   if (yi < 0) yi += LY;
-  const unsigned y = (unsigned)yi;
-  const unsigned yp1 = (yi + 1 == LY) ? 0u : (unsigned)(yi + 1);
-  const unsigned ym1 = (yi == 0) ? (unsigned)(LY - 1) : (unsigned)(yi - 1);
+  LaneOff<T> ly{(unsigned)yi * (unsigned)sizeof(T)};
+  LaneOff<T> lyp{(yi + 1 == LY ? 0u : (unsigned)(yi + 1)) * (unsigned)sizeof(T)};
+  LaneOff<T> lym{(yi == 0 ? (unsigned)(LY - 1) : (unsigned)(yi - 1)) * (unsigned)sizeof(T)};
   const bool active = lane >= kPairHalo && lane < kWave - kPairHalo && ycol < LY;
   const int x0 = chunk * g.xc;
   const int x1 = min(x0 + g.xc, LX);
@@ -85,49 +85,49 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
   auto load_qp = [&](int r) {
     const int o = roff(r);
     QP<T> q;
-    q.xx = (A.qxx_in + o)[y];
-    q.yx = (A.qyx_in + o)[y];
-    T ph = (A.phi_in + o)[y];
-    if constexpr (!EXACT) ph += (A.dphi_in + o)[y];
+    q.xx = ly.ld(A.qxx_in + o);
+    q.yx = ly.ld(A.qyx_in + o);
+    T ph = ly.ld(A.phi_in + o);
+    if constexpr (!EXACT) ph += ly.ld(A.dphi_in + o);
     q.p = ph;
     return q;
   };
   auto load_f = [&](int r, T(&f)[9]) {
     const int o = roff(r);
 #pragma unroll
-    for (int v = 0; v < 9; ++v) f[v] = (A.f_in + (v * N + o))[y];
+    for (int v = 0; v < 9; ++v) f[v] = ly.ld(A.f_in + (v * N + o));
   };
   // loads of the iteration on row x of a march in direction d
   auto load_row = [&](int x, int d, PairIn<T, EXACT>& R) {
     {
       const int o = roff(x + 2 * d);
-      R.q.xx = (A.qxx_in + o)[y];
-      R.q.yx = (A.qyx_in + o)[y];
-      R.q.p = (A.phi_in + o)[y];
-      if constexpr (!EXACT) R.dp = (A.dphi_in + o)[y];
+      R.q.xx = ly.ld(A.qxx_in + o);
+      R.q.yx = ly.ld(A.qyx_in + o);
+      R.q.p = ly.ld(A.phi_in + o);
+      if constexpr (!EXACT) R.dp = ly.ld(A.dphi_in + o);
     }
     load_f(x + d, R.f);
     if constexpr (kRebuild) {
       const int o = roff(x), om = roff(x - 1), op = roff(x + 1);
       const int rowv[9] = {o, om, op, o, o, om, op, op, om};
-      const unsigned colv[9] = {y, y, y, ym1, yp1, ym1, yp1, ym1, yp1};
+      LaneOff<T>* const colv[9] = {&ly, &ly, &ly, &lym, &lyp, &lym, &lyp, &lym, &lyp};
 #pragma unroll
-      for (int v = 0; v < 9; ++v) R.c.fn[v] = (A.fold_in + (v * N + rowv[v]))[colv[v]];
-      R.c.qnxx = (A.qoxx_in + o)[y];
-      R.c.qnyx = (A.qoyx_in + o)[y];
-      R.c.phn = (A.phn_in + o)[y];
+      for (int v = 0; v < 9; ++v) R.c.fn[v] = colv[v]->ld(A.fold_in + (v * N + rowv[v]));
+      R.c.qnxx = ly.ld(A.qoxx_in + o);
+      R.c.qnyx = ly.ld(A.qoyx_in + o);
+      R.c.phn = ly.ld(A.phn_in + o);
     } else {
       const int o = roff(x);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) R.c.fn[v] = (A.fn_in + (v * N + o))[y];
-      R.c.qnxx = (A.qnxx_in + o)[y];
-      R.c.qnyx = (A.qnyx_in + o)[y];
-      R.c.phn = (A.phn_in + o)[y];
+      for (int v = 0; v < 9; ++v) R.c.fn[v] = ly.ld(A.fn_in + (v * N + o));
+      R.c.qnxx = ly.ld(A.qnxx_in + o);
+      R.c.qnyx = ly.ld(A.qnyx_in + o);
+      R.c.phn = ly.ld(A.phn_in + o);
     }
     if constexpr (!EXACT) {
       const int o = roff(x - 2 * d);
-      R.pold = (A.phi_in + o)[y];
-      R.phic = (A.phic_in + o)[y];
+      R.pold = ly.ld(A.phi_in + o);
+      R.phic = ly.ld(A.phic_in + o);
     }
   };
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
@@ -171,8 +171,8 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
       qc = load_qp(xs);
       const QP<T> qd = load_qp(xs + D);
       if constexpr (!EXACT) {
-        rb = (A.phi_in + roff(xs))[y];
-        rc = (A.phi_in + roff(xs + D))[y];
+        rb = ly.ld(A.phi_in + roff(xs));
+        rc = ly.ld(A.phi_in + roff(xs + D));
       }
       T ft[9];
       load_f(xs - D, ft);
@@ -255,13 +255,13 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
           update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
           if (active) {
             const int ox = roff(x - 2 * D);
-            (A.qxx_out + ox)[y] = cm2.qnxx + T(.5) * Dxx;
-            (A.qyx_out + ox)[y] = cm2.qnyx + T(.5) * Dyx;
+            ly.st(A.qxx_out + ox, cm2.qnxx + T(.5) * Dxx);
+            ly.st(A.qyx_out + ox, cm2.qnyx + T(.5) * Dyx);
 #pragma unroll
             for (int v = 0; v < 9; ++v)
-              (A.f_out + (v * N + ox))[y] = cm2.fn[v] + T(.5) * om[v];
+              ly.st(A.f_out + (v * N + ox), cm2.fn[v] + T(.5) * om[v]);
             if constexpr (EXACT) {
-              (A.phi_out + ox)[y] = cm2.phn + T(.5) * Dp;
+              ly.st(A.phi_out + ox, cm2.phn + T(.5) * Dp);
             } else {
               // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, as
               // the separate last corrector
@@ -271,8 +271,8 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
               const T bp = t - pold;
               const T ap = t - bp;
               const T err = (pold - ap) + (yv - bp);
-              (A.phi_out + ox)[y] = t;
-              (A.phic_out + ox)[y] = err;
+              ly.st(A.phi_out + ox, t);
+              ly.st(A.phic_out + ox, err);
             }
           }
         }

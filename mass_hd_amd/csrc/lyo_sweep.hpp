@@ -101,6 +101,26 @@ struct SweepArgs {
   T* phic_out;
 };
 
+// Lane part of an address: a 32-bit byte offset added to a wave-uniform row
+// base, so that every access is one scalar base plus one 32-bit lane offset
+// (the saddr form of global loads and stores; no 64-bit lane arithmetic).
+// Each access first hides the offset's value from the optimiser (an empty asm
+// that "modifies" it in place, so no instruction and no copy), which would
+// otherwise fold base+offset into 64-bit per-lane pointers (hoisted out of the
+// loop, or chained from plane to plane): two VGPRs and a 64-bit add each.
+template <class T>
+struct LaneOff {
+  unsigned b;
+  __device__ __forceinline__ T ld(const T* row) {
+    asm volatile("" : "+v"(b));
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(row) + b);
+  }
+  __device__ __forceinline__ void st(T* row, T v) {
+    asm volatile("" : "+v"(b));
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(row) + b) = v;
+  }
+};
+
 template <class T>
 struct QP {
   T xx, yx, p;
@@ -183,11 +203,10 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
   const int ycol = strip * kStrip - kHalo + lane;
   int yi = ycol % LY;
   if (yi < 0) yi += LY;
-  // lane offsets are unsigned 32-bit and row bases wave-uniform, so every
-  // access is a scalar base plus a 32-bit lane offset (no 64-bit lane math)
-  const unsigned y = (unsigned)yi;
-  const unsigned yp1 = (yi + 1 == LY) ? 0u : (unsigned)(yi + 1);
-  const unsigned ym1 = (yi == 0) ? (unsigned)(LY - 1) : (unsigned)(yi - 1);
+  // byte offsets of the columns y, y+1, y-1 (periodic) in a row
+  LaneOff<T> ly{(unsigned)yi * (unsigned)sizeof(T)};
+  LaneOff<T> lyp{(yi + 1 == LY ? 0u : (unsigned)(yi + 1)) * (unsigned)sizeof(T)};
+  LaneOff<T> lym{(yi == 0 ? (unsigned)(LY - 1) : (unsigned)(yi - 1)) * (unsigned)sizeof(T)};
   const bool active = lane >= kHalo && lane < kWave - kHalo && ycol < LY;
   const int x0 = chunk * g.xc;
   const int x1 = min(x0 + g.xc, LX);
@@ -203,45 +222,45 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
   auto load_qp = [&](int r) {
     const int o = roff(r);
     QP<T> q;
-    q.xx = (A.qxx_in + o)[y];
-    q.yx = (A.qyx_in + o)[y];
-    T ph = (A.phi_in + o)[y];
-    if constexpr (!EXACT && STAGE != PRED) ph += (A.dphi_in + o)[y];
+    q.xx = ly.ld(A.qxx_in + o);
+    q.yx = ly.ld(A.qyx_in + o);
+    T ph = ly.ld(A.phi_in + o);
+    if constexpr (!EXACT && STAGE != PRED) ph += ly.ld(A.dphi_in + o);
     q.p = ph;
     return q;
   };
   auto load_f = [&](int r, T(&f)[9]) {
     const int o = roff(r);
 #pragma unroll
-    for (int v = 0; v < 9; ++v) f[v] = (A.f_in + (v * N + o))[y];
+    for (int v = 0; v < 9; ++v) f[v] = ly.ld(A.f_in + (v * N + o));
   };
   // loads of the iteration on row x of a march in direction d (+1 or -1)
   auto load_row = [&](int x, int d, RowIn<T, STAGE>& R) {
     {
       const int o = roff(x + 2 * d);
-      R.q.xx = (A.qxx_in + o)[y];
-      R.q.yx = (A.qyx_in + o)[y];
-      R.q.p = (A.phi_in + o)[y];
-      if constexpr (!EXACT && STAGE != PRED) R.dp = (A.dphi_in + o)[y];
+      R.q.xx = ly.ld(A.qxx_in + o);
+      R.q.yx = ly.ld(A.qyx_in + o);
+      R.q.p = ly.ld(A.phi_in + o);
+      if constexpr (!EXACT && STAGE != PRED) R.dp = ly.ld(A.dphi_in + o);
     }
     load_f(x + d, R.f);
     if constexpr (kRebuild) {
       // old populations at the upstream node x - c_v, old Q at x
       const int o = roff(x), om = roff(x - 1), op = roff(x + 1);
       const int rowv[9] = {o, om, op, o, o, om, op, op, om};
-      const unsigned colv[9] = {y, y, y, ym1, yp1, ym1, yp1, ym1, yp1};
+      LaneOff<T>* const colv[9] = {&ly, &ly, &ly, &lym, &lyp, &lym, &lyp, &lym, &lyp};
 #pragma unroll
-      for (int v = 0; v < 9; ++v) R.fn[v] = (A.fold_in + (v * N + rowv[v]))[colv[v]];
-      R.qnxx = (A.qoxx_in + o)[y];
-      R.qnyx = (A.qoyx_in + o)[y];
-      R.phn = (A.phn_in + o)[y];
+      for (int v = 0; v < 9; ++v) R.fn[v] = colv[v]->ld(A.fold_in + (v * N + rowv[v]));
+      R.qnxx = ly.ld(A.qoxx_in + o);
+      R.qnyx = ly.ld(A.qoyx_in + o);
+      R.phn = ly.ld(A.phn_in + o);
     } else if constexpr (STAGE != PRED) {
       const int o = roff(x);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) R.fn[v] = (A.fn_in + (v * N + o))[y];
-      R.qnxx = (A.qnxx_in + o)[y];
-      R.qnyx = (A.qnyx_in + o)[y];
-      R.phn = (A.phn_in + o)[y];
+      for (int v = 0; v < 9; ++v) R.fn[v] = ly.ld(A.fn_in + (v * N + o));
+      R.qnxx = ly.ld(A.qnxx_in + o);
+      R.qnyx = ly.ld(A.qnyx_in + o);
+      R.phn = ly.ld(A.phn_in + o);
     }
   };
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
@@ -315,34 +334,34 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
     if (active) {
       if constexpr (STAGE == PRED) {
         const T Qsx = Qxx + Dxx, Qsy = Qyx + Dyx;
-        (A.qxx_out + ox)[y] = Qsx;
-        (A.qyx_out + ox)[y] = Qsy;
+        ly.st(A.qxx_out + ox, Qsx);
+        ly.st(A.qyx_out + ox, Qsy);
         if constexpr (EXACT) {
-          (A.qnxx_out + ox)[y] = Qxx + T(.5) * Dxx;
-          (A.qnyx_out + ox)[y] = Qyx + T(.5) * Dyx;
-          (A.phn_out + ox)[y] = p + T(.5) * Dp;
-          (A.phi_out + ox)[y] = p + Dp;
+          ly.st(A.qnxx_out + ox, Qxx + T(.5) * Dxx);
+          ly.st(A.qnyx_out + ox, Qyx + T(.5) * Dyx);
+          ly.st(A.phn_out + ox, p + T(.5) * Dp);
+          ly.st(A.phi_out + ox, p + Dp);
         } else {
           if constexpr (kStoreHalf) {
-            (A.qnxx_out + ox)[y] = half_of(Qsx, Qxx);
-            (A.qnyx_out + ox)[y] = half_of(Qsy, Qyx);
+            ly.st(A.qnxx_out + ox, half_of(Qsx, Qxx));
+            ly.st(A.qnyx_out + ox, half_of(Qsy, Qyx));
           }
-          (A.phi_out + ox)[y] = Dp;
+          ly.st(A.phi_out + ox, Dp);
         }
         // collide and push to (x+cx, y+cy) (Move, lyotropic.cpp:371-386)
         const int rm = roff(x - 1), rp = roff(x + 1);
         const int rowv[9] = {ox, rp, rm, ox, ox, rp, rm, rm, rp};
-        const unsigned colv[9] = {y, y, y, yp1, ym1, yp1, ym1, yp1, ym1};
+        LaneOff<T>* const colv[9] = {&ly, &ly, &ly, &lyp, &lym, &lyp, &lym, &lyp, &lym};
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
           const int d = v * N + rowv[v];
           const T fs = fc[v] + om[v];
           if constexpr (EXACT) {
-            (A.fn_out + d)[colv[v]] = fc[v] + T(.5) * om[v];
+            colv[v]->st(A.fn_out + d, fc[v] + T(.5) * om[v]);
           } else if constexpr (kStoreHalf) {
-            (A.fn_out + d)[colv[v]] = half_of(fs, fc[v]);
+            colv[v]->st(A.fn_out + d, half_of(fs, fc[v]));
           }
-          (A.f_out + d)[colv[v]] = fs;
+          colv[v]->st(A.f_out + d, fs);
         }
       } else {
         // centre half-step values fn, QN of row x
@@ -358,25 +377,25 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
           qnx = cur.qnxx;
           qny = cur.qnyx;
         }
-        (A.qxx_out + ox)[y] = qnx + T(.5) * Dxx;
-        (A.qyx_out + ox)[y] = qny + T(.5) * Dyx;
+        ly.st(A.qxx_out + ox, qnx + T(.5) * Dxx);
+        ly.st(A.qyx_out + ox, qny + T(.5) * Dyx);
 #pragma unroll
-        for (int v = 0; v < 9; ++v) (A.f_out + (v * N + ox))[y] = fnv[v] + T(.5) * om[v];
+        for (int v = 0; v < 9; ++v) ly.st(A.f_out + (v * N + ox), fnv[v] + T(.5) * om[v]);
         if constexpr (EXACT) {
-          (A.phi_out + ox)[y] = cur.phn + T(.5) * Dp;
+          ly.st(A.phi_out + ox, cur.phn + T(.5) * Dp);
         } else if constexpr (STAGE == CORR) {
-          (A.phi_out + ox)[y] = T(.5) * (cur.phn + Dp);
+          ly.st(A.phi_out + ox, T(.5) * (cur.phn + Dp));
         } else {
           // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, so the
           // rounding of the addition is carried to the next step
-          const T pold = (A.phi_in + ox)[y];
-          const T yv = T(.5) * (cur.phn + Dp) + (A.phic_in + ox)[y];
+          const T pold = ly.ld(A.phi_in + ox);
+          const T yv = T(.5) * (cur.phn + Dp) + ly.ld(A.phic_in + ox);
           const T t = pold + yv;
           const T bp = t - pold;
           const T ap = t - bp;
           const T err = (pold - ap) + (yv - bp);
-          (A.phi_out + ox)[y] = t;
-          (A.phic_out + ox)[y] = err;
+          ly.st(A.phi_out + ox, t);
+          ly.st(A.phic_out + ox, err);
         }
       }
     }
