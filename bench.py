@@ -8,14 +8,23 @@ The same grid is also timed in FP64 (reference arithmetic).
 MLUPS = LX*LY*steps / seconds / 1e6; one step = predictor + npc correctors
 (the last two correctors run as one fused kernel).  The timed region is
 bracketed by HIP events on the library's stream; the fields are resident in
-HBM throughout.  roofline: the kernel with the longest launch, priced by its
-algorithmic traffic per node (every plane it touches read or written once):
-  predictor  read f 9, Q 2, phi 1; write f* 9, fn* 9, Q+D 2, Q+D/2 2, Dp 1
-             = 35 values = 140 B/node (FP32)
-  fused pair read f* 9, Q* 2, phi 1, Dp 1, fn* 9, QN 2, phic 1; write f 9,
-             Q 2, phi 1, phic 1 = 38 values = 152 B/node (FP32)
-against 8 TB/s.  cpu_baseline: the CPU oracle (the reference's algorithm and
-layout, OpenMP) on the host cores.
+HBM throughout.
+
+roofline: the kernel with the longest mean launch.  Its algorithmic bytes are
+the per-sweep figure of the step's data flow -- one sweep reads and writes the
+model state and the predictor's half-step state once, 36 values per node
+(predictor: read f, Q, phi = 12, write f*, fn*, Q+D, Q+D/2, phi terms = 24;
+corrector: read 24, write 12) -- times the sweeps one launch performs: 1 for
+the predictor, 2 for the fused last corrector pair.  In FP32 that is 144 B
+per node per sweep against 8 TB/s.  The fused pair keeps its first sweep's
+outputs in registers, so it moves fewer distinct bytes than two sweeps'
+worth; "distinct_bytes_per_launch" counts every plane it touches once (pair:
+read f*, f_old 18, Q*, Q_old 4, phi, Dp, phic 3; write f, Q, phi, phic 13 = 38
+values; predictor: read 12, write f*, Q+D, Dp 12 = 24 values), and "traffic"
+is the HBM bytes the counters measured per launch (profiles/).
+
+cpu_baseline: the CPU oracle (the reference's algorithm and layout, OpenMP)
+on the host cores.
 """
 import argparse
 import json
@@ -32,10 +41,12 @@ INIT = dict(config="random", seed=7, angle=90.0, noise=1.0, qamp=0.5, phinoise=0
 HBM_PEAK = 8.0e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fp32_4096_npc2.json")
 # algorithmic values per node per launch (x4 bytes in FP32)
-VALUES = {"predictor": 35, "corrector": 37, "pair": 38}
-KERNEL = {"predictor": "sweep_kernel<float, false, 0>",
-          "corrector": "sweep_kernel<float, false, 1>",
-          "pair": "pair_kernel<float, false>"}
+SWEEPS = {"predictor": 1, "corrector": 1, "pair": 2}   # sweeps per launch
+SWEEP_VALUES = 36                                        # values per node per sweep
+DISTINCT = {"predictor": 24, "corrector": 37, "pair": 38}  # FP32, npc=2 data flow
+KERNEL = {"predictor": "sweep_kernel<float, false, 0, false>",
+          "corrector": "sweep_kernel<float, false, 1, true>",
+          "pair": "pair_kernel<float, false, true>"}
 
 
 def log(msg):
@@ -153,8 +164,9 @@ def main():
                  "fp64_launch_ms": {k: round(v, 4) for k, v in k64.items() if v}}
 
     dom = max((k for k in k32 if k32[k] > 0), key=lambda k: k32[k])
-    alg_bytes = VALUES[dom] * 4 * N
+    alg_bytes = SWEEPS[dom] * SWEEP_VALUES * 4 * N
     achieved = alg_bytes / (k32[dom] / 1e3)
+    distinct = DISTINCT[dom] * 4 * N
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -195,6 +207,9 @@ def main():
                      "traffic": traffic,
                      "kernel": KERNEL[dom], "kernel_ms": round(k32[dom], 4),
                      "alg_bytes_per_launch": alg_bytes,
+                     "distinct_bytes_per_launch": distinct,
+                     "step_alg_GBps": round((1 + npc) * SWEEP_VALUES * 4 * N * args.steps
+                                            / (t32 / 1e3) / 1e9, 1),
                      "launch_ms": {k: round(v, 4) for k, v in k32.items() if v}},
         "cpu_baseline": cpu,
     }
