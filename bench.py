@@ -41,23 +41,24 @@ INIT = dict(config="random", seed=7, angle=90.0, noise=1.0, qamp=0.5, phinoise=0
 HBM_PEAK = 8.0e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fp32_4096_npc2.json")
 # algorithmic values per node per launch (x4 bytes in FP32)
-SWEEPS = {"predictor": 1, "corrector": 1, "pair": 2}   # sweeps per launch
+SWEEPS = {"predictor": 1, "corrector": 1, "pair": 2, "step": 3}   # sweeps per launch
 SWEEP_VALUES = 36                                        # values per node per sweep
-DISTINCT = {"predictor": 24, "corrector": 37, "pair": 38}  # FP32, npc=2 data flow
+DISTINCT = {"predictor": 24, "corrector": 37, "pair": 38, "step": 26}  # FP32, npc=2 data flow
 KERNEL = {"predictor": "sweep_kernel<float, false, 0, false>",
           "corrector": "sweep_kernel<float, false, 1, true>",
-          "pair": "pair_kernel<float, false, true>"}
+          "pair": "pair_kernel<float, false, true>",
+          "step": "full_kernel<float>"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-1, alt=-1):
+def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-1, alt=-1,
+            full=-1, step_chunk=0, wpb=0):
     m = lyo.Lyotropic(L, L, npc=npc, precision=precision, **PHYS)
-    if chunk:
-        m.set_launch(chunk, 0)
-    m.set_schedule(fuse, pair_chunk, alt)
+    m.set_launch(chunk, wpb)
+    m.set_schedule(fuse, pair_chunk, alt, full, step_chunk)
     m.configure(**INIT)
     if warmup:
         m.step(warmup)
@@ -65,7 +66,7 @@ def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-
     total_ms, sweeps = m.time_steps(steps, per_sweep=True)
     m.runtime_checks()
     m.close()
-    return total_ms, dict(zip(("predictor", "corrector", "pair"), sweeps))
+    return total_ms, dict(zip(("predictor", "corrector", "pair", "step"), sweeps))
 
 
 def cpu_baseline(L, steps, threads):
@@ -124,6 +125,10 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--pair-chunk", type=int, default=0)
     ap.add_argument("--no-fuse", action="store_true", help="separate corrector sweeps")
+    ap.add_argument("--no-full", action="store_true",
+                    help="predictor + fused pair instead of the one-kernel step")
+    ap.add_argument("--step-chunk", type=int, default=0)
+    ap.add_argument("--wpb", type=int, default=0, help="waves per block (0: default)")
     ap.add_argument("--alt", type=int, default=-1,
                     help="march order: 0 every chunk upwards, 1 alternating (default)")
     ap.add_argument("--skip-fp64", action="store_true")
@@ -147,7 +152,8 @@ def main():
     alt = args.alt
     log("fp32 %dx%d npc=%d: %d warmup + %d timed steps" % (L, L, npc, args.warmup, args.steps))
     t32, k32 = gpu_run(lyo, L, lyo.FP32, npc, args.steps, args.warmup, args.chunk,
-                       args.pair_chunk, fuse, alt)
+                       args.pair_chunk, fuse, alt, 0 if args.no_full else -1, args.step_chunk,
+                       args.wpb)
     mlups32 = N * args.steps / (t32 / 1e3) / 1e6
     log("fp32: %.3f ms/step, %.0f MLUPS, mean launch ms: %s"
         % (t32 / args.steps, mlups32, {k: round(v, 4) for k, v in k32.items() if v}))

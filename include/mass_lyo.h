@@ -183,9 +183,10 @@ int mass_lyo_synchronize(mass_lyo_t h);
 
 /* Timing: runs nsteps steps bracketed by HIP events on the handle's stream
  * and returns the elapsed milliseconds.  When sweep_ms is not NULL, every
- * sweep launch is also bracketed by its own event pair; sweep_ms (3 doubles)
- * gets the mean time of a predictor sweep, of a separate corrector sweep and
- * of a fused last-corrector-pair launch (0 where there was none). */
+ * sweep launch is also bracketed by its own event pair; sweep_ms (4 doubles)
+ * gets the mean time of a predictor sweep, of a separate corrector sweep, of
+ * a fused last-corrector-pair launch and of a one-kernel step (0 where there
+ * was none). */
 int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms,
                         double* sweep_ms);
 
@@ -200,12 +201,16 @@ int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_bloc
  * on (the default) or 0 off (separate sweeps); pair_chunk_rows: rows per
  * wave chunk of that kernel (0 keeps); alternate: neighbouring chunks march
  * in opposite directions so that they reach their shared rows together, 1 on
- * (the default) or 0 off (every chunk upwards).  -1 keeps a setting.  Results are bitwise the
- * same either way; derived fields of a frame are recomputed on demand after a
- * fused step.  No reference counterpart (Lyotropic::Step,
- * src/models/lyotropic.cpp:500-525, runs its correctors one after another). */
+ * (the default) or 0 off (every chunk upwards); one_kernel_step: FP32 with
+ * npc = 2 (and the pair fused) runs the whole step -- predictor and both
+ * correctors -- in one kernel, 1 on (the default) or 0 off; step_chunk_rows:
+ * rows per wave chunk of that kernel (0 keeps).  -1 keeps a setting.  Results
+ * are bitwise the same either way; derived fields of a frame are recomputed on
+ * demand after a fused step.  No reference counterpart (Lyotropic::Step,
+ * src/models/lyotropic.cpp:500-525, runs its sweeps one after another). */
 int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chunk_rows,
-                          int32_t alternate);
+                          int32_t alternate, int32_t one_kernel_step,
+                          int32_t step_chunk_rows);
 
 /* ------------------------------------------------------------------ batches
  * Many independent runs (own size and parameters; shared npc and precision)

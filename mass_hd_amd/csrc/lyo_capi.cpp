@@ -114,6 +114,13 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
   int xc2 = 64;
   while (xc2 > 4 && (long long)nstrips2 * ((p->LX + xc2 - 1) / xc2) < 16384) xc2 >>= 1;
   h->launch.xc2 = xc2;
+  // the one-kernel step runs two waves per SIMD and repeats eight rows per
+  // chunk: long chunks while there are ~4 waves per SIMD (measured at 4096^2:
+  // 64 rows > 48 > 32 > 16)
+  const int nstrips3 = (p->LY + lyo::kFullStrip - 1) / lyo::kFullStrip;
+  int xc3 = 64;
+  while (xc3 > 8 && (long long)nstrips3 * ((p->LX + xc3 - 1) / xc3) < 4096) xc3 >>= 1;
+  h->launch.xc3 = xc3;
   *out = h.release();
   return MASS_LYO_OK;
 }
@@ -390,8 +397,8 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms, double* 
   float ms = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
   if (e == hipSuccess && sweep_ms) {
-    double acc[3] = {0, 0, 0};
-    int cnt[3] = {0, 0, 0};
+    double acc[4] = {0, 0, 0, 0};
+    int cnt[4] = {0, 0, 0, 0};
     for (size_t i = 0; i < tm.used / 2 && e == hipSuccess; ++i) {
       float m = 0;
       e = hipEventElapsedTime(&m, tm.ev[2 * i], tm.ev[2 * i + 1]);
@@ -401,6 +408,7 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms, double* 
     sweep_ms[0] = cnt[0] ? acc[0] / cnt[0] : 0;
     sweep_ms[1] = cnt[1] ? acc[1] / cnt[1] : 0;
     sweep_ms[2] = cnt[2] ? acc[2] / cnt[2] : 0;
+    sweep_ms[3] = cnt[3] ? acc[3] / cnt[3] : 0;
   }
   for (auto& ev : tm.ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -425,15 +433,19 @@ int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_bloc
 }
 
 int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chunk_rows,
-                          int32_t alternate) {
+                          int32_t alternate, int32_t one_kernel_step, int32_t step_chunk_rows) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (fuse_last_pair < -1 || fuse_last_pair > 1 || pair_chunk_rows < 0 ||
-      pair_chunk_rows > 4096 || alternate < -1 || alternate > 1)
+      pair_chunk_rows > 4096 || alternate < -1 || alternate > 1 || one_kernel_step < -1 ||
+      one_kernel_step > 1 || step_chunk_rows < 0 || step_chunk_rows > 4096)
     return fail(h, MASS_LYO_EINVAL,
-                "fuse_last_pair and alternate in {-1,0,1}, pair_chunk_rows in [0,4096]");
+                "fuse_last_pair, alternate and one_kernel_step in {-1,0,1}, chunk rows in "
+                "[0,4096]");
   if (fuse_last_pair >= 0) h->launch.fuse = fuse_last_pair;
   if (pair_chunk_rows) h->launch.xc2 = pair_chunk_rows;
   if (alternate >= 0) h->launch.alt = alternate;
+  if (one_kernel_step >= 0) h->launch.full = one_kernel_step;
+  if (step_chunk_rows) h->launch.xc3 = step_chunk_rows;
   return MASS_LYO_OK;
 }
 

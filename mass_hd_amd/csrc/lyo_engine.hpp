@@ -12,6 +12,7 @@
 
 #include "lyo_aux.hpp"
 #include "lyo_diag.hpp"
+#include "lyo_full.hpp"
 #include "lyo_net_engine.hpp"
 #include "lyo_pair.hpp"
 #include "lyo_sweep.hpp"
@@ -50,6 +51,7 @@ struct Bufs {
   // stored; they are recomputed on demand from the predictor outputs and the
   // old state S[lazy_cur], which stay intact until the next step
   int lazy_cur = -1;
+  bool lazy_full = false;  // ... and the predictor's outputs too (one-kernel step)
 };
 
 struct LaunchCfg {
@@ -59,24 +61,27 @@ struct LaunchCfg {
   int fuse = 1;  // last two correctors in one kernel (npc >= 2)
   int xc2 = 16;  // rows per wave chunk of the fused pair
   int alt = 1;   // alternate the march direction between neighbouring chunks
+  int full = 1;  // FP32, npc = 2: the whole step in one kernel
+  int xc3 = 32;  // rows per wave chunk of the one-kernel step
 };
 
 // Optional per-sweep event timing.
 struct SweepTimer {
   std::vector<hipEvent_t> ev;  // pairs
-  std::vector<int> kind;       // 0 predictor, 1 corrector, 2 fused pair
+  std::vector<int> kind;       // 0 predictor, 1 corrector, 2 fused pair, 3 whole step
   size_t used = 0;
 };
 
-inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, bool pair = false) {
+// kind 0: sweep, 1: fused pair, 2: one-kernel step (strip width and chunk rows)
+inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, int kind = 0) {
   Geo g;
   g.LX = LX;
   g.LY = LY;
   g.N = LX * LY;
   g.NP = NP;
-  const int strip = pair ? kPairStrip : kStrip;
+  const int strip = kind == 2 ? kFullStrip : (kind == 1 ? kPairStrip : kStrip);
   g.nstrips = (LY + strip - 1) / strip;
-  g.xc = pair ? L.xc2 : L.xc;
+  g.xc = kind == 2 ? L.xc3 : (kind == 1 ? L.xc2 : L.xc);
   g.nchunks = (LX + g.xc - 1) / g.xc;
   g.nwaves = g.nstrips * g.nchunks;
   g.xcd = L.xcd;
@@ -112,6 +117,22 @@ inline hipError_t timed_pair(const SweepArgs<T>& a, const Params<T>& p, const Ge
     (void)hipEventRecord(e0, s);
   }
   hipError_t err = launch_pair<T, EXACT, HB>(a, p, g, L.wpb, s);
+  if (e1) (void)hipEventRecord(e1, s);
+  return err;
+}
+
+template <class T>
+inline hipError_t timed_full(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
+                             const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (tm && tm->used + 2 <= tm->ev.size()) {
+    e0 = tm->ev[tm->used];
+    e1 = tm->ev[tm->used + 1];
+    tm->kind[tm->used / 2] = 3;
+    tm->used += 2;
+    (void)hipEventRecord(e0, s);
+  }
+  hipError_t err = launch_full<T>(a, p, g, L.wpb, s);
   if (e1) (void)hipEventRecord(e1, s);
   return err;
 }
@@ -237,11 +258,39 @@ hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int
                      int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   const Geo g = make_geo(LX, LY, NP, L);
   const bool fuse = L.fuse && npc >= 2 && LX >= 8;  // the pair kernel wraps rows once
-  const Geo g2 = make_geo(LX, LY, NP, L, true);
+  const Geo g2 = make_geo(LX, LY, NP, L, 1);
+  const bool full = !EXACT && L.full && fuse && npc == 2;
+  const Geo g3 = make_geo(LX, LY, NP, L, 2);
   const bool store_half = !EXACT && pred_stores_half(npc, fuse);  // EXACT always stores
   for (int st = 0; st < nsteps; ++st) {
     const int cur = B.cur;
     B.lazy_cur = -1;
+    B.lazy_full = false;
+    if constexpr (!EXACT) {
+      if (full) {
+        // state S[cur] -> S[next]; the outputs of the predictor and of the first
+        // corrector stay in registers
+        SweepArgs<T> a{};
+        const SweepArgs<T> b = stage_args<T, EXACT>(B, npc, npc, cur);
+        a.f_in = B.Sf[cur];
+        a.qxx_in = B.Sqxx[cur];
+        a.qyx_in = B.Sqyx[cur];
+        a.phi_in = B.Sphi[cur];
+        a.phic_in = B.Sphic[cur];
+        a.f_out = b.f_out;
+        a.qxx_out = b.qxx_out;
+        a.qyx_out = b.qyx_out;
+        a.phi_out = b.phi_out;
+        a.phic_out = b.phic_out;
+        const hipError_t e = timed_full<T>(a, P, g3, L, s, tm);
+        if (e != hipSuccess) return e;
+        B.have_uq = false;
+        B.lazy_cur = cur;
+        B.lazy_full = true;
+        B.cur = cur ^ 1;
+        continue;
+      }
+    }
     for (int c = 0; c <= npc; ++c) {
       hipError_t e;
       if (fuse && c == npc - 1) {
@@ -285,12 +334,20 @@ hipError_t materialize_uq(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP
   if (B.lazy_cur < 0) return hipSuccess;
   const int cur = B.lazy_cur;
   const LaunchCfg L;
+  if (B.lazy_full) {
+    // the one-kernel step kept the predictor's outputs in registers as well:
+    // rerun the predictor sweep first (npc = 2: no half-step planes)
+    const hipError_t e = launch_sweep<T, EXACT, PRED, false>(
+        stage_args<T, EXACT>(B, npc, 0, cur), P, make_geo(LX, LY, NP, L), L.wpb, s);
+    if (e != hipSuccess) return e;
+  }
   const hipError_t e = launch_corrector<T, EXACT>(stage_args<T, EXACT>(B, npc, npc - 1, cur), P,
                                                   make_geo(LX, LY, NP, L), L, s, nullptr,
                                                   npc - 1, false);
   if (e != hipSuccess) return e;
   note_last_uq(B, stage_args<T, EXACT>(B, npc, npc, cur));
   B.lazy_cur = -1;
+  B.lazy_full = false;
   return hipSuccess;
 }
 
@@ -304,6 +361,7 @@ hipError_t pack_state(Bufs<T>& B, const double* d_ff, const double* d_qxx, const
                      rho);
   B.have_uq = false;
   B.lazy_cur = -1;
+  B.lazy_full = false;
   return hipGetLastError();
 }
 
@@ -541,6 +599,7 @@ class EngineT final : public Engine {
     const int last_cur = B_.cur ^ ((nsteps - 1) & 1);
     note_last_uq(B_, stage_args<T, EXACT>(B_, npc_, npc_, last_cur));
     B_.lazy_cur = -1;
+    B_.lazy_full = false;
     B_.cur ^= (nsteps & 1);
   }
   int parity() const override { return B_.cur; }

@@ -102,7 +102,8 @@ def load_library():
     lib.mass_lyo_synchronize.argtypes = [h]
     lib.mass_lyo_time_steps.argtypes = [h, ctypes.c_int32, dp, dp]
     lib.mass_lyo_set_launch.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
-    lib.mass_lyo_set_schedule.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    lib.mass_lyo_set_schedule.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_int32]
     lib.mass_lyo_abi_version.argtypes = []
     lib.mass_lyo_diagnostics.argtypes = [h, P(Diag), dp]
     i32 = ctypes.c_int32
@@ -242,23 +243,27 @@ class Lyotropic:
 
     def time_steps(self, nsteps, per_sweep=True):
         tot = ctypes.c_double()
-        sw = (ctypes.c_double * 3)()
+        sw = (ctypes.c_double * 4)()
         self._check(self.lib.mass_lyo_time_steps(self.h, int(nsteps), ctypes.byref(tot),
                                                  sw if per_sweep else None))
-        return tot.value, (sw[0], sw[1], sw[2])
+        return tot.value, (sw[0], sw[1], sw[2], sw[3])
 
     def set_launch(self, chunk_rows=0, waves_per_block=0, xcd_order=-1):
         self._check(self.lib.mass_lyo_set_launch(self.h, int(chunk_rows), int(waves_per_block),
                                                  int(xcd_order)))
 
-    def set_schedule(self, fuse_last_pair=-1, pair_chunk_rows=0, alternate=-1):
+    def set_schedule(self, fuse_last_pair=-1, pair_chunk_rows=0, alternate=-1,
+                     one_kernel_step=-1, step_chunk_rows=0):
         """fuse_last_pair 1: the last two correctors run as one kernel
         (default), 0: separate sweeps; pair_chunk_rows: rows per wave chunk of
         the fused kernel (0 keeps); alternate 1: neighbouring chunks march in
-        opposite directions (default), 0: all upwards.  -1 keeps a setting.
-        Results are bitwise identical either way."""
+        opposite directions (default), 0: all upwards; one_kernel_step 1: FP32
+        npc=2 runs predictor and both correctors in one kernel (default), 0:
+        predictor + fused pair; step_chunk_rows: its rows per wave chunk.  -1
+        keeps a setting.  Results are bitwise identical either way."""
         self._check(self.lib.mass_lyo_set_schedule(self.h, int(fuse_last_pair),
-                                                   int(pair_chunk_rows), int(alternate)))
+                                                   int(pair_chunk_rows), int(alternate),
+                                                   int(one_kernel_step), int(step_chunk_rows)))
 
 
 class Batch:

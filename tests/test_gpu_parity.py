@@ -103,11 +103,12 @@ def test_fused_pair_equals_separate_sweeps(require_gpu, precision, npc, LX, LY, 
     fused path recomputes on demand); asking for derived fields mid-run does
     not disturb the run."""
     runs = []
-    # (fused, march order): separate sweeps, all chunks upwards or alternating;
-    # fused, alternating
-    for fuse, alt in ((0, 0), (0, 1), (1, 1)):
+    # (fused pair, march order, one-kernel step): separate sweeps, all chunks
+    # upwards or alternating; fused pair, alternating; whole step in one kernel
+    # (FP32, npc = 2; otherwise the fused pair)
+    for fuse, alt, full in ((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 1, 1)):
         g = lyo.Lyotropic(LX, LY, npc=npc, precision=precision, **CONFIG0_PHYS)
-        g.set_schedule(fuse, xc2, alt)
+        g.set_schedule(fuse, xc2, alt, full, xc2)
         g.configure(**CONFIG0_INIT)
         g.step(4)
         mid = [g.get(n) for n in ("ux", "sigmaXY", "MU")]
@@ -123,8 +124,11 @@ def test_fused_pair_equals_separate_sweeps(require_gpu, precision, npc, LX, LY, 
 
 
 def test_fused_pair_timing_kinds(require_gpu):
-    for npc, kinds in ((1, (1, 1, 0)), (2, (1, 0, 1)), (3, (1, 1, 1))):
+    # (predictor, separate corrector, fused pair, one-kernel step)
+    for npc, full, kinds in ((1, 1, (1, 1, 0, 0)), (2, 0, (1, 0, 1, 0)), (2, 1, (0, 0, 0, 1)),
+                             (3, 1, (1, 1, 1, 0))):
         g = lyo.Lyotropic(64, 64, npc=npc, precision=lyo.FP32, **CONFIG0_PHYS)
+        g.set_schedule(one_kernel_step=full)
         g.configure(**CONFIG0_INIT)
         _, ms = g.time_steps(2)
         assert tuple(int(m > 0) for m in ms) == kinds, (npc, ms)
