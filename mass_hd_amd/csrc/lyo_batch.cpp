@@ -1,13 +1,16 @@
 // lyo_batch.cpp -- batched runs (C ABI, include/mass_lyo.h).
 //
 // A batch holds many independent lyotropic runs (any sizes, own parameters)
-// that advance together: each sweep of a step is ONE persistent launch over
-// the work items of all runs (sweep_batch_kernel), and two consecutive steps
-// (both state parities) are captured once into a HIP graph and replayed.
+// that advance together: each stage of a step's launch plan (the one-kernel
+// step, or predictor + fused pair, or separate sweeps) is ONE persistent
+// launch over the work items of all runs (full/pair/sweep_batch_kernel), and
+// two consecutive steps (both state parities) are captured once into a HIP
+// graph and replayed.
 // Per-run operations (configure, fields, sums, diagnostics) reuse the single
 // run handles; they first wait for the batch stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -20,11 +23,13 @@ struct mass_lyo_batch_s {
   int npc = 1, precision = 0;
   lyo::LaunchCfg launch;
   int grid_waves = 0;      // persistent grid size in waves (0: one wave per item)
+  std::vector<int> plan;   // stages of one step (lyo::step_plan)
+  bool store_half = false;
   std::vector<char> jobs;  // [parity][stage][run] SweepJob bytes
   void* d_jobs = nullptr;
   size_t d_jobs_bytes = 0;
   size_t job_bytes = 0;
-  int nitems = 0;
+  std::vector<int> nitems;  // work items per stage
   bool jobs_valid = false;
   hipGraphExec_t graph = nullptr;
   std::string err;
@@ -57,21 +62,28 @@ void drop_graph(mass_lyo_batch_t b) {
 int build_jobs(mass_lyo_batch_t b) {
   if (b->jobs_valid) return MASS_LYO_OK;
   drop_graph(b);
-  const int R = (int)b->runs.size(), S = b->npc + 1;
+  const int R = (int)b->runs.size();
+  int lxmin = b->runs[0]->prm.LX;
+  for (auto* h : b->runs) lxmin = std::min(lxmin, (int)h->prm.LX);
+  b->plan = b->runs[0]->eng->plan(b->launch, lxmin);
+  b->store_half = b->precision == MASS_LYO_FP32 && lyo::plan_stores_half(b->plan, b->npc);
+  const int S = (int)b->plan.size();
   b->job_bytes = b->runs[0]->eng->job_bytes();
   b->jobs.assign(2 * (size_t)S * R * b->job_bytes, 0);
+  b->nitems.assign(S, 0);
   std::vector<int> first(R);
-  int items = 0;
-  for (int r = 0; r < R; ++r) {
-    first[r] = items;
-    items += b->runs[r]->eng->job_items(b->launch);
-  }
-  b->nitems = items;
-  for (int p = 0; p < 2; ++p)
-    for (int c = 0; c < S; ++c)
+  for (int c = 0; c < S; ++c) {
+    int items = 0;
+    for (int r = 0; r < R; ++r) {
+      first[r] = items;
+      items += b->runs[r]->eng->job_items(b->plan[c], b->launch);
+    }
+    b->nitems[c] = items;
+    for (int p = 0; p < 2; ++p)
       for (int r = 0; r < R; ++r)
-        b->runs[r]->eng->fill_job(c, p, first[r], b->launch,
+        b->runs[r]->eng->fill_job(b->plan[c], p, first[r], b->launch,
                                   &b->jobs[(((size_t)p * S + c) * R + r) * b->job_bytes]);
+  }
   hipError_t e = hipSuccess;
   if (b->d_jobs_bytes < b->jobs.size()) {
     if (b->d_jobs) (void)hipFree(b->d_jobs);
@@ -88,11 +100,11 @@ int build_jobs(mass_lyo_batch_t b) {
 }
 
 hipError_t launch_step(mass_lyo_batch_t b, int parity) {
-  const int R = (int)b->runs.size(), S = b->npc + 1;
+  const int R = (int)b->runs.size(), S = (int)b->plan.size();
   for (int c = 0; c < S; ++c) {
     const char* j = static_cast<const char*>(b->d_jobs) + ((size_t)parity * S + c) * R * b->job_bytes;
-    hipError_t e = b->runs[0]->eng->launch_batch(j, R, b->nitems, c, b->grid_waves, b->launch.wpb,
-                                                 b->stream);
+    hipError_t e = b->runs[0]->eng->launch_batch(j, R, b->nitems[c], b->plan[c], b->store_half,
+                                                 b->grid_waves, b->launch.wpb, b->stream);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -139,11 +151,10 @@ int mass_lyo_batch_create(const mass_lyo_params* runs, int32_t nruns, mass_lyo_b
     }
     b->runs.push_back(h);
   }
-  // default: the chunk length a single run of the mean size would pick, one
-  // wave per work item
-  long long items = 0;
-  for (auto* h : b->runs) items += (long long)h->N;
+  // defaults (autotune picks better ones): one wave per work item
   b->launch.xc = 8;
+  b->launch.xc2 = 16;
+  b->launch.xc3 = 32;
   b->launch.wpb = 4;
   b->launch.xcd = 0;
   *out = b;
@@ -205,7 +216,7 @@ int mass_lyo_batch_set_launch(mass_lyo_batch_t b, int32_t chunk_rows, int32_t gr
     return bfail(b, MASS_LYO_EINVAL, "chunk_rows must be in [1,4096] and grid_waves >= 0");
   int rc = sync_batch(b);
   if (rc) return rc;
-  if (chunk_rows) b->launch.xc = chunk_rows;
+  if (chunk_rows) b->launch.xc = b->launch.xc2 = b->launch.xc3 = chunk_rows;
   b->grid_waves = grid_waves;
   b->jobs_valid = false;
   return MASS_LYO_OK;
@@ -240,7 +251,7 @@ int mass_lyo_batch_step(mass_lyo_batch_t b, int32_t nsteps) {
   for (; e == hipSuccess && left >= 2; left -= 2) e = hipGraphLaunch(b->graph, b->stream);
   if (e == hipSuccess && left == 1) e = launch_step(b, 0);
   if (e != hipSuccess) return bhip(b, e, "batch step");
-  for (auto* h : b->runs) h->eng->after_batch_steps(nsteps);
+  for (auto* h : b->runs) h->eng->after_batch_steps(nsteps, b->plan);
   return MASS_LYO_OK;
 }
 
@@ -281,8 +292,13 @@ int mass_lyo_batch_autotune(mass_lyo_batch_t b, int32_t* chunk_rows, int32_t* gr
     if (h->has_state)
       return bfail(b, MASS_LYO_EINVAL,
                    "autotune runs on the zeroed state of a new batch: call it before configure");
-  const int xcs[] = {4, 8, 16, 32};
-  const int grids[] = {0, 4096, 8192, 16384};
+  int rc0 = build_jobs(b);
+  if (rc0) return rc0;
+  // chunk rows per stage kind: the one-kernel step repeats eight rows per
+  // chunk, the sweeps two
+  const bool full = b->plan.size() == 1 && b->plan[0] == lyo::kStageFull;
+  const std::vector<int> xcs = full ? std::vector<int>{16, 32, 64} : std::vector<int>{4, 8, 16, 32};
+  const int grids[] = {0, 2048, 4096, 8192, 16384};
   double best = 1e300;
   int bx = b->launch.xc, bg = b->grid_waves;
   for (int xc : xcs)

@@ -234,11 +234,67 @@ SweepArgs<T> pair_args(const Bufs<T>& B, int npc, int cur) {
   return a;
 }
 
-// Which FP32 launches use the half-step shortcut (SweepArgs, flag HB): the
-// first corrector rebuilds fn and QN from its own inputs; the predictor stores
-// them only for a corrector c >= 2 that runs as a separate sweep (or as stage A
-// of a fused pair); the fused pair rebuilds them when its stage A is corrector 1.
-inline bool pred_stores_half(int npc, bool fuse) { return npc >= 3 || (npc == 2 && !fuse); }
+// Launch plan of one step (Lyotropic::Step, src/models/lyotropic.cpp:500-525):
+// a list of stages, each a separate sweep c (0 = predictor, 1..npc the
+// correctors), the fused last corrector pair, or the whole step in one kernel
+// (FP32, npc = 2).  Single runs and batches follow the same plan, so a batched
+// run equals the same run alone bit for bit.
+constexpr int kStagePair = 100;
+constexpr int kStageFull = 200;
+
+inline std::vector<int> step_plan(bool exact, int npc, const LaunchCfg& L, int LX) {
+  const bool fuse = L.fuse && npc >= 2 && LX >= 8;  // the fused kernels wrap rows once
+  std::vector<int> p;
+  if (!exact && fuse && L.full && npc == 2) {
+    p.push_back(kStageFull);
+    return p;
+  }
+  for (int c = 0; c <= npc; ++c) {
+    if (fuse && c == npc - 1) {
+      p.push_back(kStagePair);
+      break;
+    }
+    p.push_back(c);
+  }
+  return p;
+}
+
+// FP32 half-step shortcut (SweepArgs, flag HB): the first corrector rebuilds fn
+// and QN from its own inputs; the predictor stores them only when a later
+// corrector loads them (a separate sweep c >= 2, or stage A of a pair with
+// npc >= 3); the fused pair rebuilds them when its stage A is corrector 1.
+inline bool plan_stores_half(const std::vector<int>& p, int npc) {
+  for (int sc : p)
+    if ((sc >= 2 && sc <= npc) || (sc == kStagePair && npc >= 3)) return true;
+  return false;
+}
+
+inline int stage_geo_kind(int sc) { return sc == kStageFull ? 2 : (sc == kStagePair ? 1 : 0); }
+
+// Arguments of the one-kernel step: state S[cur] in, S[next] out.
+template <class T, bool EXACT>
+SweepArgs<T> full_args(const Bufs<T>& B, int npc, int cur) {
+  SweepArgs<T> a{};
+  const SweepArgs<T> b = stage_args<T, EXACT>(B, npc, npc, cur);
+  a.f_in = B.Sf[cur];
+  a.qxx_in = B.Sqxx[cur];
+  a.qyx_in = B.Sqyx[cur];
+  a.phi_in = B.Sphi[cur];
+  a.phic_in = B.Sphic[cur];
+  a.f_out = b.f_out;
+  a.qxx_out = b.qxx_out;
+  a.qyx_out = b.qyx_out;
+  a.phi_out = b.phi_out;
+  a.phic_out = b.phic_out;
+  return a;
+}
+
+template <class T, bool EXACT>
+SweepArgs<T> plan_args(const Bufs<T>& B, int npc, int sc, int cur) {
+  if (sc == kStageFull) return full_args<T, EXACT>(B, npc, cur);
+  if (sc == kStagePair) return pair_args<T, EXACT>(B, npc, cur);
+  return stage_args<T, EXACT>(B, npc, sc, cur);
+}
 
 template <class T, bool EXACT>
 hipError_t launch_corrector(const SweepArgs<T>& a, const Params<T>& P, const Geo& g,
@@ -253,73 +309,63 @@ hipError_t launch_corrector(const SweepArgs<T>& a, const Params<T>& P, const Geo
               : timed_launch<T, EXACT, CORR, false>(a, P, g, L, s, tm);
 }
 
+// One stage of a single run.
+template <class T, bool EXACT>
+hipError_t launch_stage(int sc, const SweepArgs<T>& a, const Params<T>& P, const Geo& g,
+                        const LaunchCfg& L, int npc, bool store_half, hipStream_t s,
+                        SweepTimer* tm) {
+  if (sc == kStageFull) {
+    if constexpr (!EXACT) return timed_full<T>(a, P, g, L, s, tm);
+    return hipErrorInvalidValue;
+  }
+  if (sc == kStagePair) {
+    if constexpr (!EXACT) {
+      if (npc == 2) return timed_pair<T, EXACT, true>(a, P, g, L, s, tm);
+    }
+    return timed_pair<T, EXACT, false>(a, P, g, L, s, tm);
+  }
+  if (sc == 0) {
+    if constexpr (!EXACT) {
+      if (store_half) return timed_launch<T, EXACT, PRED, true>(a, P, g, L, s, tm);
+    }
+    return timed_launch<T, EXACT, PRED, false>(a, P, g, L, s, tm);
+  }
+  return launch_corrector<T, EXACT>(a, P, g, L, s, tm, sc, sc == npc);
+}
+
+// Bookkeeping after a step whose state was S[cur]: where the inputs of the
+// last UpdateQuantities are (frame fields, surrogate), or that they have to be
+// recomputed because a fused stage kept them in registers.
+template <class T, bool EXACT>
+void note_step(Bufs<T>& B, int npc, const std::vector<int>& plan, int cur) {
+  const int last = plan.back();
+  B.lazy_cur = -1;
+  B.lazy_full = false;
+  if (last == kStagePair || last == kStageFull) {
+    B.have_uq = false;
+    B.lazy_cur = cur;
+    B.lazy_full = last == kStageFull;
+  } else {
+    note_last_uq(B, stage_args<T, EXACT>(B, npc, npc, cur));
+  }
+}
+
 template <class T, bool EXACT>
 hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int npc,
                      int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
-  const Geo g = make_geo(LX, LY, NP, L);
-  const bool fuse = L.fuse && npc >= 2 && LX >= 8;  // the pair kernel wraps rows once
-  const Geo g2 = make_geo(LX, LY, NP, L, 1);
-  const bool full = !EXACT && L.full && fuse && npc == 2;
-  const Geo g3 = make_geo(LX, LY, NP, L, 2);
-  const bool store_half = !EXACT && pred_stores_half(npc, fuse);  // EXACT always stores
+  const std::vector<int> plan = step_plan(EXACT, npc, L, LX);
+  const bool store_half = !EXACT && plan_stores_half(plan, npc);  // EXACT always stores
+  Geo geo[3];
+  for (int k = 0; k < 3; ++k) geo[k] = make_geo(LX, LY, NP, L, k);
   for (int st = 0; st < nsteps; ++st) {
     const int cur = B.cur;
-    B.lazy_cur = -1;
-    B.lazy_full = false;
-    if constexpr (!EXACT) {
-      if (full) {
-        // state S[cur] -> S[next]; the outputs of the predictor and of the first
-        // corrector stay in registers
-        SweepArgs<T> a{};
-        const SweepArgs<T> b = stage_args<T, EXACT>(B, npc, npc, cur);
-        a.f_in = B.Sf[cur];
-        a.qxx_in = B.Sqxx[cur];
-        a.qyx_in = B.Sqyx[cur];
-        a.phi_in = B.Sphi[cur];
-        a.phic_in = B.Sphic[cur];
-        a.f_out = b.f_out;
-        a.qxx_out = b.qxx_out;
-        a.qyx_out = b.qyx_out;
-        a.phi_out = b.phi_out;
-        a.phic_out = b.phic_out;
-        const hipError_t e = timed_full<T>(a, P, g3, L, s, tm);
-        if (e != hipSuccess) return e;
-        B.have_uq = false;
-        B.lazy_cur = cur;
-        B.lazy_full = true;
-        B.cur = cur ^ 1;
-        continue;
-      }
-    }
-    for (int c = 0; c <= npc; ++c) {
-      hipError_t e;
-      if (fuse && c == npc - 1) {
-        const SweepArgs<T> a = pair_args<T, EXACT>(B, npc, cur);
-        if constexpr (EXACT) {
-          e = timed_pair<T, EXACT, false>(a, P, g2, L, s, tm);
-        } else {
-          e = (npc == 2) ? timed_pair<T, EXACT, true>(a, P, g2, L, s, tm)
-                         : timed_pair<T, EXACT, false>(a, P, g2, L, s, tm);
-        }
-        if (e != hipSuccess) return e;
-        B.have_uq = false;
-        B.lazy_cur = cur;
-        break;
-      }
-      const SweepArgs<T> a = stage_args<T, EXACT>(B, npc, c, cur);
-      if (c == 0) {
-        if constexpr (EXACT) {
-          e = timed_launch<T, EXACT, PRED, false>(a, P, g, L, s, tm);
-        } else {
-          e = store_half ? timed_launch<T, EXACT, PRED, true>(a, P, g, L, s, tm)
-                         : timed_launch<T, EXACT, PRED, false>(a, P, g, L, s, tm);
-        }
-      } else {
-        if (c == npc) note_last_uq(B, a);
-        e = launch_corrector<T, EXACT>(a, P, g, L, s, tm, c, c == npc);
-      }
+    for (int sc : plan) {
+      const hipError_t e = launch_stage<T, EXACT>(sc, plan_args<T, EXACT>(B, npc, sc, cur), P,
+                                                  geo[stage_geo_kind(sc)], L, npc, store_half, s,
+                                                  tm);
       if (e != hipSuccess) return e;
     }
+    note_step<T, EXACT>(B, npc, plan, cur);
     B.cur = cur ^ 1;
   }
   return hipSuccess;
@@ -465,16 +511,19 @@ class Engine {
   virtual hipError_t sums(double* d_part, int max_part, double* d_out, hipStream_t s) = 0;
   virtual hipError_t diagnostics(DiagScratch& d, double* w_out, hipStream_t s) = 0;
   virtual size_t bytes() const = 0;
-  // batched stepping: the job of sweep c at state parity cur (host bytes of a
-  // SweepJob<T>), the bookkeeping after n batched steps, and the launch of a
-  // batched sweep over device jobs
+  // batched stepping: the step plan for runs of at least LXmin rows, the
+  // job of stage sc at state parity cur (host bytes of a SweepJob<T>), the
+  // bookkeeping after n batched steps, and the launch of a batched stage over
+  // device jobs
+  virtual std::vector<int> plan(const LaunchCfg& L, int LXmin) const = 0;
   virtual size_t job_bytes() const = 0;
-  virtual int job_items(const LaunchCfg& L) const = 0;
-  virtual void fill_job(int c, int cur, int first, const LaunchCfg& L, void* job) const = 0;
-  virtual void after_batch_steps(int nsteps) = 0;
+  virtual int job_items(int sc, const LaunchCfg& L) const = 0;
+  virtual void fill_job(int sc, int cur, int first, const LaunchCfg& L, void* job) const = 0;
+  virtual void after_batch_steps(int nsteps, const std::vector<int>& plan) = 0;
   virtual int parity() const = 0;
-  virtual hipError_t launch_batch(const void* jobs, int njobs, int nitems, int c, int grid_waves,
-                                  int wpb, hipStream_t s) const = 0;
+  virtual hipError_t launch_batch(const void* jobs, int njobs, int nitems, int sc,
+                                  bool store_half, int grid_waves, int wpb,
+                                  hipStream_t s) const = 0;
   // surrogate corrector hooks (lyo_net_engine.hpp).  src 0: current state;
   // src 1: the one-corrector state of the last plain step (needs npc >= 2)
   virtual hipError_t net_features(int src, float* feat, hipStream_t s) = 0;
@@ -581,25 +630,26 @@ class EngineT final : public Engine {
   }
   size_t bytes() const override { return bytes_; }
 
-  size_t job_bytes() const override { return sizeof(SweepJob<T>); }
-  int job_items(const LaunchCfg& L) const override {
-    return make_geo(LX_, LY_, NP_, L).nwaves;
+  std::vector<int> plan(const LaunchCfg& L, int LXmin) const override {
+    return step_plan(EXACT, npc_, L, LXmin);
   }
-  void fill_job(int c, int cur, int first, const LaunchCfg& L, void* job) const override {
+  size_t job_bytes() const override { return sizeof(SweepJob<T>); }
+  int job_items(int sc, const LaunchCfg& L) const override {
+    return make_geo(LX_, LY_, NP_, L, stage_geo_kind(sc)).nwaves;
+  }
+  void fill_job(int sc, int cur, int first, const LaunchCfg& L, void* job) const override {
     SweepJob<T> j;
-    j.A = stage_args<T, EXACT>(B_, npc_, c, cur);
+    j.A = plan_args<T, EXACT>(B_, npc_, sc, cur);
     j.P = P_;
-    j.g = make_geo(LX_, LY_, NP_, L);
+    j.g = make_geo(LX_, LY_, NP_, L, stage_geo_kind(sc));
     j.g.xcd = 0;
     j.first = first;
     *static_cast<SweepJob<T>*>(job) = j;
   }
-  void after_batch_steps(int nsteps) override {
+  void after_batch_steps(int nsteps, const std::vector<int>& plan) override {
     if (nsteps <= 0) return;
     const int last_cur = B_.cur ^ ((nsteps - 1) & 1);
-    note_last_uq(B_, stage_args<T, EXACT>(B_, npc_, npc_, last_cur));
-    B_.lazy_cur = -1;
-    B_.lazy_full = false;
+    note_step<T, EXACT>(B_, npc_, plan, last_cur);
     B_.cur ^= (nsteps & 1);
   }
   int parity() const override { return B_.cur; }
@@ -668,16 +718,31 @@ class EngineT final : public Engine {
     return run_steps<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_run, nsteps, L, s, nullptr);
   }
   int npc() const override { return npc_; }
-  hipError_t launch_batch(const void* jobs, int njobs, int nitems, int c, int grid_waves, int wpb,
-                          hipStream_t s) const override {
+  hipError_t launch_batch(const void* jobs, int njobs, int nitems, int c, bool store_half,
+                          int grid_waves, int wpb, hipStream_t s) const override {
     const SweepJob<T>* J = static_cast<const SweepJob<T>*>(jobs);
     const int waves = grid_waves > 0 && grid_waves < nitems ? grid_waves : nitems;
     const int blocks = (waves + wpb - 1) / wpb;
-    // batched steps run every corrector as its own sweep
     const dim3 grid(blocks), block(64 * wpb);
     constexpr bool HB = !EXACT;  // EXACT: the flag is ignored, one instance
+    if (c == kStageFull) {
+      if constexpr (!EXACT) {
+        const size_t lds = (size_t)wpb * kRingFloat4 * sizeof(float4);
+        hipLaunchKernelGGL((full_batch_kernel<T>), grid, block, lds, s, J, njobs, nitems);
+        return hipGetLastError();
+      }
+      return hipErrorInvalidValue;
+    }
+    if (c == kStagePair) {
+      if (HB && npc_ == 2)
+        hipLaunchKernelGGL((pair_batch_kernel<T, EXACT, HB>), grid, block, 0, s, J, njobs, nitems);
+      else
+        hipLaunchKernelGGL((pair_batch_kernel<T, EXACT, false>), grid, block, 0, s, J, njobs,
+                           nitems);
+      return hipGetLastError();
+    }
     if (c == 0) {
-      if (HB && pred_stores_half(npc_, false))
+      if (HB && store_half)
         hipLaunchKernelGGL((sweep_batch_kernel<T, EXACT, PRED, HB>), grid, block, 0, s, J, njobs,
                            nitems);
       else

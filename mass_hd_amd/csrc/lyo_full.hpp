@@ -381,6 +381,23 @@ full_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
   full_item<T>(A, P, g, wid, ring + ((int)threadIdx.x >> 6) * kRingFloat4);
 }
 
+// Batched one-kernel steps: persistent grid over the work items of many runs
+// (waves per block * kRingFloat4 float4 of dynamic LDS).
+template <class T>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(LYO_FULL_WAVES_PER_EU)))
+full_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const int nitems) {
+  extern __shared__ float4 ring[];
+  float4* r = ring + ((int)threadIdx.x >> 6) * kRingFloat4;
+  const int nw = (int)(gridDim.x * (blockDim.x >> 6));
+  int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x >> 6) +
+                                            ((int)threadIdx.x >> 6));
+  for (; item < nitems; item += nw) {
+    const SweepJob<T>& J = find_job(jobs, njobs, item);
+    full_item<T>(J.A, J.P, J.g, item - J.first, r);
+  }
+}
+
 template <class T>
 inline hipError_t launch_full(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                               int waves_per_block, hipStream_t s) {

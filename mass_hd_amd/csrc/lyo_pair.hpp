@@ -314,6 +314,20 @@ pair_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
   pair_item<T, EXACT, HB>(A, P, g, wid);
 }
 
+// Batched fused pairs: persistent grid over the work items of many runs.
+template <class T, bool EXACT, bool HB>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? LYO_PAIR_WAVES_PER_EU : 1)))
+pair_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const int nitems) {
+  const int nw = (int)(gridDim.x * (blockDim.x >> 6));
+  int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x >> 6) +
+                                            ((int)threadIdx.x >> 6));
+  for (; item < nitems; item += nw) {
+    const SweepJob<T>& J = find_job(jobs, njobs, item);
+    pair_item<T, EXACT, HB>(J.A, J.P, J.g, item - J.first);
+  }
+}
+
 template <class T, bool EXACT, bool HB>
 inline hipError_t launch_pair(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                               int waves_per_block, hipStream_t s) {

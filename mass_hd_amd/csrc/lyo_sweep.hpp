@@ -445,9 +445,23 @@ struct SweepJob {
   int first;
 };
 
+// Job of a global work item: binary search over the jobs' first items (scalar
+// loads; item is wave-uniform).
+template <class T>
+__device__ __forceinline__ const SweepJob<T>& find_job(const SweepJob<T>* __restrict__ jobs,
+                                                       const int njobs, const int item) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return jobs[lo];
+}
+
 // Persistent grid: each wave walks the global item list with a stride of the
-// whole grid, finds the job of its item (binary search over the jobs' first
-// items, scalar loads) and runs it.  Every wave reaches the end of the list.
+// whole grid, finds the job of its item and runs it.  Every wave reaches the
+// end of the list.
 template <class T, bool EXACT, int STAGE, bool HB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? LYO_SWEEP_WAVES_PER_EU : 1)))
@@ -456,13 +470,7 @@ sweep_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const 
   int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x >> 6) +
                                             ((int)threadIdx.x >> 6));
   for (; item < nitems; item += nw) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].first <= item) lo = mid;
-      else hi = mid - 1;
-    }
-    const SweepJob<T>& J = jobs[lo];
+    const SweepJob<T>& J = find_job(jobs, njobs, item);
     sweep_item<T, EXACT, STAGE, HB>(J.A, J.P, J.g, item - J.first);
   }
 }
