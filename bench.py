@@ -216,7 +216,9 @@ def main():
                      "distinct_bytes_per_launch": distinct,
                      "step_alg_GBps": round((1 + npc) * SWEEP_VALUES * 4 * N * args.steps
                                             / (t32 / 1e3) / 1e9, 1),
-                     "launch_ms": {k: round(v, 4) for k, v in k32.items() if v}},
+                     "launch_ms": {k: round(v, 4) for k, v in k32.items() if v},
+                     "traffic_GBps": (round(traffic / (k32[dom] / 1e3) / 1e9, 1)
+                                      if traffic else None)},
         "cpu_baseline": cpu,
     }
     line.update(extra)

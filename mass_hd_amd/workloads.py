@@ -124,6 +124,8 @@ def compare(L, net, steps, zeta=None, seed=7, judge_every=50, tol=1e-3, every=No
             done += k
             d = m.diagnostics()
             rows.append((done, d["u_rms"], d["defect_density"]))
+            log("  %s %d: u_rms %.4g, defect density %.4g (%.0f MLUPS)" %
+                (mode, done, d["u_rms"], d["defect_density"], L * L * done / (t_ms / 1e3) / 1e6))
         timing[mode] = {"steps": n, "ms": t_ms, "mlups": L * L * n / (t_ms / 1e3) / 1e6}
         if mode == "surrogate":
             timing[mode]["stats"] = lyo.surrogate_stats(m)
@@ -139,6 +141,8 @@ def main(argv=None):
     ap.add_argument("what", choices=["config1", "config2", "config3", "config4"])
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--iters", type=int, default=2000)
+    ap.add_argument("--fp64-steps", type=int, default=10000)
+    ap.add_argument("--every", type=int, default=1000)
     ap.add_argument("--out", default="")
     a = ap.parse_args(argv)
     if a.what == "config1":
@@ -151,7 +155,7 @@ def main(argv=None):
             r = {"training": tr, "run": compare(512, net, a.steps or 2000)}
         else:
             r = {"training": tr, "run": compare(4096, net, a.steps or 100000, zeta=0.03, seed=7,
-                                                every=1000, fp64_steps=10000)}
+                                                every=a.every, fp64_steps=a.fp64_steps)}
     txt = json.dumps(r, indent=1)
     if a.out:
         open(a.out, "w").write(txt)
