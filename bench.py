@@ -173,12 +173,13 @@ def main():
     alg_bytes = SWEEPS[dom] * SWEEP_VALUES * 4 * N
     achieved = alg_bytes / (k32[dom] / 1e3)
     distinct = DISTINCT[dom] * 4 * N
-    traffic = None
+    traffic = valu_ms = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
         if t.get("size") == L and t.get("npc") == npc and dom in t.get("kernels", {}):
             traffic = t["kernels"][dom].get("bytes_per_launch")
+            valu_ms = t["kernels"][dom].get("valu_issue_ms")
 
     cpu = None
     if not args.skip_cpu:
@@ -218,7 +219,10 @@ def main():
                                             / (t32 / 1e3) / 1e9, 1),
                      "launch_ms": {k: round(v, 4) for k, v in k32.items() if v},
                      "traffic_GBps": (round(traffic / (k32[dom] / 1e3) / 1e9, 1)
-                                      if traffic else None)},
+                                      if traffic else None),
+                     # the fused step is bound by VALU issue: profiled VALU
+                     # instructions x 4 cycles over the measured launch time
+                     "valu_busy_frac": (round(valu_ms / k32[dom], 3) if valu_ms else None)},
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -358,11 +358,10 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
       load_old(x - 3 * D, bold);
     }
   };
-  if (g.alt && !(chunk & 1)) {
-    march(std::integral_constant<int, -1>{});
-  } else {
-    march(std::integral_constant<int, 1>{});
-  }
+  // every chunk marches upwards: the step is bound by VALU issue, and one
+  // direction keeps half the code in the instruction cache (alternating
+  // directions measured no faster here)
+  march(std::integral_constant<int, 1>{});
 }
 
 template <class T>
