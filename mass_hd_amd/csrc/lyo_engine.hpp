@@ -121,7 +121,7 @@ inline hipError_t timed_pair(const SweepArgs<T>& a, const Params<T>& p, const Ge
   return err;
 }
 
-template <class T>
+template <class T, bool EXACT>
 inline hipError_t timed_full(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                              const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -132,7 +132,7 @@ inline hipError_t timed_full(const SweepArgs<T>& a, const Params<T>& p, const Ge
     tm->used += 2;
     (void)hipEventRecord(e0, s);
   }
-  hipError_t err = launch_full<T>(a, p, g, L.wpb, s);
+  hipError_t err = launch_full<T, EXACT>(a, p, g, L.wpb, s);
   if (e1) (void)hipEventRecord(e1, s);
   return err;
 }
@@ -245,7 +245,8 @@ constexpr int kStageFull = 200;
 inline std::vector<int> step_plan(bool exact, int npc, const LaunchCfg& L, int LX) {
   const bool fuse = L.fuse && npc >= 2 && LX >= 8;  // the fused kernels wrap rows once
   std::vector<int> p;
-  if (!exact && fuse && L.full && npc == 2) {
+  (void)exact;
+  if (fuse && L.full && npc == 2) {
     p.push_back(kStageFull);
     return p;
   }
@@ -315,8 +316,7 @@ hipError_t launch_stage(int sc, const SweepArgs<T>& a, const Params<T>& P, const
                         const LaunchCfg& L, int npc, bool store_half, hipStream_t s,
                         SweepTimer* tm) {
   if (sc == kStageFull) {
-    if constexpr (!EXACT) return timed_full<T>(a, P, g, L, s, tm);
-    return hipErrorInvalidValue;
+    return timed_full<T, EXACT>(a, P, g, L, s, tm);
   }
   if (sc == kStagePair) {
     if constexpr (!EXACT) {
@@ -726,12 +726,9 @@ class EngineT final : public Engine {
     const dim3 grid(blocks), block(64 * wpb);
     constexpr bool HB = !EXACT;  // EXACT: the flag is ignored, one instance
     if (c == kStageFull) {
-      if constexpr (!EXACT) {
-        const size_t lds = (size_t)wpb * kRingFloat4 * sizeof(float4);
-        hipLaunchKernelGGL((full_batch_kernel<T>), grid, block, lds, s, J, njobs, nitems);
-        return hipGetLastError();
-      }
-      return hipErrorInvalidValue;
+      const size_t lds = (size_t)wpb * ring_bytes_per_wave<T>();
+      hipLaunchKernelGGL((full_batch_kernel<T, EXACT>), grid, block, lds, s, J, njobs, nitems);
+      return hipGetLastError();
     }
     if (c == kStagePair) {
       if (HB && npc_ == 2)
