@@ -69,6 +69,44 @@ def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-
     return total_ms, dict(zip(("predictor", "corrector", "pair", "step"), sweeps))
 
 
+def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3):
+    """Surrogate corrector at LxL against the plain npc=2 step: a network
+    trained briefly on frames of a 512^2 plain run (the full training is
+    `python -m mass_hd_amd.workloads config3`), then `steps` surrogate steps
+    (the first one judged) timed with HIP events."""
+    net = lyo.Net(seed=0)
+    src = lyo.Lyotropic(512, 512, npc=2, precision=lyo.FP32, **PHYS)
+    src.configure(**INIT)
+    src.step(200)
+    for _ in range(8):
+        src.step(25)
+        net.collect(src)
+    src.close()
+    l0, l1 = net.train(500, lr=2e-3)
+    m = lyo.Lyotropic(L, L, npc=2, precision=lyo.FP32, **PHYS)
+    m.configure(**INIT)
+    m.step(5)
+    m.synchronize()
+    t_plain, _ = m.time_steps(steps, per_sweep=False)
+    lyo.enable_surrogate(m, net, judge_every, tol)
+    m.step(2)  # warm-up: a judged step and a surrogate step
+    m.synchronize()
+    t_sur, _ = m.time_steps(steps, per_sweep=False)
+    st = lyo.surrogate_stats(m)
+    m.close()
+    net.close()
+    N = L * L
+    return {"surrogate_mlups": round(N * steps / (t_sur / 1e3) / 1e6, 1),
+            "plain_mlups_same_run": round(N * steps / (t_plain / 1e3) / 1e6, 1),
+            "surrogate_over_plain": round(t_plain / t_sur, 3),
+            "judge_every": judge_every, "tol": tol,
+            "judged_velocity_error_last": st["last_error"],
+            "judged_velocity_error_max": st["max_error"],
+            "surrogate_steps": st["surrogate_steps"], "fallbacks": st["fallbacks"],
+            "net_flop_per_lu": 2 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 8),
+            "train_loss": [round(l0, 4), round(l1, 5)]}
+
+
 def cpu_baseline(L, steps, threads):
     from oracle import pyoracle as po
     o = po.Oracle(L, L, npc=2, nthreads=threads, **PHYS)
@@ -134,6 +172,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--skip-surrogate", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.calibrate:
@@ -162,7 +202,8 @@ def main():
     if not args.skip_fp64:
         s64 = max(2, args.steps // 4)
         log("fp64 %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
-        t64, k64 = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, args.chunk, 0, fuse, alt)
+        t64, k64 = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, args.chunk, 0, fuse, alt,
+                           0 if args.no_full else -1, args.step_chunk, args.wpb)
         mlups64 = N * s64 / (t64 / 1e3) / 1e6
         log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
         extra = {"fp64_mlups": round(mlups64, 1), "fp32_over_fp64": round(mlups32 / mlups64, 3),
@@ -180,6 +221,11 @@ def main():
         if t.get("size") == L and t.get("npc") == npc and dom in t.get("kernels", {}):
             traffic = t["kernels"][dom].get("bytes_per_launch")
             valu_ms = t["kernels"][dom].get("valu_issue_ms")
+
+    if not args.skip_surrogate:
+        log("surrogate %dx%d: training on 512^2 frames, %d timed steps" % (L, L, args.surrogate_steps))
+        extra["surrogate"] = surrogate_leg(lyo, L, args.surrogate_steps)
+        log("surrogate: %s" % extra["surrogate"])
 
     cpu = None
     if not args.skip_cpu:

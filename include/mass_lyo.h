@@ -186,7 +186,8 @@ int mass_lyo_synchronize(mass_lyo_t h);
  * sweep launch is also bracketed by its own event pair; sweep_ms (4 doubles)
  * gets the mean time of a predictor sweep, of a separate corrector sweep, of
  * a fused last-corrector-pair launch and of a one-kernel step (0 where there
- * was none). */
+ * was none).  In surrogate mode the steps are those of mass_lyo_step
+ * (judged steps included) and sweep_ms gets zeros. */
 int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms,
                         double* sweep_ms);
 

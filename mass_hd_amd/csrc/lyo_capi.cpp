@@ -390,11 +390,27 @@ int mass_lyo_time_steps(mass_lyo_t h, int32_t nsteps, double* total_ms, double* 
       if (e != hipSuccess) break;
     }
   }
+  int rc = MASS_LYO_OK;
   if (e == hipSuccess) e = hipEventRecord(t0, h->stream);
-  if (e == hipSuccess) e = h->eng->steps(nsteps, h->launch, h->stream, sweep_ms ? &tm : nullptr);
-  if (e == hipSuccess) e = hipEventRecord(t1, h->stream);
-  if (e == hipSuccess) e = hipEventSynchronize(t1);
+  if (e == hipSuccess) {
+    if (h->sur.net) {
+      // surrogate mode: the steps as mass_lyo_step runs them (judged steps
+      // synchronise with the host); no per-sweep times
+      for (int i = 0; i < nsteps && rc == MASS_LYO_OK; ++i) rc = surrogate_step(h);
+    } else {
+      e = h->eng->steps(nsteps, h->launch, h->stream, sweep_ms ? &tm : nullptr);
+    }
+  }
+  if (e == hipSuccess && rc == MASS_LYO_OK) e = hipEventRecord(t1, h->stream);
+  if (e == hipSuccess && rc == MASS_LYO_OK) e = hipEventSynchronize(t1);
   float ms = 0;
+  if (rc != MASS_LYO_OK) {  // the surrogate step left its message in h->err
+    for (auto& ev : tm.ev)
+      if (ev) (void)hipEventDestroy(ev);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return rc;
+  }
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
   if (e == hipSuccess && sweep_ms) {
     double acc[4] = {0, 0, 0, 0};
