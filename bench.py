@@ -55,10 +55,12 @@ def log(msg):
 
 
 def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-1, alt=-1,
-            full=-1, step_chunk=0, wpb=0):
+            full=-1, step_chunk=0, wpb=0, loops=False):
     m = lyo.Lyotropic(L, L, npc=npc, precision=precision, **PHYS)
     m.set_launch(chunk, wpb)
     m.set_schedule(fuse, pair_chunk, alt, full, step_chunk)
+    if loops:
+        m.set_loop_kernels(True)
     m.configure(**INIT)
     if warmup:
         m.step(warmup)
@@ -170,6 +172,8 @@ def main():
     ap.add_argument("--alt", type=int, default=-1,
                     help="march order: 0 every chunk upwards, 1 alternating (default)")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-fp64-loops", action="store_true",
+                    help="skip the FP64 run with one kernel per reference loop")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-surrogate", action="store_true")
@@ -207,8 +211,19 @@ def main():
         mlups64 = N * s64 / (t64 / 1e3) / 1e6
         log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
         extra = {"fp64_mlups": round(mlups64, 1), "fp32_over_fp64": round(mlups32 / mlups64, 3),
-                 "fp64_kernel_set": "strip-march sweeps, reference expression order",
+                 "fp64_kernel_set": "fused strip-march kernels (one-kernel step for npc=2), "
+                                    "reference expression order",
                  "fp64_launch_ms": {k: round(v, 4) for k, v in k64.items() if v}}
+        if not args.skip_fp64_loops:
+            log("fp64 loop kernels %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
+            tl, _ = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, loops=True)
+            mlupsl = N * s64 / (tl / 1e3) / 1e6
+            log("fp64 loop kernels: %.3f ms/step, %.0f MLUPS" % (tl / s64, mlupsl))
+            extra.update({"fp64_loops_mlups": round(mlupsl, 1),
+                          "fp32_over_fp64_loops": round(mlups32 / mlupsl, 3),
+                          "fp64_loops_kernel_set": "one kernel per reference node loop "
+                                                   "(UpdateQuantities storing 23 derived "
+                                                   "fields, UpdateFields, Move), SoA planes"})
 
     dom = max((k for k in k32 if k32[k] > 0), key=lambda k: k32[k])
     alg_bytes = SWEEPS[dom] * SWEEP_VALUES * 4 * N

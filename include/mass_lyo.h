@@ -213,6 +213,17 @@ int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chu
                           int32_t alternate, int32_t one_kernel_step,
                           int32_t step_chunk_rows);
 
+/* FP64 only: 1 runs every step as the reference's node loops, one kernel per
+ * loop -- UpdateQuantities (storing its 23 derived fields), UpdateFields(true),
+ * Move, then npc x (UpdateQuantities, UpdateFields(false)) -- and 0 (the
+ * default) returns to the fused kernels.  Results are bitwise the same; the
+ * loop kernels move the reference's ~291 values per node and step instead of
+ * the fused kernels' 26-108, and exist to measure that layout.  The first
+ * call with 1 allocates the 41 extra planes.  sweep_ms of mass_lyo_time_steps
+ * gets zeros in this mode.  Reference: Lyotropic::Step,
+ * src/models/lyotropic.cpp:343-386,500-525. */
+int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on);
+
 /* ------------------------------------------------------------------ batches
  * Many independent runs (own size and parameters; shared npc and precision)
  * advanced together.  No reference counterpart: the reference runs one model

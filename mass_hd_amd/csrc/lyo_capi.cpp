@@ -465,4 +465,17 @@ int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chu
   return MASS_LYO_OK;
 }
 
+int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (on != 0 && on != 1) return fail(h, MASS_LYO_EINVAL, "on must be 0 or 1");
+  if (on && h->prm.precision != MASS_LYO_FP64)
+    return fail(h, MASS_LYO_EINVAL, "the loop kernels run the FP64 path only");
+  if (on) {
+    const hipError_t e = h->eng->alloc_loops();
+    if (e != hipSuccess) return hip_fail(h, e, "loop kernel buffers");
+  }
+  h->launch.loops = on;
+  return MASS_LYO_OK;
+}
+
 }  // extern "C"

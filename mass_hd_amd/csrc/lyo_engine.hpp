@@ -13,6 +13,7 @@
 #include "lyo_aux.hpp"
 #include "lyo_diag.hpp"
 #include "lyo_full.hpp"
+#include "lyo_loops.hpp"
 #include "lyo_net_engine.hpp"
 #include "lyo_pair.hpp"
 #include "lyo_sweep.hpp"
@@ -52,6 +53,11 @@ struct Bufs {
   // old state S[lazy_cur], which stay intact until the next step
   int lazy_cur = -1;
   bool lazy_full = false;  // ... and the predictor's outputs too (one-kernel step)
+  // FP64 reference loop structure (lyo_loops.hpp): the 23 derived planes and
+  // the predictor's post-collision f, fn before Move
+  T* Ld = nullptr;
+  T* Lfc = nullptr;
+  T* Lfnc = nullptr;
 };
 
 struct LaunchCfg {
@@ -63,6 +69,7 @@ struct LaunchCfg {
   int alt = 1;   // alternate the march direction between neighbouring chunks
   int full = 1;  // FP32, npc = 2: the whole step in one kernel
   int xc3 = 32;  // rows per wave chunk of the one-kernel step
+  int loops = 0; // FP64: one kernel per loop of the reference's step instead
 };
 
 // Optional per-sweep event timing.
@@ -350,9 +357,77 @@ void note_step(Bufs<T>& B, int npc, const std::vector<int>& plan, int cur) {
   }
 }
 
+// The reference's loop structure (lyo_loops.hpp): UpdateQuantities,
+// UpdateFields(true), Move, then npc x (UpdateQuantities, UpdateFields(false)),
+// each a kernel over all nodes.  The predictor's outputs land where the
+// predictor sweep puts them and every corrector reads and writes the buffers
+// of the separate corrector sweeps, so the state after a step, the frame
+// fields and the sums are those of the fused kernels.
+template <class T, bool EXACT>
+hipError_t run_loop_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int npc,
+                          int nsteps, hipStream_t s) {
+  if constexpr (!EXACT) {
+    return hipErrorInvalidValue;
+  } else {
+    if (!B.Ld || !B.Lfc || !B.Lfnc) return hipErrorInvalidValue;
+    const int N = LX * LY, bs = 256, nb = (N + bs - 1) / bs;
+    for (int st = 0; st < nsteps; ++st) {
+      const int cur = B.cur;
+      const SweepArgs<T> a0 = stage_args<T, EXACT>(B, npc, 0, cur);
+      hipLaunchKernelGGL((loop_uq_kernel<T, EXACT>), dim3(nb), dim3(bs), 0, s, a0.f_in, a0.qxx_in,
+                         a0.qyx_in, a0.phi_in, P, LX, LY, NP, B.Ld);
+      LoopUF<T> u{};
+      u.D = B.Ld;
+      u.f = a0.f_in;
+      u.qxx = a0.qxx_in;
+      u.qyx = a0.qyx_in;
+      u.phi = a0.phi_in;
+      u.f_out = B.Lfc;
+      u.fn_out = B.Lfnc;
+      u.qxx_out = a0.qxx_out;
+      u.qyx_out = a0.qyx_out;
+      u.qnxx_out = a0.qnxx_out;
+      u.qnyx_out = a0.qnyx_out;
+      u.phi_out = a0.phi_out;
+      u.phn_out = a0.phn_out;
+      hipLaunchKernelGGL((loop_uf_kernel<T, EXACT, true>), dim3(nb), dim3(bs), 0, s, u, P, LX, LY,
+                         NP);
+      hipLaunchKernelGGL((loop_move_kernel<T>), dim3(nb), dim3(bs), 0, s, (const T*)B.Lfc,
+                         (const T*)B.Lfnc, a0.f_out, a0.fn_out, LX, LY, NP);
+      for (int c = 1; c <= npc; ++c) {
+        const SweepArgs<T> a = stage_args<T, EXACT>(B, npc, c, cur);
+        hipLaunchKernelGGL((loop_uq_kernel<T, EXACT>), dim3(nb), dim3(bs), 0, s, a.f_in,
+                           a.qxx_in, a.qyx_in, a.phi_in, P, LX, LY, NP, B.Ld);
+        LoopUF<T> w{};
+        w.D = B.Ld;
+        w.f = a.f_in;
+        w.qxx = a.qxx_in;
+        w.qyx = a.qyx_in;
+        w.phi = a.phi_in;
+        w.fn = a.fn_in;
+        w.qnxx = a.qnxx_in;
+        w.qnyx = a.qnyx_in;
+        w.phn = a.phn_in;
+        w.f_out = a.f_out;
+        w.qxx_out = a.qxx_out;
+        w.qyx_out = a.qyx_out;
+        w.phi_out = a.phi_out;
+        hipLaunchKernelGGL((loop_uf_kernel<T, EXACT, false>), dim3(nb), dim3(bs), 0, s, w, P, LX,
+                           LY, NP);
+      }
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      note_step<T, EXACT>(B, npc, std::vector<int>{npc}, cur);
+      B.cur = cur ^ 1;
+    }
+    return hipSuccess;
+  }
+}
+
 template <class T, bool EXACT>
 hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int npc,
                      int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
+  if (L.loops) return run_loop_steps<T, EXACT>(B, P, LX, LY, NP, npc, nsteps, s);
   const std::vector<int> plan = step_plan(EXACT, npc, L, LX);
   const bool store_half = !EXACT && plan_stores_half(plan, npc);  // EXACT always stores
   Geo geo[3];
@@ -533,6 +608,9 @@ class Engine {
   // nsteps steps with npc_run correctors (npc_run <= npc of the engine)
   virtual hipError_t steps_with(int npc_run, int nsteps, const LaunchCfg& L, hipStream_t s) = 0;
   virtual int npc() const = 0;
+  // buffers of the reference loop structure (FP64 only): allocated on first
+  // use, kept until release
+  virtual hipError_t alloc_loops() = 0;
 };
 
 template <class T, bool EXACT>
@@ -546,15 +624,7 @@ class EngineT final : public Engine {
     NP_ = NP;
     npc_ = npc;
     const size_t n1 = (size_t)NP, n9 = 9 * (size_t)NP;
-    auto get = [&](size_t n, T** p) -> hipError_t {
-      hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-      if (e == hipSuccess) {
-        ptrs_.push_back(*p);
-        bytes_ += n * sizeof(T);
-        e = hipMemset(*p, 0, n * sizeof(T));
-      }
-      return e;
-    };
+    auto get = [&](size_t n, T** p) { return get_(n, p); };
     hipError_t e = hipSuccess;
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
       if (e == hipSuccess) e = get(n9, &B_.Sf[i]);
@@ -718,6 +788,15 @@ class EngineT final : public Engine {
     return run_steps<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_run, nsteps, L, s, nullptr);
   }
   int npc() const override { return npc_; }
+  hipError_t alloc_loops() override {
+    if (!EXACT) return hipErrorInvalidValue;
+    if (B_.Ld) return hipSuccess;
+    const size_t n1 = (size_t)NP_;
+    hipError_t e = get_(kLoopPlanes * n1, &B_.Ld);
+    if (e == hipSuccess) e = get_(9 * n1, &B_.Lfc);
+    if (e == hipSuccess) e = get_(9 * n1, &B_.Lfnc);
+    return e;
+  }
   hipError_t launch_batch(const void* jobs, int njobs, int nitems, int c, bool store_half,
                           int grid_waves, int wpb, hipStream_t s) const override {
     const SweepJob<T>* J = static_cast<const SweepJob<T>*>(jobs);
@@ -763,6 +842,16 @@ class EngineT final : public Engine {
   }
 
  private:
+  hipError_t get_(size_t n, T** p) {
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e == hipSuccess) {
+      ptrs_.push_back(*p);
+      bytes_ += n * sizeof(T);
+      e = hipMemset(*p, 0, n * sizeof(T));
+    }
+    return e;
+  }
+
   Bufs<T> B_;
   Params<T> P_{};
   double rho_ = 0;

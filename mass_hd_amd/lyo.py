@@ -104,6 +104,7 @@ def load_library():
     lib.mass_lyo_set_launch.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.mass_lyo_set_schedule.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_int32]
+    lib.mass_lyo_set_loop_kernels.argtypes = [h, ctypes.c_int32]
     lib.mass_lyo_abi_version.argtypes = []
     lib.mass_lyo_diagnostics.argtypes = [h, P(Diag), dp]
     i32 = ctypes.c_int32
@@ -264,6 +265,12 @@ class Lyotropic:
         self._check(self.lib.mass_lyo_set_schedule(self.h, int(fuse_last_pair),
                                                    int(pair_chunk_rows), int(alternate),
                                                    int(one_kernel_step), int(step_chunk_rows)))
+
+    def set_loop_kernels(self, on=True):
+        """FP64: run each step as the reference's node loops, one kernel per
+        loop (UpdateQuantities storing its 23 derived fields, UpdateFields,
+        Move); False returns to the fused kernels.  Bitwise the same results."""
+        self._check(self.lib.mass_lyo_set_loop_kernels(self.h, 1 if on else 0))
 
 
 class Batch:

@@ -123,6 +123,38 @@ def test_fused_pair_equals_separate_sweeps(require_gpu, precision, npc, LX, LY, 
             assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
 
 
+@pytest.mark.parametrize("npc", [1, 2, 3])
+@pytest.mark.parametrize("LX,LY", [(128, 128), (45, 67), (5, 3), (2, 60)])
+def test_fp64_loop_kernels_bit_exact(require_gpu, LX, LY, npc):
+    """The reference's loop structure (one kernel per node loop, derived fields
+    stored) gives the oracle's fields bit for bit, and the fused kernels'
+    results when the two alternate within one run."""
+    g, o, _, _ = pair(LX, LY, npc=npc)
+    g.set_loop_kernels(True)
+    g.step(1)
+    o.step(1)
+    assert_bitwise(g, o, STATE)
+    assert_bitwise(g, o, DERIVED)
+    f = lyo.Lyotropic(LX, LY, npc=npc, precision=lyo.FP64, **CONFIG0_PHYS)
+    f.configure(**CONFIG0_INIT)
+    f.step(1)
+    for on in (False, True, False):
+        g.set_loop_kernels(on)
+        g.step(2)
+        f.step(2)
+    for n in STATE + DERIVED:
+        x, y = g.get(n), f.get(n)
+        assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
+
+
+def test_loop_kernels_are_fp64_only(require_gpu):
+    g = lyo.Lyotropic(16, 16, npc=2, precision=lyo.FP32, **CONFIG0_PHYS)
+    with pytest.raises(lyo.MassLyoError) as e:
+        g.set_loop_kernels(True)
+    assert e.value.code == 1 and "FP64" in str(e.value)
+    g.set_loop_kernels(False)
+
+
 def test_fused_pair_timing_kinds(require_gpu):
     # (predictor, separate corrector, fused pair, one-kernel step)
     for npc, full, kinds in ((1, 1, (1, 1, 0, 0)), (2, 0, (1, 0, 1, 0)), (2, 1, (0, 0, 0, 1)),
