@@ -285,6 +285,10 @@ int mass_lyo_net_normalization(mass_lyo_net_t n, float* out32);
 /* network on host data already in network units: x, y are [LX*LY][8] FP32
  * (node k = y + LY*x), LY a multiple of 16 */
 int mass_lyo_net_forward_host(mass_lyo_net_t n, const float* x, int32_t LX, int32_t LY, float* y);
+/* the surrogate's inference on host data in physical units: x [LX*LY][8]
+ * features are normalised with the network's statistics, the output is
+ * scaled back to increments (as surrogate mode applies them) */
+int mass_lyo_net_predict_host(mass_lyo_net_t n, const float* x, int32_t LX, int32_t LY, float* y);
 /* loss sum((out - y)^2)/(2N) and its gradient wrt the parameters */
 int mass_lyo_net_grad_host(mass_lyo_net_t n, const float* x, const float* y, int32_t LX,
                            int32_t LY, double* loss, float* grads);

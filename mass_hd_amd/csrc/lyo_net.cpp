@@ -47,7 +47,8 @@ __global__ void pack_frag_kernel(const float* __restrict__ W, int cin_f, int cou
   const int j = e & 7, lane = (e >> 3) & 63, sn = e >> 9;
   const int nb = sn % nblk, s = sn / nblk;
   const int q = lane >> 4, i = lane & 15;
-  const int co = nb * 16 + i;
+  // cout = 8: M rows 8..15 repeat channels 0..7 (see conv_rows_kernel)
+  const int co = cout == 8 ? (i & 7) : nb * 16 + i;
   int t, ci;
   if (cin == 32) {
     t = s;
@@ -197,38 +198,88 @@ hipError_t pack_all(mass_lyo_net_t n) {
   return hipGetLastError();
 }
 
-int grid_for(int LX, int LY) {
-  const int tiles = LX * (LY / 16);
-  const int waves = tiles < 16384 ? tiles : 16384;
-  return (waves + 3) / 4;
+// output rows per conv task: Cin = 32 streams ROWS + 2 input rows once per
+// task; Cin = 8 (raw FP32 fragments in layer 0) keeps fewer rows in flight
+constexpr int kRows32 = 8;
+constexpr int kRows8 = 4;
+
+int conv_waves(int LX, int LY, int rows) {
+  const int tasks = ((LX + rows - 1) / rows) * (LY / 16);
+  return tasks < 8192 ? tasks : 8192;
 }
 
+template <int CIN, int COUT, int ROWS, bool F32, int EPI, bool ZOUT>
+void launch_conv(const lyo::net::ConvArgs& a, hipStream_t s) {
+  using namespace lyo::net;
+  const dim3 grid(conv_waves(a.LX, a.LY, ROWS)), block(64);
+  if (a.LX % ROWS)
+    hipLaunchKernelGGL((conv_rows_kernel<CIN, COUT, ROWS, F32, true, EPI, ZOUT>), grid, block, 0,
+                       s, a);
+  else
+    hipLaunchKernelGGL((conv_rows_kernel<CIN, COUT, ROWS, F32, false, EPI, ZOUT>), grid, block,
+                       0, s, a);
+}
+
+// One convolution launch.  in_f32 (layer 0 of inference): FP32 features,
+// normalised on load with the network's mean / inverse deviation; out_scale
+// (last layer of inference): output channels scaled to physical increments.
 hipError_t conv(mass_lyo_net_t n, const __bf16* in, int l, bool bwd, int epi, float* zout,
-                const float* zin, __bf16* out_bf, float* out_f32, int LX, int LY, hipStream_t s) {
+                const float* zin, __bf16* out_bf, float* out_f32, int LX, int LY, hipStream_t s,
+                const float* in_f32 = nullptr, const float* out_scale = nullptr) {
+  using namespace lyo::net;
   ConvArgs a;
   a.in = in;
+  a.in_f32 = in_f32;
+  a.in_norm = in_f32 ? n->d_norm : nullptr;
   a.wfrag = reinterpret_cast<const bf16x8*>(bwd ? n->d_wb[l] : n->d_wf[l]);
   a.bias = bwd ? nullptr : n->d_params + layer_woff(l) + 9 * kCh[l] * kCh[l + 1];
   a.zout = zout;
   a.zin = zin;
   a.out_bf = out_bf;
   a.out_f32 = out_f32;
+  a.out_scale = out_scale;
   a.cin = bwd ? kCh[l + 1] : kCh[l];
   a.cout = bwd ? kCh[l] : kCh[l + 1];
   a.LX = LX;
   a.LY = LY;
   a.epi = epi;
-  hipLaunchKernelGGL(lyo::net::conv_mfma_kernel, dim3(grid_for(LX, LY)), dim3(256), 0, s, a);
+  // the layer shapes and epilogues of the network (forward: 8->32, 32->32
+  // with GELU, 32->8 linear; data gradient: 32->32 and 8->32)
+  const bool z = zout != nullptr;
+  if (a.cin == 32 && a.cout == 32 && epi == EPI_GELU && !z && out_bf)
+    launch_conv<32, 32, kRows32, false, EPI_GELU, false>(a, s);
+  else if (a.cin == 32 && a.cout == 32 && epi == EPI_GELU && z && out_bf)
+    launch_conv<32, 32, kRows32, false, EPI_GELU, true>(a, s);
+  else if (a.cin == 32 && a.cout == 32 && epi == EPI_GELU_GRAD && zin && out_bf)
+    launch_conv<32, 32, kRows32, false, EPI_GELU_GRAD, false>(a, s);
+  else if (a.cin == 32 && a.cout == 8 && epi == EPI_LINEAR && out_f32)
+    launch_conv<32, 8, kRows32, false, EPI_LINEAR, false>(a, s);
+  else if (a.cin == 8 && a.cout == 32 && epi == EPI_GELU && in_f32 && !z && out_bf)
+    launch_conv<8, 32, kRows8, true, EPI_GELU, false>(a, s);
+  else if (a.cin == 8 && a.cout == 32 && epi == EPI_GELU && !in_f32 && !z && out_bf)
+    launch_conv<8, 32, kRows8, false, EPI_GELU, false>(a, s);
+  else if (a.cin == 8 && a.cout == 32 && epi == EPI_GELU && !in_f32 && z && out_bf)
+    launch_conv<8, 32, kRows8, false, EPI_GELU, true>(a, s);
+  else if (a.cin == 8 && a.cout == 32 && epi == EPI_GELU_GRAD && zin && out_bf)
+    launch_conv<8, 32, kRows8, false, EPI_GELU_GRAD, false>(a, s);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-// forward of normalised bf16 input a0 -> out (FP32 [N][8], normalised units)
-hipError_t forward(mass_lyo_net_t n, int LX, int LY, bool keep, hipStream_t s) {
+// forward of normalised bf16 input a0 (or, inference, of FP32 features feat
+// normalised on load) -> out (FP32 [N][8]; normalised units unless out_scale)
+hipError_t forward(mass_lyo_net_t n, int LX, int LY, bool keep, hipStream_t s,
+                   const float* feat = nullptr, float* out = nullptr,
+                   const float* out_scale = nullptr) {
   using namespace lyo::net;
-  hipError_t e = conv(n, n->a0, 0, false, EPI_GELU, keep ? n->z1 : nullptr, nullptr, n->a1, nullptr, LX, LY, s);
+  hipError_t e = conv(n, n->a0, 0, false, EPI_GELU, keep ? n->z1 : nullptr, nullptr, n->a1,
+                      nullptr, LX, LY, s, feat);
   if (e == hipSuccess) e = conv(n, n->a1, 1, false, EPI_GELU, keep ? n->z2 : nullptr, nullptr, n->a2, nullptr, LX, LY, s);
   if (e == hipSuccess) e = conv(n, n->a2, 2, false, EPI_GELU, keep ? n->z3 : nullptr, nullptr, n->a3, nullptr, LX, LY, s);
-  if (e == hipSuccess) e = conv(n, n->a3, 3, false, EPI_LINEAR, nullptr, nullptr, nullptr, n->out, LX, LY, s);
+  if (e == hipSuccess)
+    e = conv(n, n->a3, 3, false, EPI_LINEAR, nullptr, nullptr, nullptr, out ? out : n->out, LX,
+             LY, s, nullptr, out_scale);
   return e;
 }
 
@@ -272,13 +323,8 @@ hipError_t net_predict(mass_lyo_net_t n, const float* feat, int LX, int LY, floa
   const int N = LX * LY;
   hipError_t e = ensure_ws(n, (size_t)N);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(normalize_kernel, dim3((N * 8 + 255) / 256), dim3(256), 0, s, feat,
-                     n->d_norm, n->d_norm + 8, N, n->a0);
-  e = forward(n, LX, LY, false, s);
-  if (e == hipSuccess)
-    hipLaunchKernelGGL(scale_kernel, dim3((N * 8 + 255) / 256), dim3(256), 0, s, n->out,
-                       n->d_norm + 16, N, delta);
-  return e == hipSuccess ? hipGetLastError() : e;
+  // normalisation folded into layer 0's loads, output scaling into layer 3
+  return forward(n, LX, LY, false, s, feat, delta, n->d_norm + 16);
 }
 
 bool net_ready(mass_lyo_net_t n) { return n && n->have_norm; }
@@ -390,6 +436,20 @@ int mass_lyo_net_forward_host(mass_lyo_net_t n, const float* x, int32_t LX, int3
   if (e == hipSuccess) e = hipMemcpyAsync(y, n->out, N * 32, hipMemcpyDeviceToHost, n->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
   return e == hipSuccess ? MASS_LYO_OK : nhip(n, e, "forward");
+}
+
+int mass_lyo_net_predict_host(mass_lyo_net_t n, const float* x, int32_t LX, int32_t LY, float* y) {
+  if (!n || !x || !y) return nfail(n, MASS_LYO_EINVAL, "null argument");
+  if (LX < 1 || LY < 16 || LY % 16) return nfail(n, MASS_LYO_EINVAL, "LY must be a multiple of 16");
+  const size_t N = (size_t)LX * LY;
+  float* d = nullptr;
+  hipError_t e = hipMalloc((void**)&d, N * 64);
+  if (e == hipSuccess) e = hipMemcpyAsync(d, x, N * 32, hipMemcpyHostToDevice, n->stream);
+  if (e == hipSuccess) e = lyo::net_predict(n, d, LX, LY, d + N * 8, n->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(y, d + N * 8, N * 32, hipMemcpyDeviceToHost, n->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+  if (d) (void)hipFree(d);
+  return e == hipSuccess ? MASS_LYO_OK : nhip(n, e, "predict");
 }
 
 int mass_lyo_net_grad_host(mass_lyo_net_t n, const float* x, const float* y, int32_t LX, int32_t LY,

@@ -392,6 +392,7 @@ def _net_api(lib):
     lib.mass_lyo_net_set_params.argtypes = [h, fp]
     lib.mass_lyo_net_normalization.argtypes = [h, fp]
     lib.mass_lyo_net_forward_host.argtypes = [h, fp, i32, i32, fp]
+    lib.mass_lyo_net_predict_host.argtypes = [h, fp, i32, i32, fp]
     lib.mass_lyo_net_grad_host.argtypes = [h, fp, fp, i32, i32, dp, fp]
     lib.mass_lyo_net_collect.argtypes = [h, h]
     lib.mass_lyo_net_dataset_size.argtypes = [h]
@@ -472,6 +473,15 @@ class Net:
         xi = np.ascontiguousarray(x, dtype=np.float32)
         y = np.empty((LX, LY, 8), dtype=np.float32)
         self._check(self.lib.mass_lyo_net_forward_host(self.h, _fptr(xi), LX, LY, _fptr(y)))
+        return y
+
+    def predict(self, x):
+        """surrogate inference on features in physical units (normalised with
+        the network's statistics, output scaled to increments)"""
+        LX, LY, _ = x.shape
+        xi = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((LX, LY, 8), dtype=np.float32)
+        self._check(self.lib.mass_lyo_net_predict_host(self.h, _fptr(xi), LX, LY, _fptr(y)))
         return y
 
     def grad(self, x, y):

@@ -30,6 +30,22 @@ def test_forward_matches_oracle(require_gpu):
     assert err < 3e-2, err  # BF16 weights and activations, FP32 accumulation
 
 
+def test_inference_path_matches_forward(require_gpu):
+    """Surrogate inference (FP32 features normalised inside the first layer's
+    loads, output scaled inside the last layer) equals the plain forward pass
+    bit for bit under the identity normalisation of a fresh network, on
+    ragged row counts (tasks of 8 rows with a partial last one)."""
+    rng = np.random.default_rng(2)
+    net = lyo.Net(seed=4)
+    p = net.get_params()
+    p += rng.normal(0, 0.05, p.shape).astype(np.float32)
+    net.set_params(p)
+    assert np.array_equal(net.normalization()[:8], np.zeros(8, np.float32))
+    for LX, LY in ((7, 32), (9, 16), (16, 48)):
+        x = rng.normal(size=(LX, LY, 8)).astype(np.float32)
+        assert np.array_equal(net.predict(x), net.forward(x)), (LX, LY)
+
+
 def test_gradients_match_oracle(require_gpu):
     rng = np.random.default_rng(1)
     net = lyo.Net(seed=5)
@@ -79,6 +95,15 @@ def test_training_reduces_loss_and_beats_no_network(require_gpu):
     assert net.dataset_size == 8
     l0, l1 = net.train(300, lr=2e-3)
     assert l1 < 0.5 * l0, (l0, l1)
+    # inference with the trained statistics against the oracle
+    nrm = net.normalization().astype(np.float64)
+    xr = np.random.default_rng(3).normal(size=(5, 16, 8))
+    x = (xr / nrm[8:16] + nrm[:8]).astype(np.float32)
+    got = net.predict(x)
+    ref, _ = no.forward((x.astype(np.float64) - nrm[:8]) * nrm[8:16], oracle_params(net))
+    ref = ref * nrm[16:24]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 3e-2, err
 
     # From one state S (a run after 40 steps), one more step three ways:
     # (a) plain npc=2 -- the reference; (b) predictor + one corrector with no
