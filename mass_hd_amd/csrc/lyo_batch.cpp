@@ -295,8 +295,8 @@ int mass_lyo_batch_autotune(mass_lyo_batch_t b, int32_t* chunk_rows, int32_t* gr
   int rc0 = build_jobs(b);
   if (rc0) return rc0;
   // chunk rows per stage kind: the one-kernel step repeats eight rows per
-  // chunk, the sweeps two
-  const bool full = b->plan.size() == 1 && b->plan[0] == lyo::kStageFull;
+  // chunk (four with one corrector), the sweeps two
+  const bool full = b->plan.size() == 1 && lyo::is_full_stage(b->plan[0]);
   const std::vector<int> xcs = full ? std::vector<int>{16, 32, 64} : std::vector<int>{4, 8, 16, 32};
   const int grids[] = {0, 2048, 4096, 8192, 16384};
   double best = 1e300;

@@ -79,16 +79,19 @@ struct SweepTimer {
   size_t used = 0;
 };
 
-// kind 0: sweep, 1: fused pair, 2: one-kernel step (strip width and chunk rows)
+// kind 0: sweep, 1: fused pair, 2: one-kernel step, 3: one-kernel step with
+// one corrector (strip width and chunk rows)
+constexpr int kGeoKinds = 4;
 inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, int kind = 0) {
   Geo g;
   g.LX = LX;
   g.LY = LY;
   g.N = LX * LY;
   g.NP = NP;
-  const int strip = kind == 2 ? kFullStrip : (kind == 1 ? kPairStrip : kStrip);
+  const int strip =
+      kind == 3 ? kFull1Strip : (kind == 2 ? kFullStrip : (kind == 1 ? kPairStrip : kStrip));
   g.nstrips = (LY + strip - 1) / strip;
-  g.xc = kind == 2 ? L.xc3 : (kind == 1 ? L.xc2 : L.xc);
+  g.xc = kind >= 2 ? L.xc3 : (kind == 1 ? L.xc2 : L.xc);
   g.nchunks = (LX + g.xc - 1) / g.xc;
   g.nwaves = g.nstrips * g.nchunks;
   g.xcd = L.xcd;
@@ -128,7 +131,7 @@ inline hipError_t timed_pair(const SweepArgs<T>& a, const Params<T>& p, const Ge
   return err;
 }
 
-template <class T, bool EXACT>
+template <class T, bool EXACT, int NPC>
 inline hipError_t timed_full(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                              const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -139,7 +142,7 @@ inline hipError_t timed_full(const SweepArgs<T>& a, const Params<T>& p, const Ge
     tm->used += 2;
     (void)hipEventRecord(e0, s);
   }
-  hipError_t err = launch_full<T, EXACT>(a, p, g, L.wpb, s);
+  hipError_t err = launch_full<T, EXACT, NPC>(a, p, g, L.wpb, s);
   if (e1) (void)hipEventRecord(e1, s);
   return err;
 }
@@ -248,6 +251,7 @@ SweepArgs<T> pair_args(const Bufs<T>& B, int npc, int cur) {
 // run equals the same run alone bit for bit.
 constexpr int kStagePair = 100;
 constexpr int kStageFull = 200;
+constexpr int kStageFull1 = 201;  // the whole step with npc = 1
 
 inline std::vector<int> step_plan(bool exact, int npc, const LaunchCfg& L, int LX) {
   const bool fuse = L.fuse && npc >= 2 && LX >= 8;  // the fused kernels wrap rows once
@@ -255,6 +259,10 @@ inline std::vector<int> step_plan(bool exact, int npc, const LaunchCfg& L, int L
   (void)exact;
   if (fuse && L.full && npc == 2) {
     p.push_back(kStageFull);
+    return p;
+  }
+  if (L.full && npc == 1 && LX >= 8) {
+    p.push_back(kStageFull1);
     return p;
   }
   for (int c = 0; c <= npc; ++c) {
@@ -277,7 +285,11 @@ inline bool plan_stores_half(const std::vector<int>& p, int npc) {
   return false;
 }
 
-inline int stage_geo_kind(int sc) { return sc == kStageFull ? 2 : (sc == kStagePair ? 1 : 0); }
+inline int stage_geo_kind(int sc) {
+  return sc == kStageFull1 ? 3 : (sc == kStageFull ? 2 : (sc == kStagePair ? 1 : 0));
+}
+static_assert(kGeoKinds == 4, "stage_geo_kind returns kinds 0..3");
+inline bool is_full_stage(int sc) { return sc == kStageFull || sc == kStageFull1; }
 
 // Arguments of the one-kernel step: state S[cur] in, S[next] out.
 template <class T, bool EXACT>
@@ -299,7 +311,7 @@ SweepArgs<T> full_args(const Bufs<T>& B, int npc, int cur) {
 
 template <class T, bool EXACT>
 SweepArgs<T> plan_args(const Bufs<T>& B, int npc, int sc, int cur) {
-  if (sc == kStageFull) return full_args<T, EXACT>(B, npc, cur);
+  if (is_full_stage(sc)) return full_args<T, EXACT>(B, npc, cur);
   if (sc == kStagePair) return pair_args<T, EXACT>(B, npc, cur);
   return stage_args<T, EXACT>(B, npc, sc, cur);
 }
@@ -322,9 +334,8 @@ template <class T, bool EXACT>
 hipError_t launch_stage(int sc, const SweepArgs<T>& a, const Params<T>& P, const Geo& g,
                         const LaunchCfg& L, int npc, bool store_half, hipStream_t s,
                         SweepTimer* tm) {
-  if (sc == kStageFull) {
-    return timed_full<T, EXACT>(a, P, g, L, s, tm);
-  }
+  if (sc == kStageFull) return timed_full<T, EXACT, 2>(a, P, g, L, s, tm);
+  if (sc == kStageFull1) return timed_full<T, EXACT, 1>(a, P, g, L, s, tm);
   if (sc == kStagePair) {
     if constexpr (!EXACT) {
       if (npc == 2) return timed_pair<T, EXACT, true>(a, P, g, L, s, tm);
@@ -348,10 +359,10 @@ void note_step(Bufs<T>& B, int npc, const std::vector<int>& plan, int cur) {
   const int last = plan.back();
   B.lazy_cur = -1;
   B.lazy_full = false;
-  if (last == kStagePair || last == kStageFull) {
+  if (last == kStagePair || is_full_stage(last)) {
     B.have_uq = false;
     B.lazy_cur = cur;
-    B.lazy_full = last == kStageFull;
+    B.lazy_full = is_full_stage(last);
   } else {
     note_last_uq(B, stage_args<T, EXACT>(B, npc, npc, cur));
   }
@@ -430,8 +441,8 @@ hipError_t run_steps(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP, int
   if (L.loops) return run_loop_steps<T, EXACT>(B, P, LX, LY, NP, npc, nsteps, s);
   const std::vector<int> plan = step_plan(EXACT, npc, L, LX);
   const bool store_half = !EXACT && plan_stores_half(plan, npc);  // EXACT always stores
-  Geo geo[3];
-  for (int k = 0; k < 3; ++k) geo[k] = make_geo(LX, LY, NP, L, k);
+  Geo geo[kGeoKinds];
+  for (int k = 0; k < kGeoKinds; ++k) geo[k] = make_geo(LX, LY, NP, L, k);
   for (int st = 0; st < nsteps; ++st) {
     const int cur = B.cur;
     for (int sc : plan) {
@@ -457,15 +468,17 @@ hipError_t materialize_uq(Bufs<T>& B, const Params<T>& P, int LX, int LY, int NP
   const LaunchCfg L;
   if (B.lazy_full) {
     // the one-kernel step kept the predictor's outputs in registers as well:
-    // rerun the predictor sweep first (npc = 2: no half-step planes)
+    // rerun the predictor sweep first (npc <= 2: no half-step planes)
     const hipError_t e = launch_sweep<T, EXACT, PRED, false>(
         stage_args<T, EXACT>(B, npc, 0, cur), P, make_geo(LX, LY, NP, L), L.wpb, s);
     if (e != hipSuccess) return e;
   }
-  const hipError_t e = launch_corrector<T, EXACT>(stage_args<T, EXACT>(B, npc, npc - 1, cur), P,
-                                                  make_geo(LX, LY, NP, L), L, s, nullptr,
-                                                  npc - 1, false);
-  if (e != hipSuccess) return e;
+  if (npc >= 2) {
+    const hipError_t e = launch_corrector<T, EXACT>(stage_args<T, EXACT>(B, npc, npc - 1, cur),
+                                                    P, make_geo(LX, LY, NP, L), L, s, nullptr,
+                                                    npc - 1, false);
+    if (e != hipSuccess) return e;
+  }
   note_last_uq(B, stage_args<T, EXACT>(B, npc, npc, cur));
   B.lazy_cur = -1;
   B.lazy_full = false;
@@ -805,8 +818,12 @@ class EngineT final : public Engine {
     const dim3 grid(blocks), block(64 * wpb);
     constexpr bool HB = !EXACT;  // EXACT: the flag is ignored, one instance
     if (c == kStageFull) {
-      const size_t lds = (size_t)wpb * ring_bytes_per_wave<T>();
-      hipLaunchKernelGGL((full_batch_kernel<T, EXACT>), grid, block, lds, s, J, njobs, nitems);
+      const size_t lds = full_lds_bytes<T, 2>(wpb);
+      hipLaunchKernelGGL((full_batch_kernel<T, EXACT, 2>), grid, block, lds, s, J, njobs, nitems);
+      return hipGetLastError();
+    }
+    if (c == kStageFull1) {
+      hipLaunchKernelGGL((full_batch_kernel<T, EXACT, 1>), grid, block, 0, s, J, njobs, nitems);
       return hipGetLastError();
     }
     if (c == kStagePair) {

@@ -1,5 +1,5 @@
-// lyo_full.hpp -- one whole step (predictor + two correctors) in one kernel
-// (gfx950), FP32 and FP64.
+// lyo_full.hpp -- one whole step (predictor + one or two correctors) in one
+// kernel (gfx950), FP32 and FP64.
 //
 // Lyotropic::Step (src/models/lyotropic.cpp:500-525) with npc = 2 is a
 // predictor sweep (UpdateQuantities, UpdateFields(true), Move) followed by two
@@ -19,6 +19,12 @@
 // bitwise those of the predictor sweep + fused pair.  Each stage needs its
 // inputs one lane further in, so the halo is six lanes per side (52 interior
 // columns), and a chunk repeats eight rows.
+//
+// With npc = 1 (NPC = 1) stage A is the last corrector: it stores the new
+// state (FP32: phi with the TwoSum of the separate last corrector) and there
+// is no stage B and no LDS ring; the halo is four lanes (56 interior columns)
+// and a chunk repeats four rows.  The results are bitwise those of the
+// predictor sweep + the separate corrector sweep.
 #pragma once
 
 #include "lyo_pair.hpp"
@@ -27,6 +33,8 @@ namespace lyo {
 
 constexpr int kFullHalo = 6;
 constexpr int kFullStrip = kWave - 2 * kFullHalo;
+constexpr int kFull1Halo = 4;  // npc = 1
+constexpr int kFull1Strip = kWave - 2 * kFull1Halo;
 
 // Per-row outputs of P that A needs two rows later: half-step Q and the phi
 // terms (FP32: Dp and old phi; FP64: phi + Dp/2).
@@ -64,7 +72,8 @@ constexpr size_t ring_bytes_per_wave() {
 
 // Arguments: f_in, qxx_in, qyx_in, phi_in (and FP32 phic_in) = the state
 // S[cur]; f_out, qxx_out, qyx_out, phi_out (and FP32 phic_out) = S[next].
-template <class T, bool EXACT>
+// NPC: correctors per step, 1 or 2.
+template <class T, bool EXACT, int NPC>
 __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>& P,
                                           const Geo& g, const int wid,
                                           typename Ring<T>::V* ring) {
@@ -90,15 +99,17 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
       for (int j = 0; j < W; ++j) v[q * W + j] = e[j];
     }
   };
+  static_assert(NPC == 1 || NPC == 2, "one or two correctors");
+  constexpr int HALO = NPC == 2 ? kFullHalo : kFull1Halo;
   const int strip = wid % g.nstrips;
   const int chunk = wid / g.nstrips;
   const int LX = g.LX, LY = g.LY, N = g.NP;
 
-  const int ycol = strip * kFullStrip - kFullHalo + lane;
+  const int ycol = strip * (kWave - 2 * HALO) - HALO + lane;
   int yi = ycol % LY;
   if (yi < 0) yi += LY;
   LaneOff<T> ly{(unsigned)yi * (unsigned)sizeof(T)};
-  const bool active = lane >= kFullHalo && lane < kWave - kFullHalo && ycol < LY;
+  const bool active = lane >= HALO && lane < kWave - HALO && ycol < LY;
   const int x0 = chunk * g.xc;
   const int x1 = min(x0 + g.xc, LX);
 
@@ -129,6 +140,30 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
       R.phic = ly.ld(A.phic_in + o);
     }
   };
+  // The last corrector's outputs of row r (Lyotropic::UpdateFieldsAtNode with
+  // first = false, lyotropic.cpp:331-340) from its half-step values hv (fn 9,
+  // QN 2, phi term) and increments; FP32 phi: phi_new = phi_old + phic +
+  // (Dp_pred + Dp)/2 with a TwoSum, as the separate last corrector.
+  auto store_state = [&](int r, const T(&hv)[12], T Dxx, T Dyx, T Dp, const T(&om)[9],
+                         const OldPhi<T>& old) {
+    const int ox = roff(r);
+    ly.st(A.qxx_out + ox, hv[9] + T(.5) * Dxx);
+    ly.st(A.qyx_out + ox, hv[10] + T(.5) * Dyx);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) ly.st(A.f_out + (v * N + ox), hv[v] + T(.5) * om[v]);
+    if constexpr (EXACT) {
+      ly.st(A.phi_out + ox, hv[11] + T(.5) * Dp);
+    } else {
+      const T pold = old.pold;
+      const T yv = T(.5) * (hv[11] + Dp) + old.phic;
+      const T t = pold + yv;
+      const T bp = t - pold;
+      const T ap = t - bp;
+      const T err = (pold - ap) + (yv - bp);
+      ly.st(A.phi_out + ox, t);
+      ly.st(A.phic_out + ox, err);
+    }
+  };
   // UpdateQuantities on the centre row of three stencil rows (x-1, x, x+1)
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
                 NbrVals<T>& nv, CenterVals<T>& cv) {
@@ -141,9 +176,10 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
   constexpr int cxv[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
   constexpr int cyv[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
   // P visits rows xs, xs+1, ...: four rows before the chunk, the chunk and
-  // four rows after it; A trails by two rows and B by four
-  const int xs = x0 - 4;
-  const int nI = x1 - x0 + 8;
+  // four rows after it; A trails by two rows and B by four (NPC = 1: two rows
+  // before and after, A the last stage)
+  const int xs = x0 - 2 * NPC;
+  const int nI = x1 - x0 + 4 * NPC;
 
   // ---- stage P rolling state: stencil rows x (pqb), x+1 (pqc), x+2 (cq);
   // UpdateQuantities of rows x-1 (pum), x (pu0); populations of rows x (pfc),
@@ -163,7 +199,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     load_f(xs, pfc);
     cq = load_qp(xs + 2);
     load_f(xs + 1, cf);
-    load_old(xs - 4, bold);
+    load_old(xs - 2 * NPC, bold);
     CenterVals<T> ct;
     uq(qa, pqb, pqc, ft, pum, ct);
     uq(pqb, pqc, qd, pfc, pu0, pc0);
@@ -279,34 +315,39 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
       CenterVals<T> acn;
       uq(aqb, aqc, aq_new, fst, aun, acn);
       if (i >= 4) {
-        // half-step values of row x-2, kept for B two rows later
+        // half-step values of row x-2 (NPC = 2: kept for B two rows later)
         T cen[12];
 #pragma unroll
         for (int v = 0; v < 9; ++v) cen[v] = ahn[v];
         cen[9] = po2.qnx;
         cen[10] = po2.qny;
         cen[11] = po2.dp;
-        ring_put(1, s0, cen);
+        if constexpr (NPC == 2) ring_put(1, s0, cen);
         const UFStencil<T> s = uf_stencil(aum, au0, aun);
         T Dxx, Dyx, Dp, om[9];
         update_fields<T, EXACT>(ac0, aqb.xx, aqb.yx, aqb.p, afc, s, P, Dxx, Dyx, Dp, om);
-        T ao[12];
-        qa_out.xx = cen[9] + T(.5) * Dxx;
-        qa_out.yx = cen[10] + T(.5) * Dyx;
+        if constexpr (NPC == 2) {
+          T ao[12];
+          qa_out.xx = cen[9] + T(.5) * Dxx;
+          qa_out.yx = cen[10] + T(.5) * Dyx;
 #pragma unroll
-        for (int v = 0; v < 9; ++v) ao[v] = cen[v] + T(.5) * om[v];
-        if constexpr (EXACT) {
-          qa_out.p = cen[11] + T(.5) * Dp;
-        } else {
-          // the separate sweep stores the increment, the next one adds it to
-          // old phi: the same two roundings here
-          const T inc = T(.5) * (cen[11] + Dp);
-          qa_out.p = po2.raw + inc;
+          for (int v = 0; v < 9; ++v) ao[v] = cen[v] + T(.5) * om[v];
+          if constexpr (EXACT) {
+            qa_out.p = cen[11] + T(.5) * Dp;
+          } else {
+            // the separate sweep stores the increment, the next one adds it
+            // to old phi: the same two roundings here
+            const T inc = T(.5) * (cen[11] + Dp);
+            qa_out.p = po2.raw + inc;
+          }
+          ao[9] = qa_out.xx;
+          ao[10] = qa_out.yx;
+          ao[11] = qa_out.p;
+          ring_put(0, s0, ao);
+        } else if (active) {
+          // the last corrector: the new state of row x-2
+          store_state(x - 2, cen, Dxx, Dyx, Dp, om, bold);
         }
-        ao[9] = qa_out.xx;
-        ao[10] = qa_out.yx;
-        ao[11] = qa_out.p;
-        ring_put(0, s0, ao);
       }
       aum = au0;
       au0 = aun;
@@ -323,53 +364,36 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     po1 = po;
 
     // ---- B: UpdateQuantities on row x-3, UpdateFields on row x-4
-    if (i >= 6) {  // wave-uniform
-      T ao1[12], ao2[12];  // A's outputs of rows x-3, x-4
-      ring_get(0, s1, ao1);
-      ring_get(0, s2, ao2);
-      const QP<T> bqc{ao1[9], ao1[10], ao1[11]}, bqb{ao2[9], ao2[10], ao2[11]};
-      T bfp[9], bfc[9];
+    if constexpr (NPC == 2) {
+      if (i >= 6) {  // wave-uniform
+        T ao1[12], ao2[12];  // A's outputs of rows x-3, x-4
+        ring_get(0, s1, ao1);
+        ring_get(0, s2, ao2);
+        const QP<T> bqc{ao1[9], ao1[10], ao1[11]}, bqb{ao2[9], ao2[10], ao2[11]};
+        T bfp[9], bfc[9];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        bfp[v] = ao1[v];
-        bfc[v] = ao2[v];
-      }
-      NbrVals<T> bun;
-      CenterVals<T> bcn;
-      uq(bqb, bqc, qa_out, bfp, bun, bcn);
-      if (i >= 8) {
-        const UFStencil<T> s = uf_stencil(bum, bu0, bun);
-        T Dxx, Dyx, Dp, om[9];
-        update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
-        T cmv[12];  // half-step values of row x-4
-        ring_get(1, s2, cmv);
-        if (active) {
-          const int ox = roff(x - 4);
-          ly.st(A.qxx_out + ox, cmv[9] + T(.5) * Dxx);
-          ly.st(A.qyx_out + ox, cmv[10] + T(.5) * Dyx);
-#pragma unroll
-          for (int v = 0; v < 9; ++v) ly.st(A.f_out + (v * N + ox), cmv[v] + T(.5) * om[v]);
-          if constexpr (EXACT) {
-            ly.st(A.phi_out + ox, cmv[11] + T(.5) * Dp);
-          } else {
-            // phi_new = phi_old + phic + (Dp_pred + Dp)/2 with a TwoSum, as
-            // the separate last corrector
-            const T pold = bold.pold;
-            const T yv = T(.5) * (cmv[11] + Dp) + bold.phic;
-            const T t = pold + yv;
-            const T bp = t - pold;
-            const T ap = t - bp;
-            const T err = (pold - ap) + (yv - bp);
-            ly.st(A.phi_out + ox, t);
-            ly.st(A.phic_out + ox, err);
-          }
+        for (int v = 0; v < 9; ++v) {
+          bfp[v] = ao1[v];
+          bfc[v] = ao2[v];
         }
+        NbrVals<T> bun;
+        CenterVals<T> bcn;
+        uq(bqb, bqc, qa_out, bfp, bun, bcn);
+        if (i >= 8) {
+          const UFStencil<T> s = uf_stencil(bum, bu0, bun);
+          T Dxx, Dyx, Dp, om[9];
+          update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
+          T cmv[12];  // half-step values of row x-4
+          ring_get(1, s2, cmv);
+          if (active) store_state(x - 4, cmv, Dxx, Dyx, Dp, om, bold);
+        }
+        bum = bu0;
+        bu0 = bun;
+        bc0 = bcn;
       }
-      bum = bu0;
-      bu0 = bun;
-      bc0 = bcn;
     }
-    load_old(x - 3, bold);
+    // old phi of the next iteration's output row
+    load_old(x + 1 - 2 * NPC, bold);
   }
 }
 
@@ -381,7 +405,7 @@ __device__ __forceinline__ typename Ring<T>::V* wave_ring() {
          ((int)threadIdx.x >> 6) * Ring<T>::kVecs;
 }
 
-template <class T, bool EXACT>
+template <class T, bool EXACT, int NPC>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : LYO_FULL_WAVES_PER_EU)))
 full_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
@@ -393,11 +417,11 @@ full_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
   const int wid = __builtin_amdgcn_readfirstlane(b * ((int)blockDim.x >> 6) +
                                                  ((int)threadIdx.x >> 6));
   if (wid >= g.nwaves) return;
-  full_item<T, EXACT>(A, P, g, wid, wave_ring<T>());
+  full_item<T, EXACT, NPC>(A, P, g, wid, wave_ring<T>());
 }
 
 // Batched one-kernel steps: persistent grid over the work items of many runs.
-template <class T, bool EXACT>
+template <class T, bool EXACT, int NPC>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : LYO_FULL_WAVES_PER_EU)))
 full_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const int nitems) {
@@ -406,17 +430,25 @@ full_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const i
                                             ((int)threadIdx.x >> 6));
   for (; item < nitems; item += nw) {
     const SweepJob<T>& J = find_job(jobs, njobs, item);
-    full_item<T, EXACT>(J.A, J.P, J.g, item - J.first, wave_ring<T>());
+    full_item<T, EXACT, NPC>(J.A, J.P, J.g, item - J.first, wave_ring<T>());
   }
 }
 
-template <class T, bool EXACT>
+// LDS of a block: the rings of its waves (NPC = 2 only)
+template <class T, int NPC>
+constexpr size_t full_lds_bytes(int waves_per_block) {
+  return NPC == 2 ? (size_t)waves_per_block * ring_bytes_per_wave<T>() : 0;
+}
+
+template <class T, bool EXACT, int NPC>
 inline hipError_t launch_full(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                               int waves_per_block, hipStream_t s) {
+  if (!geo_ok(g, NPC == 2 ? kFullStrip : kFull1Strip, 8) || waves_per_block < 1)
+    return hipErrorInvalidValue;
   const int threads = 64 * waves_per_block;
   const int blocks = (g.nwaves + waves_per_block - 1) / waves_per_block;
-  const size_t lds = (size_t)waves_per_block * ring_bytes_per_wave<T>();
-  hipLaunchKernelGGL((full_kernel<T, EXACT>), dim3(blocks), dim3(threads), lds, s, a, p, g);
+  const size_t lds = full_lds_bytes<T, NPC>(waves_per_block);
+  hipLaunchKernelGGL((full_kernel<T, EXACT, NPC>), dim3(blocks), dim3(threads), lds, s, a, p, g);
   return hipGetLastError();
 }
 

@@ -331,6 +331,7 @@ pair_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const i
 template <class T, bool EXACT, bool HB>
 inline hipError_t launch_pair(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                               int waves_per_block, hipStream_t s) {
+  if (!geo_ok(g, kPairStrip, 8) || waves_per_block < 1) return hipErrorInvalidValue;
   const int threads = 64 * waves_per_block;
   const int blocks = (g.nwaves + waves_per_block - 1) / waves_per_block;
   hipLaunchKernelGGL((pair_kernel<T, EXACT, HB>), dim3(blocks), dim3(threads), 0, s, a, p, g);

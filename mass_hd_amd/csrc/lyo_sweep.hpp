@@ -56,6 +56,15 @@ struct Geo {
   int alt;  // 1: even chunks march downwards, odd upwards; 0: all upwards
 };
 
+// Host-side check of a launch geometry against the strip width of its kernel
+// (and the rows a kernel's single wrap needs): the grid, the chunk and strip
+// counts and the plane stride all have to agree with what the kernel indexes.
+inline bool geo_ok(const Geo& g, int strip, int min_lx) {
+  if (g.LX < min_lx || g.LY < 1 || g.xc < 1 || g.N != g.LX * g.LY || g.NP < g.N) return false;
+  return g.nstrips == (g.LY + strip - 1) / strip && g.nchunks == (g.LX + g.xc - 1) / g.xc &&
+         g.nwaves == g.nstrips * g.nchunks;
+}
+
 enum Stage { PRED = 0, CORR = 1, CORR_LAST = 2 };
 
 // Buffer roles per stage (planes of N values; populations are 9 planes).
@@ -479,6 +488,7 @@ sweep_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const 
 template <class T, bool EXACT, int STAGE, bool HB>
 inline hipError_t launch_sweep(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                                int waves_per_block, hipStream_t s) {
+  if (!geo_ok(g, kStrip, 2) || waves_per_block < 1) return hipErrorInvalidValue;
   const int threads = 64 * waves_per_block;
   const int blocks = (g.nwaves + waves_per_block - 1) / waves_per_block;
   hipLaunchKernelGGL((sweep_kernel<T, EXACT, STAGE, HB>), dim3(blocks), dim3(threads), 0, s, a,

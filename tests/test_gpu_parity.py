@@ -155,10 +155,33 @@ def test_loop_kernels_are_fp64_only(require_gpu):
     g.set_loop_kernels(False)
 
 
+@pytest.mark.parametrize("precision", [lyo.FP64, lyo.FP32])
+@pytest.mark.parametrize("LX,LY,xc", [(128, 128, 32), (45, 67, 7), (5, 3, 1), (8, 60, 3),
+                                      (70, 113, 64)])
+def test_one_kernel_step_npc1_equals_sweeps(require_gpu, precision, LX, LY, xc):
+    """npc = 1: predictor and corrector in one kernel give the two sweeps'
+    state and frame fields bit for bit (LX < 8 runs the sweeps)."""
+    runs = []
+    for full in (0, 1):
+        g = lyo.Lyotropic(LX, LY, npc=1, precision=precision, **CONFIG0_PHYS)
+        g.set_schedule(one_kernel_step=full, step_chunk_rows=xc)
+        g.configure(**CONFIG0_INIT)
+        g.step(3)
+        mid = [g.get(n) for n in ("ux", "sigmaXY", "MU")]
+        g.step(4)
+        runs.append((g, mid))
+    (a, ma), (b, mb) = runs
+    for x, y in zip(ma, mb):
+        assert np.array_equal(x, y)
+    for n in STATE + DERIVED:
+        x, y = a.get(n), b.get(n)
+        assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
+
+
 def test_fused_pair_timing_kinds(require_gpu):
     # (predictor, separate corrector, fused pair, one-kernel step)
-    for npc, full, kinds in ((1, 1, (1, 1, 0, 0)), (2, 0, (1, 0, 1, 0)), (2, 1, (0, 0, 0, 1)),
-                             (3, 1, (1, 1, 1, 0))):
+    for npc, full, kinds in ((1, 0, (1, 1, 0, 0)), (1, 1, (0, 0, 0, 1)), (2, 0, (1, 0, 1, 0)),
+                             (2, 1, (0, 0, 0, 1)), (3, 1, (1, 1, 1, 0))):
         g = lyo.Lyotropic(64, 64, npc=npc, precision=lyo.FP32, **CONFIG0_PHYS)
         g.set_schedule(one_kernel_step=full)
         g.configure(**CONFIG0_INIT)
