@@ -133,7 +133,7 @@ void mass_lyo_destroy(mass_lyo_t h) {
   if (h->d_stage) (void)hipFree(h->d_stage);
   if (h->d_part) (void)hipFree(h->d_part);
   for (void* p : h->diag_allocs) (void)hipFree(p);
-  for (void* p : {(void*)h->sur.feat, (void*)h->sur.delta, (void*)h->sur.part})
+  for (void* p : {(void*)h->sur.delta, (void*)h->sur.part})
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -198,8 +198,11 @@ static int surrogate_step(mass_lyo_t h) {
   const bool judge = (S.count % S.judge_every) == 0;
   if (judge) {
     e = h->eng->steps(1, h->launch, h->stream, nullptr);
-    if (e == hipSuccess) e = h->eng->net_features(1, S.feat, h->stream);
-    if (e == hipSuccess) e = lyo::net_predict(S.net, S.feat, LX, LY, S.delta, h->stream);
+    __bf16* in = nullptr;
+    const float* nrm = nullptr;
+    if (e == hipSuccess) e = lyo::net_input(S.net, (size_t)LX * LY, &in, &nrm);
+    if (e == hipSuccess) e = h->eng->net_features(1, nullptr, nrm, in, h->stream);
+    if (e == hipSuccess) e = lyo::net_predict_input(S.net, LX, LY, S.delta, h->stream);
     const int nparts = 256;
     if (e == hipSuccess) e = h->eng->net_judge(S.delta, S.part, nparts, h->stream);
     double part[2 * 256];
@@ -229,8 +232,11 @@ static int surrogate_step(mass_lyo_t h) {
     S.stats.plain_steps++;
   } else {
     e = h->eng->steps_with(1, 1, h->launch, h->stream);
-    if (e == hipSuccess) e = h->eng->net_features(0, S.feat, h->stream);
-    if (e == hipSuccess) e = lyo::net_predict(S.net, S.feat, LX, LY, S.delta, h->stream);
+    __bf16* in = nullptr;
+    const float* nrm = nullptr;
+    if (e == hipSuccess) e = lyo::net_input(S.net, (size_t)LX * LY, &in, &nrm);
+    if (e == hipSuccess) e = h->eng->net_features(0, nullptr, nrm, in, h->stream);
+    if (e == hipSuccess) e = lyo::net_predict_input(S.net, LX, LY, S.delta, h->stream);
     if (e == hipSuccess) e = h->eng->net_apply(S.delta, h->stream);
     S.stats.surrogate_steps++;
   }
@@ -265,8 +271,7 @@ int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_ev
   auto& S = h->sur;
   hipError_t e = hipSuccess;
   const size_t N = (size_t)h->N;
-  if (!S.feat) e = hipMalloc((void**)&S.feat, N * 8 * sizeof(float));
-  if (e == hipSuccess && !S.delta) e = hipMalloc((void**)&S.delta, N * 8 * sizeof(float));
+  if (!S.delta) e = hipMalloc((void**)&S.delta, N * 8 * sizeof(float));
   if (e == hipSuccess && !S.part) e = hipMalloc((void**)&S.part, 2 * 256 * sizeof(double));
   if (e != hipSuccess) return hip_fail(h, e, "surrogate buffers");
   S.net = net;

@@ -613,8 +613,10 @@ class Engine {
                                   bool store_half, int grid_waves, int wpb,
                                   hipStream_t s) const = 0;
   // surrogate corrector hooks (lyo_net_engine.hpp).  src 0: current state;
-  // src 1: the one-corrector state of the last plain step (needs npc >= 2)
-  virtual hipError_t net_features(int src, float* feat, hipStream_t s) = 0;
+  // src 1: the one-corrector state of the last plain step (needs npc >= 2).
+  // Features FP32 into feat, or normalised bf16 into feat_bf when given
+  virtual hipError_t net_features(int src, float* feat, const float* norm, __bf16* feat_bf,
+                                  hipStream_t s) = 0;
   virtual hipError_t net_targets(float* tgt, hipStream_t s) = 0;
   virtual hipError_t net_apply(const float* delta, hipStream_t s) = 0;
   virtual hipError_t net_judge(const float* delta, double* part, int nparts, hipStream_t s) = 0;
@@ -737,7 +739,8 @@ class EngineT final : public Engine {
   }
   int parity() const override { return B_.cur; }
 
-  hipError_t net_features(int src, float* feat, hipStream_t s) override {
+  hipError_t net_features(int src, float* feat, const float* norm, __bf16* feat_bf,
+                          hipStream_t s) override {
     if (src != 0) {
       const hipError_t e = materialize_uq<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_, s);
       if (e != hipSuccess) return e;
@@ -763,8 +766,9 @@ class EngineT final : public Engine {
         dphi = B_.Cphi[0];
       }
     }
+    if (feat_bf && !norm) return hipErrorInvalidValue;
     hipLaunchKernelGGL((features_kernel<T, EXACT>), dim3(nb), dim3(bs), 0, s, f, qxx, qyx, phi,
-                       dphi, phic, P_, LX_, LY_, NP_, feat);
+                       dphi, phic, P_, LX_, LY_, NP_, feat, norm, feat_bf);
     return hipGetLastError();
   }
   hipError_t net_targets(float* tgt, hipStream_t s) override {

@@ -31,7 +31,6 @@ struct mass_lyo_s {
     double tol = 1e-3;
     long long count = 0;      // steps taken in surrogate mode
     int plain_left = 0;       // plain steps still owed after a failed judgement
-    float* feat = nullptr;    // [N][8]
     float* delta = nullptr;   // [N][8]
     double* part = nullptr;   // judge partial sums
     mass_lyo_surrogate_stats stats{};

@@ -327,6 +327,19 @@ hipError_t net_predict(mass_lyo_net_t n, const float* feat, int LX, int LY, floa
   return forward(n, LX, LY, false, s, feat, delta, n->d_norm + 16);
 }
 
+hipError_t net_input(mass_lyo_net_t n, size_t N, __bf16** in, const float** norm) {
+  const hipError_t e = ensure_ws(n, N);
+  if (e != hipSuccess) return e;
+  *in = n->a0;
+  *norm = n->d_norm;
+  return hipSuccess;
+}
+
+hipError_t net_predict_input(mass_lyo_net_t n, int LX, int LY, float* delta, hipStream_t s) {
+  if ((size_t)LX * LY > n->wsN) return hipErrorInvalidValue;
+  return forward(n, LX, LY, false, s, nullptr, delta, n->d_norm + 16);
+}
+
 bool net_ready(mass_lyo_net_t n) { return n && n->have_norm; }
 
 }  // namespace lyo
@@ -490,7 +503,7 @@ int mass_lyo_net_collect(mass_lyo_net_t n, mass_lyo_t h) {
   if (e == hipSuccess) e = hipMalloc((void**)&f, N * 32);
   if (e == hipSuccess) e = hipMalloc((void**)&t, N * 32);
   // the one-corrector state of the last (plain) step and its increments
-  if (e == hipSuccess) e = h->eng->net_features(1, f, h->stream);
+  if (e == hipSuccess) e = h->eng->net_features(1, f, nullptr, nullptr, h->stream);
   if (e == hipSuccess) e = h->eng->net_targets(t, h->stream);
   const int nb = 1024;
   std::vector<double> part(24 * nb);

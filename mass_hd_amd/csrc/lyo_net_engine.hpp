@@ -37,11 +37,15 @@ __device__ void uq_at(const T* f, const T* qxx, const T* qyx, const T* phi, cons
                               ph(kyp), ph(kym), P, nv, cv);
 }
 
+// Features of every node: FP32 [N][8] (feat), or -- surrogate inference --
+// normalised bf16 [N][8] (feat_bf, (x - mean[c]) * inv_std[c] with norm =
+// mean[8], inv_std[8]), the network's first layer input.
 template <class T, bool EXACT>
 __global__ void features_kernel(const T* __restrict__ f, const T* __restrict__ qxx,
                                 const T* __restrict__ qyx, const T* __restrict__ phi,
                                 const T* __restrict__ dphi, const T* __restrict__ phic,
-                                Params<T> P, int LX, int LY, int NP, float* __restrict__ feat) {
+                                Params<T> P, int LX, int LY, int NP, float* __restrict__ feat,
+                                const float* __restrict__ norm, __bf16* __restrict__ feat_bf) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= LX * LY) return;
   const int x = k / LY, y = k - x * LY;
@@ -52,10 +56,10 @@ __global__ void features_kernel(const T* __restrict__ f, const T* __restrict__ q
   uq_at<T, EXACT>(f, qxx, qyx, phi, dphi, NP, LX, LY, x == 0 ? LX - 1 : x - 1, y, P, xm, cv);
   uq_at<T, EXACT>(f, qxx, qyx, phi, dphi, NP, LX, LY, x, y + 1 == LY ? 0 : y + 1, P, yp, cv);
   uq_at<T, EXACT>(f, qxx, qyx, phi, dphi, NP, LX, LY, x, y == 0 ? LY - 1 : y - 1, P, ym, cv);
-  float* o = feat + (size_t)k * kFeat;
   double p = (double)phi[k];
   if (dphi) p += (double)dphi[k];
   if (phic) p += (double)phic[k];
+  float o[kFeat];
   o[0] = (float)c.ux;
   o[1] = (float)c.uy;
   o[2] = (float)qxx[k];
@@ -64,6 +68,17 @@ __global__ void features_kernel(const T* __restrict__ f, const T* __restrict__ q
   o[5] = (float)c.mu;
   o[6] = (float)(T(.5) * (xp.sxx - xm.sxx) + T(.5) * (yp.sxy - ym.sxy));
   o[7] = (float)(T(.5) * (xp.syx - xm.syx) + T(.5) * (yp.syy - ym.syy));
+  if (feat_bf) {
+    typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+    bf16x8_t v;
+#pragma unroll
+    for (int j = 0; j < kFeat; ++j) v[j] = (__bf16)((o[j] - norm[j]) * norm[8 + j]);
+    *reinterpret_cast<bf16x8_t*>(feat_bf + (size_t)k * kFeat) = v;
+  } else {
+    float* q = feat + (size_t)k * kFeat;
+#pragma unroll
+    for (int j = 0; j < kFeat; ++j) q[j] = o[j];
+  }
 }
 
 template <class T, bool SHIFT>

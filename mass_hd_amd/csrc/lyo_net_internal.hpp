@@ -9,6 +9,12 @@ namespace lyo {
 // delta (FP32 [N][8], physical units) = network(features FP32 [N][8]) on s
 hipError_t net_predict(mass_lyo_net_t n, const float* feat, int LX, int LY, float* delta,
                        hipStream_t s);
+// the network's first-layer input buffer for N nodes (normalised bf16
+// [N][8], allocated on first use) and its input normalisation (device:
+// mean[8], inv_std[8])
+hipError_t net_input(mass_lyo_net_t n, size_t N, __bf16** in, const float** norm);
+// delta (FP32 [N][8], physical units) = network(the input buffer) on s
+hipError_t net_predict_input(mass_lyo_net_t n, int LX, int LY, float* delta, hipStream_t s);
 // trained (normalisation known)
 bool net_ready(mass_lyo_net_t n);
 }  // namespace lyo
