@@ -71,7 +71,7 @@ def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-
     return total_ms, dict(zip(("predictor", "corrector", "pair", "step"), sweeps))
 
 
-def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3):
+def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500):
     """Surrogate corrector at LxL against the plain npc=2 step: a network
     trained briefly on frames of a 512^2 plain run (the full training is
     `python -m mass_hd_amd.workloads config3`), then `steps` surrogate steps
@@ -84,7 +84,7 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3):
         src.step(25)
         net.collect(src)
     src.close()
-    l0, l1 = net.train(500, lr=2e-3)
+    l0, l1 = net.train(train_iters, lr=2e-3)
     m = lyo.Lyotropic(L, L, npc=2, precision=lyo.FP32, **PHYS)
     m.configure(**INIT)
     m.step(5)
@@ -178,6 +178,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=20)
+    ap.add_argument("--surrogate-train", type=int, default=500,
+                    help="Adam iterations of the surrogate leg's training")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.calibrate:
@@ -226,7 +228,9 @@ def main():
                                                    "fields, UpdateFields, Move), SoA planes"})
 
     dom = max((k for k in k32 if k32[k] > 0), key=lambda k: k32[k])
-    alg_bytes = SWEEPS[dom] * SWEEP_VALUES * 4 * N
+    # the one-kernel step performs all 1 + npc sweeps of a step
+    sweeps = (1 + npc) if dom == "step" else SWEEPS[dom]
+    alg_bytes = sweeps * SWEEP_VALUES * 4 * N
     achieved = alg_bytes / (k32[dom] / 1e3)
     distinct = DISTINCT[dom] * 4 * N
     traffic = valu_ms = None
@@ -239,7 +243,8 @@ def main():
 
     if not args.skip_surrogate:
         log("surrogate %dx%d: training on 512^2 frames, %d timed steps" % (L, L, args.surrogate_steps))
-        extra["surrogate"] = surrogate_leg(lyo, L, args.surrogate_steps)
+        extra["surrogate"] = surrogate_leg(lyo, L, args.surrogate_steps,
+                                           train_iters=args.surrogate_train)
         log("surrogate: %s" % extra["surrogate"])
 
     cpu = None
@@ -254,7 +259,7 @@ def main():
         log("cpu: %.2f MLUPS" % v)
 
     line = {
-        "metric": "MLUPS at 4096^2 (FP32 plain, npc=2)",
+        "metric": "MLUPS at %d^2 (FP32 plain, npc=%d)" % (L, npc),
         "value": round(mlups32, 1),
         "unit": "MLUPS",
         "n_gpus": 1,
