@@ -213,6 +213,23 @@ int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chu
                           int32_t alternate, int32_t one_kernel_step,
                           int32_t step_chunk_rows);
 
+/* Binary checkpoint of a run's state, for restarting long runs exactly.  No
+ * reference counterpart: the reference writes text frames only
+ * (src/serialization.hpp:280-286), whose six digits cannot resume a run.  The
+ * image is a 64-byte header (magic "MASSLYO1", format version 1, precision,
+ * LX, LY, npc, number of planes, rho, byte count of the planes; little-endian
+ * host) followed by the raw device state as planes of LX*LY values (node
+ * k = y + LY*x): FP64 f (9 planes, directions of src/geometry.hpp:17-27), Qxx,
+ * Qyx, phi; FP32 the shifted populations f - w_i*rho (9), Qxx, Qyx, phi and
+ * phi's compensation term.  Loading needs the same precision, LX, LY and rho
+ * (npc may differ); a run continued from a checkpoint is bitwise the run that
+ * was not interrupted.  After a load the derived frame fields are zero until
+ * the next step, as after mass_lyo_set_state.  size: bytes of the image;
+ * save: bytes >= size; load: the image as saved. */
+int mass_lyo_checkpoint_size(mass_lyo_t h, uint64_t* bytes);
+int mass_lyo_checkpoint_save(mass_lyo_t h, void* buf, uint64_t bytes);
+int mass_lyo_checkpoint_load(mass_lyo_t h, const void* buf, uint64_t bytes);
+
 /* FP64 only: 1 runs every step as the reference's node loops, one kernel per
  * loop -- UpdateQuantities (storing its 23 derived fields), UpdateFields(true),
  * Move, then npc x (UpdateQuantities, UpdateFields(false)) -- and 0 (the

@@ -470,6 +470,75 @@ int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chu
   return MASS_LYO_OK;
 }
 
+namespace {
+struct CheckpointHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t precision;
+  int32_t LX, LY, npc, planes;
+  double rho;
+  uint64_t state_bytes;
+  uint8_t reserved[16];
+};
+static_assert(sizeof(CheckpointHeader) == 64, "64-byte checkpoint header");
+constexpr char kCheckpointMagic[8] = {'M', 'A', 'S', 'S', 'L', 'Y', 'O', '1'};
+}  // namespace
+
+int mass_lyo_checkpoint_size(mass_lyo_t h, uint64_t* bytes) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!bytes) return fail(h, MASS_LYO_EINVAL, "null size pointer");
+  *bytes = sizeof(CheckpointHeader) + h->eng->state_bytes();
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_checkpoint_save(mass_lyo_t h, void* buf, uint64_t bytes) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!buf) return fail(h, MASS_LYO_EINVAL, "null buffer");
+  if (!h->has_state) return fail(h, MASS_LYO_EINVAL, "no state yet: configure or set_state first");
+  const uint64_t need = sizeof(CheckpointHeader) + h->eng->state_bytes();
+  if (bytes < need) return fail(h, MASS_LYO_EINVAL, "checkpoint buffer too small");
+  CheckpointHeader hd{};
+  std::memcpy(hd.magic, kCheckpointMagic, 8);
+  hd.version = 1;
+  hd.precision = (uint32_t)h->prm.precision;
+  hd.LX = h->prm.LX;
+  hd.LY = h->prm.LY;
+  hd.npc = h->prm.npc;
+  hd.planes = h->eng->state_planes();
+  hd.rho = h->prm.rho;
+  hd.state_bytes = h->eng->state_bytes();
+  std::memcpy(buf, &hd, sizeof(hd));
+  const hipError_t e = h->eng->save_state(static_cast<char*>(buf) + sizeof(hd), h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "checkpoint save");
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_checkpoint_load(mass_lyo_t h, const void* buf, uint64_t bytes) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!buf) return fail(h, MASS_LYO_EINVAL, "null buffer");
+  CheckpointHeader hd;
+  if (bytes < sizeof(hd)) return fail(h, MASS_LYO_EINVAL, "not a checkpoint (too short)");
+  std::memcpy(&hd, buf, sizeof(hd));
+  if (std::memcmp(hd.magic, kCheckpointMagic, 8) != 0 || hd.version != 1)
+    return fail(h, MASS_LYO_EINVAL, "not a checkpoint of this format");
+  if ((int)hd.precision != h->prm.precision)
+    return fail(h, MASS_LYO_EINVAL, "checkpoint precision differs from the run's");
+  if (hd.LX != h->prm.LX || hd.LY != h->prm.LY)
+    return fail(h, MASS_LYO_EINVAL,
+                "checkpoint grid " + std::to_string(hd.LX) + "x" + std::to_string(hd.LY) +
+                    " differs from the run's");
+  if (hd.rho != h->prm.rho)
+    return fail(h, MASS_LYO_EINVAL, "checkpoint rho differs from the run's");
+  if (hd.planes != h->eng->state_planes() || hd.state_bytes != h->eng->state_bytes() ||
+      bytes < sizeof(hd) + hd.state_bytes)
+    return fail(h, MASS_LYO_EINVAL, "checkpoint truncated or of another layout");
+  const hipError_t e =
+      h->eng->load_state(static_cast<const char*>(buf) + sizeof(hd), h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "checkpoint load");
+  h->has_state = true;
+  return MASS_LYO_OK;
+}
+
 int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (on != 0 && on != 1) return fail(h, MASS_LYO_EINVAL, "on must be 0 or 1");

@@ -626,6 +626,14 @@ class Engine {
   // buffers of the reference loop structure (FP64 only): allocated on first
   // use, kept until release
   virtual hipError_t alloc_loops() = 0;
+  // raw state S[cur] (f 9 planes, Qxx, Qyx, phi, FP32: phi compensation) as
+  // contiguous planes of LX*LY values: its size, and copies to / from host
+  // memory (synchronous; after a load the frame's derived fields restart at
+  // zero)
+  virtual int state_planes() const = 0;
+  virtual size_t state_bytes() const = 0;
+  virtual hipError_t save_state(void* host, hipStream_t s) = 0;
+  virtual hipError_t load_state(const void* host, hipStream_t s) = 0;
 };
 
 template <class T, bool EXACT>
@@ -805,6 +813,34 @@ class EngineT final : public Engine {
     return run_steps<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_run, nsteps, L, s, nullptr);
   }
   int npc() const override { return npc_; }
+  int state_planes() const override { return EXACT ? 12 : 13; }
+  size_t state_bytes() const override {
+    return (size_t)state_planes() * (size_t)LX_ * (size_t)LY_ * sizeof(T);
+  }
+  hipError_t save_state(void* host, hipStream_t s) override {
+    T* planes[13];
+    const int n = state_plane_ptrs(planes);
+    const size_t pb = (size_t)LX_ * LY_ * sizeof(T);
+    char* h = static_cast<char*>(host);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n && e == hipSuccess; ++i, h += pb)
+      e = hipMemcpyAsync(h, planes[i], pb, hipMemcpyDeviceToHost, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+  }
+  hipError_t load_state(const void* host, hipStream_t s) override {
+    T* planes[13];
+    const int n = state_plane_ptrs(planes);
+    const size_t pb = (size_t)LX_ * LY_ * sizeof(T);
+    const char* h = static_cast<const char*>(host);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n && e == hipSuccess; ++i, h += pb)
+      e = hipMemcpyAsync(planes[i], h, pb, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    B_.have_uq = false;
+    B_.lazy_cur = -1;
+    B_.lazy_full = false;
+    return e;
+  }
   hipError_t alloc_loops() override {
     if (!EXACT) return hipErrorInvalidValue;
     if (B_.Ld) return hipSuccess;
@@ -863,6 +899,17 @@ class EngineT final : public Engine {
   }
 
  private:
+  // the state planes of S[cur] in checkpoint order
+  int state_plane_ptrs(T* (&planes)[13]) const {
+    const int cur = B_.cur;
+    int n = 0;
+    for (int v = 0; v < 9; ++v) planes[n++] = B_.Sf[cur] + (size_t)v * NP_;
+    planes[n++] = B_.Sqxx[cur];
+    planes[n++] = B_.Sqyx[cur];
+    planes[n++] = B_.Sphi[cur];
+    if (!EXACT) planes[n++] = B_.Sphic[cur];
+    return n;
+  }
   hipError_t get_(size_t n, T** p) {
     hipError_t e = hipMalloc((void**)p, n * sizeof(T));
     if (e == hipSuccess) {

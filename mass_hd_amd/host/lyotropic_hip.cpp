@@ -2,7 +2,9 @@
 #include "lyotropic_hip.hpp"
 
 #include <cmath>
+#include <fstream>
 #include <iostream>
+#include <iterator>
 #include <map>
 
 #include "error_msg.hpp"
@@ -70,6 +72,18 @@ void LyotropicHip::Initialize() {
 }
 
 void LyotropicHip::Configure() {
+  if (!restart_file.empty()) {
+    // state from a binary checkpoint; the check totals are its sums
+    std::ifstream in(restart_file, std::ios::binary);
+    if (!in) throw error_msg("cannot open restart file '", restart_file, "'");
+    const std::vector<char> image((std::istreambuf_iterator<char>(in)),
+                                  std::istreambuf_iterator<char>());
+    if (mass_lyo_checkpoint_load(h_, image.data(), image.size()) != MASS_LYO_OK)
+      throw error_msg("restart file '", restart_file, "': ", mass_lyo_last_error(h_));
+    if (mass_lyo_sums(h_, &ftot, &ptot) != MASS_LYO_OK) throw error_msg(mass_lyo_last_error(h_));
+    totalphi = ptot;
+    return;
+  }
   // lyotropic.cpp:69-166
   auto it = config_ids().find(init_config);
   if (it == config_ids().end())
@@ -111,6 +125,18 @@ void LyotropicHip::RuntimeChecks() {
   std::cout << "pcheck: " << pcheck << "/" << ptot << '\n';
 }
 
+void LyotropicHip::Post() {
+  if (checkpoint_file.empty()) return;
+  uint64_t n = 0;
+  if (mass_lyo_checkpoint_size(h_, &n) != MASS_LYO_OK) throw error_msg(mass_lyo_last_error(h_));
+  std::vector<char> image(n);
+  if (mass_lyo_checkpoint_save(h_, image.data(), n) != MASS_LYO_OK)
+    throw error_msg(mass_lyo_last_error(h_));
+  std::ofstream out(checkpoint_file, std::ios::binary | std::ios::trunc);
+  if (!out.write(image.data(), (std::streamsize)n))
+    throw error_msg("cannot write checkpoint file '", checkpoint_file, "'");
+}
+
 void LyotropicHip::RuntimeStats() {}
 
 option_list LyotropicHip::GetOptions() {
@@ -147,6 +173,9 @@ option_list LyotropicHip::GetOptions() {
   device.add_string("precision", &precision, "fp64 (reference arithmetic) or fp32")
       .default_value("fp64")
       .add_uint("chunk-rows", &chunk_rows, "rows per wavefront chunk of the sweeps (0 = auto)")
-      .default_value("0");
+      .default_value("0")
+      .add_string("restart", &restart_file,
+                  "binary checkpoint to start from instead of the initial configuration")
+      .add_string("checkpoint", &checkpoint_file, "write a binary checkpoint after the run");
   return {model, config, device};
 }

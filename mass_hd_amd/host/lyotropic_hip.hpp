@@ -30,6 +30,9 @@ class LyotropicHip : public Model {
   double Qamp = 1.0;
   double phinoise = 0.0;
   unsigned chunk_rows = 0;
+  // binary checkpoints (no reference counterpart): state loaded instead of the
+  // initial configuration / written after the run
+  std::string restart_file, checkpoint_file;
   // check totals (lyotropic.hpp:41-44)
   double ftot = 0, ptot = 0;
 
@@ -44,6 +47,7 @@ class LyotropicHip : public Model {
   void Initialize() override;
   void Configure() override;
   void Step() override;
+  void Post() override;
   void RuntimeChecks() override;
   void RuntimeStats() override;
   option_list GetOptions() override;

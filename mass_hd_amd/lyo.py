@@ -105,6 +105,9 @@ def load_library():
     lib.mass_lyo_set_schedule.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_int32]
     lib.mass_lyo_set_loop_kernels.argtypes = [h, ctypes.c_int32]
+    lib.mass_lyo_checkpoint_size.argtypes = [h, ctypes.POINTER(ctypes.c_uint64)]
+    lib.mass_lyo_checkpoint_save.argtypes = [h, ctypes.c_void_p, ctypes.c_uint64]
+    lib.mass_lyo_checkpoint_load.argtypes = [h, ctypes.c_void_p, ctypes.c_uint64]
     lib.mass_lyo_abi_version.argtypes = []
     lib.mass_lyo_diagnostics.argtypes = [h, P(Diag), dp]
     i32 = ctypes.c_int32
@@ -266,6 +269,21 @@ class Lyotropic:
         self._check(self.lib.mass_lyo_set_schedule(self.h, int(fuse_last_pair),
                                                    int(pair_chunk_rows), int(alternate),
                                                    int(one_kernel_step), int(step_chunk_rows)))
+
+    def checkpoint(self):
+        """binary image of the run's state (header + raw device planes);
+        restore() on a run of the same grid, precision and rho continues
+        bitwise as this run would"""
+        n = ctypes.c_uint64()
+        self._check(self.lib.mass_lyo_checkpoint_size(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(self.lib.mass_lyo_checkpoint_save(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, image):
+        image = bytes(image)
+        self._check(self.lib.mass_lyo_checkpoint_load(
+            self.h, ctypes.cast(ctypes.c_char_p(image), ctypes.c_void_p), len(image)))
 
     def set_loop_kernels(self, on=True):
         """FP64: run each step as the reference's node loops, one kernel per

@@ -197,3 +197,27 @@ def test_host_run_matches_oracle(card, tmp_path, require_gpu):
     for k in ("QQxx", "QQyx", "phi", "sigmaXY", "sigma_active_xx"):
         assert np.allclose(f20[k]["value"], o.get(k), rtol=1e-5, atol=1e-6), k
     assert np.allclose(np.array(f20["ff"]["value"]), o.get("ff"), rtol=1e-5)
+
+
+@pytest.mark.gpu
+def test_host_restart_from_checkpoint(card, tmp_path, require_gpu):
+    """20 steps + checkpoint, then a run restarted from it for 10 steps, ends
+    in the frame of one 30-step run (the same state: identical text)."""
+    full = tmp_path / "full"
+    rc, _, err = run([card(RUNCARD.replace("nsteps = 20", "nsteps = 30")), "-o", str(full), "-f"])
+    assert rc == 0, err
+    ckp = tmp_path / "state.ckp"
+    rc, _, err = run([card(RUNCARD + "checkpoint = %s\n" % ckp), "--no-write"])
+    assert rc == 0, err
+    assert ckp.exists() and ckp.read_bytes()[:8] == b"MASSLYO1"
+    part = tmp_path / "part"
+    rc, _, err = run([card(RUNCARD.replace("nsteps = 20", "nsteps = 10") +
+                           "restart = %s\n" % ckp), "-o", str(part), "-f"])
+    assert rc == 0, err
+    a = json.load(open(full / "frame30.json"))["data"]
+    b = json.load(open(part / "frame10.json"))["data"]
+    for k in ("ff", "QQxx", "QQyx", "phi"):
+        assert a[k]["value"] == b[k]["value"], k
+    rc, _, err = run([card(RUNCARD + "restart = %s\n" % (tmp_path / "missing")), "--no-write"])
+    assert rc == 1 and "restart file" in err
+
