@@ -158,8 +158,10 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="time plain 1 GiB device copies (HBM ceiling, PMC scale)")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # 200 + 500 steps: the throughput of a long run (the first few dozen steps
+    # after start-up run ~8% slower at 4096^2)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--npc", type=int, default=2)
     ap.add_argument("--chunk", type=int, default=0)
@@ -208,7 +210,8 @@ def main():
     if not args.skip_fp64:
         s64 = max(2, args.steps // 4)
         log("fp64 %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
-        t64, k64 = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, args.chunk, 0, fuse, alt,
+        w64 = max(2, args.warmup // 4)
+        t64, k64 = gpu_run(lyo, L, lyo.FP64, npc, s64, w64, args.chunk, 0, fuse, alt,
                            0 if args.no_full else -1, args.step_chunk, args.wpb)
         mlups64 = N * s64 / (t64 / 1e3) / 1e6
         log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
@@ -218,7 +221,7 @@ def main():
                  "fp64_launch_ms": {k: round(v, 4) for k, v in k64.items() if v}}
         if not args.skip_fp64_loops:
             log("fp64 loop kernels %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
-            tl, _ = gpu_run(lyo, L, lyo.FP64, npc, s64, 2, loops=True)
+            tl, _ = gpu_run(lyo, L, lyo.FP64, npc, s64, w64, loops=True)
             mlupsl = N * s64 / (tl / 1e3) / 1e6
             log("fp64 loop kernels: %.3f ms/step, %.0f MLUPS" % (tl / s64, mlupsl))
             extra.update({"fp64_loops_mlups": round(mlupsl, 1),
