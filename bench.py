@@ -71,11 +71,12 @@ def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-
     return total_ms, dict(zip(("predictor", "corrector", "pair", "step"), sweeps))
 
 
-def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500):
+def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warmup=200):
     """Surrogate corrector at LxL against the plain npc=2 step: a network
     trained briefly on frames of a 512^2 plain run (the full training is
     `python -m mass_hd_amd.workloads config3`), then `steps` surrogate steps
-    (the first one judged) timed with HIP events."""
+    timed with HIP events (one in judge_every of them a judged plain step),
+    against `steps` plain steps of the same run after `warmup` steps."""
     net = lyo.Net(seed=0)
     src = lyo.Lyotropic(512, 512, npc=2, precision=lyo.FP32, **PHYS)
     src.configure(**INIT)
@@ -87,11 +88,13 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500):
     l0, l1 = net.train(train_iters, lr=2e-3)
     m = lyo.Lyotropic(L, L, npc=2, precision=lyo.FP32, **PHYS)
     m.configure(**INIT)
-    m.step(5)
+    m.step(warmup)
     m.synchronize()
     t_plain, _ = m.time_steps(steps, per_sweep=False)
     lyo.enable_surrogate(m, net, judge_every, tol)
-    m.step(2)  # warm-up: a judged step and a surrogate step
+    # a judged step and a surrogate step; the timed steps then hold one
+    # judged step per judge_every steps
+    m.step(2)
     m.synchronize()
     t_sur, _ = m.time_steps(steps, per_sweep=False)
     st = lyo.surrogate_stats(m)
@@ -179,7 +182,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-surrogate", action="store_true")
-    ap.add_argument("--surrogate-steps", type=int, default=20)
+    ap.add_argument("--surrogate-steps", type=int, default=50,
+                    help="timed steps of the surrogate leg (50: one judged step among them)")
     ap.add_argument("--surrogate-train", type=int, default=500,
                     help="Adam iterations of the surrogate leg's training")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -247,7 +251,8 @@ def main():
     if not args.skip_surrogate:
         log("surrogate %dx%d: training on 512^2 frames, %d timed steps" % (L, L, args.surrogate_steps))
         extra["surrogate"] = surrogate_leg(lyo, L, args.surrogate_steps,
-                                           train_iters=args.surrogate_train)
+                                           train_iters=args.surrogate_train,
+                                           warmup=args.warmup)
         log("surrogate: %s" % extra["surrogate"])
 
     cpu = None
