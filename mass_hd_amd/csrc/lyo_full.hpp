@@ -27,6 +27,8 @@
 // predictor sweep + the separate corrector sweep.
 #pragma once
 
+#include <type_traits>
+
 #include "lyo_pair.hpp"
 
 namespace lyo {
@@ -222,7 +224,12 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
   NbrVals<T> bum{}, bu0{};
   CenterVals<T> bc0{};
 
-  for (int i = 0, x = xs, s0 = 0; i < nI; ++i, ++x, s0 = (s0 == 2) ? 0 : s0 + 1) {
+  // One row of the march (P on row x).  WARM: the first 4*NPC rows, where
+  // the later stages have no inputs yet; afterwards every stage runs and the
+  // rows go in threes, so that the rolling rows (three deep) and the ring
+  // slots (period three) are renamed rather than copied.
+  auto row = [&](const int i, const int x, const int s0, auto warm) {
+    constexpr bool WARM = decltype(warm)::value;
     const int s1 = (s0 == 0) ? 2 : s0 - 1;  // slot of iteration i-1
     const int s2 = (s1 == 0) ? 2 : s1 - 1;  // slot of iteration i-2
     // ---- P: UpdateQuantities on row x+1, UpdateFields on row x
@@ -310,11 +317,11 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
 
     // ---- A: UpdateQuantities on row x-1, UpdateFields on row x-2
     QP<T> qa_out{};
-    if (i >= 2) {  // wave-uniform
+    if (!WARM || i >= 2) {  // wave-uniform
       NbrVals<T> aun;
       CenterVals<T> acn;
       uq(aqb, aqc, aq_new, fst, aun, acn);
-      if (i >= 4) {
+      if (!WARM || i >= 4) {
         // half-step values of row x-2 (NPC = 2: kept for B two rows later)
         T cen[12];
 #pragma unroll
@@ -365,7 +372,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
 
     // ---- B: UpdateQuantities on row x-3, UpdateFields on row x-4
     if constexpr (NPC == 2) {
-      if (i >= 6) {  // wave-uniform
+      if (!WARM || i >= 6) {  // wave-uniform
         T ao1[12], ao2[12];  // A's outputs of rows x-3, x-4
         ring_get(0, s1, ao1);
         ring_get(0, s2, ao2);
@@ -379,7 +386,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         NbrVals<T> bun;
         CenterVals<T> bcn;
         uq(bqb, bqc, qa_out, bfp, bun, bcn);
-        if (i >= 8) {
+        if (!WARM || i >= 8) {
           const UFStencil<T> s = uf_stencil(bum, bu0, bun);
           T Dxx, Dyx, Dp, om[9];
           update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
@@ -394,7 +401,16 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     }
     // old phi of the next iteration's output row
     load_old(x + 1 - 2 * NPC, bold);
+  };
+  constexpr int kWarm = 4 * NPC;  // nI >= kWarm + 1: a chunk has a row
+  int i = 0;
+  for (; i < kWarm; ++i) row(i, xs + i, i % 3, std::true_type{});
+  for (; i + 3 <= nI; i += 3) {
+    row(i, xs + i, kWarm % 3, std::false_type{});
+    row(i + 1, xs + i + 1, (kWarm + 1) % 3, std::false_type{});
+    row(i + 2, xs + i + 2, (kWarm + 2) % 3, std::false_type{});
   }
+  for (; i < nI; ++i) row(i, xs + i, i % 3, std::false_type{});
 }
 
 // Dynamic LDS of a launch: one pair of rings per wave of the block.
