@@ -53,8 +53,9 @@ typedef struct mass_lyo_params {
   int32_t npc;          /* corrector sweeps, >= 1 */
   int32_t precision;    /* enum mass_lyo_precision */
   int32_t conserve_phi; /* must be 0: plain Lyotropic (lyotropic.hpp:46) */
-  int32_t plane_pad;    /* extra elements between SoA planes (0 = none);
-                           results do not depend on it */
+  int32_t plane_pad;    /* extra elements between SoA planes, beyond the
+                           library's own stagger of 256 (0 = none); results
+                           do not depend on it */
   double GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
   double AA, CC, LL, KK, zeta, zetaI, conc;
 } mass_lyo_params;
