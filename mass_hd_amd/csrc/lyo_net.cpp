@@ -242,7 +242,6 @@ hipError_t conv(mass_lyo_net_t n, const __bf16* in, int l, bool bwd, int epi, fl
   a.cout = bwd ? kCh[l] : kCh[l + 1];
   a.LX = LX;
   a.LY = LY;
-  a.epi = epi;
   // the layer shapes and epilogues of the network (forward: 8->32, 32->32
   // with GELU, 32->8 linear; data gradient: 32->32 and 8->32)
   const bool z = zout != nullptr;

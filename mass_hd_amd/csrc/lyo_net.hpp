@@ -74,7 +74,6 @@ struct ConvArgs {
   const float* out_scale;  // EPI_LINEAR: [cout] factors of the output, or null
   int cin, cout;           // 8 or 32
   int LX, LY;              // LY % 16 == 0
-  int epi;
 };
 
 // Raw input fragment of one lane: 8 channels of one pixel, bf16 or (layer 0
