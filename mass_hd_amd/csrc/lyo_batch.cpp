@@ -297,7 +297,8 @@ int mass_lyo_batch_autotune(mass_lyo_batch_t b, int32_t* chunk_rows, int32_t* gr
   // chunk rows per stage kind: the one-kernel step repeats eight rows per
   // chunk (four with one corrector), the sweeps two
   const bool full = b->plan.size() == 1 && lyo::is_full_stage(b->plan[0]);
-  const std::vector<int> xcs = full ? std::vector<int>{16, 32, 64} : std::vector<int>{4, 8, 16, 32};
+  const std::vector<int> xcs = full ? std::vector<int>{16, 24, 32, 48, 64, 96, 128}
+                                    : std::vector<int>{4, 8, 16, 32};
   const int grids[] = {0, 2048, 4096, 8192, 16384};
   double best = 1e300;
   int bx = b->launch.xc, bg = b->grid_waves;

@@ -51,7 +51,7 @@ def test_batch_equals_solo_runs(require_gpu, precision, npc, launch):
 def test_batch_autotune_then_run(require_gpu):
     b = lyo.Batch(RUNS, npc=2, precision="fp32")
     chunk, grid = b.autotune()
-    assert chunk in (16, 32, 64)  # candidates of the one-kernel step
+    assert chunk in (16, 24, 32, 48, 64, 96, 128)  # candidates of the one-kernel step
     for i in range(len(RUNS)):
         b.configure(i, **dict(CONFIG0_INIT, seed=7 + i))
     b.step(4)
