@@ -94,7 +94,7 @@ def train_surrogate(frames_per_run=8, spacing=50, warm=200, iters=2000, lr=2e-3,
     log("  trained %d iterations in %.1fs: loss %.4g -> %.4g" % (iters, time.time() - t0, l0, l1))
     b.close()
     return net, {"frames": net.dataset_size, "pairs": pairs, "runs": len(runs), "iters": iters,
-                 "loss_first": l0, "loss_last": l1}
+                 "warm": warm, "spacing": spacing, "loss_first": l0, "loss_last": l1}
 
 
 def compare(L, net, steps, zeta=None, seed=7, judge_every=50, tol=1e-3, every=None,
@@ -144,13 +144,26 @@ def main(argv=None):
     ap.add_argument("--fp64-steps", type=int, default=10000)
     ap.add_argument("--every", type=int, default=1000)
     ap.add_argument("--out", default="")
+    # training frames: config3 (2,000 steps) from the first 600 steps of the
+    # 512^2 runs; config4 (10^5 steps) from steps 1,500-5,000, the developed
+    # flow (frames from the first 600 steps only let the long 4096^2 run drift:
+    # u_rms +26%, defect density +33% at 5x10^4 steps, against +0.4% / +4%)
+    ap.add_argument("--warm", type=int, default=0,
+                    help="plain steps of the training runs before their first frame "
+                         "(0: 200 for config3, 1000 for config4)")
+    ap.add_argument("--spacing", type=int, default=0,
+                    help="steps between training frames (0: 50 for config3, 500 for config4)")
+    ap.add_argument("--frames", type=int, default=8, help="training frames per run")
     a = ap.parse_args(argv)
     if a.what == "config1":
         r = precision_report(1024, a.steps or 10000, 2, 10, progress=log)
     elif a.what == "config2":
         r = config2(a.steps or 5000)
     else:
-        net, tr = train_surrogate(iters=a.iters)
+        late = a.what == "config4"
+        net, tr = train_surrogate(frames_per_run=a.frames,
+                                  spacing=a.spacing or (500 if late else 50),
+                                  warm=a.warm or (1000 if late else 200), iters=a.iters)
         if a.what == "config3":
             r = {"training": tr, "run": compare(512, net, a.steps or 2000)}
         else:
