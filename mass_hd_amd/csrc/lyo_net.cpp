@@ -24,6 +24,10 @@ using lyo::net::ConvArgs;
 namespace {
 
 constexpr int kCh[5] = {8, 32, 32, 32, 8};
+// weight gradients: waves of the partial sums, and the largest layer's
+// parameter count (9 * 32 * 32 + 32)
+constexpr int kWgradWaves = 512;
+constexpr int kWgradMaxPar = 9 * 32 * 32 + 32;
 constexpr int kOut = 5;  // used output channels (du_x, du_y, dQxx, dQyx, dphi)
 
 int layer_woff(int l) {
@@ -131,6 +135,7 @@ struct mass_lyo_net_s {
   __bf16 *a0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr, *dz = nullptr, *dz2 = nullptr,
          *d4 = nullptr;
   float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *out = nullptr, *y = nullptr;
+  float* wpart = nullptr;  // per-wave partial weight gradients
   double* part = nullptr;
   hipStream_t stream = nullptr;
   std::string err;
@@ -152,10 +157,10 @@ int nhip(mass_lyo_net_t n, hipError_t e, const char* what) {
 void free_ws(mass_lyo_net_t n) {
   for (void* p : {(void*)n->a0, (void*)n->a1, (void*)n->a2, (void*)n->a3, (void*)n->dz,
                   (void*)n->dz2, (void*)n->d4, (void*)n->z1, (void*)n->z2, (void*)n->z3,
-                  (void*)n->out, (void*)n->y})
+                  (void*)n->out, (void*)n->y, (void*)n->wpart})
     if (p) (void)hipFree(p);
   n->a0 = n->a1 = n->a2 = n->a3 = n->dz = n->dz2 = n->d4 = nullptr;
-  n->z1 = n->z2 = n->z3 = n->out = n->y = nullptr;
+  n->z1 = n->z2 = n->z3 = n->out = n->y = n->wpart = nullptr;
   n->wsN = 0;
 }
 
@@ -178,6 +183,7 @@ hipError_t ensure_ws(mass_lyo_net_t n, size_t N) {
   get(N * 32 * 4, (void**)&n->z3);
   get(N * 8 * 4, (void**)&n->out);
   get(N * 8 * 4, (void**)&n->y);
+  get(sizeof(float) * (size_t)kWgradWaves * kWgradMaxPar, (void**)&n->wpart);
   if (e == hipSuccess) n->wsN = N;
   return e;
 }
@@ -290,17 +296,26 @@ hipError_t backward(mass_lyo_net_t n, int LX, int LY, int nparts) {
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(loss_grad_kernel, dim3(nparts), dim3(256), 0, n->stream, n->out, n->y, N,
                      n->d4, n->part);
-  const int tiles = LX * (LY / 16);
-  const int tpb = 16, wblocks = (tiles + tpb - 1) / tpb;
+  const int segs = LX * (LY / 16);
+  const int nw = segs < kWgradWaves ? segs : kWgradWaves;
   const __bf16* ins[4] = {n->a0, n->a1, n->a2, n->a3};
   const float* zs[4] = {nullptr, n->z1, n->z2, n->z3};
   const __bf16* d = n->d4;
   __bf16* bufs[2] = {n->dz, n->dz2};
   for (int l = 3; l >= 0 && e == hipSuccess; --l) {
-    float* dW = n->d_grads + layer_woff(l);
-    float* db = dW + 9 * kCh[l] * kCh[l + 1];
-    hipLaunchKernelGGL(conv_wgrad_kernel, dim3(wblocks), dim3(256), 0, n->stream, ins[l], d, kCh[l],
-                       kCh[l + 1], LX, LY, tpb, dW, db);
+    // weights and bias of layer l: per-wave partial sums, then their sum
+    const int npar = 9 * kCh[l] * kCh[l + 1] + kCh[l + 1];
+    if (l == 0)
+      hipLaunchKernelGGL((conv_wgrad_mfma_kernel<8, 32>), dim3(nw), dim3(64), 0, n->stream,
+                         ins[l], d, LX, LY, n->wpart);
+    else if (l == 3)
+      hipLaunchKernelGGL((conv_wgrad_mfma_kernel<32, 8>), dim3(nw), dim3(64), 0, n->stream,
+                         ins[l], d, LX, LY, n->wpart);
+    else
+      hipLaunchKernelGGL((conv_wgrad_mfma_kernel<32, 32>), dim3(nw), dim3(64), 0, n->stream,
+                         ins[l], d, LX, LY, n->wpart);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((npar + 31) / 32), dim3(256), 0, n->stream,
+                       n->wpart, nw, npar, n->d_grads + layer_woff(l));
     e = hipGetLastError();
     if (l > 0 && e == hipSuccess) {
       __bf16* dn = bufs[l & 1];

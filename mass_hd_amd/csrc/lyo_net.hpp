@@ -300,67 +300,144 @@ __global__ void __launch_bounds__(64) conv_rows_kernel(const ConvArgs a) {
   }
 }
 
-// Weight gradient (FP32, LDS-staged): dW[t][co][ci] += sum_p d[p][co] *
-// in[p + off(t)][ci] and db[co] += sum_p d[p][co] over the 16-pixel row tiles
-// [tile0, tile0 + tiles) of one block.  One thread per (co, ci) pair.
-__global__ void __launch_bounds__(256)
-conv_wgrad_kernel(const __bf16* __restrict__ in, const __bf16* __restrict__ d, int cin, int cout,
-                  int LX, int LY, int tiles_per_block, float* __restrict__ dW,
-                  float* __restrict__ db) {
-  __shared__ float sd[16][32];
-  __shared__ float sin_[3][18][32];  // rows x-1..x+1, columns y0-1..y0+16
-  const int tiles_per_row = LY >> 4;
-  const int ntiles = LX * tiles_per_row;
-  const int pairs = cin * cout;
-  float accw[4][kTaps];  // up to 4 (co,ci) pairs per thread (32*32/256)
-  float accb = 0.f;
+// Weight and bias gradients of one layer on MFMA:
+//   dW[t][co][ci] = sum_p d[p][co] * in[p + off(t)][ci],  db[co] = sum_p d[p][co]
+// with v_mfma_f32_16x16x16bf16_1k, M = 16 output channels, N = 16 input
+// channels and K = 16 consecutive pixels of a row.  Those operands need four
+// consecutive pixels of one channel per lane (lane l = 16q + i: A[i][4q..4q+3],
+// B[4q..4q+3][i]), while the activations are [pixel][channel]: each 16-pixel x
+// 32-channel fragment (lane: pixel i, channels 8q..8q+7) is transposed by a
+// 16x16x32 MFMA against a 0/1 matrix selecting 16 channels, which leaves lane
+// (q, i) with channel i of pixels 4q..4q+3 -- exactly, one product per sum.
+// Each wave accumulates a grid-stride set of 16-pixel row segments and writes
+// its partial sums to part[wave][9*COUT*CIN + COUT] (the layer's parameter
+// order); wgrad_reduce_kernel adds the partials of all waves in wave order.
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(64)
+conv_wgrad_mfma_kernel(const __bf16* __restrict__ in, const __bf16* __restrict__ d, int LX, int LY,
+                       float* __restrict__ part) {
+  constexpr int MB = COUT >= 16 ? COUT / 16 : 1;  // blocks of output channels
+  constexpr int NB = CIN >= 16 ? CIN / 16 : 1;    // blocks of input channels
+  constexpr int NPAR = kTaps * COUT * CIN + COUT;
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, i = lane & 15;
+  // selection matrices: B[k][n] = 1 where k = 16*blk + n (lane: B[8q..8q+7][i])
+  bf16x8 sel[2];
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int blk = 0; blk < 2; ++blk)
 #pragma unroll
-    for (int t = 0; t < kTaps; ++t) accw[j][t] = 0.f;
-  const int t0 = blockIdx.x * tiles_per_block;
-  for (int tile = t0; tile < min(t0 + tiles_per_block, ntiles); ++tile) {
-    const int x = tile / tiles_per_row;
-    const int y0 = (tile - x * tiles_per_row) << 4;
-    __syncthreads();
-    for (int e = threadIdx.x; e < 16 * cout; e += 256) {
-      const int p = e / cout, c = e - p * cout;
-      sd[p][c] = (float)d[((size_t)x * LY + y0 + p) * cout + c];
-    }
-    for (int e = threadIdx.x; e < 3 * 18 * cin; e += 256) {
-      const int c = e % cin, r = e / cin;
-      const int row = r / 18, col = r % 18;
-      int xx = x + row - 1, yy = y0 + col - 1;
-      xx += (xx < 0) ? LX : 0;
-      xx -= (xx >= LX) ? LX : 0;
-      yy += (yy < 0) ? LY : 0;
-      yy -= (yy >= LY) ? LY : 0;
-      sin_[row][col][c] = (float)in[((size_t)xx * LY + yy) * cin + c];
-    }
-    __syncthreads();
+    for (int j = 0; j < 8; ++j) sel[blk][j] = (__bf16)(8 * q + j == 16 * blk + i ? 1.f : 0.f);
+  // the transpose of a natural fragment, as a bf16 operand of the K=16 MFMA
+  auto transpose = [&](const bf16x8& nat, int blk) {
+    const f32x4 t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nat, sel[blk], f32x4{0.f, 0.f, 0.f, 0.f},
+                                                           0, 0, 0);
+    s16x4 o;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int pr = threadIdx.x + 256 * j;
-      if (pr >= pairs) break;
-      const int co = pr / cin, ci = pr - co * cin;
-      for (int p = 0; p < 16; ++p) {
-        const float dv = sd[p][co];
+    for (int r = 0; r < 4; ++r) o[r] = __builtin_bit_cast(short, (__bf16)t[r]);
+    return o;
+  };
+  f32x4 acc[kTaps][MB][NB];
 #pragma unroll
-        for (int t = 0; t < kTaps; ++t) accw[j][t] += dv * sin_[t / 3][p + t % 3][ci];
+  for (int t = 0; t < kTaps; ++t)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[t][mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) bsum[mb] = 0.f;
+
+  const int ytiles = LY >> 4;
+  const int nseg = LX * ytiles;
+  for (int seg = (int)blockIdx.x; seg < nseg; seg += (int)gridDim.x) {
+    const int x = seg / ytiles;
+    const int y0 = (seg - x * ytiles) << 4;
+    // d of the segment: pixel y0+i, channels 8q..8q+7 (zero past COUT)
+    bf16x8 dn = bf16x8{};
+    if (8 * q < COUT)
+      dn = *reinterpret_cast<const bf16x8*>(d + ((size_t)x * LY + y0 + i) * COUT + 8 * q);
+    s16x4 dt[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const f32x4 t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(dn, sel[mb],
+                                                             f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        dt[mb][r] = __builtin_bit_cast(short, (__bf16)t[r]);
+        bsum[mb] += t[r];
       }
     }
-    if (threadIdx.x < cout)
-      for (int p = 0; p < 16; ++p) accb += sd[p][threadIdx.x];
-  }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int pr = threadIdx.x + 256 * j;
-    if (pr >= pairs) break;
-    const int co = pr / cin, ci = pr - co * cin;
+    for (int dxi = 0; dxi < 3; ++dxi) {
+      int xr = x + dxi - 1;
+      xr += xr < 0 ? LX : 0;
+      xr -= xr >= LX ? LX : 0;
 #pragma unroll
-    for (int t = 0; t < kTaps; ++t) atomicAdd(&dW[(t * cout + co) * cin + ci], accw[j][t]);
+      for (int dyi = 0; dyi < 3; ++dyi) {
+        int yy = y0 + i + dyi - 1;
+        yy += yy < 0 ? LY : 0;
+        yy -= yy >= LY ? LY : 0;
+        bf16x8 an = bf16x8{};
+        if (8 * q < CIN)
+          an = *reinterpret_cast<const bf16x8*>(in + ((size_t)xr * LY + yy) * CIN + 8 * q);
+        const int t = 3 * dxi + dyi;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          const s16x4 at = transpose(an, nb);
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb)
+            acc[t][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(dt[mb], at, acc[t][mb][nb],
+                                                                       0, 0, 0);
+        }
+      }
+    }
   }
-  if (threadIdx.x < cout) atomicAdd(&db[threadIdx.x], accb);
+  // lane (q, i) holds dW_t[co = 16 mb + 4q + r][ci = 16 nb + i]
+  float* o = part + (size_t)blockIdx.x * NPAR;
+#pragma unroll
+  for (int t = 0; t < kTaps; ++t)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int co = 16 * mb + 4 * q + r, ci = 16 * nb + i;
+          if (co < COUT && ci < CIN) o[(t * COUT + co) * CIN + ci] = acc[t][mb][nb][r];
+        }
+  // bias: channel 16 mb + i, summed over the four lane quarters
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) {
+    float v = bsum[mb];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    const int co = 16 * mb + i;
+    if (q == 0 && co < COUT) o[kTaps * COUT * CIN + co] = v;
+  }
+}
+
+// out[e] = sum over waves w of part[w][e], e < n: a block takes 32
+// consecutive elements; its eight groups of 32 threads add the waves
+// w = g, g + 8, ... (coalesced rows), then the eight group sums are added in
+// group order (a fixed order: the result does not depend on timing).
+__global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float* __restrict__ part, int nw,
+                                                           int n, float* __restrict__ out) {
+  __shared__ float red[8][32];
+  const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int e = blockIdx.x * 32 + c;
+  float s = 0.f;
+  if (e < n)
+    for (int w = g; w < nw; w += 8) s += part[(size_t)w * n + e];
+  red[g][c] = s;
+  __syncthreads();
+  if (g == 0 && e < n) {
+    float t = red[0][c];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) t += red[k][c];
+    out[e] = t;
+  }
 }
 
 // Loss of the output layer: out (f32 [N][8]) against y (f32 [N][8]);
