@@ -261,6 +261,86 @@ def test_active_force_closed_form():
     assert np.allclose(ff1.sum(1), ff0.sum(1), rtol=1e-15)
 
 
+def test_isotropic_active_force_closed_form():
+    """Isotropic activity only: sigma_xx = sigma_yy = zetaI (conc - phi), so on
+    a phi mode each node's momentum grows by F = div sigma = -zetaI grad phi
+    (central differences), and the isotropic frame stress carries the term."""
+    LX, LY = 24, 5
+    x, y, k = _mode(LX, LY)
+    eps, zi, conc, p0 = 0.03, 0.07, 0.6, 0.4
+    phi = p0 + eps * np.cos(k * x)
+    o = po.Oracle(LX, LY, **phys(zetaI=zi, conc=conc, tauNem=1.3, tauIso=1.3))
+    ff0 = rest_state(LX, LY)
+    o.set_state(ff0.reshape(-1), np.zeros(LX * LY), np.zeros(LX * LY), phi)
+    o.update_quantities()
+    iso = zi * (conc - phi)
+    for f in ("sigmaXX", "sigmaYY", "sigma_bulk"):
+        assert np.allclose(o.get(f), iso, atol=1e-17), f
+    assert np.allclose(o.get("sigmaXY"), 0, atol=1e-18)
+    o.update_fields(True)
+    ff1 = o.get("ff")
+    Fx = zi * eps * math.sin(k) * np.sin(k * x)  # -zetaI * (phi(x+1) - phi(x-1)) / 2
+    assert np.allclose(((ff1 - ff0) * C[:, 0]).sum(1), Fx, atol=1e-15)
+    assert np.allclose(((ff1 - ff0) * C[:, 1]).sum(1), 0, atol=1e-15)
+
+
+def test_bulk_q_term_is_heun_on_the_uniform_ode():
+    """Uniform, motionless state with the bulk coupling CC: nothing but Q
+    moves, by the ODE dQ/dt = GammaQ H, H = -CC |Q|^2 Q (|Q|^2 = Qxx^2 + Qyx^2).
+    A step with one corrector is one Heun (explicit trapezoid) step of it, and
+    many steps follow the exact solution Q0 / sqrt(1 + 2 GammaQ CC |Q0|^2 t)
+    to Heun's second order."""
+    LX, LY = 4, 3
+    N = LX * LY
+    G, cc = 0.1, 0.5
+    q0 = np.array([0.6, -0.3])
+    o = po.Oracle(LX, LY, npc=1, **phys(GammaQ=G, CC=cc, tauNem=1.0, tauIso=1.0))
+    o.set_state(rest_state(LX, LY).reshape(-1), np.full(N, q0[0]), np.full(N, q0[1]),
+                np.full(N, 0.3))
+
+    def F(q):
+        return G * (-cc * (q @ q)) * q
+
+    q1 = q0 + 0.5 * (F(q0) + F(q0 + F(q0)))
+    o.step(1)
+    assert np.allclose(o.get("QQxx"), q1[0], rtol=1e-14)
+    assert np.allclose(o.get("QQyx"), q1[1], rtol=1e-14)
+    assert np.allclose(o.get("ff"), rest_state(LX, LY), rtol=1e-14)
+    assert np.allclose(o.get("phi"), 0.3, rtol=1e-15)
+    steps = 2000
+    o.step(steps - 1)
+    a = G * cc
+    exact = q0 / math.sqrt(1 + 2 * a * (q0 @ q0) * steps)
+    got = np.array([o.get("QQxx")[0], o.get("QQyx")[0]])
+    err = np.abs(got - exact).max() / np.abs(exact).max()
+    assert err < 2e-4, err  # a wrong coefficient or sign is off by O(1)
+
+
+def test_bulk_stress_q_part_form():
+    """The Q part of the isotropic bulk stress is linear in CC, quartic in |Q|
+    and positive: sigma_bulk(CC, Q) - sigma_bulk(0, Q) = c CC |Q|^4, c > 0,
+    and mu does not depend on Q."""
+    LX, LY = 3, 4
+    N = LX * LY
+
+    def sb(cc, q):
+        o = po.Oracle(LX, LY, **phys(AA=0.2, CC=cc))
+        o.set_state(rest_state(LX, LY).reshape(-1), np.full(N, q[0]), np.full(N, q[1]),
+                    np.full(N, 0.35))
+        o.update_quantities()
+        return o.get("sigma_bulk")[0], o.get("MU")[0]
+
+    base, mu0 = sb(0.0, (0.3, 0.4))
+    d1, mu1 = sb(0.8, (0.3, 0.4))
+    d2, _ = sb(1.6, (0.3, 0.4))
+    d3, _ = sb(0.8, (0.6, 0.8))
+    c = (d1 - base) / (0.8 * 0.25 ** 2)  # |Q|^2 = 0.25
+    assert c > 0
+    assert d2 - base == pytest.approx(2 * (d1 - base), rel=1e-12)
+    assert d3 - base == pytest.approx(16 * (d1 - base), rel=1e-12)
+    assert mu1 == mu0
+
+
 def test_elastic_stress_linear_order():
     """Small Q: sigma_xx = -2 xi H_xx + O(Q^2), sigma_xy - sigma_yx =
     2 sigmaA = 4 (Qxx Hyx - Qyx Hxx)."""
