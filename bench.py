@@ -2,26 +2,31 @@
 
 Workload (the large single-domain case): 4096 x 4096 periodic D2Q9, FP32 with
 shifted populations and compensated phi, npc = 2 (predictor + 2 correctors),
-zeta = 0.03, init seed 7 (random director, phi = 0.5 + U(-0.05, 0.05)).
-The same grid is also timed in FP64 (reference arithmetic).
+zeta = 0.03, init seed 7 (random director, phi = 0.5 + U(-0.05, 0.05)),
+200 warm-up + 500 timed steps.  The same grid is also timed in FP64 (the
+reference arithmetic) with the fused kernels and with one kernel per
+reference loop, and in surrogate mode (predictor + one corrector + the
+network, judged every 50 steps).
 
 MLUPS = LX*LY*steps / seconds / 1e6; one step = predictor + npc correctors
-(the last two correctors run as one fused kernel).  The timed region is
-bracketed by HIP events on the library's stream; the fields are resident in
-HBM throughout.
+(with npc <= 2 the whole step is one kernel).  The timed region is bracketed
+by HIP events on the library's stream; the fields are resident in HBM
+throughout.
 
 roofline: the kernel with the longest mean launch.  Its algorithmic bytes are
 the per-sweep figure of the step's data flow -- one sweep reads and writes the
 model state and the predictor's half-step state once, 36 values per node
 (predictor: read f, Q, phi = 12, write f*, fn*, Q+D, Q+D/2, phi terms = 24;
 corrector: read 24, write 12) -- times the sweeps one launch performs: 1 for
-the predictor, 2 for the fused last corrector pair.  In FP32 that is 144 B
-per node per sweep against 8 TB/s.  The fused pair keeps its first sweep's
-outputs in registers, so it moves fewer distinct bytes than two sweeps'
-worth; "distinct_bytes_per_launch" counts every plane it touches once (pair:
-read f*, f_old 18, Q*, Q_old 4, phi, Dp, phic 3; write f, Q, phi, phic 13 = 38
-values; predictor: read 12, write f*, Q+D, Dp 12 = 24 values), and "traffic"
-is the HBM bytes the counters measured per launch (profiles/).
+a predictor or corrector sweep, 2 for the fused last corrector pair, 1 + npc
+for the one-kernel step.  In FP32 that is 144 B per node per sweep against
+8 TB/s.  The fused kernels keep intermediate results in registers and LDS, so
+they move fewer distinct bytes than their sweeps' worth:
+"distinct_bytes_per_launch" counts every plane a launch touches once (one-
+kernel step: read and write f', Q, phi, phic = 26 values; pair: 38;
+predictor: 24), "traffic" is the HBM bytes the counters measured per launch
+and "valu_busy_frac" the counted VALU instructions' issue time over the
+launch time (profiles/).
 
 cpu_baseline: the CPU oracle (the reference's algorithm and layout, OpenMP)
 on the host cores.
