@@ -197,7 +197,7 @@ def main():
         calibrate()
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
               "(the x-slab decomposition is design only)", file=sys.stderr)
         sys.exit(2)
 

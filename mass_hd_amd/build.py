@@ -26,7 +26,8 @@ ARCH = "gfx950"
 COMMON = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
     "-Wall", "-Wno-unused-function",
-    # packed FP32 instructions stay off ("compiler flags")
+    # no packed FP32 instructions (v_pk_*_f32): some of their forms have given
+    # wrong results on this GPU type, so every HIP unit is built without them
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"),
 ]

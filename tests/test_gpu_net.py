@@ -1,7 +1,8 @@
 """Surrogate corrector network on the GPU: forward and gradients against the
 FP64 restatement (oracle/net_oracle.py) to BF16 tolerance, training reduces
-the loss, the trained network beats the method without it, and the judge
-hands every step back to the plain corrector when the tolerance cannot be met.
+the loss, the trained network beats the method without it, and the judged
+plain step takes every step back to the plain corrector when the tolerance
+cannot be met.
 """
 import numpy as np
 import pytest
