@@ -13,6 +13,10 @@
  *   UpdateQuantities/Fields, Move   src/models/lyotropic.cpp:343-386
  *   Step                            src/models/lyotropic.cpp:500-525
  *   RuntimeChecks sums              src/models/lyotropic.cpp:527-549
+ *   ghost-layer grid (bc != 0)      src/geometry.cpp:16-30,140-183,
+ *                                   src/geometry.hpp:17-89,148-199,306-316,475-501
+ *   boundary conditions             src/models/lyotropic.cpp:388-498,
+ *                                   src/fields.hpp:619-905
  *   ConfigureAtNode                 src/models/lyotropic.cpp:69-166
  *   mt19937 + random_real           src/random.cpp:12,21-24,43-46 (libstdc++
  *                                   generate_canonical<double,53>: two 32-bit
@@ -40,9 +44,10 @@ static const int YDIR[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
 
 struct lyo_oracle {
   lyo_oracle_params p;
-  uint32_t N;
-  uint32_t* nb; /* N*9 */
-  double *ff, *fn, *ff_tmp, *fn_tmp; /* N*9 each */
+  uint32_t N;   /* domain nodes */
+  uint32_t T;   /* all nodes: N, or (LX+2)*(LY+2) with the ghost layer */
+  uint32_t* nb; /* T*9 */
+  double *ff, *fn, *ff_tmp, *fn_tmp; /* T*9 each */
   double* s[LYO_O_NFIELDS];          /* scalar fields (index >= 2) */
   double* phi_tmp;
   double totalphi, countphi, ftot, ptot;
@@ -59,7 +64,7 @@ static double modu_d(double num, double div) {
 
 static inline const uint32_t* get_neighbours(const lyo_oracle* o, uint32_t k) {
   /* std::vector::at bounds check of geometry.hpp:544-545 */
-  if (k >= o->N) {
+  if (k >= o->T) {
     fprintf(stderr, "lyo_oracle: neighbour index %u out of range\n", k);
     abort();
   }
@@ -155,28 +160,63 @@ void lyo_oracle_random_real(uint32_t seed, int n, double lo, double hi, double* 
 
 /* ------------------------------------------------------------- lifecycle */
 
+/* src/geometry.cpp:150-171: node (x, y) of the (LX+2)*(LY+2) grid -> index
+ * (domain first, then walls Left, Right, Front, Back, then the corners;
+ * src/geometry.hpp:475-501) */
+static uint32_t layer_index(uint32_t LX, uint32_t LY, uint32_t x, uint32_t y) {
+  const uint32_t N = LX * LY, T = (LX + 2) * (LY + 2);
+  if (x == 0 && y == 0) return T - 4;              /* LeftFront */
+  if (x == LX + 1 && y == 0) return T - 3;         /* RightFront */
+  if (x == 0 && y == LY + 1) return T - 2;         /* LeftBack */
+  if (x == LX + 1 && y == LY + 1) return T - 1;    /* RightBack */
+  if (x == 0) return N + (y - 1);                  /* Left wall */
+  if (x == LX + 1) return N + LY + (y - 1);        /* Right wall */
+  if (y == 0) return N + 2 * LY + (x - 1);         /* Front wall */
+  if (y == LY + 1) return N + 2 * LY + LX + (x - 1); /* Back wall */
+  return (y - 1) + LY * (x - 1);
+}
+
 lyo_oracle* lyo_oracle_create(const lyo_oracle_params* p) {
-  if (p->LX < 2 || p->LY < 2) return NULL;
+  if (p->LX < 2 || p->LY < 2 || p->bc < 0 || p->bc > 4) return NULL;
   lyo_oracle* o = (lyo_oracle*)calloc(1, sizeof(lyo_oracle));
   o->p = *p;
   o->N = (uint32_t)p->LX * (uint32_t)p->LY;
-  const size_t N = o->N;
-  o->nb = (uint32_t*)malloc(sizeof(uint32_t) * 9 * N);
+  /* src/geometry.cpp:24-30: Lyotropic uses the Layer grid for bc != 0
+   * (lyotropic.cpp:15-17) */
+  o->T = p->bc ? ((uint32_t)p->LX + 2) * ((uint32_t)p->LY + 2) : o->N;
+  const size_t T = o->T;
+  o->nb = (uint32_t*)malloc(sizeof(uint32_t) * 9 * T);
   /* Field::SetSize value-initialises: every field starts at zero */
-  o->ff = (double*)calloc(9 * N, sizeof(double));
-  o->fn = (double*)calloc(9 * N, sizeof(double));
-  o->ff_tmp = (double*)calloc(9 * N, sizeof(double));
-  o->fn_tmp = (double*)calloc(9 * N, sizeof(double));
-  for (int f = 2; f < LYO_O_NFIELDS; ++f) o->s[f] = (double*)calloc(N, sizeof(double));
-  o->phi_tmp = (double*)calloc(N, sizeof(double));
-  /* geometry.cpp:100-110: k = y + LY*x, neighbour v at ((x+cx) mod LX, (y+cy) mod LY) */
-  for (uint32_t k = 0; k < o->N; ++k) {
-    const long x = k / (uint32_t)p->LY;
-    const long y = k % (uint32_t)p->LY;
-    for (int v = 0; v < 9; ++v) {
-      const unsigned xx = (unsigned)modu_d((double)(x + XDIR[v]), (double)(unsigned)p->LX);
-      const unsigned yy = (unsigned)modu_d((double)(y + YDIR[v]), (double)(unsigned)p->LY);
-      o->nb[9 * (size_t)k + v] = yy + (unsigned)p->LY * xx;
+  o->ff = (double*)calloc(9 * T, sizeof(double));
+  o->fn = (double*)calloc(9 * T, sizeof(double));
+  o->ff_tmp = (double*)calloc(9 * T, sizeof(double));
+  o->fn_tmp = (double*)calloc(9 * T, sizeof(double));
+  for (int f = 2; f < LYO_O_NFIELDS; ++f) o->s[f] = (double*)calloc(T, sizeof(double));
+  o->phi_tmp = (double*)calloc(T, sizeof(double));
+  if (!p->bc) {
+    /* geometry.cpp:100-110: k = y + LY*x, neighbour v at ((x+cx) mod LX, (y+cy) mod LY) */
+    for (uint32_t k = 0; k < o->N; ++k) {
+      const long x = k / (uint32_t)p->LY;
+      const long y = k % (uint32_t)p->LY;
+      for (int v = 0; v < 9; ++v) {
+        const unsigned xx = (unsigned)modu_d((double)(x + XDIR[v]), (double)(unsigned)p->LX);
+        const unsigned yy = (unsigned)modu_d((double)(y + YDIR[v]), (double)(unsigned)p->LY);
+        o->nb[9 * (size_t)k + v] = yy + (unsigned)p->LY * xx;
+      }
+    }
+  } else {
+    /* geometry.cpp:173-182: the periodic table of the (LX+2)*(LY+2) grid,
+     * converted to the layer's indices */
+    const unsigned LX = (unsigned)p->LX, LY = (unsigned)p->LY;
+    for (uint32_t k = 0; k < o->T; ++k) {
+      const long x = k / (LY + 2);
+      const long y = k % (LY + 2);
+      const uint32_t kk = layer_index(LX, LY, (uint32_t)x, (uint32_t)y);
+      for (int v = 0; v < 9; ++v) {
+        const unsigned xx = (unsigned)modu_d((double)(x + XDIR[v]), (double)(LX + 2));
+        const unsigned yy = (unsigned)modu_d((double)(y + YDIR[v]), (double)(LY + 2));
+        o->nb[9 * (size_t)kk + v] = layer_index(LX, LY, xx, yy);
+      }
     }
   }
   return o;
@@ -192,8 +232,10 @@ void lyo_oracle_destroy(lyo_oracle* o) {
 }
 
 void lyo_oracle_neighbours(const lyo_oracle* o, uint32_t* out) {
-  memcpy(out, o->nb, sizeof(uint32_t) * 9 * (size_t)o->N);
+  memcpy(out, o->nb, sizeof(uint32_t) * 9 * (size_t)o->T);
 }
+
+uint32_t lyo_oracle_total_size(const lyo_oracle* o) { return o->T; }
 
 #define F(id) (o->s[id])
 
@@ -231,9 +273,21 @@ int lyo_oracle_configure(lyo_oracle* o, const lyo_oracle_init* in) {
       case 4: /* random: uniform order, phi drawn per node */
         nematicOrder = 1;
         break;
-      case 5: /* boxatwall, bc=0 branch */
-      case 6: /* cuttingatwall, bc=0 branch */
-        if (y < level && udiff(LX / 2, x) < level) nematicOrder = 1;
+      case 5: /* boxatwall */
+        if (o->p.bc == 4) {
+          if (x < level && udiff(LY / 2, y) < level / 2) nematicOrder = 1;
+        } else if (y < level && udiff(LX / 2, x) < level) {
+          nematicOrder = 1;
+        }
+        break;
+      case 6: /* cuttingatwall (unsigned arithmetic as in the reference) */
+        if (o->p.bc == 4) {
+          const unsigned off = (unsigned)(int)(radius * 1.5);
+          if (pow(x, 2) + pow(uwrap(udiff(y, LY / 2 + off), LY), 2) <= radius * radius) nematicOrder = 1;
+          if (pow(x, 2) + pow(uwrap(udiff(y, LY / 2 - off), LY), 2) <= radius * radius) nematicOrder = 1;
+        } else if (y < level && udiff(LX / 2, x) < level) {
+          nematicOrder = 1;
+        }
         break;
       case 7: /* circleatwall, bc=0 branch */
         if (pow(uwrap(udiff(x, LX / 2), LX), 2) + pow(udiff(y, 0 * LY), 2) <= radius * radius) nematicOrder = 1;
@@ -282,6 +336,15 @@ int lyo_oracle_get(const lyo_oracle* o, int field, double* out) {
   if (field == LYO_O_FN) { memcpy(out, o->fn, sizeof(double) * 9 * N); return 0; }
   if (field < 2 || field >= LYO_O_NFIELDS) return -1;
   memcpy(out, o->s[field], sizeof(double) * N);
+  return 0;
+}
+
+int lyo_oracle_get_all(const lyo_oracle* o, int field, double* out) {
+  const size_t T = o->T;
+  if (field == LYO_O_FF) { memcpy(out, o->ff, sizeof(double) * 9 * T); return 0; }
+  if (field == LYO_O_FN) { memcpy(out, o->fn, sizeof(double) * 9 * T); return 0; }
+  if (field < 2 || field >= LYO_O_NFIELDS) return -1;
+  memcpy(out, o->s[field], sizeof(double) * T);
   return 0;
 }
 
@@ -461,13 +524,14 @@ void lyo_oracle_update_fields(lyo_oracle* o, int first) {
   o->phi_tmp = t;
 }
 
-/* src/models/lyotropic.cpp:371-386 (periodic grid: TotalSize == DomainSize) */
+/* src/models/lyotropic.cpp:371-386: every node of the grid, ghost layer
+ * included */
 void lyo_oracle_move(lyo_oracle* o) {
   const int nt = o->p.nthreads;
-  const unsigned N = o->N;
+  const unsigned T = o->T;
   (void)nt;
 #pragma omp parallel for num_threads(nt) if (nt)
-  for (unsigned k = 0; k < N; ++k) {
+  for (unsigned k = 0; k < T; ++k) {
     const uint32_t* d = get_neighbours(o, k);
     for (int v = 0; v < 9; ++v) {
       o->ff_tmp[9 * (size_t)d[v] + v] = o->ff[9 * (size_t)k + v];
@@ -478,15 +542,229 @@ void lyo_oracle_move(lyo_oracle* o) {
   t = o->fn; o->fn = o->fn_tmp; o->fn_tmp = t;
 }
 
+/* ------------------------------------------------ boundary conditions */
+
+/* src/geometry.hpp:148-199: walls and corners carry the number of the
+ * direction that points out of the domain through them */
+enum { W_RIGHT = 1, W_LEFT = 2, W_BACK = 3, W_FRONT = 4 };
+enum { C_RIGHTBACK = 5, C_LEFTFRONT = 6, C_LEFTBACK = 7, C_RIGHTFRONT = 8 };
+
+/* src/geometry.hpp:40-59 (reflection about the plane with normal i; row 0:
+ * bounce back) and :65-84 (projection onto that plane) */
+static const int REFL[9][9] = {
+    {0, 2, 1, 4, 3, 6, 5, 8, 7}, {0, 2, 1, 3, 4, 7, 8, 5, 6}, {0, 2, 1, 3, 4, 7, 8, 5, 6},
+    {0, 1, 2, 4, 3, 8, 7, 6, 5}, {0, 1, 2, 4, 3, 8, 7, 6, 5}, {0, 4, 3, 2, 1, 6, 5, 7, 8},
+    {0, 4, 3, 2, 1, 6, 5, 7, 8}, {0, 3, 4, 1, 2, 5, 6, 8, 7}, {0, 3, 4, 1, 2, 5, 6, 8, 7}};
+static const int PROJ[9][9] = {
+    {0, 0, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 3, 4, 3, 4, 3, 4}, {0, 0, 0, 3, 4, 3, 4, 3, 4},
+    {0, 1, 2, 0, 0, 1, 2, 2, 1}, {0, 1, 2, 0, 0, 1, 2, 2, 1}, {0, 5, 6, 5, 6, 5, 6, 0, 0},
+    {0, 1, 2, 4, 3, 8, 7, 6, 5}, {0, 8, 7, 7, 8, 0, 0, 7, 8}, {0, 8, 7, 7, 8, 0, 0, 7, 8}};
+
+static inline int opp(int d) { return REFL[0][d]; }
+static inline uint32_t nxt(const lyo_oracle* o, uint32_t k, int d) {
+  return get_neighbours(o, k)[d];
+}
+/* src/geometry.hpp:475-521 */
+static uint32_t wall_size(const lyo_oracle* o, int w) {
+  return (w == W_LEFT || w == W_RIGHT) ? (uint32_t)o->p.LY : (uint32_t)o->p.LX;
+}
+static uint32_t wall_index(const lyo_oracle* o, int w, uint32_t i) {
+  const uint32_t LX = (uint32_t)o->p.LX, LY = (uint32_t)o->p.LY;
+  switch (w) {
+    case W_LEFT: return o->N + i;
+    case W_RIGHT: return o->N + LY + i;
+    case W_FRONT: return o->N + 2 * LY + i;
+    default: return o->N + 2 * LY + LX + i;
+  }
+}
+static uint32_t corner_index(const lyo_oracle* o, int c) {
+  switch (c) {
+    case C_LEFTFRONT: return o->T - 4;
+    case C_RIGHTFRONT: return o->T - 3;
+    case C_LEFTBACK: return o->T - 2;
+    default: return o->T - 1;
+  }
+}
+/* src/geometry.hpp:312-316 */
+static inline int is_incoming(int v, int w) {
+  return XDIR[w] * XDIR[v] == -1 || YDIR[w] * YDIR[v] == -1;
+}
+
+/* scalar fields, src/fields.hpp:619-625 (PBC wall), 704-747 (Neumann,
+ * Dirichlet), 738-768 (copy derivative) */
+static void s_pbc_wall(const lyo_oracle* o, double* a, int w) {
+  for (uint32_t i = 0; i < wall_size(o, w); ++i)
+    a[wall_index(o, w, i)] = a[nxt(o, wall_index(o, opp(w), i), w)];
+}
+static void s_neumann_wall(const lyo_oracle* o, double* a, int w) {
+  for (uint32_t i = 0; i < wall_size(o, w); ++i) {
+    const uint32_t k = wall_index(o, w, i);
+    a[k] = a[nxt(o, k, opp(w))];
+  }
+}
+static void s_dirichlet_wall(const lyo_oracle* o, double* a, int w, double value) {
+  for (uint32_t i = 0; i < wall_size(o, w); ++i) {
+    const uint32_t k = wall_index(o, w, i);
+    a[k] = 2 * value - a[nxt(o, k, opp(w))];
+  }
+}
+static void s_copyder_wall(const lyo_oracle* o, double* a, int w) {
+  for (uint32_t i = 0; i < wall_size(o, w); ++i) {
+    const uint32_t k = wall_index(o, w, i);
+    const uint32_t j = nxt(o, k, opp(w));
+    a[k] = 2 * a[j] - a[nxt(o, j, opp(w))];
+  }
+}
+/* corners: towards d = opp(c) (both walls closed) or opp(w) (w closed, the
+ * other wall periodic) */
+static void s_neumann_corner(const lyo_oracle* o, double* a, int c, int d) {
+  const uint32_t k = corner_index(o, c);
+  a[k] = a[nxt(o, k, d)];
+}
+static void s_dirichlet_corner(const lyo_oracle* o, double* a, int c, int d, double value) {
+  const uint32_t k = corner_index(o, c);
+  a[k] = 2 * value - a[nxt(o, k, d)];
+}
+static void s_copyder_corner(const lyo_oracle* o, double* a, int c, int d) {
+  const uint32_t k = corner_index(o, c);
+  const uint32_t j = nxt(o, k, d);
+  a[k] = 2 * a[j] - a[nxt(o, j, d)];
+}
+
+/* kind 0 Neumann, 1 Dirichlet (value), 2 copy derivative; box:
+ * fields.hpp:770-780,805-829, channel: 844-885 */
+static void s_apply(const lyo_oracle* o, double* a, int kind, double value, int channel) {
+  static const int box_corners[4] = {C_LEFTFRONT, C_RIGHTFRONT, C_LEFTBACK, C_RIGHTBACK};
+  static const int ch_corners[4][2] = {{C_RIGHTBACK, W_BACK}, {C_RIGHTFRONT, W_FRONT},
+                                       {C_LEFTBACK, W_BACK}, {C_LEFTFRONT, W_FRONT}};
+  if (channel) {
+    s_pbc_wall(o, a, W_LEFT);
+    s_pbc_wall(o, a, W_RIGHT);
+  }
+  const int walls[4] = {W_LEFT, W_RIGHT, W_FRONT, W_BACK};
+  for (int i = channel ? 2 : 0; i < 4; ++i) {
+    if (kind == 0) s_neumann_wall(o, a, walls[i]);
+    else if (kind == 1) s_dirichlet_wall(o, a, walls[i], value);
+    else s_copyder_wall(o, a, walls[i]);
+  }
+  for (int i = 0; i < 4; ++i) {
+    const int c = channel ? ch_corners[i][0] : box_corners[i];
+    const int d = channel ? opp(ch_corners[i][1]) : opp(c);
+    if (kind == 0) s_neumann_corner(o, a, c, d);
+    else if (kind == 1) s_dirichlet_corner(o, a, c, d, value);
+    else s_copyder_corner(o, a, c, d);
+  }
+}
+
+/* LB fields (AoS, 9 per node), src/fields.hpp:619-625 (PBC wall: the whole
+ * node), 647-701 (free slip, no slip) */
+static void lb_pbc_wall(const lyo_oracle* o, double* f, int w) {
+  for (uint32_t i = 0; i < wall_size(o, w); ++i) {
+    const uint32_t k = wall_index(o, w, i);
+    const uint32_t src = nxt(o, wall_index(o, opp(w), i), w);
+    for (int v = 0; v < 9; ++v) f[9 * (size_t)k + v] = f[9 * (size_t)src + v];
+  }
+}
+static void lb_free_slip_wall(const lyo_oracle* o, double* f, int w) {
+  for (uint32_t i = 0; i < wall_size(o, w); ++i) {
+    const uint32_t k = wall_index(o, w, i);
+    for (int v = 0; v < 9; ++v)
+      if (is_incoming(v, w))
+        f[9 * (size_t)k + v] = f[9 * (size_t)nxt(o, k, opp(w)) + REFL[w][v]];
+  }
+}
+static void lb_no_slip_wall(const lyo_oracle* o, double* f, int w) {
+  for (uint32_t i = 0; i < wall_size(o, w); ++i) {
+    const uint32_t k = wall_index(o, w, i);
+    for (int v = 0; v < 9; ++v)
+      if (is_incoming(v, w)) f[9 * (size_t)k + v] = f[9 * (size_t)nxt(o, k, v) + opp(v)];
+  }
+}
+/* fields.hpp:689-701: bounce back at a corner (no slip, or free slip on
+ * both walls) */
+static void lb_bounce_corner(const lyo_oracle* o, double* f, int c) {
+  const uint32_t k = corner_index(o, c);
+  f[9 * (size_t)k + opp(c)] = f[9 * (size_t)nxt(o, k, opp(c)) + c];
+}
+/* fields.hpp:675-687: free slip on wall w next to a periodic wall */
+static void lb_free_slip_corner(const lyo_oracle* o, double* f, int c, int w) {
+  const int p = PROJ[w][c];
+  const uint32_t k = corner_index(o, c);
+  const int v = opp(c);
+  const uint32_t l = nxt(o, corner_index(o, REFL[p][c]), p);
+  f[9 * (size_t)k + v] = f[9 * (size_t)l + v];
+}
+
+/* bc 1 FreeSlipChannel, 2 NoSlipChannel, 3 FreeSlip, 4 NoSlip
+ * (fields.hpp:781-803,887-918) */
+static void lb_apply(const lyo_oracle* o, double* f, int bc) {
+  const int channel = bc <= 2, slip = (bc == 1 || bc == 3);
+  if (channel) {
+    lb_pbc_wall(o, f, W_LEFT);
+    lb_pbc_wall(o, f, W_RIGHT);
+  }
+  const int walls[4] = {W_LEFT, W_RIGHT, W_FRONT, W_BACK};
+  for (int i = channel ? 2 : 0; i < 4; ++i) {
+    if (slip) lb_free_slip_wall(o, f, walls[i]);
+    else lb_no_slip_wall(o, f, walls[i]);
+  }
+  if (bc == 1) {
+    lb_free_slip_corner(o, f, C_RIGHTBACK, W_BACK);
+    lb_free_slip_corner(o, f, C_RIGHTFRONT, W_FRONT);
+    lb_free_slip_corner(o, f, C_LEFTBACK, W_BACK);
+    lb_free_slip_corner(o, f, C_LEFTFRONT, W_FRONT);
+  } else if (bc == 2) {
+    lb_bounce_corner(o, f, C_RIGHTBACK);
+    lb_bounce_corner(o, f, C_RIGHTFRONT);
+    lb_bounce_corner(o, f, C_LEFTBACK);
+    lb_bounce_corner(o, f, C_LEFTFRONT);
+  } else {
+    lb_bounce_corner(o, f, C_LEFTFRONT);
+    lb_bounce_corner(o, f, C_RIGHTFRONT);
+    lb_bounce_corner(o, f, C_LEFTBACK);
+    lb_bounce_corner(o, f, C_RIGHTBACK);
+  }
+}
+
+/* src/models/lyotropic.cpp:388-498 */
+void lyo_oracle_boundary(lyo_oracle* o, int which) {
+  const int bc = o->p.bc;
+  if (!bc) return;
+  const int channel = bc <= 2;
+  if (which == 0) {
+    s_apply(o, F(LYO_O_QQXX), 0, 0., channel);
+    s_apply(o, F(LYO_O_QQYX), 0, 0., channel);
+    s_apply(o, F(LYO_O_PHI), 0, 0., channel);
+  } else if (which == 1) {
+    s_apply(o, F(LYO_O_UY), 2, 0., channel);
+    s_apply(o, F(LYO_O_UX), 2, 0., channel);
+    s_apply(o, F(LYO_O_UY_PHI), 1, 0., channel);
+    s_apply(o, F(LYO_O_UX_PHI), 1, 0., channel);
+    s_apply(o, F(LYO_O_MU), 0, 0., channel);
+    s_apply(o, F(LYO_O_SIGMAXX), 0, 0., channel);
+    s_apply(o, F(LYO_O_SIGMAYY), 0, 0., channel);
+    s_apply(o, F(LYO_O_SIGMAYX), 0, 0., channel);
+    s_apply(o, F(LYO_O_SIGMAXY), 0, 0., channel);
+  } else {
+    lb_apply(o, o->ff, bc);
+    lb_apply(o, o->fn, bc);
+  }
+}
+
 /* src/models/lyotropic.cpp:500-525 (bc=0: the boundary-condition calls are
  * no-ops) */
 void lyo_oracle_step(lyo_oracle* o, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
+    lyo_oracle_boundary(o, 0);
     lyo_oracle_update_quantities(o);
+    lyo_oracle_boundary(o, 1);
     lyo_oracle_update_fields(o, 1);
+    lyo_oracle_boundary(o, 2);
     lyo_oracle_move(o);
     for (int n = 1; n <= o->p.npc; ++n) {
+      lyo_oracle_boundary(o, 0);
       lyo_oracle_update_quantities(o);
+      lyo_oracle_boundary(o, 1);
       lyo_oracle_update_fields(o, 0);
     }
   }

@@ -25,7 +25,9 @@ typedef struct lyo_oracle_params {
   int32_t npc;          /* corrector sweeps per step, lyotropic.hpp:40 */
   int32_t nthreads;     /* 0 = serial (reference default, options.cpp:51-52) */
   int32_t conserve_phi; /* lyotropic.hpp:46 (false for plain Lyotropic) */
-  int32_t pad_;
+  int32_t bc;           /* boundary conditions, lyotropic.cpp:388-498: 0 periodic,
+                           1 free-slip channel, 2 no-slip channel, 3 free-slip
+                           box, 4 no-slip box (1-4 on a ghost layer) */
   double GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
   double AA, CC, LL, KK, zeta, zetaI, conc;
 } lyo_oracle_params;
@@ -82,8 +84,19 @@ void lyo_oracle_sums(const lyo_oracle* o, double* fsum, double* phisum,
                      double* ftot, double* ptot, double* totalphi,
                      double* countphi);
 
-/* periodic neighbour table (geometry.cpp:100-110), N*9 */
+/* neighbour table, lyo_oracle_total_size(o)*9: periodic (geometry.cpp:
+ * 100-110) for bc 0, the ghost-layer grid (geometry.cpp:140-183) otherwise */
 void lyo_oracle_neighbours(const lyo_oracle* o, uint32_t* out);
+/* nodes including the ghost layer: LX*LY (bc 0) or (LX+2)*(LY+2) */
+uint32_t lyo_oracle_total_size(const lyo_oracle* o);
+/* the boundary-condition calls of the step (lyotropic.cpp:388-498), exposed
+ * for unit checks: which 0 = BoundaryConditionsFields, 1 = ...Fields2,
+ * 2 = ...LB */
+void lyo_oracle_boundary(lyo_oracle* o, int which);
+/* whole-grid access including the ghost layer (total_size values; ff/fn
+ * total_size*9), index as in the reference: domain, then walls Left, Right,
+ * Front, Back, then the corners LeftFront, RightFront, LeftBack, RightBack */
+int lyo_oracle_get_all(const lyo_oracle* o, int field, double* out);
 
 /* raw mt19937 outputs and the canonical double of the reference's
  * random_real(), for pinning the RNG restatement */

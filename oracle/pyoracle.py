@@ -34,7 +34,7 @@ CONFIGS = {
 class _Params(ctypes.Structure):
     _fields_ = ([("LX", ctypes.c_int32), ("LY", ctypes.c_int32), ("npc", ctypes.c_int32),
                  ("nthreads", ctypes.c_int32), ("conserve_phi", ctypes.c_int32),
-                 ("pad_", ctypes.c_int32)] + [(k, ctypes.c_double) for k in PHYS_KEYS])
+                 ("bc", ctypes.c_int32)] + [(k, ctypes.c_double) for k in PHYS_KEYS])
 
 
 class _Init(ctypes.Structure):
@@ -75,6 +75,11 @@ def lib():
         L.lyo_oracle_sums.restype = None
         L.lyo_oracle_neighbours.argtypes = [vp, up]
         L.lyo_oracle_neighbours.restype = None
+        L.lyo_oracle_total_size.argtypes = [vp]
+        L.lyo_oracle_total_size.restype = ctypes.c_uint32
+        L.lyo_oracle_boundary.argtypes = [vp, ctypes.c_int]
+        L.lyo_oracle_boundary.restype = None
+        L.lyo_oracle_get_all.argtypes = [vp, ctypes.c_int, dp]
         L.lyo_oracle_mt19937.argtypes = [ctypes.c_uint32, ctypes.c_int, up]
         L.lyo_oracle_mt19937.restype = None
         L.lyo_oracle_random_real.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_double,
@@ -93,14 +98,17 @@ def _dp(a):
 class Oracle:
     """FP64 CPU restatement of the reference Lyotropic model."""
 
-    def __init__(self, LX, LY, npc=1, nthreads=0, **phys):
+    def __init__(self, LX, LY, npc=1, nthreads=0, bc=0, **phys):
+        """bc: 0 periodic, 1 free-slip channel, 2 no-slip channel, 3 free-slip
+        box, 4 no-slip box (lyotropic.cpp:388-498)."""
         self.L = lib()
-        self.LX, self.LY, self.N = LX, LY, LX * LY
-        p = _Params(LX=LX, LY=LY, npc=npc, nthreads=nthreads, conserve_phi=0, pad_=0,
+        self.LX, self.LY, self.N, self.bc = LX, LY, LX * LY, bc
+        p = _Params(LX=LX, LY=LY, npc=npc, nthreads=nthreads, conserve_phi=0, bc=bc,
                     **{k: float(phys[k]) for k in PHYS_KEYS})
         self.o = self.L.lyo_oracle_create(ctypes.byref(p))
         if not self.o:
-            raise ValueError("oracle: bad grid size")
+            raise ValueError("oracle: bad grid size or bc")
+        self.T = int(self.L.lyo_oracle_total_size(self.o))
 
     def close(self):
         if self.o:
@@ -155,10 +163,22 @@ class Oracle:
         return dict(zip(["fsum", "phisum", "ftot", "ptot", "totalphi", "countphi"],
                         [x.value for x in v]))
 
+    def boundary(self, which):
+        """0: BoundaryConditionsFields, 1: ...Fields2, 2: ...LB."""
+        self.L.lyo_oracle_boundary(self.o, which)
+
+    def get_all(self, name):
+        """Whole grid with the ghost layer (domain, walls L, R, F, B, corners
+        LF, RF, LB, RB)."""
+        fid = FIELDS[name]
+        out = np.empty(self.T * (9 if fid < 2 else 1))
+        self.L.lyo_oracle_get_all(self.o, fid, _dp(out))
+        return out.reshape(self.T, 9) if fid < 2 else out
+
     def neighbours(self):
-        out = np.empty(self.N * 9, dtype=np.uint32)
+        out = np.empty(self.T * 9, dtype=np.uint32)
         self.L.lyo_oracle_neighbours(self.o, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
-        return out.reshape(self.N, 9)
+        return out.reshape(self.T, 9)
 
 
 def mt19937(seed, n):
