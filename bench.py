@@ -60,8 +60,8 @@ def log(msg):
 
 
 def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-1, alt=-1,
-            full=-1, step_chunk=0, wpb=0, loops=False):
-    m = lyo.Lyotropic(L, L, npc=npc, precision=precision, **PHYS)
+            full=-1, step_chunk=0, wpb=0, loops=False, bc=0):
+    m = lyo.Lyotropic(L, L, npc=npc, precision=precision, bc=bc, **PHYS)
     m.set_launch(chunk, wpb)
     m.set_schedule(fuse, pair_chunk, alt, full, step_chunk)
     if loops:
@@ -184,6 +184,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-fp64-loops", action="store_true",
                     help="skip the FP64 run with one kernel per reference loop")
+    ap.add_argument("--skip-walls", action="store_true",
+                    help="skip the FP64 no-slip box (bc=4) leg")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-surrogate", action="store_true")
@@ -238,6 +240,14 @@ def main():
                           "fp64_loops_kernel_set": "one kernel per reference node loop "
                                                    "(UpdateQuantities storing 23 derived "
                                                    "fields, UpdateFields, Move), SoA planes"})
+        if not args.skip_walls:
+            log("fp64 no-slip box (bc=4) %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
+            tw, _ = gpu_run(lyo, L, lyo.FP64, npc, s64, w64, bc=4)
+            mlupsw = N * s64 / (tw / 1e3) / 1e6
+            log("fp64 walls: %.3f ms/step, %.0f MLUPS" % (tw / s64, mlupsw))
+            extra.update({"fp64_walls_bc4_mlups": round(mlupsw, 1),
+                          "fp64_walls_kernel_set": "ghost layer, boundary-condition kernels "
+                                                   "between the reference's node loops"})
 
     dom = max((k for k in k32 if k32[k] > 0), key=lambda k: k32[k])
     # the one-kernel step performs all 1 + npc sweeps of a step

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define MASS_LYO_ABI_VERSION 1
+#define MASS_LYO_ABI_VERSION 2
 
 enum mass_lyo_status {
   MASS_LYO_OK = 0,
@@ -56,6 +56,14 @@ typedef struct mass_lyo_params {
   int32_t plane_pad;    /* extra elements between SoA planes, beyond the
                            library's own stagger of 256 (0 = none); results
                            do not depend on it */
+  int32_t bc;           /* boundary conditions (Lyotropic(LX,LY,BC),
+                           lyotropic.cpp:15-17,388-498): 0 periodic,
+                           1 free-slip channel, 2 no-slip channel (walls at
+                           y = -1/2 and LY-1/2, periodic in x), 3 free-slip
+                           box, 4 no-slip box.  bc 1-4 keep a ghost layer
+                           (geometry.cpp:140-183), need FP64 and run the
+                           reference's loop structure; batches, the
+                           surrogate and the diagnostics need bc = 0 */
   double GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
   double AA, CC, LL, KK, zeta, zetaI, conc;
 } mass_lyo_params;

@@ -133,6 +133,9 @@ int mass_lyo_batch_create(const mass_lyo_params* runs, int32_t nruns, mass_lyo_b
   for (int i = 1; i < nruns; ++i)
     if (runs[i].npc != runs[0].npc || runs[i].precision != runs[0].precision)
       return bfail(nullptr, MASS_LYO_EINVAL, "all runs of a batch share npc and precision");
+  for (int i = 0; i < nruns; ++i)
+    if (runs[i].bc != 0)
+      return bfail(nullptr, MASS_LYO_EINVAL, "batched runs are periodic (bc 0)");
   auto* b = new mass_lyo_batch_s();
   b->npc = runs[0].npc;
   b->precision = runs[0].precision;

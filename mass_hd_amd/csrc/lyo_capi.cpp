@@ -80,6 +80,11 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
   if (p->conserve_phi)
     return fail(nullptr, MASS_LYO_EINVAL,
                 "conserve_phi is not part of the plain lyotropic model");
+  if (p->bc < 0 || p->bc > 4)
+    return fail(nullptr, MASS_LYO_EINVAL,
+                "bc must be 0 (periodic), 1/2 (free-/no-slip channel) or 3/4 (free-/no-slip box)");
+  if (p->bc != 0 && p->precision != MASS_LYO_FP64)
+    return fail(nullptr, MASS_LYO_EINVAL, "walled grids (bc 1-4) run in FP64 only");
   const double* v = param_block(*p);
   for (int i = 0; i < 14; ++i)
     if (!std::isfinite(v[i])) return fail(nullptr, MASS_LYO_EINVAL, "non-finite parameter");
@@ -115,7 +120,11 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
   h->device = dev;
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(nullptr, e, "hipStreamCreate");
-  h->eng.reset(p->precision == MASS_LYO_FP64 ? lyo::make_engine_fp64() : lyo::make_engine_fp32());
+  if (p->bc != 0)
+    h->eng.reset(lyo::make_engine_walls(p->bc));
+  else
+    h->eng.reset(p->precision == MASS_LYO_FP64 ? lyo::make_engine_fp64()
+                                               : lyo::make_engine_fp32());
   e = h->eng->alloc(p->LX, p->LY, h->NP, p->npc);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_stage, sizeof(double) * 12 * (size_t)N);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_part, sizeof(double) * (2 * kMaxPart + 2));
@@ -206,6 +215,7 @@ int mass_lyo_configure(mass_lyo_t h, const mass_lyo_init* init, double* ftot, do
   hi.phinoise = init->phinoise;
   hi.conc = h->prm.conc;
   hi.rho = h->prm.rho;
+  hi.bc = h->prm.bc;
   std::vector<double> ff, qxx, qyx, phi;
   std::string msg;
   if (lyo::configure_host(hi, ff, qxx, qyx, phi, ftot, ptot, msg) != 0)
@@ -292,6 +302,7 @@ int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_ev
   if (!net || !lyo::net_ready(net))
     return fail(h, MASS_LYO_EINVAL, "the network is not trained (no normalisation yet)");
   if (h->prm.npc < 2) return fail(h, MASS_LYO_EINVAL, "the surrogate replaces correctors 2..npc: npc must be >= 2");
+  if (h->prm.bc != 0) return fail(h, MASS_LYO_EINVAL, "the surrogate needs a periodic grid (bc 0)");
   if (h->prm.LY % 16) return fail(h, MASS_LYO_EINVAL, "the surrogate needs LY to be a multiple of 16");
   if (judge_every < 1 || !(tol > 0)) return fail(h, MASS_LYO_EINVAL, "judge_every >= 1 and tol > 0");
   auto& S = h->sur;
@@ -359,6 +370,8 @@ int mass_lyo_sums(mass_lyo_t h, double* fsum, double* phisum) {
 int mass_lyo_diagnostics(mass_lyo_t h, mass_lyo_diag* out, double* charge) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (!out) return fail(h, MASS_LYO_EINVAL, "null output");
+  if (h->prm.bc != 0)
+    return fail(h, MASS_LYO_EINVAL, "the diagnostics (defect winding numbers) need a periodic grid");
   hipError_t e = hipSuccess;
   lyo::DiagScratch& d = h->diag;
   const size_t N = (size_t)h->N;
@@ -504,7 +517,8 @@ struct CheckpointHeader {
   int32_t LX, LY, npc, planes;
   double rho;
   uint64_t state_bytes;
-  uint8_t reserved[16];
+  int32_t bc;  // 0 in checkpoints of periodic runs
+  uint8_t reserved[12];
 };
 static_assert(sizeof(CheckpointHeader) == 64, "64-byte checkpoint header");
 constexpr char kCheckpointMagic[8] = {'M', 'A', 'S', 'S', 'L', 'Y', 'O', '1'};
@@ -533,6 +547,7 @@ int mass_lyo_checkpoint_save(mass_lyo_t h, void* buf, uint64_t bytes) {
   hd.planes = h->eng->state_planes();
   hd.rho = h->prm.rho;
   hd.state_bytes = h->eng->state_bytes();
+  hd.bc = h->prm.bc;
   std::memcpy(buf, &hd, sizeof(hd));
   const hipError_t e = h->eng->save_state(static_cast<char*>(buf) + sizeof(hd), h->stream);
   if (e != hipSuccess) return hip_fail(h, e, "checkpoint save");
@@ -555,6 +570,8 @@ int mass_lyo_checkpoint_load(mass_lyo_t h, const void* buf, uint64_t bytes) {
                     " differs from the run's");
   if (hd.rho != h->prm.rho)
     return fail(h, MASS_LYO_EINVAL, "checkpoint rho differs from the run's");
+  if (hd.bc != h->prm.bc)
+    return fail(h, MASS_LYO_EINVAL, "checkpoint boundary conditions differ from the run's");
   if (hd.planes != h->eng->state_planes() || hd.state_bytes != h->eng->state_bytes() ||
       bytes < sizeof(hd) + hd.state_bytes)
     return fail(h, MASS_LYO_EINVAL, "checkpoint truncated or of another layout");
@@ -570,6 +587,8 @@ int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on) {
   if (on != 0 && on != 1) return fail(h, MASS_LYO_EINVAL, "on must be 0 or 1");
   if (on && h->prm.precision != MASS_LYO_FP64)
     return fail(h, MASS_LYO_EINVAL, "the loop kernels run the FP64 path only");
+  if (!on && h->prm.bc != 0)
+    return fail(h, MASS_LYO_EINVAL, "walled grids (bc 1-4) run the loop kernels only");
   if (on) {
     const hipError_t e = h->eng->alloc_loops();
     if (e != hipSuccess) return hip_fail(h, e, "loop kernel buffers");

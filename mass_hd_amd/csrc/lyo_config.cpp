@@ -59,9 +59,23 @@ int configure_host(const HostInit& in, std::vector<double>& ff, std::vector<doub
       case 4:  // random
         order = 1;
         break;
-      case 5:  // boxatwall (periodic branch)
-      case 6:  // cuttingatwall (periodic branch)
-        if (y < level && udiff(LX / 2, x) < level) order = 1;
+      case 5:  // boxatwall
+        if (in.bc == 4) {
+          if (x < level && udiff(LY / 2, y) < level / 2) order = 1;
+        } else if (y < level && udiff(LX / 2, x) < level) {
+          order = 1;
+        }
+        break;
+      case 6:  // cuttingatwall (unsigned arithmetic, as the reference's)
+        if (in.bc == 4) {
+          const unsigned off = (unsigned)int(radius * 1.5);
+          if (std::pow(x, 2) + std::pow(uwrap(udiff(y, LY / 2 + off), LY), 2) <= radius * radius)
+            order = 1;
+          if (std::pow(x, 2) + std::pow(uwrap(udiff(y, LY / 2 - off), LY), 2) <= radius * radius)
+            order = 1;
+        } else if (y < level && udiff(LX / 2, x) < level) {
+          order = 1;
+        }
         break;
       case 7:  // circleatwall (periodic branch)
         if (std::pow(uwrap(udiff(x, LX / 2), LX), 2) + std::pow(udiff(y, 0 * LY), 2) <=

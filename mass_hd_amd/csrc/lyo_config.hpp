@@ -13,6 +13,7 @@ struct HostInit {
   uint32_t seed;
   double level, radius, angle_deg, noise, qamp, phinoise;
   double conc, rho;
+  int bc;  // boundary conditions: bc 4 places boxatwall / cuttingatwall at x = 0
 };
 
 // Fills ff (N*9 AoS), qxx, qyx, phi; returns the check totals ftot, ptot.

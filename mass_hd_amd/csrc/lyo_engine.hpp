@@ -930,5 +930,6 @@ class EngineT final : public Engine {
 
 Engine* make_engine_fp64();
 Engine* make_engine_fp32();
+Engine* make_engine_walls(int bc);  // FP64, bc 1-4 (lyo_walls.hpp)
 
 }  // namespace lyo

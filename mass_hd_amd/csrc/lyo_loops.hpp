@@ -28,7 +28,7 @@ enum LoopPlane {
   kLoopPlanes
 };
 
-// Periodic neighbours of node k = y + LY*x (src/geometry.cpp:87-110).
+// Storage index of a node and of its four axis neighbours.
 struct LoopNbr {
   int k, kxp, kxm, kyp, kym;
 };
@@ -44,18 +44,44 @@ __device__ __forceinline__ LoopNbr loop_nbr(int k, int LX, int LY) {
   return n;
 }
 
+// Node t = y + LY*x of the domain is stored at t on the periodic grid
+// (src/geometry.cpp:100-110) ...
+struct PeriodicGrid {
+  __device__ __forceinline__ static LoopNbr at(int t, int LX, int LY) {
+    return loop_nbr(t, LX, LY);
+  }
+};
+
+// ... and at (y+1) + (LY+2)*(x+1) on the grid with a ghost layer
+// (src/geometry.cpp:140-183), whose ghost nodes hold the walls' values: a
+// domain node's neighbours are never wrapped there.
+struct LayerGrid {
+  __device__ __forceinline__ static LoopNbr at(int t, int LX, int LY) {
+    (void)LX;
+    const int x = t / LY, y = t - x * LY, W = LY + 2;
+    LoopNbr n;
+    n.k = (y + 1) + W * (x + 1);
+    n.kxp = n.k + W;
+    n.kxm = n.k - W;
+    n.kyp = n.k + 1;
+    n.kym = n.k - 1;
+    return n;
+  }
+};
+
 // UpdateQuantities (lyotropic.cpp:168-247, loop :343-360): all 23 derived
-// planes of every node, plane i at D + i*NP.
-template <class T, bool EXACT>
+// planes of every domain node, plane i at D + i*NP.
+template <class T, bool EXACT, class G = PeriodicGrid>
 __global__ void __launch_bounds__(256) loop_uq_kernel(const T* __restrict__ f,
                                                       const T* __restrict__ qxx,
                                                       const T* __restrict__ qyx,
                                                       const T* __restrict__ phi, Params<T> P,
                                                       int LX, int LY, int NP, T* __restrict__ D) {
   static_assert(EXACT, "the loop kernels run the reference's FP64 arithmetic");
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= LX * LY) return;
-  const LoopNbr n = loop_nbr(k, LX, LY);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= LX * LY) return;
+  const LoopNbr n = G::at(t, LX, LY);
+  const int k = n.k;
   T fl[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) fl[v] = f[v * NP + k];
@@ -100,13 +126,14 @@ struct LoopUF {
   T* phn_out;     // predictor
 };
 
-template <class T, bool EXACT, bool FIRST>
+template <class T, bool EXACT, bool FIRST, class G = PeriodicGrid>
 __global__ void __launch_bounds__(256) loop_uf_kernel(LoopUF<T> a, Params<T> P, int LX, int LY,
                                                       int NP) {
   static_assert(EXACT, "the loop kernels run the reference's FP64 arithmetic");
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= LX * LY) return;
-  const LoopNbr n = loop_nbr(k, LX, LY);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= LX * LY) return;
+  const LoopNbr n = G::at(t, LX, LY);
+  const int k = n.k;
   auto pl = [&](int i) { return a.D + (size_t)i * NP; };
   CenterVals<T> c;
   c.nn = pl(LP_N)[k];

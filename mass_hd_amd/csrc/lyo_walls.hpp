@@ -1,0 +1,612 @@
+// lyo_walls.hpp -- walled grids (bc 1-4): free-slip / no-slip channels and
+// boxes on a ghost layer, FP64 in the reference's expression order.
+//
+// Lyotropic with bc != 0 (src/models/lyotropic.cpp:15-17) stores its fields on
+// a grid with one layer of ghost nodes around the domain
+// (src/geometry.cpp:140-183): the (LX+2)*(LY+2) torus, whose ghost nodes the
+// boundary conditions (lyotropic.cpp:388-498, src/fields.hpp:619-918) fill
+// from the domain before each use, and through which Move streams the
+// populations like any other node (lyotropic.cpp:371-386 runs over the whole
+// grid).  Here every plane is that torus, node (X, Y) at Y + (LY+2)*X with
+// X, Y in [0, LX+1] x [0, LY+2); domain node (x, y) is (x+1, y+1).  The step
+// runs the reference's loop structure -- the node loops of lyo_loops.hpp on
+// LayerGrid -- with the boundary conditions as kernels in between, so results
+// are bitwise those of the oracle's restatement.
+//
+// The boundary conditions run in three launches per call, in the
+// reference's order: Left/Right walls, then Front/Back walls, then the four
+// corners.  Within a launch no ghost a thread writes is read by another
+// thread (walls read the domain, the previous launch's ghosts, or -- the
+// no-slip rule at a wall's ends -- ghosts of walls that come later, before
+// those are written, as in the reference), so each launch is one thread per
+// ghost node.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "lyo_aux.hpp"
+#include "lyo_engine.hpp"
+#include "lyo_loops.hpp"
+
+namespace lyo {
+
+// src/geometry.hpp:148-199: walls and corners carry the number of the
+// direction that points out of the domain through them.
+enum WallId { kWallRight = 1, kWallLeft = 2, kWallBack = 3, kWallFront = 4 };
+enum CornerId { kCornerRB = 5, kCornerLF = 6, kCornerLB = 7, kCornerRF = 8 };
+
+// src/geometry.hpp:17-27 (directions), 40-59 (reflection about the plane of
+// normal i; row 0: bounce back), 65-84 (projection onto that plane)
+__constant__ int kWallCX[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
+__constant__ int kWallCY[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+__constant__ int kRefl[9][9] = {
+    {0, 2, 1, 4, 3, 6, 5, 8, 7}, {0, 2, 1, 3, 4, 7, 8, 5, 6}, {0, 2, 1, 3, 4, 7, 8, 5, 6},
+    {0, 1, 2, 4, 3, 8, 7, 6, 5}, {0, 1, 2, 4, 3, 8, 7, 6, 5}, {0, 4, 3, 2, 1, 6, 5, 7, 8},
+    {0, 4, 3, 2, 1, 6, 5, 7, 8}, {0, 3, 4, 1, 2, 5, 6, 8, 7}, {0, 3, 4, 1, 2, 5, 6, 8, 7}};
+__constant__ int kProj[9][9] = {
+    {0, 0, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 3, 4, 3, 4, 3, 4}, {0, 0, 0, 3, 4, 3, 4, 3, 4},
+    {0, 1, 2, 0, 0, 1, 2, 2, 1}, {0, 1, 2, 0, 0, 1, 2, 2, 1}, {0, 5, 6, 5, 6, 5, 6, 0, 0},
+    {0, 1, 2, 4, 3, 8, 7, 6, 5}, {0, 8, 7, 7, 8, 0, 0, 7, 8}, {0, 8, 7, 7, 8, 0, 0, 7, 8}};
+
+// Geometry of the ghost-layer torus.
+struct WallGeo {
+  int LX, LY;
+  int W;   // LY + 2: stride of X
+  int NP;  // plane stride (>= (LX+2)*(LY+2))
+  int bc;  // 1 free-slip channel, 2 no-slip channel, 3 free-slip box, 4 no-slip box
+};
+
+__device__ __forceinline__ int wopp(int d) { return kRefl[0][d]; }
+
+// node index of (X, Y) and its neighbour in direction d on the torus
+__device__ __forceinline__ int wnode(const WallGeo& g, int X, int Y) { return Y + g.W * X; }
+__device__ __forceinline__ int wnext(const WallGeo& g, int p, int d) {
+  int X = p / g.W, Y = p - X * g.W;
+  X += kWallCX[d];
+  Y += kWallCY[d];
+  X = X < 0 ? X + g.LX + 2 : (X > g.LX + 1 ? X - (g.LX + 2) : X);
+  Y = Y < 0 ? Y + g.W : (Y >= g.W ? Y - g.W : Y);
+  return wnode(g, X, Y);
+}
+
+// The ghost node of thread i in a launch (phase 0: Left/Right walls, i <
+// 2*LY; phase 1: Front/Back walls, i < 2*LX; phase 2: the four corners in
+// the order the box calls them, LeftFront, RightFront, LeftBack, RightBack)
+// and its wall (phases 0, 1) or corner (phase 2).  False past the end.
+__device__ __forceinline__ bool wall_ghost(const WallGeo& g, int phase, int i, int& p, int& id) {
+  if (phase == 0) {
+    if (i >= 2 * g.LY) return false;
+    const bool right = i >= g.LY;
+    const int Y = (right ? i - g.LY : i) + 1;
+    p = wnode(g, right ? g.LX + 1 : 0, Y);
+    id = right ? kWallRight : kWallLeft;
+  } else if (phase == 1) {
+    if (i >= 2 * g.LX) return false;
+    const bool back = i >= g.LX;
+    const int X = (back ? i - g.LX : i) + 1;
+    p = wnode(g, X, back ? g.LY + 1 : 0);
+    id = back ? kWallBack : kWallFront;
+  } else {
+    if (i >= 4) return false;
+    const int c[4] = {kCornerLF, kCornerRF, kCornerLB, kCornerRB};
+    id = c[i];
+    const int X = (id == kCornerLF || id == kCornerLB) ? 0 : g.LX + 1;
+    const int Y = (id == kCornerLF || id == kCornerRF) ? 0 : g.LY + 1;
+    p = wnode(g, X, Y);
+  }
+  return true;
+}
+
+// channel corners: the closed wall next to corner c (the other wall is
+// periodic; fields.hpp:844-885)
+__device__ __forceinline__ int channel_wall(int c) {
+  return (c == kCornerLF || c == kCornerRF) ? kWallFront : kWallBack;
+}
+
+// Scalar rules.  kind 0: Neumann (fields.hpp:704-723), 1: Dirichlet with
+// value 0 (:725-747), 2: copy derivative (:749-768); PBC walls of a channel
+// copy the opposite domain column (:619-625).
+template <class T>
+__device__ __forceinline__ void scalar_ghost(const WallGeo& g, T* a, int kind, int phase, int p,
+                                             int id) {
+  const bool channel = g.bc <= 2;
+  if (phase == 0 && channel) {  // ApplyPBC(Left), ApplyPBC(Right)
+    const int Y = p % g.W;
+    const int src = id == kWallLeft ? wnode(g, g.LX, Y) : wnode(g, 1, Y);
+    a[p] = a[src];
+    return;
+  }
+  // direction towards the domain: away from the wall, or for a corner away
+  // from the corner (box) or from its closed wall (channel)
+  const int d = phase < 2 ? wopp(id) : wopp(channel ? channel_wall(id) : id);
+  const int j = wnext(g, p, d);
+  if (kind == 0) {
+    a[p] = a[j];
+  } else if (kind == 1) {
+    a[p] = T(2) * T(0) - a[j];
+  } else {
+    a[p] = T(2) * a[j] - a[wnext(g, j, d)];
+  }
+}
+
+// src/geometry.hpp:312-316
+__device__ __forceinline__ bool incoming(int v, int w) {
+  return kWallCX[w] * kWallCX[v] == -1 || kWallCY[w] * kWallCY[v] == -1;
+}
+
+// LB rules on one population field (9 planes): PBC walls copy the whole
+// node (fields.hpp:619-625), free slip reflects the incoming directions
+// about the wall (:647-661), no slip bounces them back (:663-674); corners
+// bounce back (:689-701) except the free-slip channel's (:675-687).
+template <class T>
+__device__ __forceinline__ void lb_ghost(const WallGeo& g, T* f, int phase, int p, int id) {
+  const int NP = g.NP;
+  const bool channel = g.bc <= 2, slip = g.bc == 1 || g.bc == 3;
+  if (phase == 0 && channel) {
+    const int Y = p % g.W;
+    const int src = id == kWallLeft ? wnode(g, g.LX, Y) : wnode(g, 1, Y);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) f[v * NP + p] = f[v * NP + src];
+    return;
+  }
+  if (phase < 2) {
+    const int w = id;
+#pragma unroll
+    for (int v = 1; v < 9; ++v) {
+      if (!incoming(v, w)) continue;
+      if (slip)
+        f[v * NP + p] = f[kRefl[w][v] * NP + wnext(g, p, wopp(w))];
+      else
+        f[v * NP + p] = f[wopp(v) * NP + wnext(g, p, v)];
+    }
+    return;
+  }
+  const int c = id;
+  if (g.bc == 1) {
+    const int w = channel_wall(c);
+    const int pr = kProj[w][c];
+    int X = 0, Y = 0;
+    const int rc = kRefl[pr][c];  // the other corner along the closed wall
+    X = (rc == kCornerLF || rc == kCornerLB) ? 0 : g.LX + 1;
+    Y = (rc == kCornerLF || rc == kCornerRF) ? 0 : g.LY + 1;
+    const int l = wnext(g, wnode(g, X, Y), pr);
+    f[wopp(c) * NP + p] = f[wopp(c) * NP + l];
+  } else {
+    f[wopp(c) * NP + p] = f[c * NP + wnext(g, p, wopp(c))];
+  }
+}
+
+// BoundaryConditionsFields (lyotropic.cpp:422-449): Neumann on Qxx, Qyx, phi.
+template <class T>
+__global__ void __launch_bounds__(256) wall_fields_kernel(WallGeo g, int phase, T* qxx, T* qyx,
+                                                          T* phi) {
+  int p, id;
+  if (!wall_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id)) return;
+  scalar_ghost(g, qxx, 0, phase, p, id);
+  scalar_ghost(g, qyx, 0, phase, p, id);
+  scalar_ghost(g, phi, 0, phase, p, id);
+}
+
+// BoundaryConditionsFields2 (lyotropic.cpp:451-498) on the derived planes:
+// copy derivative on u, Dirichlet 0 on u*phi, Neumann on MU and the stresses.
+template <class T>
+__global__ void __launch_bounds__(256) wall_fields2_kernel(WallGeo g, int phase, T* D) {
+  int p, id;
+  if (!wall_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id)) return;
+  auto pl = [&](int i) { return D + (size_t)i * g.NP; };
+  scalar_ghost(g, pl(LP_UY), 2, phase, p, id);
+  scalar_ghost(g, pl(LP_UX), 2, phase, p, id);
+  scalar_ghost(g, pl(LP_UYP), 1, phase, p, id);
+  scalar_ghost(g, pl(LP_UXP), 1, phase, p, id);
+  scalar_ghost(g, pl(LP_MU), 0, phase, p, id);
+  scalar_ghost(g, pl(LP_SXX), 0, phase, p, id);
+  scalar_ghost(g, pl(LP_SYY), 0, phase, p, id);
+  scalar_ghost(g, pl(LP_SYX), 0, phase, p, id);
+  scalar_ghost(g, pl(LP_SXY), 0, phase, p, id);
+}
+
+// BoundaryConditionsLB (lyotropic.cpp:388-420) on ff and fn.
+template <class T>
+__global__ void __launch_bounds__(256) wall_lb_kernel(WallGeo g, int phase, T* f, T* fn) {
+  int p, id;
+  if (!wall_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id)) return;
+  lb_ghost(g, f, phase, p, id);
+  lb_ghost(g, fn, phase, p, id);
+}
+
+// Move over every node of the torus (lyotropic.cpp:371-386), as a pull:
+// node p takes direction v from p - c_v, the node whose push lands on p.
+template <class T>
+__global__ void __launch_bounds__(256) wall_move_kernel(WallGeo g, const T* __restrict__ f,
+                                                        const T* __restrict__ fn,
+                                                        T* __restrict__ f_to,
+                                                        T* __restrict__ fn_to) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (g.LX + 2) * g.W) return;
+  f_to[p] = f[p];
+  fn_to[p] = fn[p];
+#pragma unroll
+  for (int v = 1; v < 9; ++v) {
+    const int src = wnext(g, p, wopp(v));
+    f_to[v * g.NP + p] = f[v * g.NP + src];
+    fn_to[v * g.NP + p] = fn[v * g.NP + src];
+  }
+}
+
+// Host FP64 AoS domain state -> torus planes (ghost nodes stay as they are).
+template <class T>
+__global__ void wall_pack_kernel(WallGeo g, const double* __restrict__ ff,
+                                 const double* __restrict__ qxx, const double* __restrict__ qyx,
+                                 const double* __restrict__ phi, T* f, T* oqxx, T* oqyx, T* ophi) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.LX * g.LY) return;
+  const int p = LayerGrid::at(t, g.LX, g.LY).k;
+#pragma unroll
+  for (int v = 0; v < 9; ++v) f[v * g.NP + p] = (T)ff[9 * (size_t)t + v];
+  oqxx[p] = (T)qxx[t];
+  oqyx[p] = (T)qyx[t];
+  ophi[p] = (T)phi[t];
+}
+
+// Domain of torus planes -> FP64 (nplanes = 9: AoS populations; 1: scalar).
+template <class T>
+__global__ void wall_unpack_kernel(WallGeo g, const T* __restrict__ a, int nplanes,
+                                   double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.LX * g.LY) return;
+  const int p = LayerGrid::at(t, g.LX, g.LY).k;
+  for (int v = 0; v < nplanes; ++v) out[nplanes * (size_t)t + v] = (double)a[v * g.NP + p];
+}
+
+// Torus planes <-> contiguous (LX+2)*(LY+2) host-order planes (checkpoints).
+template <class T>
+__global__ void wall_copy_planes_kernel(const T* __restrict__ src, int src_stride,
+                                        T* __restrict__ dst, int dst_stride, int n, int nplanes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int v = 0; v < nplanes; ++v) dst[(size_t)v * dst_stride + i] = src[(size_t)v * src_stride + i];
+}
+
+// Block b sums domain nodes [b*per, min((b+1)*per, N)) of sum_v f and phi in
+// a fixed order (the partial sums of lyo_aux.hpp on the torus layout).
+template <class T>
+__global__ void __launch_bounds__(256)
+wall_partial_sums_kernel(WallGeo g, const T* __restrict__ f, const T* __restrict__ phi, int per,
+                         double* __restrict__ part_f, double* __restrict__ part_p) {
+  __shared__ double sf[4], sp[4];
+  const int N = g.LX * g.LY;
+  const int beg = blockIdx.x * per;
+  const int end = min(beg + per, N);
+  double af = 0, ap = 0;
+  for (int t = beg + (int)threadIdx.x; t < end; t += 256) {
+    const int p = LayerGrid::at(t, g.LX, g.LY).k;
+    double s = 0;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) s += (double)f[v * g.NP + p];
+    af += s;
+    ap += (double)phi[p];
+  }
+  af = wave_sum(af);
+  ap = wave_sum(ap);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sf[w] = af;
+    sp[w] = ap;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part_f[blockIdx.x] = (sf[0] + sf[1]) + (sf[2] + sf[3]);
+    part_p[blockIdx.x] = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+  }
+}
+
+// The run of a walled grid.  FP64 only (the reference's arithmetic); the
+// fused strip-march kernels, batches, the surrogate and the diagnostics
+// assume a periodic grid and are not offered here.
+class WallEngine final : public Engine {
+ public:
+  using T = double;
+  explicit WallEngine(int bc) : bc_(bc) {}
+  ~WallEngine() override { release(); }
+
+  hipError_t alloc(int LX, int LY, int NP, int npc) override {
+    (void)NP;
+    LX_ = LX;
+    LY_ = LY;
+    npc_ = npc;
+    TP_ = (LX + 2) * (LY + 2);
+    // planes on 256-byte boundaries, staggered by 256 values like the
+    // periodic layout
+    NPw_ = ((TP_ + 256 + 63) / 64) * 64;
+    const size_t n1 = (size_t)NPw_, n9 = 9 * n1;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      if (e == hipSuccess) e = get_(n9, &F_[i]);
+      if (e == hipSuccess) e = get_(n9, &Fn_[i]);
+      if (e == hipSuccess) e = get_(n1, &phi_[i]);
+    }
+    if (e == hipSuccess) e = get_(n1, &qxx_);
+    if (e == hipSuccess) e = get_(n1, &qyx_);
+    if (e == hipSuccess) e = get_(n1, &qnxx_);
+    if (e == hipSuccess) e = get_(n1, &qnyx_);
+    if (e == hipSuccess) e = get_(n1, &phn_);
+    if (e == hipSuccess) e = get_((size_t)kLoopPlanes * n1, &D_);
+    return e;
+  }
+
+  void release() override {
+    for (T* p : ptrs_) (void)hipFree(p);
+    ptrs_.clear();
+    bytes_ = 0;
+    F_[0] = F_[1] = Fn_[0] = Fn_[1] = phi_[0] = phi_[1] = nullptr;
+    qxx_ = qyx_ = qnxx_ = qnyx_ = phn_ = D_ = nullptr;
+  }
+
+  void set_params(const double* v) override {
+    P_.GammaQ = v[0];
+    P_.GammaP = v[1];
+    P_.xi = v[2];
+    P_.tauNem = v[3];
+    P_.tauIso = v[4];
+    P_.rho = v[5];
+    P_.friction = v[6];
+    P_.AA = v[7];
+    P_.CC = v[8];
+    P_.LL = v[9];
+    P_.KK = v[10];
+    P_.zeta = v[11];
+    P_.zetaI = v[12];
+    P_.conc = v[13];
+  }
+
+  // Lyotropic::Step (lyotropic.cpp:500-525)
+  hipError_t steps(int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) override {
+    (void)L;
+    (void)tm;
+    const int N = LX_ * LY_, bs = 256, nb = (N + bs - 1) / bs;
+    const int nbt = (TP_ + bs - 1) / bs;
+    for (int st = 0; st < nsteps; ++st) {
+      T* f = F_[cur_];
+      T* fn = Fn_[cur_];
+      fields_bc(s);
+      hipLaunchKernelGGL((loop_uq_kernel<T, true, LayerGrid>), dim3(nb), dim3(bs), 0, s, f, qxx_,
+                         qyx_, phi_[pc_], P_, LX_, LY_, NPw_, D_);
+      fields2_bc(s);
+      LoopUF<T> u{};
+      u.D = D_;
+      u.f = f;
+      u.qxx = qxx_;
+      u.qyx = qyx_;
+      u.phi = phi_[pc_];
+      u.f_out = f;  // in place, as UpdateFieldsAtNode writes ff[k]
+      u.fn_out = fn;
+      u.qxx_out = qxx_;
+      u.qyx_out = qyx_;
+      u.qnxx_out = qnxx_;
+      u.qnyx_out = qnyx_;
+      u.phi_out = phi_[pc_ ^ 1];
+      u.phn_out = phn_;
+      hipLaunchKernelGGL((loop_uf_kernel<T, true, true, LayerGrid>), dim3(nb), dim3(bs), 0, s, u,
+                         P_, LX_, LY_, NPw_);
+      pc_ ^= 1;
+      for (int ph = 0; ph < 3; ++ph)
+        hipLaunchKernelGGL((wall_lb_kernel<T>), dim3(phase_blocks(ph)), dim3(bs), 0, s, geo(),
+                           ph, f, fn);
+      hipLaunchKernelGGL((wall_move_kernel<T>), dim3(nbt), dim3(bs), 0, s, geo(), (const T*)f,
+                         (const T*)fn, F_[cur_ ^ 1], Fn_[cur_ ^ 1]);
+      cur_ ^= 1;
+      f = F_[cur_];
+      fn = Fn_[cur_];
+      for (int c = 1; c <= npc_; ++c) {
+        fields_bc(s);
+        hipLaunchKernelGGL((loop_uq_kernel<T, true, LayerGrid>), dim3(nb), dim3(bs), 0, s, f,
+                           qxx_, qyx_, phi_[pc_], P_, LX_, LY_, NPw_, D_);
+        fields2_bc(s);
+        LoopUF<T> w{};
+        w.D = D_;
+        w.f = f;
+        w.qxx = qxx_;
+        w.qyx = qyx_;
+        w.phi = phi_[pc_];
+        w.fn = fn;
+        w.qnxx = qnxx_;
+        w.qnyx = qnyx_;
+        w.phn = phn_;
+        w.f_out = f;
+        w.qxx_out = qxx_;
+        w.qyx_out = qyx_;
+        w.phi_out = phi_[pc_ ^ 1];
+        hipLaunchKernelGGL((loop_uf_kernel<T, true, false, LayerGrid>), dim3(nb), dim3(bs), 0, s,
+                           w, P_, LX_, LY_, NPw_);
+        pc_ ^= 1;
+      }
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+
+  // Every buffer restarts at zero (ghost layer included, as after
+  // Field::SetSize), then the domain takes the given state.
+  hipError_t pack(const double* d_ff, const double* d_qxx, const double* d_qyx,
+                  const double* d_phi, hipStream_t s) override {
+    hipError_t e = zero_all(s);
+    if (e != hipSuccess) return e;
+    cur_ = 0;
+    pc_ = 0;
+    const int N = LX_ * LY_, bs = 256;
+    hipLaunchKernelGGL((wall_pack_kernel<T>), dim3((N + bs - 1) / bs), dim3(bs), 0, s, geo(), d_ff,
+                       d_qxx, d_qyx, d_phi, F_[0], qxx_, qyx_, phi_[0]);
+    return hipGetLastError();
+  }
+
+  // mass_lyo_field numbering; derived fields are the planes of the last
+  // UpdateQuantities call (zero before the first step)
+  hipError_t field(int id, double* d_out, hipStream_t s) override {
+    const T* a = nullptr;
+    int np = 1;
+    switch (id) {
+      case 0: a = F_[cur_]; np = 9; break;
+      case 1: a = qxx_; break;
+      case 2: a = qyx_; break;
+      case 3: a = phi_[pc_]; break;
+      case D_N: a = plane(LP_N); break;
+      case D_UX: a = plane(LP_UX); break;
+      case D_UY: a = plane(LP_UY); break;
+      case D_SXX: a = plane(LP_SXX); break;
+      case D_SXY: a = plane(LP_SXY); break;
+      case D_SYX: a = plane(LP_SYX); break;
+      case D_SYY: a = plane(LP_SYY); break;
+      case D_BULK: a = plane(LP_BULK); break;
+      case D_EL_XX: a = plane(LP_ELXX); break;
+      case D_EL_YX: a = plane(LP_ELYX); break;
+      case D_PF_XX: a = plane(LP_PFXX); break;
+      case D_PF_YX: a = plane(LP_PFYX); break;
+      case D_ACT_XX: a = plane(LP_ACTXX); break;
+      case D_ACT_YX: a = plane(LP_ACTYX); break;
+      case D_MU: a = plane(LP_MU); break;
+      case D_HXX: a = plane(LP_HXX); break;
+      case D_HYX: a = plane(LP_HYX); break;
+      default: return hipErrorInvalidValue;
+    }
+    const int N = LX_ * LY_, bs = 256;
+    hipLaunchKernelGGL((wall_unpack_kernel<T>), dim3((N + bs - 1) / bs), dim3(bs), 0, s, geo(), a,
+                       np, d_out);
+    return hipGetLastError();
+  }
+
+  hipError_t sums(double* d_part, int max_part, double* d_out, hipStream_t s) override {
+    const int N = LX_ * LY_;
+    int nparts = (N + 4095) / 4096;
+    if (nparts > max_part) nparts = max_part;
+    const int per = (N + nparts - 1) / nparts;
+    hipLaunchKernelGGL((wall_partial_sums_kernel<T>), dim3(nparts), dim3(256), 0, s, geo(),
+                       F_[cur_], phi_[pc_], per, d_part, d_part + max_part);
+    hipLaunchKernelGGL(final_sums_kernel, dim3(1), dim3(256), 0, s, d_part, d_part + max_part,
+                       nparts, 0.0, d_out);
+    return hipGetLastError();
+  }
+
+  hipError_t diagnostics(DiagScratch&, double*, hipStream_t) override {
+    return hipErrorNotSupported;
+  }
+  size_t bytes() const override { return bytes_; }
+
+  std::vector<int> plan(const LaunchCfg&, int) const override { return {}; }
+  size_t job_bytes() const override { return 0; }
+  int job_items(int, const LaunchCfg&) const override { return 0; }
+  void fill_job(int, int, int, const LaunchCfg&, void*) const override {}
+  void after_batch_steps(int, const std::vector<int>&) override {}
+  int parity() const override { return cur_; }
+  hipError_t launch_batch(const void*, int, int, int, bool, int, int,
+                          hipStream_t) const override {
+    return hipErrorNotSupported;
+  }
+  hipError_t net_features(int, float*, const float*, __bf16*, hipStream_t) override {
+    return hipErrorNotSupported;
+  }
+  hipError_t net_targets(float*, hipStream_t) override { return hipErrorNotSupported; }
+  hipError_t net_apply(const float*, hipStream_t) override { return hipErrorNotSupported; }
+  hipError_t net_judge(const float*, double*, int, hipStream_t) override {
+    return hipErrorNotSupported;
+  }
+  hipError_t steps_with(int, int, const LaunchCfg&, hipStream_t) override {
+    return hipErrorNotSupported;
+  }
+  int npc() const override { return npc_; }
+  hipError_t alloc_loops() override { return hipSuccess; }
+
+  // checkpoint state: f (9), Qxx, Qyx, phi on the whole torus, ghost layer
+  // included, each plane (LX+2)*(LY+2) values
+  int state_planes() const override { return 12; }
+  size_t state_bytes() const override { return 12 * (size_t)TP_ * sizeof(T); }
+  hipError_t save_state(void* host, hipStream_t s) override {
+    return copy_state(host, nullptr, s);
+  }
+  hipError_t load_state(const void* host, hipStream_t s) override {
+    hipError_t e = zero_all(s);
+    if (e != hipSuccess) return e;
+    cur_ = 0;
+    pc_ = 0;
+    return copy_state(nullptr, host, s);
+  }
+
+ private:
+  WallGeo geo() const { return WallGeo{LX_, LY_, LY_ + 2, NPw_, bc_}; }
+  const T* plane(int i) const { return D_ + (size_t)i * NPw_; }
+  unsigned phase_blocks(int phase) const {
+    const int n = phase == 0 ? 2 * LY_ : (phase == 1 ? 2 * LX_ : 4);
+    return (unsigned)((n + 255) / 256);
+  }
+  void fields_bc(hipStream_t s) {
+    for (int ph = 0; ph < 3; ++ph)
+      hipLaunchKernelGGL((wall_fields_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s, geo(),
+                         ph, qxx_, qyx_, phi_[pc_]);
+  }
+  void fields2_bc(hipStream_t s) {
+    for (int ph = 0; ph < 3; ++ph)
+      hipLaunchKernelGGL((wall_fields2_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s, geo(),
+                         ph, D_);
+  }
+  hipError_t zero_all(hipStream_t s) {
+    for (size_t i = 0; i < ptrs_.size(); ++i) {
+      const hipError_t e = hipMemsetAsync(ptrs_[i], 0, sizes_[i] * sizeof(T), s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  // to host (out) or from host (in), through a device staging copy
+  hipError_t copy_state(void* out, const void* in, hipStream_t s) {
+    const size_t bytes = state_bytes();
+    T* stage = nullptr;
+    hipError_t e = hipMalloc((void**)&stage, bytes);
+    if (e != hipSuccess) return e;
+    T* planes[4] = {F_[cur_], qxx_, qyx_, phi_[pc_]};
+    const int counts[4] = {9, 1, 1, 1};
+    const int bs = 256, nb = (TP_ + bs - 1) / bs;
+    if (in) e = hipMemcpyAsync(stage, in, bytes, hipMemcpyHostToDevice, s);
+    size_t off = 0;
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) {
+      if (in)
+        hipLaunchKernelGGL((wall_copy_planes_kernel<T>), dim3(nb), dim3(bs), 0, s,
+                           (const T*)(stage + off), TP_, planes[i], NPw_, TP_, counts[i]);
+      else
+        hipLaunchKernelGGL((wall_copy_planes_kernel<T>), dim3(nb), dim3(bs), 0, s,
+                           (const T*)planes[i], NPw_, stage + off, TP_, TP_, counts[i]);
+      off += (size_t)counts[i] * TP_;
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && out) e = hipMemcpyAsync(out, stage, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(stage);
+    return e;
+  }
+  hipError_t get_(size_t n, T** p) {
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e == hipSuccess) {
+      ptrs_.push_back(*p);
+      sizes_.push_back(n);
+      bytes_ += n * sizeof(T);
+      e = hipMemset(*p, 0, n * sizeof(T));
+    }
+    return e;
+  }
+
+  int bc_;
+  Params<T> P_{};
+  int LX_ = 0, LY_ = 0, npc_ = 1, TP_ = 0, NPw_ = 0;
+  int cur_ = 0;  // ff/fn parity (Move swaps ff with ff_tmp)
+  int pc_ = 0;   // phi parity (UpdateFields swaps phi with phi_tmp)
+  T* F_[2] = {};
+  T* Fn_[2] = {};
+  T* phi_[2] = {};
+  T *qxx_ = nullptr, *qyx_ = nullptr, *qnxx_ = nullptr, *qnyx_ = nullptr, *phn_ = nullptr;
+  T* D_ = nullptr;
+  std::vector<T*> ptrs_;
+  std::vector<size_t> sizes_;
+  size_t bytes_ = 0;
+};
+
+}  // namespace lyo

@@ -36,9 +36,10 @@ void LyotropicHip::Initialize() {
   angle = angle_deg * M_PI / 180.;
   if (nsubsteps > 1)
     throw error_msg("time stepping not implemented for this model, please set nsubsteps=1.");
-  if (BC != 0)
+  if (BC > 4)
     throw error_msg("boundary conditions bc=", BC,
-                    " are not available on the GPU path (periodic boundaries, bc=0, only).");
+                    " are not available on the GPU path (bc 0 periodic, 1/2 free-/no-slip "
+                    "channel, 3/4 free-/no-slip box).");
   mass_lyo_params p{};
   p.LX = (int32_t)LX;
   p.LY = (int32_t)LY;
@@ -51,6 +52,7 @@ void LyotropicHip::Initialize() {
     throw error_msg("precision must be fp64 or fp32 (got '", precision, "')");
   p.conserve_phi = 0;
   p.plane_pad = 0;
+  p.bc = (int32_t)BC;
   p.GammaQ = GammaQ;
   p.GammaP = GammaP;
   p.xi = xi;

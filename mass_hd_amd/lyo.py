@@ -45,7 +45,8 @@ PHYS_KEYS = ["GammaQ", "GammaP", "xi", "tauNem", "tauIso", "rho", "friction",
 class Params(ctypes.Structure):
     _fields_ = ([("LX", ctypes.c_int32), ("LY", ctypes.c_int32), ("npc", ctypes.c_int32),
                  ("precision", ctypes.c_int32), ("conserve_phi", ctypes.c_int32),
-                 ("plane_pad", ctypes.c_int32)] + [(k, ctypes.c_double) for k in PHYS_KEYS])
+                 ("plane_pad", ctypes.c_int32), ("bc", ctypes.c_int32)]
+                + [(k, ctypes.c_double) for k in PHYS_KEYS])
 
 
 class Init(ctypes.Structure):
@@ -135,9 +136,12 @@ def _dptr(a):
 
 
 class Lyotropic:
-    """One lyotropic run on the GPU (the reference's ``Lyotropic`` model)."""
+    """One lyotropic run on the GPU (the reference's ``Lyotropic`` model).
 
-    def __init__(self, LX, LY, npc=1, precision=FP64, plane_pad=0, **phys):
+    bc: 0 periodic, 1 free-slip channel, 2 no-slip channel, 3 free-slip box,
+    4 no-slip box (walled grids: FP64 only)."""
+
+    def __init__(self, LX, LY, npc=1, precision=FP64, plane_pad=0, bc=0, **phys):
         self.lib = load_library()
         if isinstance(precision, str):
             precision = {"fp64": FP64, "fp32": FP32}[precision]
@@ -148,10 +152,10 @@ class Lyotropic:
         if unknown:
             raise ValueError("unknown model parameters: %s" % ", ".join(sorted(unknown)))
         self.LX, self.LY, self.N = int(LX), int(LY), int(LX) * int(LY)
-        self.npc, self.precision = int(npc), precision
+        self.npc, self.precision, self.bc = int(npc), precision, int(bc)
         self.phys = dict(phys)
         p = Params(LX=self.LX, LY=self.LY, npc=self.npc, precision=precision,
-                   conserve_phi=0, plane_pad=int(plane_pad),
+                   conserve_phi=0, plane_pad=int(plane_pad), bc=self.bc,
                    **{k: float(phys[k]) for k in PHYS_KEYS})
         h = ctypes.c_void_p()
         rc = self.lib.mass_lyo_create(ctypes.byref(p), ctypes.byref(h))
@@ -306,7 +310,7 @@ class Batch:
         for i, r in enumerate(runs):
             phys = {k: float(r[k]) for k in PHYS_KEYS}
             arr[i] = Params(LX=int(r["LX"]), LY=int(r["LY"]), npc=int(npc), precision=precision,
-                            conserve_phi=0, plane_pad=0, **phys)
+                            conserve_phi=0, plane_pad=0, bc=int(r.get("bc", 0)), **phys)
             self.sizes.append((int(r["LX"]), int(r["LY"])))
         h = ctypes.c_void_p()
         rc = self.lib.mass_lyo_batch_create(arr, len(runs), ctypes.byref(h))

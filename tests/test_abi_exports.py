@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     lib = ctypes.CDLL(LIB)
-    assert lib.mass_lyo_abi_version() == 1
+    assert lib.mass_lyo_abi_version() == 2
 
 
 def test_header_compiles_as_c():

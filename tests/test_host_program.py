@@ -108,17 +108,18 @@ def test_nsubsteps_rejected(card):
 
 
 def test_boundary_conditions_rejected(card):
-    rc, _, err = run([card(RUNCARD + "bc = 1\n"), "--no-write"])
-    assert rc == 1 and "periodic" in err
+    # bc 0-4 (periodic, channels, boxes) are offered; the other values are not
+    rc, _, err = run([card(RUNCARD + "bc = 5\n"), "--no-write"])
+    assert rc == 1 and "bc=5" in err and "channel" in err
 
 
 def test_command_line_overrides_runcard(card):
     # the command line is stored first, so it wins; bc on the command line
-    rc, _, err = run([card(), "--bc", "3", "--no-write"])
-    assert rc == 1 and "bc=3" in err
+    rc, _, err = run([card(RUNCARD + "bc = 6\n"), "--bc", "7", "--no-write"])
+    assert rc == 1 and "bc=7" in err
     # and a runcard value alone is used
-    rc, _, err = run([card(RUNCARD + "bc = 4\n"), "--no-write"])
-    assert rc == 1 and "bc=4" in err
+    rc, _, err = run([card(RUNCARD + "bc = 6\n"), "--no-write"])
+    assert rc == 1 and "bc=6" in err
 
 
 def test_bad_precision(card):
@@ -195,6 +196,26 @@ def test_host_run_matches_oracle(card, tmp_path, require_gpu):
     o.step(20)
     f20 = json.load(open(outdir / "frame20.json"))["data"]
     for k in ("QQxx", "QQyx", "phi", "sigmaXY", "sigma_active_xx"):
+        assert np.allclose(f20[k]["value"], o.get(k), rtol=1e-5, atol=1e-6), k
+    assert np.allclose(np.array(f20["ff"]["value"]), o.get("ff"), rtol=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bc", [2, 4])
+def test_host_walled_run_matches_oracle(card, tmp_path, require_gpu, bc):
+    """A runcard with walls (bc 2: no-slip channel, bc 4: no-slip box) gives
+    the oracle's frames to the archive's six digits."""
+    from oracle import pyoracle as po
+    outdir = tmp_path / "out"
+    rc, out, err = run([card(RUNCARD + "bc = %d\n" % bc), "-o", str(outdir), "-f"])
+    assert rc == 0, err
+    o = po.Oracle(32, 32, npc=1, bc=bc, GammaQ=0.1, GammaP=0.1, xi=0.7, tauNem=2.0, tauIso=2.0,
+                  rho=40.0, friction=0.0, AA=0.01, CC=0.0, LL=0.04, KK=0.04, zeta=0.02,
+                  zetaI=0.0, conc=0.5)
+    o.configure(config="random", seed=1, angle=90, noise=1, qamp=0.5, phinoise=0.05)
+    o.step(20)
+    f20 = json.load(open(outdir / "frame20.json"))["data"]
+    for k in ("QQxx", "QQyx", "phi", "sigmaXX", "sigmaXY"):
         assert np.allclose(f20[k]["value"], o.get(k), rtol=1e-5, atol=1e-6), k
     assert np.allclose(np.array(f20["ff"]["value"]), o.get("ff"), rtol=1e-5)
 
