@@ -17,6 +17,11 @@
  *                                   src/geometry.hpp:17-89,148-199,306-316,475-501
  *   boundary conditions             src/models/lyotropic.cpp:388-498,
  *                                   src/fields.hpp:619-905
+ *   go-or-grow                      src/models/go-or-grow.cpp:17-404,
+ *                                   src/models/lyotropic-with-division.cpp:18-156
+ *   random_normal                   src/random.cpp:27-30 (libstdc++
+ *                                   normal_distribution: Marsaglia polar
+ *                                   method, a fresh distribution per draw)
  *   ConfigureAtNode                 src/models/lyotropic.cpp:69-166
  *   mt19937 + random_real           src/random.cpp:12,21-24,43-46 (libstdc++
  *                                   generate_canonical<double,53>: two 32-bit
@@ -42,6 +47,9 @@ static const double SB = 1/12., SD = 1/12.;
 static const int XDIR[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
 static const int YDIR[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
 
+/* The generator fixed by the C++ standard ([rand.predef]: mt19937). */
+typedef struct { uint32_t mt[624]; int idx; } mt_state;
+
 struct lyo_oracle {
   lyo_oracle_params p;
   uint32_t N;   /* domain nodes */
@@ -51,6 +59,13 @@ struct lyo_oracle {
   double* s[LYO_O_NFIELDS];          /* scalar fields (index >= 2) */
   double* phi_tmp;
   double totalphi, countphi, ftot, ptot;
+  mt_state rng; /* the reference's global generator (random.cpp:12), seeded at configure */
+  /* go-or-grow */
+  int growth;
+  lyo_oracle_growth g;
+  double* m_tmp;
+  uint8_t *div_mask, *death_mask;
+  uint32_t div_count, death_count;
 };
 
 /* ---------------------------------------------------------------- helpers */
@@ -101,8 +116,6 @@ void lyo_oracle_equilibrium(double vx, double vy, double nn, double* fe9) {
 }
 
 /* ------------------------------------------------------------ mt19937 RNG */
-/* The generator fixed by the C++ standard ([rand.predef]: mt19937). */
-typedef struct { uint32_t mt[624]; int idx; } mt_state;
 
 static void mt_seed(mt_state* s, uint32_t seed) {
   s->mt[0] = seed;
@@ -144,6 +157,27 @@ static double mt_canonical(mt_state* s) {
 /* uniform_real_distribution<>(lo, hi)(gen) */
 static double mt_real(mt_state* s, double lo, double hi) {
   return (mt_canonical(s) * (hi - lo)) + lo;
+}
+
+/* normal_distribution<>(0, 1)(gen) of a fresh distribution (libstdc++'s
+ * polar method: pairs of 2U-1 until 0 < r2 <= 1, returns y*sqrt(-2 ln r2/r2),
+ * then *stddev + mean) */
+static double mt_normal(mt_state* s) {
+  double x, y, r2;
+  do {
+    x = 2.0 * mt_canonical(s) - 1.0;
+    y = 2.0 * mt_canonical(s) - 1.0;
+    r2 = x * x + y * y;
+  } while (r2 > 1.0 || r2 == 0.0);
+  const double mult = sqrt(-2 * log(r2) / r2);
+  const double ret = y * mult;
+  return ret * 1.0 + 0.0;
+}
+
+void lyo_oracle_random_normal(uint32_t seed, int n, double* out) {
+  mt_state s;
+  mt_seed(&s, seed);
+  for (int i = 0; i < n; ++i) out[i] = mt_normal(&s);
 }
 
 void lyo_oracle_mt19937(uint32_t seed, int n, uint32_t* out) {
@@ -193,6 +227,9 @@ lyo_oracle* lyo_oracle_create(const lyo_oracle_params* p) {
   o->fn_tmp = (double*)calloc(9 * T, sizeof(double));
   for (int f = 2; f < LYO_O_NFIELDS; ++f) o->s[f] = (double*)calloc(T, sizeof(double));
   o->phi_tmp = (double*)calloc(T, sizeof(double));
+  o->m_tmp = (double*)calloc(T, sizeof(double));
+  o->div_mask = (uint8_t*)calloc(o->N, 1);
+  o->death_mask = (uint8_t*)calloc(o->N, 1);
   if (!p->bc) {
     /* geometry.cpp:100-110: k = y + LY*x, neighbour v at ((x+cx) mod LX, (y+cy) mod LY) */
     for (uint32_t k = 0; k < o->N; ++k) {
@@ -228,6 +265,9 @@ void lyo_oracle_destroy(lyo_oracle* o) {
   free(o->ff); free(o->fn); free(o->ff_tmp); free(o->fn_tmp);
   for (int f = 2; f < LYO_O_NFIELDS; ++f) free(o->s[f]);
   free(o->phi_tmp);
+  free(o->m_tmp);
+  free(o->div_mask);
+  free(o->death_mask);
   free(o);
 }
 
@@ -239,6 +279,8 @@ uint32_t lyo_oracle_total_size(const lyo_oracle* o) { return o->T; }
 
 #define F(id) (o->s[id])
 
+static void growth_configure(lyo_oracle* o);
+
 /* ------------------------------------------------------------- configure */
 
 static unsigned udiff(unsigned a, unsigned b) { return a > b ? a - b : b - a; }
@@ -249,8 +291,8 @@ static unsigned uwrap(unsigned x, unsigned L) {
 
 int lyo_oracle_configure(lyo_oracle* o, const lyo_oracle_init* in) {
   const unsigned LX = (unsigned)o->p.LX, LY = (unsigned)o->p.LY;
-  mt_state rng;
-  mt_seed(&rng, in->seed);
+  mt_state* rng = &o->rng;
+  mt_seed(rng, in->seed);
   const double angle = in->angle_deg * M_PI / 180.;
   const double level = in->level, radius = in->radius;
   o->totalphi = 0; o->ftot = 0; o->ptot = 0;
@@ -298,11 +340,11 @@ int lyo_oracle_configure(lyo_oracle* o, const lyo_oracle_init* in) {
       default:
         return -1;
     }
-    const double theta = angle + in->noise * M_PI * (mt_real(&rng, 0., 1.) - .5);
+    const double theta = angle + in->noise * M_PI * (mt_real(rng, 0., 1.) - .5);
     F(LYO_O_QQXX)[k] = in->qamp * nematicOrder * (cos(2 * theta));
     F(LYO_O_QQYX)[k] = in->qamp * nematicOrder * (sin(2 * theta));
     if (in->config == 4)
-      F(LYO_O_PHI)[k] = o->p.conc + mt_real(&rng, -in->phinoise, in->phinoise);
+      F(LYO_O_PHI)[k] = o->p.conc + mt_real(rng, -in->phinoise, in->phinoise);
     else
       F(LYO_O_PHI)[k] = nematicOrder * o->p.conc;
     o->totalphi += F(LYO_O_PHI)[k];
@@ -312,6 +354,7 @@ int lyo_oracle_configure(lyo_oracle* o, const lyo_oracle_init* in) {
     for (int v = 0; v < 9; ++v) o->ftot = o->ftot + o->ff[9 * (size_t)k + v];
     o->ptot += F(LYO_O_PHI)[k];
   }
+  if (o->growth) growth_configure(o);
   return 0;
 }
 
@@ -498,6 +541,355 @@ static void update_fields_at_node(lyo_oracle* o, unsigned k, int first) {
   }
 }
 
+/* ------------------------------------------------------------- go-or-grow */
+
+static void s_apply(const lyo_oracle* o, double* a, int kind, double value, int channel);
+
+int lyo_oracle_set_growth(lyo_oracle* o, const lyo_oracle_growth* g) {
+  /* go-or-grow.cpp:21-26 */
+  if (g->chi_noise < 0 || g->chi_length < 0 || g->relax_dt < 0) return -1;
+  if (g->chi_config != 0 && g->chi_config != 1) return -1;
+  o->growth = 1;
+  o->g = *g;
+  /* lyotropic-with-division.cpp:23-31 */
+  o->div_count = o->death_count = 0u;
+  o->p.conserve_phi = ((g->alpha == 0 || g->division_rate == 0) &&
+                       (g->beta == 0 || g->death_rate == 0));
+  return 0;
+}
+
+void lyo_oracle_masks(const lyo_oracle* o, uint8_t* division, uint8_t* death) {
+  memcpy(division, o->div_mask, o->N);
+  memcpy(death, o->death_mask, o->N);
+}
+
+/* go-or-grow.cpp:365-385: Lyotropic's field conditions, then m and chi */
+static void growth_boundary_fields(lyo_oracle* o) {
+  lyo_oracle_boundary(o, 0);
+  if (o->growth && o->p.bc) {
+    s_apply(o, F(LYO_O_M), 0, 0., o->p.bc <= 2);
+    s_apply(o, F(LYO_O_CHI), 0, 0., o->p.bc <= 2);
+  }
+}
+
+/* go-or-grow.cpp:201-209 */
+static void update_phenotype(lyo_oracle* o) {
+  const double phi_epsilon = 1e-12;
+  for (unsigned k = 0; k < o->N; ++k)
+    F(LYO_O_CHI)[k] = F(LYO_O_PHI)[k] > phi_epsilon ? F(LYO_O_M)[k] / F(LYO_O_PHI)[k] : 0.;
+  growth_boundary_fields(o);
+}
+
+/* go-or-grow.cpp:211-222 */
+static void project_m(lyo_oracle* o) {
+  for (unsigned k = 0; k < o->N; ++k) {
+    const double upper = F(LYO_O_PHI)[k] > 0 ? F(LYO_O_PHI)[k] : 0.;
+    if (F(LYO_O_M)[k] < 0)
+      F(LYO_O_M)[k] = 0;
+    else if (F(LYO_O_M)[k] > upper)
+      F(LYO_O_M)[k] = upper;
+  }
+}
+
+/* go-or-grow.cpp:42-119: dry free-energy descent of phi and Q */
+static void relax_free_energy(lyo_oracle* o) {
+  const lyo_oracle_growth* g = &o->g;
+  const lyo_oracle_params* P = &o->p;
+  if (g->relax_steps == 0 || (!g->relax_phi && !g->relax_Q)) return;
+  for (unsigned step = 0; step < g->relax_steps; ++step) {
+    lyo_oracle_boundary(o, 0);
+    for (unsigned k = 0; k < o->N; ++k) {
+      const uint32_t* d = get_neighbours(o, k);
+      const double Qxx = F(LYO_O_QQXX)[k];
+      const double Qyx = F(LYO_O_QQYX)[k];
+      const double p = F(LYO_O_PHI)[k];
+      const double q2 = Qxx * Qxx + Qyx * Qyx;
+      const double term = g->Snem - q2;
+      const double dp_critical = p - g->phi_critical;
+      const double mu_compress = dp_critical > 0 ? g->B * dp_critical : 0.;
+      F(LYO_O_HHXX)[k] = 2 * P->CC * term * Qxx + P->LL * laplacian(F(LYO_O_QQXX), d, SD);
+      F(LYO_O_HHYX)[k] = 2 * P->CC * term * Qyx + P->LL * laplacian(F(LYO_O_QQYX), d, SD);
+      F(LYO_O_MU)[k] = P->AA * p * (1 - p) * (1 - 2 * p) + mu_compress
+                       - P->KK * laplacian(F(LYO_O_PHI), d, SD);
+    }
+    if (P->bc) s_apply(o, F(LYO_O_MU), 0, 0., P->bc <= 2);
+    for (unsigned k = 0; k < o->N; ++k) {
+      const uint32_t* d = get_neighbours(o, k);
+      if (g->relax_phi)
+        o->phi_tmp[k] = F(LYO_O_PHI)[k] + g->relax_dt * P->GammaP * laplacian(F(LYO_O_MU), d, SD);
+      else
+        o->phi_tmp[k] = F(LYO_O_PHI)[k];
+      if (g->relax_Q) {
+        F(LYO_O_QNXX)[k] = F(LYO_O_QQXX)[k] + g->relax_dt * P->GammaQ * F(LYO_O_HHXX)[k];
+        F(LYO_O_QNYX)[k] = F(LYO_O_QQYX)[k] + g->relax_dt * P->GammaQ * F(LYO_O_HHYX)[k];
+      } else {
+        F(LYO_O_QNXX)[k] = F(LYO_O_QQXX)[k];
+        F(LYO_O_QNYX)[k] = F(LYO_O_QQYX)[k];
+      }
+    }
+    double* t = o->s[LYO_O_PHI]; o->s[LYO_O_PHI] = o->phi_tmp; o->phi_tmp = t;
+    t = o->s[LYO_O_QQXX]; o->s[LYO_O_QQXX] = o->s[LYO_O_QNXX]; o->s[LYO_O_QNXX] = t;
+    t = o->s[LYO_O_QQYX]; o->s[LYO_O_QQYX] = o->s[LYO_O_QNYX]; o->s[LYO_O_QNYX] = t;
+  }
+  lyo_oracle_boundary(o, 0);
+  double relaxed_phi = 0.;
+  for (unsigned k = 0; k < o->N; ++k) relaxed_phi += F(LYO_O_PHI)[k];
+  o->totalphi = relaxed_phi;
+  o->countphi = relaxed_phi;
+  o->ptot = relaxed_phi;
+}
+
+/* go-or-grow.cpp:121-141 */
+static void reset_hydrodynamics(lyo_oracle* o) {
+  double ftotal = 0.;
+  double fe[9];
+  for (unsigned k = 0; k < o->N; ++k) {
+    F(LYO_O_UX)[k] = F(LYO_O_UY)[k] = F(LYO_O_UX_PHI)[k] = F(LYO_O_UY_PHI)[k] = 0.;
+    F(LYO_O_N)[k] = o->p.rho;
+    equilibrium(0., 0., o->p.rho, fe);
+    for (int v = 0; v < 9; ++v) {
+      o->ff[9 * (size_t)k + v] = fe[v];
+      o->fn[9 * (size_t)k + v] = fe[v];
+      o->ff_tmp[9 * (size_t)k + v] = fe[v];
+      o->fn_tmp[9 * (size_t)k + v] = fe[v];
+    }
+    for (int v = 0; v < 9; ++v) ftotal = ftotal + fe[v];
+  }
+  o->ftot = ftotal;
+}
+
+/* go-or-grow.cpp:143-199 */
+static void configure_phenotype(lyo_oracle* o) {
+  const lyo_oracle_growth* g = &o->g;
+  const unsigned N = o->N;
+  double* chi = NULL;
+  if (g->chi_config == 0) { /* noise */
+    for (unsigned k = 0; k < N; ++k) F(LYO_O_CHI)[k] = mt_normal(&o->rng);
+    growth_boundary_fields(o);
+    const double smooth_rate = 0.2;
+    const unsigned smooth_steps =
+        g->chi_length > 0 ? (unsigned)ceil(g->chi_length * g->chi_length / (2 * smooth_rate)) : 0u;
+    for (unsigned s = 0; s < smooth_steps; ++s) {
+      chi = F(LYO_O_CHI);
+      for (unsigned k = 0; k < N; ++k) {
+        const uint32_t* d = get_neighbours(o, k);
+        o->m_tmp[k] = chi[k] + smooth_rate * laplacian(chi, d, SD);
+      }
+      o->s[LYO_O_CHI] = o->m_tmp;
+      o->m_tmp = chi;
+      growth_boundary_fields(o);
+    }
+    chi = F(LYO_O_CHI);
+    double mean = 0.;
+    for (unsigned k = 0; k < N; ++k) mean += chi[k];
+    mean /= N;
+    double variance = 0.;
+    for (unsigned k = 0; k < N; ++k) {
+      chi[k] -= mean;
+      variance += chi[k] * chi[k];
+    }
+    variance /= N;
+    const double amplitude = sqrt(g->chi_noise);
+    const double norm = variance > 0 ? amplitude / sqrt(variance) : 0.;
+    for (unsigned k = 0; k < N; ++k) chi[k] = g->chi0 + norm * chi[k];
+  } else { /* front */
+    const double length = g->chi_length > 0 ? g->chi_length : 1.;
+    const double center = .5 * (unsigned)(o->p.LX - 1);
+    chi = F(LYO_O_CHI);
+    for (unsigned k = 0; k < N; ++k) {
+      const double x = k / (unsigned)o->p.LY;
+      chi[k] = .5 * (1. - tanh((x - center) / length));
+    }
+  }
+  for (unsigned k = 0; k < N; ++k) F(LYO_O_M)[k] = F(LYO_O_PHI)[k] * F(LYO_O_CHI)[k];
+  project_m(o);
+  update_phenotype(o);
+  growth_boundary_fields(o);
+}
+
+static void growth_configure(lyo_oracle* o) {
+  relax_free_energy(o);
+  reset_hydrodynamics(o);
+  configure_phenotype(o);
+}
+
+/* lyotropic-with-division.cpp:34-108: one mask */
+static void draw_mask(lyo_oracle* o, uint8_t* mask, uint32_t* count, double life, double rate,
+                      double radius, int division) {
+  if ((*count)--) return;
+  const unsigned LX = (unsigned)o->p.LX, LY = (unsigned)o->p.LY;
+  memset(mask, 0, o->N);
+  *count = (uint32_t)life;
+  unsigned nc = 0, cap = 64;
+  unsigned* cx = (unsigned*)malloc(sizeof(unsigned) * cap);
+  unsigned* cy = (unsigned*)malloc(sizeof(unsigned) * cap);
+  for (unsigned k = 0; k < o->N; ++k) {
+    const double p = F(LYO_O_PHI)[k];
+    if (p >= .5) {
+      const double adjusted = division ? rate * p * (1 - p / o->g.phi_critical) : rate * p;
+      if (mt_real(&o->rng, 0., 1.) < adjusted) {
+        if (nc == cap) {
+          cap *= 2;
+          cx = (unsigned*)realloc(cx, sizeof(unsigned) * cap);
+          cy = (unsigned*)realloc(cy, sizeof(unsigned) * cap);
+        }
+        cx[nc] = k / LY;
+        cy[nc] = k % LY;
+        ++nc;
+      }
+    }
+  }
+  const double r2 = ceil(radius * radius);
+  for (unsigned k = 0; k < o->N; ++k) {
+    const unsigned yk = k % LY, xk = k / LY;
+    for (unsigned c = 0; c < nc; ++c) {
+      if (pow(uwrap(udiff(yk, cy[c]), LY), 2) + pow(uwrap(udiff(xk, cx[c]), LX), 2) <= r2) {
+        mask[k] = 1;
+        break;
+      }
+    }
+  }
+  free(cx);
+  free(cy);
+}
+
+/* lyotropic-with-division.cpp:34-108 */
+static void set_masks(lyo_oracle* o) {
+  const lyo_oracle_growth* g = &o->g;
+  draw_mask(o, o->div_mask, &o->div_count, g->division_time, g->division_rate,
+            g->division_radius, 1);
+  draw_mask(o, o->death_mask, &o->death_count, g->death_time, g->death_rate, g->death_radius, 0);
+}
+
+/* go-or-grow.cpp:224-303 */
+static void growth_uq_at_node(lyo_oracle* o, unsigned k) {
+  const lyo_oracle_params* P = &o->p;
+  const lyo_oracle_growth* g = &o->g;
+  const uint32_t* d = get_neighbours(o, k);
+  const double* f = o->ff + 9 * (size_t)k;
+  const double* QQxx = F(LYO_O_QQXX);
+  const double* QQyx = F(LYO_O_QQYX);
+  const double* phi = F(LYO_O_PHI);
+  const double Qxx = QQxx[k];
+  const double Qyx = QQyx[k];
+  const double p = phi[k];
+
+  const double nn = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+  const double vx = (f[1] - f[2] + f[5] - f[6] - f[7] + f[8]) / nn;
+  const double vy = (f[3] - f[4] + f[5] - f[6] + f[7] - f[8]) / nn;
+
+  const double del2Qxx = laplacian(QQxx, d, SD);
+  const double dxQxx = derivX(QQxx, d, SB);
+  const double dyQxx = derivY(QQxx, d, SB);
+  const double del2Qyx = laplacian(QQyx, d, SD);
+  const double dxQyx = derivX(QQyx, d, SB);
+  const double dyQyx = derivY(QQyx, d, SB);
+  const double del2p = laplacian(phi, d, SD);
+  const double dxPhi = derivX(phi, d, SB);
+  const double dyPhi = derivY(phi, d, SB);
+
+  const double q2 = Qxx * Qxx + Qyx * Qyx;
+  const double term = g->Snem - q2;
+  const double Hxx = 2 * P->CC * term * Qxx + P->LL * del2Qxx;
+  const double Hyx = 2 * P->CC * term * Qyx + P->LL * del2Qyx;
+  const double dp_critical = p - g->phi_critical;
+  const double f_compress = dp_critical > 0 ? .5 * g->B * dp_critical * dp_critical : 0.;
+  const double mu_compress = dp_critical > 0 ? g->B * dp_critical : 0.;
+  const double mu = P->AA * p * (1 - p) * (1 - 2 * p) + mu_compress - P->KK * del2p;
+
+  const double sigmaB = .5 * P->AA * p * p * (1 - p) * (1 - p) + f_compress
+                        + .5 * P->CC * term * term - mu * p;
+  const double active_prefactor = P->zeta * p * (1 - F(LYO_O_CHI)[k]);
+  const double sigmaF = 2 * P->xi * ((Qxx * Qxx - 1) * Hxx + Qxx * Qyx * Hyx)
+                        - active_prefactor * Qxx + .5 * P->KK * (dyPhi * dyPhi - dxPhi * dxPhi)
+                        + P->LL * (dyQxx * dyQxx + dyQyx * dyQyx - dxQxx * dxQxx - dxQyx * dxQyx);
+  const double sigmaS = 2 * P->xi * (Qyx * Qxx * Hxx + (Qyx * Qyx - 1) * Hyx)
+                        - active_prefactor * Qyx - P->KK * dxPhi * dyPhi
+                        - 2 * P->LL * (dxQxx * dyQxx + dxQyx * dyQyx);
+  const double sigmaA = 2 * (Qxx * Hyx - Qyx * Hxx);
+
+  F(LYO_O_UX)[k] = vx;
+  F(LYO_O_UY)[k] = vy;
+  F(LYO_O_UX_PHI)[k] = vx * p;
+  F(LYO_O_UY_PHI)[k] = vy * p;
+  F(LYO_O_N)[k] = nn;
+  F(LYO_O_HHXX)[k] = Hxx;
+  F(LYO_O_HHYX)[k] = Hyx;
+  F(LYO_O_MU)[k] = mu;
+  F(LYO_O_DXQQXX)[k] = dxQxx;
+  F(LYO_O_DXQQYX)[k] = dxQyx;
+  F(LYO_O_DYQQXX)[k] = dyQxx;
+  F(LYO_O_DYQQYX)[k] = dyQyx;
+  F(LYO_O_SIGMAXX)[k] = sigmaF + sigmaB + P->zetaI * (P->conc - p);
+  F(LYO_O_SIGMAYY)[k] = -sigmaF + sigmaB + P->zetaI * (P->conc - p);
+  F(LYO_O_SIGMAXY)[k] = sigmaS + sigmaA;
+  F(LYO_O_SIGMAYX)[k] = sigmaS - sigmaA;
+  F(LYO_O_SIGMA_BULK)[k] = sigmaB + P->zetaI * (P->conc - p);
+  F(LYO_O_SIGMA_EL_XX)[k] = sigmaF - .5 * P->KK * (dyPhi * dyPhi - dxPhi * dxPhi) + active_prefactor * Qxx;
+  F(LYO_O_SIGMA_EL_YX)[k] = sigmaS + P->KK * dxPhi * dyPhi + active_prefactor * Qyx;
+  F(LYO_O_SIGMA_PF_XX)[k] = .5 * P->KK * (dyPhi * dyPhi - dxPhi * dxPhi);
+  F(LYO_O_SIGMA_PF_YX)[k] = -P->KK * dxPhi * dyPhi;
+  F(LYO_O_SIGMA_ACT_XX)[k] = -active_prefactor * Qxx;
+  F(LYO_O_SIGMA_ACT_YX)[k] = -active_prefactor * Qyx;
+}
+
+static void update_fields_at_node(lyo_oracle* o, unsigned k, int first);
+
+/* go-or-grow.cpp:322-362: the phenotype density's increment, Lyotropic's
+ * update, then division/death and the bounded m */
+static void growth_uf_at_node(lyo_oracle* o, unsigned k, int first) {
+  const lyo_oracle_params* P = &o->p;
+  const lyo_oracle_growth* g = &o->g;
+  const uint32_t* d = get_neighbours(o, k);
+  const double* phi = F(LYO_O_PHI);
+  const double* chi = F(LYO_O_CHI);
+  const double* m = F(LYO_O_M);
+  const double* ux = F(LYO_O_UX);
+  const double* uy = F(LYO_O_UY);
+  const double* MU = F(LYO_O_MU);
+
+  const double division_m = o->div_mask[k] ? 1. : 0.;
+  const double death_m = o->death_mask[k] ? 1. : 0.;
+  const double growth_rate = g->alpha * division_m - g->beta * death_m;
+  const double chi_eff = chi[k];
+  const double R = chi_eff * phi[k] * growth_rate;
+
+  const double mFlux = .5 * (ux[d[1]] * m[d[1]] - ux[d[2]] * m[d[2]])
+                     + .5 * (uy[d[3]] * m[d[3]] - uy[d[4]] * m[d[4]]);
+  const double chi_px = chi[d[1]];
+  const double chi_mx = chi[d[2]];
+  const double chi_py = chi[d[3]];
+  const double chi_my = chi[d[4]];
+  const double diffusiveMFlux = .5 * (chi_px + chi_eff) * (MU[d[1]] - MU[k])
+                              - .5 * (chi_eff + chi_mx) * (MU[k] - MU[d[2]])
+                              + .5 * (chi_py + chi_eff) * (MU[d[3]] - MU[k])
+                              - .5 * (chi_eff + chi_my) * (MU[k] - MU[d[4]]);
+  const double phenotypeDiffusion = .5 * (phi[d[1]] + phi[k]) * (chi_px - chi_eff)
+                                  - .5 * (phi[k] + phi[d[2]]) * (chi_eff - chi_mx)
+                                  + .5 * (phi[d[3]] + phi[k]) * (chi_py - chi_eff)
+                                  - .5 * (phi[k] + phi[d[4]]) * (chi_eff - chi_my);
+  const double dVdChi = 2 * g->Achi * chi_eff * (1 - chi_eff) * (1 - 2 * chi_eff)
+                      + g->Ochi * (phi[k] - g->phiswitch);
+  const double Sswitch = -phi[k] * dVdChi;
+  const double mGrowth = g->growTogether ? chi_eff * R : R;
+  const double Dm = P->GammaP * diffusiveMFlux + g->Dchi * phenotypeDiffusion - mFlux + Sswitch
+                    + mGrowth;
+
+  update_fields_at_node(o, k, first);
+  o->phi_tmp[k] += R;
+  if (first) {
+    F(LYO_O_MN)[k] = m[k] + .5 * Dm;
+    o->m_tmp[k] = m[k] + Dm;
+  } else {
+    o->m_tmp[k] = F(LYO_O_MN)[k] + .5 * Dm;
+  }
+  const double upper = o->phi_tmp[k] > 0 ? o->phi_tmp[k] : 0.;
+  if (o->m_tmp[k] < 0)
+    o->m_tmp[k] = 0;
+  else if (o->m_tmp[k] > upper)
+    o->m_tmp[k] = upper;
+}
+
 /* src/models/lyotropic.cpp:343-360 */
 void lyo_oracle_update_quantities(lyo_oracle* o) {
   double sum = 0;
@@ -507,9 +899,13 @@ void lyo_oracle_update_quantities(lyo_oracle* o) {
 #pragma omp parallel for reduction(+ : sum) num_threads(nt) if (nt)
   for (unsigned k = 0; k < N; ++k) {
     sum = sum + F(LYO_O_PHI)[k];
-    update_quantities_at_node(o, k);
+    if (o->growth)
+      growth_uq_at_node(o, k);
+    else
+      update_quantities_at_node(o, k);
   }
   o->countphi = sum;
+  if (o->growth) update_phenotype(o); /* go-or-grow.cpp:305-320 */
 }
 
 /* src/models/lyotropic.cpp:362-369 */
@@ -517,11 +913,22 @@ void lyo_oracle_update_fields(lyo_oracle* o, int first) {
   const int nt = o->p.nthreads;
   const unsigned N = o->N;
   (void)nt;
+  if (o->growth) {
 #pragma omp parallel for num_threads(nt) if (nt)
-  for (unsigned k = 0; k < N; ++k) update_fields_at_node(o, k, first);
+    for (unsigned k = 0; k < N; ++k) growth_uf_at_node(o, k, first);
+  } else {
+#pragma omp parallel for num_threads(nt) if (nt)
+    for (unsigned k = 0; k < N; ++k) update_fields_at_node(o, k, first);
+  }
   double* t = o->s[LYO_O_PHI];
   o->s[LYO_O_PHI] = o->phi_tmp;
   o->phi_tmp = t;
+  if (o->growth) { /* go-or-grow.cpp:358-362 */
+    t = o->s[LYO_O_M];
+    o->s[LYO_O_M] = o->m_tmp;
+    o->m_tmp = t;
+    update_phenotype(o);
+  }
 }
 
 /* src/models/lyotropic.cpp:371-386: every node of the grid, ghost layer
@@ -755,14 +1162,15 @@ void lyo_oracle_boundary(lyo_oracle* o, int which) {
  * no-ops) */
 void lyo_oracle_step(lyo_oracle* o, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
-    lyo_oracle_boundary(o, 0);
+    if (o->growth) set_masks(o); /* lyotropic-with-division.cpp:150-156 */
+    growth_boundary_fields(o);
     lyo_oracle_update_quantities(o);
     lyo_oracle_boundary(o, 1);
     lyo_oracle_update_fields(o, 1);
     lyo_oracle_boundary(o, 2);
     lyo_oracle_move(o);
     for (int n = 1; n <= o->p.npc; ++n) {
-      lyo_oracle_boundary(o, 0);
+      growth_boundary_fields(o);
       lyo_oracle_update_quantities(o);
       lyo_oracle_boundary(o, 1);
       lyo_oracle_update_fields(o, 0);

@@ -56,8 +56,32 @@ enum {
   LYO_O_SIGMAXX, LYO_O_SIGMAYY, LYO_O_SIGMAYX, LYO_O_SIGMAXY,
   LYO_O_SIGMA_BULK, LYO_O_SIGMA_EL_XX, LYO_O_SIGMA_EL_YX,
   LYO_O_SIGMA_PF_XX, LYO_O_SIGMA_PF_YX, LYO_O_SIGMA_ACT_XX, LYO_O_SIGMA_ACT_YX,
+  LYO_O_M, LYO_O_MN, LYO_O_CHI, /* go-or-grow: phenotype density, its half step, phenotype */
   LYO_O_NFIELDS
 };
+
+/* go-or-grow (src/models/go-or-grow.cpp, lyotropic-with-division.cpp):
+ * parameters of go-or-grow.hpp:10-28 and lyotropic-with-division.hpp:19-34 */
+typedef struct lyo_oracle_growth {
+  int32_t chi_config;   /* 0 noise, 1 front */
+  int32_t growTogether, relax_phi, relax_Q;
+  uint32_t relax_steps;
+  int32_t pad_;
+  double B, Snem, Dchi, Achi, Ochi, phiswitch;
+  double chi0, chi_noise, chi_length, relax_dt;
+  double alpha, beta, division_rate, division_time, division_radius;
+  double death_rate, death_time, death_radius, phi_critical;
+} lyo_oracle_growth;
+
+/* Makes the run a go-or-grow run (call before configure): Initialize of
+ * go-or-grow.cpp:17-32 and lyotropic-with-division.cpp:18-32 (phi is
+ * conserved when neither division nor death can happen). */
+int lyo_oracle_set_growth(lyo_oracle* o, const lyo_oracle_growth* g);
+/* division and death masks (N bytes each, 0/1) of the last step */
+void lyo_oracle_masks(const lyo_oracle* o, uint8_t* division, uint8_t* death);
+/* std::normal_distribution<>(0, 1)(gen), a fresh distribution per draw as
+ * random_normal does (random.cpp:27-30) */
+void lyo_oracle_random_normal(uint32_t seed, int n, double* out);
 
 lyo_oracle* lyo_oracle_create(const lyo_oracle_params* p);
 void lyo_oracle_destroy(lyo_oracle* o);

@@ -22,7 +22,7 @@ FIELDS = {
     "sigmaXX": 20, "sigmaYY": 21, "sigmaYX": 22, "sigmaXY": 23,
     "sigma_bulk": 24, "sigma_elastic_xx": 25, "sigma_elastic_yx": 26,
     "sigma_phase_field_xx": 27, "sigma_phase_field_yx": 28,
-    "sigma_active_xx": 29, "sigma_active_yx": 30,
+    "sigma_active_xx": 29, "sigma_active_yx": 30, "m": 31, "mN": 32, "chi": 33,
 }
 
 CONFIGS = {
@@ -35,6 +35,22 @@ class _Params(ctypes.Structure):
     _fields_ = ([("LX", ctypes.c_int32), ("LY", ctypes.c_int32), ("npc", ctypes.c_int32),
                  ("nthreads", ctypes.c_int32), ("conserve_phi", ctypes.c_int32),
                  ("bc", ctypes.c_int32)] + [(k, ctypes.c_double) for k in PHYS_KEYS])
+
+
+GROWTH_DOUBLES = ["B", "Snem", "Dchi", "Achi", "Ochi", "phiswitch", "chi0", "chi_noise",
+                  "chi_length", "relax_dt", "alpha", "beta", "division_rate", "division_time",
+                  "division_radius", "death_rate", "death_time", "death_radius", "phi_critical"]
+# go-or-grow defaults (go-or-grow.hpp:10-28)
+GROWTH_DEFAULTS = dict(chi_config="noise", growTogether=0, relax_phi=0, relax_Q=0, relax_steps=0,
+                       B=0.0, Snem=1.0, Dchi=0.0, Achi=0.0, Ochi=0.0, phiswitch=0.0, chi0=0.0,
+                       chi_noise=0.0, chi_length=0.0, relax_dt=1.0)
+
+
+class _Growth(ctypes.Structure):
+    _fields_ = ([("chi_config", ctypes.c_int32), ("growTogether", ctypes.c_int32),
+                 ("relax_phi", ctypes.c_int32), ("relax_Q", ctypes.c_int32),
+                 ("relax_steps", ctypes.c_uint32), ("pad_", ctypes.c_int32)]
+                + [(k, ctypes.c_double) for k in GROWTH_DOUBLES])
 
 
 class _Init(ctypes.Structure):
@@ -80,6 +96,11 @@ def lib():
         L.lyo_oracle_boundary.argtypes = [vp, ctypes.c_int]
         L.lyo_oracle_boundary.restype = None
         L.lyo_oracle_get_all.argtypes = [vp, ctypes.c_int, dp]
+        L.lyo_oracle_set_growth.argtypes = [vp, ctypes.POINTER(_Growth)]
+        L.lyo_oracle_masks.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p]
+        L.lyo_oracle_masks.restype = None
+        L.lyo_oracle_random_normal.argtypes = [ctypes.c_uint32, ctypes.c_int, dp]
+        L.lyo_oracle_random_normal.restype = None
         L.lyo_oracle_mt19937.argtypes = [ctypes.c_uint32, ctypes.c_int, up]
         L.lyo_oracle_mt19937.restype = None
         L.lyo_oracle_random_real.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_double,
@@ -163,6 +184,26 @@ class Oracle:
         return dict(zip(["fsum", "phisum", "ftot", "ptot", "totalphi", "countphi"],
                         [x.value for x in v]))
 
+    def set_growth(self, **kw):
+        """Makes this a go-or-grow run (before configure).  Keys: those of
+        GROWTH_DEFAULTS (chi_config "noise" or "front") and alpha, beta,
+        division_rate/time/radius, death_rate/time/radius, phi_critical."""
+        g = dict(GROWTH_DEFAULTS)
+        g.update(kw)
+        cfg = {"noise": 0, "front": 1}[g["chi_config"]]
+        s = _Growth(chi_config=cfg, growTogether=int(g["growTogether"]),
+                    relax_phi=int(g["relax_phi"]), relax_Q=int(g["relax_Q"]),
+                    relax_steps=int(g["relax_steps"]), pad_=0,
+                    **{k: float(g[k]) for k in GROWTH_DOUBLES})
+        if self.L.lyo_oracle_set_growth(self.o, ctypes.byref(s)) != 0:
+            raise ValueError("oracle: bad growth parameters")
+
+    def masks(self):
+        div = np.zeros(self.N, dtype=np.uint8)
+        death = np.zeros(self.N, dtype=np.uint8)
+        self.L.lyo_oracle_masks(self.o, div.ctypes.data, death.ctypes.data)
+        return div, death
+
     def boundary(self, which):
         """0: BoundaryConditionsFields, 1: ...Fields2, 2: ...LB."""
         self.L.lyo_oracle_boundary(self.o, which)
@@ -190,6 +231,12 @@ def mt19937(seed, n):
 def random_real(seed, n, lo=0.0, hi=1.0):
     out = np.empty(n)
     lib().lyo_oracle_random_real(seed, n, lo, hi, _dp(out))
+    return out
+
+
+def random_normal(seed, n):
+    out = np.empty(n)
+    lib().lyo_oracle_random_normal(seed, n, _dp(out))
     return out
 
 
