@@ -117,8 +117,28 @@ enum mass_lyo_field {
   MASS_LYO_MU = 18,
   MASS_LYO_HHXX = 19,
   MASS_LYO_HHYX = 20,
-  MASS_LYO_NUM_FIELDS = 21
+  /* go-or-grow runs (mass_lyo_set_growth) also have: */
+  MASS_LYO_M = 21,             /* phenotype density m */
+  MASS_LYO_CHI = 22,           /* phenotype chi = m/phi */
+  MASS_LYO_DIVISION_MASK = 23, /* 0/1 */
+  MASS_LYO_DEATH_MASK = 24,    /* 0/1 */
+  MASS_LYO_NUM_FIELDS = 25
 };
+
+/* go-or-grow parameters (go-or-grow.hpp:10-28, lyotropic-with-division.hpp:
+ * 19-34; their runcard keys go-or-grow.cpp:387-423,
+ * lyotropic-with-division.cpp:128-148). */
+typedef struct mass_lyo_growth {
+  int32_t chi_config;   /* 0 noise, 1 front */
+  int32_t growTogether; /* m's growth source: 0 R, 1 chi*R */
+  int32_t relax_phi, relax_Q;
+  uint32_t relax_steps; /* dry free-energy relaxation before the dynamics */
+  int32_t reserved;     /* 0 */
+  double B, Snem, Dchi, Achi, Ochi, phiswitch;
+  double chi0, chi_noise, chi_length, relax_dt;
+  double alpha, beta, division_rate, division_time, division_radius;
+  double death_rate, death_time, death_radius, phi_critical;
+} mass_lyo_growth;
 
 typedef struct mass_lyo_s* mass_lyo_t;
 
@@ -249,6 +269,23 @@ int mass_lyo_checkpoint_load(mass_lyo_t h, const void* buf, uint64_t bytes);
  * gets zeros in this mode.  Reference: Lyotropic::Step,
  * src/models/lyotropic.cpp:343-386,500-525. */
 int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on);
+
+/* Makes h a go-or-grow run: replaces the GoOrGrow(LX,LY,BC) factory +
+ * GoOrGrow::Initialize (go-or-grow.cpp:13-32, lyotropic-with-division.cpp:
+ * 14-32).  Call after mass_lyo_create and before mass_lyo_configure; FP64
+ * only, any bc.  mass_lyo_configure then also runs Configure's dry
+ * relaxation, hydrodynamic reset and phenotype set-up (go-or-grow.cpp:34-40)
+ * and returns ftot of the reset and ptot of the relaxed phi; every
+ * mass_lyo_step first redraws the division/death masks when due
+ * (lyotropic-with-division.cpp:34-108, 150-156) from the run's mt19937 stream
+ * on the host, reading phi back from the device for that.  phi is conserved
+ * (conserve_phi, lyotropic.cpp:297-298) when neither division nor death can
+ * happen; its sum is then the device's fixed-tree sum, not the reference's
+ * serial one, so such runs match the reference to rounding, not bitwise.
+ * The run runs the reference's loop structure on the ghost-layer grid
+ * (periodic ghosts for bc 0); the diagnostics, surrogate, batches and
+ * checkpoints are not offered for it. */
+int mass_lyo_set_growth(mass_lyo_t h, const mass_lyo_growth* g);
 
 /* ------------------------------------------------------------------ batches
  * Many independent runs (own size and parameters; shared npc and precision)

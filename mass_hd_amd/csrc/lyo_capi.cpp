@@ -218,9 +218,20 @@ int mass_lyo_configure(mass_lyo_t h, const mass_lyo_init* init, double* ftot, do
   hi.bc = h->prm.bc;
   std::vector<double> ff, qxx, qyx, phi;
   std::string msg;
-  if (lyo::configure_host(hi, ff, qxx, qyx, phi, ftot, ptot, msg) != 0)
+  double ft = 0, pt = 0;
+  if (lyo::configure_host(hi, h->gen, ff, qxx, qyx, phi, &ft, &pt, msg) != 0)
     return fail(h, MASS_LYO_EINVAL, msg);
-  return upload_state(h, ff.data(), qxx.data(), qyx.data(), phi.data());
+  const int rc = upload_state(h, ff.data(), qxx.data(), qyx.data(), phi.data());
+  if (rc == MASS_LYO_OK && h->growth) {
+    // GoOrGrow::Configure (go-or-grow.cpp:34-40)
+    const hipError_t e = h->eng->growth_configure(&ft, &pt, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "go-or-grow configure");
+  }
+  if (rc == MASS_LYO_OK) {
+    if (ftot) *ftot = ft;
+    if (ptot) *ptot = pt;
+  }
+  return rc;
 }
 
 // One step in surrogate mode: a judged plain step every judge_every steps
@@ -302,7 +313,8 @@ int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_ev
   if (!net || !lyo::net_ready(net))
     return fail(h, MASS_LYO_EINVAL, "the network is not trained (no normalisation yet)");
   if (h->prm.npc < 2) return fail(h, MASS_LYO_EINVAL, "the surrogate replaces correctors 2..npc: npc must be >= 2");
-  if (h->prm.bc != 0) return fail(h, MASS_LYO_EINVAL, "the surrogate needs a periodic grid (bc 0)");
+  if (h->prm.bc != 0 || h->growth)
+    return fail(h, MASS_LYO_EINVAL, "the surrogate needs a plain periodic run (bc 0)");
   if (h->prm.LY % 16) return fail(h, MASS_LYO_EINVAL, "the surrogate needs LY to be a multiple of 16");
   if (judge_every < 1 || !(tol > 0)) return fail(h, MASS_LYO_EINVAL, "judge_every >= 1 and tol > 0");
   auto& S = h->sur;
@@ -343,6 +355,8 @@ int mass_lyo_get_field(mass_lyo_t h, int32_t field, double* out) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (!out) return fail(h, MASS_LYO_EINVAL, "null output");
   if (field < 0 || field >= MASS_LYO_NUM_FIELDS) return fail(h, MASS_LYO_EINVAL, "unknown field id");
+  if (field >= MASS_LYO_M && !h->growth)
+    return fail(h, MASS_LYO_EINVAL, "m, chi and the masks belong to go-or-grow runs");
   const size_t n = (field == MASS_LYO_FF ? 9 : 1) * (size_t)h->N;
   hipError_t e = h->eng->field(field, h->d_stage, h->stream);
   if (e == hipSuccess)
@@ -370,8 +384,9 @@ int mass_lyo_sums(mass_lyo_t h, double* fsum, double* phisum) {
 int mass_lyo_diagnostics(mass_lyo_t h, mass_lyo_diag* out, double* charge) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (!out) return fail(h, MASS_LYO_EINVAL, "null output");
-  if (h->prm.bc != 0)
-    return fail(h, MASS_LYO_EINVAL, "the diagnostics (defect winding numbers) need a periodic grid");
+  if (h->prm.bc != 0 || h->growth)
+    return fail(h, MASS_LYO_EINVAL,
+                "the diagnostics (defect winding numbers) need a plain periodic run");
   hipError_t e = hipSuccess;
   lyo::DiagScratch& d = h->diag;
   const size_t N = (size_t)h->N;
@@ -526,6 +541,7 @@ constexpr char kCheckpointMagic[8] = {'M', 'A', 'S', 'S', 'L', 'Y', 'O', '1'};
 
 int mass_lyo_checkpoint_size(mass_lyo_t h, uint64_t* bytes) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (h->growth) return fail(h, MASS_LYO_EINVAL, "go-or-grow runs have no checkpoints");
   if (!bytes) return fail(h, MASS_LYO_EINVAL, "null size pointer");
   *bytes = sizeof(CheckpointHeader) + h->eng->state_bytes();
   return MASS_LYO_OK;
@@ -533,6 +549,7 @@ int mass_lyo_checkpoint_size(mass_lyo_t h, uint64_t* bytes) {
 
 int mass_lyo_checkpoint_save(mass_lyo_t h, void* buf, uint64_t bytes) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (h->growth) return fail(h, MASS_LYO_EINVAL, "go-or-grow runs have no checkpoints");
   if (!buf) return fail(h, MASS_LYO_EINVAL, "null buffer");
   if (!h->has_state) return fail(h, MASS_LYO_EINVAL, "no state yet: configure or set_state first");
   const uint64_t need = sizeof(CheckpointHeader) + h->eng->state_bytes();
@@ -556,6 +573,7 @@ int mass_lyo_checkpoint_save(mass_lyo_t h, void* buf, uint64_t bytes) {
 
 int mass_lyo_checkpoint_load(mass_lyo_t h, const void* buf, uint64_t bytes) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (h->growth) return fail(h, MASS_LYO_EINVAL, "go-or-grow runs have no checkpoints");
   if (!buf) return fail(h, MASS_LYO_EINVAL, "null buffer");
   CheckpointHeader hd;
   if (bytes < sizeof(hd)) return fail(h, MASS_LYO_EINVAL, "not a checkpoint (too short)");
@@ -582,13 +600,47 @@ int mass_lyo_checkpoint_load(mass_lyo_t h, const void* buf, uint64_t bytes) {
   return MASS_LYO_OK;
 }
 
+int mass_lyo_set_growth(mass_lyo_t h, const mass_lyo_growth* g) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (!g) return fail(h, MASS_LYO_EINVAL, "null growth parameters");
+  if (h->prm.precision != MASS_LYO_FP64)
+    return fail(h, MASS_LYO_EINVAL, "go-or-grow runs in FP64 only");
+  if (h->sur.net) return fail(h, MASS_LYO_EINVAL, "disable the surrogate first");
+  // go-or-grow.cpp:21-26
+  if (g->chi_noise < 0) return fail(h, MASS_LYO_EINVAL, "chi-noise must be non-negative.");
+  if (g->chi_length < 0) return fail(h, MASS_LYO_EINVAL, "chi-length must be non-negative.");
+  if (g->relax_dt < 0) return fail(h, MASS_LYO_EINVAL, "relax-dt must be non-negative.");
+  if (g->chi_config != 0 && g->chi_config != 1)
+    return fail(h, MASS_LYO_EINVAL, "chi_config must be 0 (noise) or 1 (front)");
+  // the counters take these as unsigned (lyotropic-with-division.cpp:41,79)
+  if (!(g->division_time >= 0) || !(g->death_time >= 0) || g->division_time > 4294967295. ||
+      g->death_time > 4294967295.)
+    return fail(h, MASS_LYO_EINVAL, "division-time and death-time must be in [0, 2^32)");
+  if (!h->growth && h->prm.bc == 0) {
+    // the model runs on the ghost-layer engine, with periodic ghosts here
+    std::unique_ptr<lyo::Engine> eng(lyo::make_engine_walls(0));
+    hipError_t e = eng->alloc(h->prm.LX, h->prm.LY, h->NP, h->prm.npc);
+    if (e != hipSuccess) return hip_fail(h, e, "go-or-grow buffers");
+    eng->set_params(param_block(h->prm));
+    h->eng->release();
+    h->eng = std::move(eng);
+  }
+  const hipError_t e = h->eng->set_growth(*g, &h->gen);
+  if (e != hipSuccess) return hip_fail(h, e, "go-or-grow buffers");
+  h->growth = true;
+  h->has_state = false;
+  h->launch.loops = 1;
+  return MASS_LYO_OK;
+}
+
 int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (on != 0 && on != 1) return fail(h, MASS_LYO_EINVAL, "on must be 0 or 1");
   if (on && h->prm.precision != MASS_LYO_FP64)
     return fail(h, MASS_LYO_EINVAL, "the loop kernels run the FP64 path only");
-  if (!on && h->prm.bc != 0)
-    return fail(h, MASS_LYO_EINVAL, "walled grids (bc 1-4) run the loop kernels only");
+  if (!on && (h->prm.bc != 0 || h->growth))
+    return fail(h, MASS_LYO_EINVAL,
+                "walled grids (bc 1-4) and go-or-grow run the loop kernels only");
   if (on) {
     const hipError_t e = h->eng->alloc_loops();
     if (e != hipSuccess) return hip_fail(h, e, "loop kernel buffers");

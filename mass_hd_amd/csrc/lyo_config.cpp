@@ -24,15 +24,15 @@ unsigned uwrap(unsigned x, unsigned L) {                                   // to
 }
 }  // namespace
 
-int configure_host(const HostInit& in, std::vector<double>& ff, std::vector<double>& qxx,
-                   std::vector<double>& qyx, std::vector<double>& phi, double* ftot,
-                   double* ptot, std::string& err) {
+int configure_host(const HostInit& in, std::mt19937& gen, std::vector<double>& ff,
+                   std::vector<double>& qxx, std::vector<double>& qyx, std::vector<double>& phi,
+                   double* ftot, double* ptot, std::string& err) {
   const unsigned LX = in.LX, LY = in.LY, N = LX * LY;
   ff.assign(9 * (size_t)N, 0.0);
   qxx.assign(N, 0.0);
   qyx.assign(N, 0.0);
   phi.assign(N, 0.0);
-  std::mt19937 gen(in.seed);
+  gen.seed(in.seed);
   auto random_real = [&](double lo, double hi) {
     return std::uniform_real_distribution<>(lo, hi)(gen);
   };

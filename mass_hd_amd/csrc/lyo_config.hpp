@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -17,8 +18,10 @@ struct HostInit {
 };
 
 // Fills ff (N*9 AoS), qxx, qyx, phi; returns the check totals ftot, ptot.
-int configure_host(const HostInit& in, std::vector<double>& ff, std::vector<double>& qxx,
-                   std::vector<double>& qyx, std::vector<double>& phi, double* ftot,
-                   double* ptot, std::string& err);
+// gen is seeded with in.seed and left where the draws end (go-or-grow draws
+// on from there).
+int configure_host(const HostInit& in, std::mt19937& gen, std::vector<double>& ff,
+                   std::vector<double>& qxx, std::vector<double>& qyx, std::vector<double>& phi,
+                   double* ftot, double* ptot, std::string& err);
 
 }  // namespace lyo

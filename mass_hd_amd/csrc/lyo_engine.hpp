@@ -8,8 +8,10 @@
 //   C[2]  intermediate corrector outputs (npc >= 2) -- ping-pong
 #pragma once
 
+#include <random>
 #include <vector>
 
+#include "../../include/mass_lyo.h"
 #include "lyo_aux.hpp"
 #include "lyo_diag.hpp"
 #include "lyo_full.hpp"
@@ -634,6 +636,16 @@ class Engine {
   virtual size_t state_bytes() const = 0;
   virtual hipError_t save_state(void* host, hipStream_t s) = 0;
   virtual hipError_t load_state(const void* host, hipStream_t s) = 0;
+  // go-or-grow (lyo_growth.hpp; the ghost-layer engine only): the model's
+  // parameters and the run's generator (before configure), and Configure's
+  // relaxation, reset and phenotype set-up after the initial state is packed
+  // (ptot: in, Lyotropic's; out, the relaxed one)
+  virtual hipError_t set_growth(const mass_lyo_growth&, std::mt19937*) {
+    return hipErrorNotSupported;
+  }
+  virtual hipError_t growth_configure(double*, double*, hipStream_t) {
+    return hipErrorNotSupported;
+  }
 };
 
 template <class T, bool EXACT>

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -24,6 +25,8 @@ struct mass_lyo_s {
   lyo::DiagScratch diag;
   std::vector<void*> diag_allocs;
   bool has_state = false;     // configured or set_state called
+  std::mt19937 gen;           // the run's generator (src/random.cpp:12), seeded by configure
+  bool growth = false;        // a go-or-grow run (mass_lyo_set_growth)
   // surrogate corrector
   struct Surrogate {
     mass_lyo_net_t net = nullptr;

@@ -508,7 +508,8 @@ int mass_lyo_net_grad_host(mass_lyo_net_t n, const float* x, const float* y, int
 int mass_lyo_net_collect(mass_lyo_net_t n, mass_lyo_t h) {
   if (!n || !h) return nfail(n, MASS_LYO_EINVAL, "null argument");
   if (h->prm.npc < 2) return nfail(n, MASS_LYO_EINVAL, "collecting needs a run with npc >= 2");
-  if (h->prm.bc != 0) return nfail(n, MASS_LYO_EINVAL, "collecting needs a periodic run (bc 0)");
+  if (h->prm.bc != 0 || h->growth)
+    return nfail(n, MASS_LYO_EINVAL, "collecting needs a plain periodic run (bc 0)");
   if (h->prm.LY % 16) return nfail(n, MASS_LYO_EINVAL, "LY must be a multiple of 16");
   if (!n->feats.empty() && (h->prm.LX != n->dLX || h->prm.LY != n->dLY))
     return nfail(n, MASS_LYO_EINVAL, "all frames of a dataset have one grid size");

@@ -24,10 +24,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <random>
+#include <utility>
 #include <vector>
 
 #include "lyo_aux.hpp"
 #include "lyo_engine.hpp"
+#include "lyo_growth.hpp"
 #include "lyo_loops.hpp"
 
 namespace lyo {
@@ -55,7 +59,8 @@ struct WallGeo {
   int LX, LY;
   int W;   // LY + 2: stride of X
   int NP;  // plane stride (>= (LX+2)*(LY+2))
-  int bc;  // 1 free-slip channel, 2 no-slip channel, 3 free-slip box, 4 no-slip box
+  int bc;  // 1 free-slip channel, 2 no-slip channel, 3 free-slip box, 4 no-slip box;
+           // 0: periodic ghosts (go-or-grow on a periodic grid)
 };
 
 __device__ __forceinline__ int wopp(int d) { return kRefl[0][d]; }
@@ -99,6 +104,16 @@ __device__ __forceinline__ bool wall_ghost(const WallGeo& g, int phase, int i, i
   return true;
 }
 
+// The domain node a ghost node is the periodic image of.  With periodic
+// ghosts every condition copies it (ApplyPBC, fields.hpp:619-631,749-760);
+// the domain then sees exactly the periodic grid's neighbours and streams.
+__device__ __forceinline__ int periodic_image(const WallGeo& g, int p) {
+  int X = p / g.W, Y = p - X * g.W;
+  X = X == 0 ? g.LX : (X == g.LX + 1 ? 1 : X);
+  Y = Y == 0 ? g.LY : (Y == g.LY + 1 ? 1 : Y);
+  return wnode(g, X, Y);
+}
+
 // channel corners: the closed wall next to corner c (the other wall is
 // periodic; fields.hpp:844-885)
 __device__ __forceinline__ int channel_wall(int c) {
@@ -111,6 +126,10 @@ __device__ __forceinline__ int channel_wall(int c) {
 template <class T>
 __device__ __forceinline__ void scalar_ghost(const WallGeo& g, T* a, int kind, int phase, int p,
                                              int id) {
+  if (g.bc == 0) {
+    a[p] = a[periodic_image(g, p)];
+    return;
+  }
   const bool channel = g.bc <= 2;
   if (phase == 0 && channel) {  // ApplyPBC(Left), ApplyPBC(Right)
     const int Y = p % g.W;
@@ -143,6 +162,12 @@ __device__ __forceinline__ bool incoming(int v, int w) {
 template <class T>
 __device__ __forceinline__ void lb_ghost(const WallGeo& g, T* f, int phase, int p, int id) {
   const int NP = g.NP;
+  if (g.bc == 0) {
+    const int src = periodic_image(g, p);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) f[v * NP + p] = f[v * NP + src];
+    return;
+  }
   const bool channel = g.bc <= 2, slip = g.bc == 1 || g.bc == 3;
   if (phase == 0 && channel) {
     const int Y = p % g.W;
@@ -207,6 +232,16 @@ __global__ void __launch_bounds__(256) wall_fields2_kernel(WallGeo g, int phase,
   scalar_ghost(g, pl(LP_SXY), 0, phase, p, id);
 }
 
+// Neumann on up to two more scalar planes: m and chi (go-or-grow.cpp:
+// 365-385), or MU of the dry relaxation (:70-83).
+template <class T>
+__global__ void __launch_bounds__(256) wall_scalars_kernel(WallGeo g, int phase, T* a, T* b) {
+  int p, id;
+  if (!wall_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id)) return;
+  scalar_ghost(g, a, 0, phase, p, id);
+  if (b) scalar_ghost(g, b, 0, phase, p, id);
+}
+
 // BoundaryConditionsLB (lyotropic.cpp:388-420) on ff and fn.
 template <class T>
 __global__ void __launch_bounds__(256) wall_lb_kernel(WallGeo g, int phase, T* f, T* fn) {
@@ -233,6 +268,22 @@ __global__ void __launch_bounds__(256) wall_move_kernel(WallGeo g, const T* __re
     f_to[v * g.NP + p] = f[v * g.NP + src];
     fn_to[v * g.NP + p] = fn[v * g.NP + src];
   }
+}
+
+// Host FP64 domain array -> the domain of a torus plane.
+static __global__ void wall_scatter_kernel(WallGeo g, const double* __restrict__ in,
+                                           double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.LX * g.LY) return;
+  out[LayerGrid::at(t, g.LX, g.LY).k] = in[t];
+}
+
+// Domain of a 0/1 byte mask -> FP64.
+static __global__ void wall_unpack_mask_kernel(WallGeo g, const unsigned char* __restrict__ mask,
+                                               double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.LX * g.LY) return;
+  out[t] = mask[LayerGrid::at(t, g.LX, g.LY).k] ? 1.0 : 0.0;
 }
 
 // Host FP64 AoS domain state -> torus planes (ghost nodes stay as they are).
@@ -339,9 +390,18 @@ class WallEngine final : public Engine {
   void release() override {
     for (T* p : ptrs_) (void)hipFree(p);
     ptrs_.clear();
+    sizes_.clear();
     bytes_ = 0;
     F_[0] = F_[1] = Fn_[0] = Fn_[1] = phi_[0] = phi_[1] = nullptr;
     qxx_ = qyx_ = qnxx_ = qnyx_ = phn_ = D_ = nullptr;
+    m_[0] = m_[1] = mn_ = chi_ = part_ = stage_ = fe_ = nullptr;
+    if (div_) (void)hipFree(div_);
+    if (death_) (void)hipFree(death_);
+    if (cent_) (void)hipFree(cent_);
+    div_ = death_ = nullptr;
+    cent_ = nullptr;
+    cent_cap_ = 0;
+    growth_ = false;
   }
 
   void set_params(const double* v) override {
@@ -361,66 +421,32 @@ class WallEngine final : public Engine {
     P_.conc = v[13];
   }
 
-  // Lyotropic::Step (lyotropic.cpp:500-525)
+  // Lyotropic::Step (lyotropic.cpp:500-525); go-or-grow redraws its masks
+  // first (lyotropic-with-division.cpp:150-156)
   hipError_t steps(int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) override {
     (void)L;
     (void)tm;
-    const int N = LX_ * LY_, bs = 256, nb = (N + bs - 1) / bs;
-    const int nbt = (TP_ + bs - 1) / bs;
+    const int bs = 256, nbt = (TP_ + bs - 1) / bs;
     for (int st = 0; st < nsteps; ++st) {
-      T* f = F_[cur_];
-      T* fn = Fn_[cur_];
+      if (growth_) {
+        const hipError_t e = set_masks(s);
+        if (e != hipSuccess) return e;
+      }
       fields_bc(s);
-      hipLaunchKernelGGL((loop_uq_kernel<T, true, LayerGrid>), dim3(nb), dim3(bs), 0, s, f, qxx_,
-                         qyx_, phi_[pc_], P_, LX_, LY_, NPw_, D_);
+      update_quantities(s);
       fields2_bc(s);
-      LoopUF<T> u{};
-      u.D = D_;
-      u.f = f;
-      u.qxx = qxx_;
-      u.qyx = qyx_;
-      u.phi = phi_[pc_];
-      u.f_out = f;  // in place, as UpdateFieldsAtNode writes ff[k]
-      u.fn_out = fn;
-      u.qxx_out = qxx_;
-      u.qyx_out = qyx_;
-      u.qnxx_out = qnxx_;
-      u.qnyx_out = qnyx_;
-      u.phi_out = phi_[pc_ ^ 1];
-      u.phn_out = phn_;
-      hipLaunchKernelGGL((loop_uf_kernel<T, true, true, LayerGrid>), dim3(nb), dim3(bs), 0, s, u,
-                         P_, LX_, LY_, NPw_);
-      pc_ ^= 1;
+      update_fields(true, s);
       for (int ph = 0; ph < 3; ++ph)
         hipLaunchKernelGGL((wall_lb_kernel<T>), dim3(phase_blocks(ph)), dim3(bs), 0, s, geo(),
-                           ph, f, fn);
-      hipLaunchKernelGGL((wall_move_kernel<T>), dim3(nbt), dim3(bs), 0, s, geo(), (const T*)f,
-                         (const T*)fn, F_[cur_ ^ 1], Fn_[cur_ ^ 1]);
+                           ph, F_[cur_], Fn_[cur_]);
+      hipLaunchKernelGGL((wall_move_kernel<T>), dim3(nbt), dim3(bs), 0, s, geo(),
+                         (const T*)F_[cur_], (const T*)Fn_[cur_], F_[cur_ ^ 1], Fn_[cur_ ^ 1]);
       cur_ ^= 1;
-      f = F_[cur_];
-      fn = Fn_[cur_];
       for (int c = 1; c <= npc_; ++c) {
         fields_bc(s);
-        hipLaunchKernelGGL((loop_uq_kernel<T, true, LayerGrid>), dim3(nb), dim3(bs), 0, s, f,
-                           qxx_, qyx_, phi_[pc_], P_, LX_, LY_, NPw_, D_);
+        update_quantities(s);
         fields2_bc(s);
-        LoopUF<T> w{};
-        w.D = D_;
-        w.f = f;
-        w.qxx = qxx_;
-        w.qyx = qyx_;
-        w.phi = phi_[pc_];
-        w.fn = fn;
-        w.qnxx = qnxx_;
-        w.qnyx = qnyx_;
-        w.phn = phn_;
-        w.f_out = f;
-        w.qxx_out = qxx_;
-        w.qyx_out = qyx_;
-        w.phi_out = phi_[pc_ ^ 1];
-        hipLaunchKernelGGL((loop_uf_kernel<T, true, false, LayerGrid>), dim3(nb), dim3(bs), 0, s,
-                           w, P_, LX_, LY_, NPw_);
-        pc_ ^= 1;
+        update_fields(false, s);
       }
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
@@ -436,6 +462,7 @@ class WallEngine final : public Engine {
     if (e != hipSuccess) return e;
     cur_ = 0;
     pc_ = 0;
+    mc_ = 0;
     const int N = LX_ * LY_, bs = 256;
     hipLaunchKernelGGL((wall_pack_kernel<T>), dim3((N + bs - 1) / bs), dim3(bs), 0, s, geo(), d_ff,
                        d_qxx, d_qyx, d_phi, F_[0], qxx_, qyx_, phi_[0]);
@@ -469,8 +496,20 @@ class WallEngine final : public Engine {
       case D_MU: a = plane(LP_MU); break;
       case D_HXX: a = plane(LP_HXX); break;
       case D_HYX: a = plane(LP_HYX); break;
+      case MASS_LYO_M: a = growth_ ? m_[mc_] : nullptr; break;
+      case MASS_LYO_CHI: a = chi_; break;
+      case MASS_LYO_DIVISION_MASK:
+      case MASS_LYO_DEATH_MASK: {
+        if (!growth_) return hipErrorInvalidValue;
+        const int N = LX_ * LY_;
+        hipLaunchKernelGGL(wall_unpack_mask_kernel, dim3((N + 255) / 256), dim3(256), 0, s, geo(),
+                           (const unsigned char*)(id == MASS_LYO_DIVISION_MASK ? div_ : death_),
+                           d_out);
+        return hipGetLastError();
+      }
       default: return hipErrorInvalidValue;
     }
+    if (!a) return hipErrorInvalidValue;
     const int N = LX_ * LY_, bs = 256;
     hipLaunchKernelGGL((wall_unpack_kernel<T>), dim3((N + bs - 1) / bs), dim3(bs), 0, s, geo(), a,
                        np, d_out);
@@ -540,10 +579,103 @@ class WallEngine final : public Engine {
     const int n = phase == 0 ? 2 * LY_ : (phase == 1 ? 2 * LX_ : 4);
     return (unsigned)((n + 255) / 256);
   }
-  void fields_bc(hipStream_t s) {
-    for (int ph = 0; ph < 3; ++ph)
+  // BoundaryConditionsFields (lyotropic.cpp:422-449; go-or-grow.cpp:365-385
+  // adds m and chi)
+  void fields_bc(hipStream_t s, bool with_growth = true) {
+    for (int ph = 0; ph < 3; ++ph) {
       hipLaunchKernelGGL((wall_fields_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s, geo(),
                          ph, qxx_, qyx_, phi_[pc_]);
+      if (growth_ && with_growth)
+        hipLaunchKernelGGL((wall_scalars_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s,
+                           geo(), ph, m_[mc_], chi_);
+    }
+  }
+  unsigned node_blocks() const { return (unsigned)((LX_ * LY_ + 255) / 256); }
+  // UpdatePhenotypeQuantities (go-or-grow.cpp:201-209)
+  void phenotype(hipStream_t s) {
+    hipLaunchKernelGGL((phenotype_kernel<LayerGrid>), dim3(node_blocks()), dim3(256), 0, s,
+                       (const T*)m_[mc_], (const T*)phi_[pc_], chi_, LX_, LY_);
+    fields_bc(s);
+  }
+  // UpdateQuantities (lyotropic.cpp:343-360; go-or-grow.cpp:305-320)
+  void update_quantities(hipStream_t s) {
+    const T* f = F_[cur_];
+    if (!growth_) {
+      hipLaunchKernelGGL((loop_uq_kernel<T, true, LayerGrid>), dim3(node_blocks()), dim3(256), 0,
+                         s, f, qxx_, qyx_, phi_[pc_], P_, LX_, LY_, NPw_, D_);
+      return;
+    }
+    hipLaunchKernelGGL((growth_uq_kernel<LayerGrid>), dim3(node_blocks()), dim3(256), 0, s, f,
+                       (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], (const T*)chi_, P_,
+                       gd_, LX_, LY_, NPw_, D_);
+    if (conserve_) {  // countphi: the sum of phi this UpdateQuantities sees
+      const int N = LX_ * LY_;
+      int nparts = (N + 4095) / 4096;
+      if (nparts > kGrowthParts) nparts = kGrowthParts;
+      const int per = (N + nparts - 1) / nparts;
+      hipLaunchKernelGGL((wall_partial_sums_kernel<T>), dim3(nparts), dim3(256), 0, s, geo(),
+                         (const T*)F_[cur_], (const T*)phi_[pc_], per, part_,
+                         part_ + kGrowthParts);
+      hipLaunchKernelGGL(final_sums_kernel, dim3(1), dim3(256), 0, s, part_,
+                         part_ + kGrowthParts, nparts, 0.0, part_ + 2 * kGrowthParts);
+    }
+    phenotype(s);
+  }
+  // UpdateFields (lyotropic.cpp:362-369; go-or-grow.cpp:322-362)
+  void update_fields(bool first, hipStream_t s) {
+    T* f = F_[cur_];
+    LoopUF<T> u{};
+    u.D = D_;
+    u.f = f;
+    u.qxx = qxx_;
+    u.qyx = qyx_;
+    u.phi = phi_[pc_];
+    u.f_out = f;  // in place, as UpdateFieldsAtNode writes ff[k]
+    u.qxx_out = qxx_;
+    u.qyx_out = qyx_;
+    u.phi_out = phi_[pc_ ^ 1];
+    if (first) {
+      u.fn_out = Fn_[cur_];
+      u.qnxx_out = qnxx_;
+      u.qnyx_out = qnyx_;
+      u.phn_out = phn_;
+    } else {
+      u.fn = Fn_[cur_];
+      u.qnxx = qnxx_;
+      u.qnyx = qnyx_;
+      u.phn = phn_;
+    }
+    if (!growth_) {
+      if (first)
+        hipLaunchKernelGGL((loop_uf_kernel<T, true, true, LayerGrid>), dim3(node_blocks()),
+                           dim3(256), 0, s, u, P_, LX_, LY_, NPw_);
+      else
+        hipLaunchKernelGGL((loop_uf_kernel<T, true, false, LayerGrid>), dim3(node_blocks()),
+                           dim3(256), 0, s, u, P_, LX_, LY_, NPw_);
+      pc_ ^= 1;
+      return;
+    }
+    GrowthUF a{};
+    a.l = u;
+    a.m = m_[mc_];
+    a.chi = chi_;
+    a.mn = mn_;
+    a.division = div_;
+    a.death = death_;
+    a.m_out = m_[mc_ ^ 1];
+    a.mn_out = mn_;
+    a.countphi = part_ + 2 * kGrowthParts + 1;
+    a.totalphi = totalphi_;
+    a.conserve = conserve_ ? 1 : 0;
+    if (first)
+      hipLaunchKernelGGL((growth_uf_kernel<true, LayerGrid>), dim3(node_blocks()), dim3(256), 0,
+                         s, a, P_, gd_, LX_, LY_, NPw_);
+    else
+      hipLaunchKernelGGL((growth_uf_kernel<false, LayerGrid>), dim3(node_blocks()), dim3(256),
+                         0, s, a, P_, gd_, LX_, LY_, NPw_);
+    pc_ ^= 1;
+    mc_ ^= 1;
+    phenotype(s);
   }
   void fields2_bc(hipStream_t s) {
     for (int ph = 0; ph < 3; ++ph)
@@ -555,7 +687,10 @@ class WallEngine final : public Engine {
       const hipError_t e = hipMemsetAsync(ptrs_[i], 0, sizes_[i] * sizeof(T), s);
       if (e != hipSuccess) return e;
     }
-    return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (div_) e = hipMemsetAsync(div_, 0, (size_t)TP_, s);
+    if (e == hipSuccess && death_) e = hipMemsetAsync(death_, 0, (size_t)TP_, s);
+    return e;
   }
   // to host (out) or from host (in), through a device staging copy
   hipError_t copy_state(void* out, const void* in, hipStream_t s) {
@@ -593,6 +728,253 @@ class WallEngine final : public Engine {
     }
     return e;
   }
+
+  // ---------------------------------------------------------- go-or-grow
+  static constexpr int kGrowthParts = 1024;
+  // domain of a torus plane -> host (synchronous), host -> domain
+  hipError_t download(const T* plane, std::vector<double>& out, hipStream_t s) {
+    const int N = LX_ * LY_;
+    out.resize(N);
+    hipLaunchKernelGGL((wall_unpack_kernel<T>), dim3((N + 255) / 256), dim3(256), 0, s, geo(),
+                       plane, 1, stage_);
+    hipError_t e = hipMemcpyAsync(out.data(), stage_, sizeof(double) * N, hipMemcpyDeviceToHost,
+                                  s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+  }
+  hipError_t upload(const std::vector<double>& in, T* plane, hipStream_t s) {
+    const int N = LX_ * LY_;
+    hipError_t e = hipMemcpyAsync(stage_, in.data(), sizeof(double) * N, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(wall_scatter_kernel, dim3((N + 255) / 256), dim3(256), 0, s, geo(),
+                       (const double*)stage_, plane);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // `in` may go away
+    return e;
+  }
+  // One mask of LyotropicWithDivision::SetMasks (lyotropic-with-division.cpp:
+  // 34-108): due when its counter was 0; the centres are drawn on the host
+  // from phi in node order, the disks marked on the device.
+  hipError_t redraw(unsigned char* mask, const std::vector<double>& phi, double rate,
+                    double radius, bool division, hipStream_t s) {
+    std::vector<int> centres;
+    for (int k = 0; k < LX_ * LY_; ++k) {
+      const double p = phi[k];
+      if (p >= .5) {
+        const double adjusted =
+            division ? rate * p * (1 - p / gp_.phi_critical) : rate * p;
+        if (std::uniform_real_distribution<>(0., 1.)(*gen_) < adjusted) {
+          centres.push_back(k / LY_);
+          centres.push_back(k % LY_);
+        }
+      }
+    }
+    hipError_t e = hipMemsetAsync(mask, 0, (size_t)TP_, s);
+    const int nc = (int)centres.size() / 2;
+    if (e != hipSuccess || nc == 0) return e;
+    if ((size_t)nc * 2 > cent_cap_) {
+      if (cent_) (void)hipFree(cent_);
+      cent_ = nullptr;
+      cent_cap_ = 0;
+      e = hipMalloc((void**)&cent_, sizeof(int) * 2 * (size_t)nc);
+      if (e != hipSuccess) return e;
+      cent_cap_ = 2 * (size_t)nc;
+    }
+    e = hipMemcpyAsync(cent_, centres.data(), sizeof(int) * 2 * (size_t)nc,
+                       hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    const double r2 = std::ceil(radius * radius);
+    int R = 0;
+    while ((double)(R + 1) * (double)(R + 1) <= r2) ++R;
+    const long long span = 2 * R + 1, threads = (long long)nc * span * span;
+    hipLaunchKernelGGL(mask_disk_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       s, (const int*)cent_, nc, R, r2, LX_, LY_, mask);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // `centres` goes away
+    return e;
+  }
+  hipError_t set_masks(hipStream_t s) {
+    const uint32_t div_old = div_count_--;
+    const uint32_t death_old = death_count_--;
+    if (div_old != 0 && death_old != 0) return hipSuccess;
+    std::vector<double> phi;
+    hipError_t e = download(phi_[pc_], phi, s);
+    if (e == hipSuccess && div_old == 0) {
+      div_count_ = (uint32_t)gp_.division_time;
+      e = redraw(div_, phi, gp_.division_rate, gp_.division_radius, true, s);
+    }
+    if (e == hipSuccess && death_old == 0) {
+      death_count_ = (uint32_t)gp_.death_time;
+      e = redraw(death_, phi, gp_.death_rate, gp_.death_radius, false, s);
+    }
+    return e;
+  }
+
+ public:
+  hipError_t set_growth(const mass_lyo_growth& g, std::mt19937* gen) override {
+    const size_t n1 = (size_t)NPw_;
+    hipError_t e = hipSuccess;
+    if (!growth_) {
+      for (int i = 0; i < 2 && e == hipSuccess; ++i) e = get_(n1, &m_[i]);
+      if (e == hipSuccess) e = get_(n1, &mn_);
+      if (e == hipSuccess) e = get_(n1, &chi_);
+      if (e == hipSuccess) e = get_(2 * kGrowthParts + 2, &part_);
+      if (e == hipSuccess) e = get_((size_t)LX_ * LY_, &stage_);
+      if (e == hipSuccess) e = get_(9, &fe_);
+      if (e == hipSuccess) e = hipMalloc((void**)&div_, (size_t)TP_);
+      if (e == hipSuccess) e = hipMalloc((void**)&death_, (size_t)TP_);
+      if (e == hipSuccess) e = hipMemset(div_, 0, (size_t)TP_);
+      if (e == hipSuccess) e = hipMemset(death_, 0, (size_t)TP_);
+      if (e != hipSuccess) return e;
+    }
+    growth_ = true;
+    gp_ = g;
+    gen_ = gen;
+    gd_.B = g.B;
+    gd_.Snem = g.Snem;
+    gd_.Dchi = g.Dchi;
+    gd_.Achi = g.Achi;
+    gd_.Ochi = g.Ochi;
+    gd_.phiswitch = g.phiswitch;
+    gd_.alpha = g.alpha;
+    gd_.beta = g.beta;
+    gd_.phi_critical = g.phi_critical;
+    gd_.growTogether = g.growTogether;
+    // lyotropic-with-division.cpp:23-31
+    div_count_ = death_count_ = 0u;
+    conserve_ = (g.alpha == 0 || g.division_rate == 0) && (g.beta == 0 || g.death_rate == 0);
+    return hipSuccess;
+  }
+
+  // GoOrGrow::Configure after Lyotropic::Configure (go-or-grow.cpp:34-40)
+  hipError_t growth_configure(double* ftot, double* ptot, hipStream_t s) override {
+    if (!growth_) return hipErrorNotSupported;
+    const int N = LX_ * LY_;
+    const unsigned nb = node_blocks();
+    hipError_t e = hipSuccess;
+    totalphi_ = *ptot;
+    // RelaxFreeEnergy (:42-119)
+    if (gp_.relax_steps != 0 && (gp_.relax_phi || gp_.relax_Q)) {
+      T* hxx = D_ + (size_t)LP_HXX * NPw_;
+      T* hyx = D_ + (size_t)LP_HYX * NPw_;
+      T* mu = D_ + (size_t)LP_MU * NPw_;
+      for (uint32_t st = 0; st < gp_.relax_steps; ++st) {
+        fields_bc(s, false);
+        hipLaunchKernelGGL((relax_mu_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
+                           (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], P_, gd_, LX_, LY_,
+                           hxx, hyx, mu);
+        for (int ph = 0; ph < 3; ++ph)
+          hipLaunchKernelGGL((wall_scalars_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s,
+                             geo(), ph, mu, (T*)nullptr);
+        hipLaunchKernelGGL((relax_update_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
+                           (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], (const T*)hxx,
+                           (const T*)hyx, (const T*)mu, P_, gp_.relax_dt, gp_.relax_phi,
+                           gp_.relax_Q, LX_, LY_, phi_[pc_ ^ 1], qnxx_, qnyx_);
+        pc_ ^= 1;
+        std::swap(qxx_, qnxx_);
+        std::swap(qyx_, qnyx_);
+      }
+      fields_bc(s, false);
+      std::vector<double> phi;
+      e = download(phi_[pc_], phi, s);
+      if (e != hipSuccess) return e;
+      double relaxed_phi = 0.;
+      for (int k = 0; k < N; ++k) relaxed_phi += phi[k];
+      totalphi_ = relaxed_phi;
+      *ptot = relaxed_phi;
+    }
+    // ResetHydrodynamics (:121-141): fe(0, 0, rho) of src/lb.hpp:24-41
+    {
+      const double nn = P_.rho, vx = 0., vy = 0.;
+      const double v2 = vx * vx + vy * vy;
+      double fe[9];
+      fe[1] = nn * (1. / 9. + 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
+      fe[2] = nn * (1. / 9. - 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
+      fe[3] = nn * (1. / 9. + 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
+      fe[4] = nn * (1. / 9. - 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
+      fe[5] = nn * (1. / 36. + 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
+      fe[6] = nn * (1. / 36. - 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
+      fe[7] = nn * (1. / 36. - 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
+      fe[8] = nn * (1. / 36. + 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
+      fe[0] = nn - fe[1] - fe[2] - fe[3] - fe[4] - fe[5] - fe[6] - fe[7] - fe[8];
+      e = hipMemcpy(fe_, fe, sizeof(fe), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((reset_hydro_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s, F_[0], F_[1],
+                         Fn_[0], Fn_[1], D_, P_.rho, (const T*)fe_, LX_, LY_, NPw_);
+      double ftotal = 0.;
+      for (int k = 0; k < N; ++k)
+        for (int v = 0; v < 9; ++v) ftotal = ftotal + fe[v];
+      *ftot = ftotal;
+    }
+    // ConfigurePhenotype (:143-199)
+    std::vector<double> chi(N);
+    if (gp_.chi_config == 0) {
+      for (int k = 0; k < N; ++k) chi[k] = std::normal_distribution<>(0., 1.)(*gen_);
+      e = upload(chi, chi_, s);
+      if (e != hipSuccess) return e;
+      fields_bc(s);
+      const double smooth_rate = 0.2;
+      const unsigned smooth_steps =
+          gp_.chi_length > 0
+              ? static_cast<unsigned>(std::ceil(gp_.chi_length * gp_.chi_length / (2 * smooth_rate)))
+              : 0u;
+      for (unsigned st = 0; st < smooth_steps; ++st) {
+        T* tmp = m_[mc_ ^ 1];
+        hipLaunchKernelGGL((chi_smooth_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
+                           (const T*)chi_, tmp, LX_, LY_);
+        m_[mc_ ^ 1] = chi_;  // swap(chi, m_tmp)
+        chi_ = tmp;
+        fields_bc(s);
+      }
+      e = download(chi_, chi, s);
+      if (e != hipSuccess) return e;
+      double mean = 0.;
+      for (int k = 0; k < N; ++k) mean += chi[k];
+      mean /= N;
+      double variance = 0.;
+      for (int k = 0; k < N; ++k) {
+        chi[k] -= mean;
+        variance += chi[k] * chi[k];
+      }
+      variance /= N;
+      const double amplitude = std::sqrt(gp_.chi_noise);
+      const double norm = variance > 0 ? amplitude / std::sqrt(variance) : 0.;
+      for (int k = 0; k < N; ++k) chi[k] = gp_.chi0 + norm * chi[k];
+    } else {
+      const double length = gp_.chi_length > 0 ? gp_.chi_length : 1.;
+      const double center = .5 * (unsigned)(LX_ - 1);
+      for (int k = 0; k < N; ++k) {
+        const double x = (unsigned)k / (unsigned)LY_;
+        chi[k] = .5 * (1. - std::tanh((x - center) / length));
+      }
+    }
+    e = upload(chi, chi_, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((phenotype_init_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
+                       (const T*)phi_[pc_], (const T*)chi_, m_[mc_], LX_, LY_);
+    phenotype(s);
+    fields_bc(s);
+    return hipGetLastError();
+  }
+
+ private:
+  bool growth_ = false, conserve_ = false;
+  mass_lyo_growth gp_{};
+  GrowthDev gd_{};
+  std::mt19937* gen_ = nullptr;
+  uint32_t div_count_ = 0, death_count_ = 0;
+  double totalphi_ = 0;
+  int mc_ = 0;  // m parity (UpdateFields swaps m with m_tmp)
+  T* m_[2] = {};
+  T* mn_ = nullptr;
+  T* chi_ = nullptr;
+  T* part_ = nullptr;   // fixed-tree sum partials; [2*parts+1]: sum of phi
+  T* stage_ = nullptr;  // N doubles: host <-> domain staging
+  T* fe_ = nullptr;
+  unsigned char* div_ = nullptr;
+  unsigned char* death_ = nullptr;
+  int* cent_ = nullptr;
+  size_t cent_cap_ = 0;
 
   int bc_;
   Params<T> P_{};

@@ -30,7 +30,23 @@ FIELDS = {
     "sigma_bulk": 11, "sigma_elastic_xx": 12, "sigma_elastic_yx": 13,
     "sigma_phase_field_xx": 14, "sigma_phase_field_yx": 15,
     "sigma_active_xx": 16, "sigma_active_yx": 17, "MU": 18, "HHxx": 19, "HHyx": 20,
+    "m": 21, "chi": 22, "division_mask": 23, "death_mask": 24,
 }
+
+# go-or-grow parameters (mass_lyo_growth; go-or-grow.hpp:10-28 defaults)
+GROWTH_DOUBLES = ["B", "Snem", "Dchi", "Achi", "Ochi", "phiswitch", "chi0", "chi_noise",
+                  "chi_length", "relax_dt", "alpha", "beta", "division_rate", "division_time",
+                  "division_radius", "death_rate", "death_time", "death_radius", "phi_critical"]
+GROWTH_DEFAULTS = dict(chi_config="noise", growTogether=0, relax_phi=0, relax_Q=0, relax_steps=0,
+                       B=0.0, Snem=1.0, Dchi=0.0, Achi=0.0, Ochi=0.0, phiswitch=0.0, chi0=0.0,
+                       chi_noise=0.0, chi_length=0.0, relax_dt=1.0)
+
+
+class Growth(ctypes.Structure):
+    _fields_ = ([("chi_config", ctypes.c_int32), ("growTogether", ctypes.c_int32),
+                 ("relax_phi", ctypes.c_int32), ("relax_Q", ctypes.c_int32),
+                 ("relax_steps", ctypes.c_uint32), ("reserved", ctypes.c_int32)]
+                + [(k, ctypes.c_double) for k in GROWTH_DOUBLES])
 
 # the frame of Lyotropic::serialize_frame (src/models/lyotropic.hpp:121-138)
 FRAME_FIELDS = ["ff", "QQxx", "QQyx", "phi", "sigmaXX", "sigmaXY", "sigmaYX", "sigmaYY",
@@ -106,6 +122,7 @@ def load_library():
     lib.mass_lyo_set_schedule.argtypes = [h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_int32]
     lib.mass_lyo_set_loop_kernels.argtypes = [h, ctypes.c_int32]
+    lib.mass_lyo_set_growth.argtypes = [h, ctypes.POINTER(Growth)]
     lib.mass_lyo_checkpoint_size.argtypes = [h, ctypes.POINTER(ctypes.c_uint64)]
     lib.mass_lyo_checkpoint_save.argtypes = [h, ctypes.c_void_p, ctypes.c_uint64]
     lib.mass_lyo_checkpoint_load.argtypes = [h, ctypes.c_void_p, ctypes.c_uint64]
@@ -288,6 +305,22 @@ class Lyotropic:
         image = bytes(image)
         self._check(self.lib.mass_lyo_checkpoint_load(
             self.h, ctypes.cast(ctypes.c_char_p(image), ctypes.c_void_p), len(image)))
+
+    def set_growth(self, **kw):
+        """Makes this a go-or-grow run (before configure): keys of
+        GROWTH_DEFAULTS (chi_config "noise" or "front") and alpha, beta,
+        division_rate/time/radius, death_rate/time/radius, phi_critical."""
+        g = dict(GROWTH_DEFAULTS)
+        g.update(kw)
+        missing = [k for k in GROWTH_DOUBLES if k not in g]
+        if missing:
+            raise ValueError("missing go-or-grow parameters: %s" % ", ".join(missing))
+        cfg = {"noise": 0, "front": 1}[g["chi_config"]]
+        s = Growth(chi_config=cfg, growTogether=int(g["growTogether"]),
+                   relax_phi=int(g["relax_phi"]), relax_Q=int(g["relax_Q"]),
+                   relax_steps=int(g["relax_steps"]), reserved=0,
+                   **{k: float(g[k]) for k in GROWTH_DOUBLES})
+        self._check(self.lib.mass_lyo_set_growth(self.h, ctypes.byref(s)))
 
     def set_loop_kernels(self, on=True):
         """FP64: run each step as the reference's node loops, one kernel per

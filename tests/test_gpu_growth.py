@@ -1,0 +1,138 @@
+"""go-or-grow on the GPU against the CPU oracle.
+
+The go-or-grow kernels (mass_hd_amd/csrc/lyo_growth.hpp) follow the
+reference's loop structure in FP64 and draw the masks from the same mt19937
+stream on the host, so every field -- the Lyotropic state, the derived frame
+fields, m, chi and the division/death masks -- must equal the oracle's bit for
+bit after configure (dry relaxation, hydrodynamic reset, phenotype set-up) and
+after steps that redraw the masks, on periodic grids (ghost layer with
+periodic ghosts) and walled ones.  With conserve_phi (no division, no death)
+phi's sum enters every step; the device sums it in a fixed tree where the
+reference sums serially, so those runs are compared at 1e-12.
+"""
+import numpy as np
+import pytest
+
+from mass_hd_amd import lyo
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+PHYS = dict(GammaQ=0.3, GammaP=5e-2, xi=0.3, tauNem=1.0, tauIso=1.0, rho=40.0, friction=0.0,
+            AA=0.03, CC=0.1, LL=0.002, KK=0.06, zeta=0.02, zetaI=0.0, conc=1.0)
+# the proliferation cases of the reference's runcards, scaled down
+GROWTH = dict(B=0.1, Snem=1.0, relax_steps=20, relax_phi=1, relax_Q=1, relax_dt=1.0,
+              chi_config="noise", chi0=1.0, chi_noise=0.0, chi_length=0.0, Dchi=0.0,
+              Achi=0.0, Ochi=0.0, phiswitch=0.0, growTogether=0, beta=0.0, death_rate=0.0,
+              death_time=1.0, death_radius=0.0, division_rate=0.1, division_time=2.0,
+              division_radius=3.0, alpha=0.001, phi_critical=1.2)
+INIT = dict(config="circle", seed=1001, radius=10.0, level=20.0, noise=1.0, angle=0.0)
+STATE = ["ff", "QQxx", "QQyx", "phi", "m", "chi"]
+DERIVED = ["n", "ux", "uy", "sigmaXX", "sigmaXY", "sigmaYX", "sigmaYY", "sigma_bulk",
+           "sigma_elastic_xx", "sigma_elastic_yx", "sigma_phase_field_xx",
+           "sigma_phase_field_yx", "sigma_active_xx", "sigma_active_yx", "MU", "HHxx", "HHyx"]
+
+
+def pair(LX, LY, bc=0, npc=1, phys=PHYS, growth=None, init=INIT):
+    g = dict(GROWTH)
+    g.update(growth or {})
+    a = lyo.Lyotropic(LX, LY, npc=npc, precision=lyo.FP64, bc=bc, **phys)
+    a.set_growth(**g)
+    o = po.Oracle(LX, LY, npc=npc, bc=bc, **phys)
+    o.set_growth(**g)
+    ta = a.configure(**init)
+    to = o.configure(**init)
+    return a, o, ta, to
+
+
+def check(a, o, names, what=""):
+    for n in names:
+        x, y = a.get(n), o.get(n)
+        assert np.array_equal(x, y), "%s %s differs: max |d| = %g" % (what, n, np.abs(x - y).max())
+    da, ea = a.get("division_mask"), a.get("death_mask")
+    do, eo = o.masks()
+    assert np.array_equal(da, do.astype(float)) and np.array_equal(ea, eo.astype(float)), what
+
+
+@pytest.mark.parametrize("bc", [0, 2, 4])
+@pytest.mark.parametrize("npc", [1, 2])
+@pytest.mark.parametrize("LX,LY", [(40, 36), (7, 5)])
+def test_growth_bit_exact(require_gpu, bc, npc, LX, LY):
+    print("go-or-grow bc=%d npc=%d %dx%d" % (bc, npc, LX, LY), flush=True)
+    a, o, ta, to = pair(LX, LY, bc, npc)
+    assert ta == to  # ftot of the reset, ptot of the relaxed phi
+    check(a, o, STATE, "configure")
+    for s in range(3):
+        a.step(4)
+        o.step(4)
+        check(a, o, STATE + DERIVED, "after %d steps" % (4 * s + 4))
+    if LX > 10:
+        assert a.get("division_mask").any()  # the masks were drawn and used
+        assert a.get("phi").sum() > ta[1]     # division grew the colony
+
+
+@pytest.mark.parametrize("growth", [
+    dict(chi_noise=0.01, chi_length=2.0, chi0=0.6, Dchi=0.05, Achi=0.02, Ochi=0.01,
+         phiswitch=0.5, growTogether=1),
+    dict(chi_config="front", chi_length=4.0, beta=0.0005, death_rate=0.05, death_time=3.0,
+         death_radius=2.0),
+])
+@pytest.mark.parametrize("bc", [0, 3])
+def test_growth_phenotype_terms_bit_exact(require_gpu, growth, bc):
+    a, o, ta, to = pair(48, 40, bc, npc=2, growth=growth)
+    assert ta == to
+    check(a, o, STATE, "configure")
+    a.step(7)
+    o.step(7)
+    check(a, o, STATE + DERIVED, "7 steps")
+
+
+def test_growth_conserve_phi_matches_to_rounding(require_gpu):
+    # no division or death: conserve_phi adds (countphi - totalphi)/N to Dp
+    a, o, ta, to = pair(40, 36, 0, npc=1, growth=dict(alpha=0.0, division_rate=0.0))
+    assert ta == to
+    a.step(10)
+    o.step(10)
+    for n in ["ff", "QQxx", "QQyx", "phi", "m"]:
+        x, y = a.get(n), o.get(n)
+        assert np.allclose(x, y, rtol=1e-12, atol=1e-14), n
+    # chi = m/phi amplifies those differences where phi is small
+    phi = o.get("phi")
+    big = phi > 1e-3
+    assert np.allclose(a.get("chi")[big], o.get("chi")[big], rtol=1e-8, atol=0)
+
+
+def test_growth_512_bounds_and_repeats(require_gpu):
+    print("go-or-grow 512^2", flush=True)
+    runs = []
+    for _ in range(2):
+        a = lyo.Lyotropic(512, 512, npc=1, precision=lyo.FP64, **PHYS)
+        a.set_growth(**dict(GROWTH, relax_steps=50))
+        a.configure(**dict(INIT, radius=120.0))
+        a.step(30)
+        phi, m = a.get("phi"), a.get("m")
+        assert np.all(m >= 0) and np.all(m <= np.maximum(phi, 0))  # ProjectM's bounds
+        assert np.isfinite(a.get("ff")).all()
+        runs.append([a.get(n) for n in STATE])
+    for x, y in zip(*runs):
+        assert np.array_equal(x, y)
+
+
+def test_growth_refusals(require_gpu):
+    f32 = lyo.Lyotropic(16, 16, npc=1, precision=lyo.FP32, **PHYS)
+    with pytest.raises(lyo.MassLyoError):
+        f32.set_growth(**GROWTH)
+    plain = lyo.Lyotropic(16, 16, npc=1, precision=lyo.FP64, **PHYS)
+    plain.configure(**INIT)
+    with pytest.raises(lyo.MassLyoError):
+        plain.get("m")
+    a = lyo.Lyotropic(16, 16, npc=1, precision=lyo.FP64, **PHYS)
+    with pytest.raises(lyo.MassLyoError):
+        a.set_growth(**dict(GROWTH, chi_noise=-1.0))
+    a.set_growth(**GROWTH)
+    a.configure(**INIT)
+    for call in (a.diagnostics, a.checkpoint, lambda: a.set_loop_kernels(False)):
+        with pytest.raises(lyo.MassLyoError):
+            call()
+    a.step(2)
+    assert np.isfinite(a.get("m")).all()
