@@ -5,8 +5,10 @@ shifted populations and compensated phi, npc = 2 (predictor + 2 correctors),
 zeta = 0.03, init seed 7 (random director, phi = 0.5 + U(-0.05, 0.05)),
 200 warm-up + 500 timed steps.  The same grid is also timed in FP64 (the
 reference arithmetic) with the fused kernels and with one kernel per
-reference loop, and in surrogate mode (predictor + one corrector + the
-network, judged every 50 steps).
+reference loop, in FP64 as a no-slip box (bc = 4, ghost layer), and in
+surrogate mode (predictor + one corrector + the network, judged every 50
+steps); go-or-grow is timed on the reference's own 500 x 500 proliferation
+runcard.
 
 MLUPS = LX*LY*steps / seconds / 1e6; one step = predictor + npc correctors
 (with npc <= 2 the whole step is one kernel).  The timed region is bracketed
@@ -74,6 +76,41 @@ def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-
     m.runtime_checks()
     m.close()
     return total_ms, dict(zip(("predictor", "corrector", "pair", "step"), sweeps))
+
+
+# the reference's go-or-grow proliferation runcard
+# (cases/20may/proliferation_p2_bulk/alpha0p00100_pc1p20/run.dat)
+GROWTH_PHYS = dict(GammaQ=0.3, GammaP=5e-2, xi=0.3, tauNem=1.0, tauIso=1.0, rho=40.0,
+                   friction=0.0, AA=0.03, CC=0.1, LL=0.002, KK=0.06, zeta=0.0, zetaI=0.0,
+                   conc=1.0)
+GROWTH_PARAMS = dict(B=0.1, Snem=1.0, relax_steps=2000, relax_dt=1.0, relax_phi=1, relax_Q=1,
+                     chi_config="noise", chi0=1.0, chi_noise=0.0, chi_length=0.0, Dchi=0.0,
+                     Achi=0.0, Ochi=0.0, phiswitch=0.0, growTogether=0, beta=0.0,
+                     death_rate=0.0, death_time=1.0, death_radius=0.0, division_rate=0.1,
+                     division_time=2.0, division_radius=3.0, alpha=0.001, phi_critical=1.2)
+GROWTH_INIT = dict(config="circle", seed=1001, radius=60.0, level=250.0, noise=1.0, angle=0.0)
+
+
+def growth_leg(lyo, steps, L=500):
+    """go-or-grow at 500^2, npc=1 (the reference's proliferation runcard):
+    configure (2000 dry relaxation steps, reset, phenotype), `steps` untimed,
+    then `steps` timed steps, whose masks are redrawn every third step from
+    phi read back to the host."""
+    m = lyo.Lyotropic(L, L, npc=1, precision=lyo.FP64, **GROWTH_PHYS)
+    m.set_growth(**GROWTH_PARAMS)
+    t0 = time.time()
+    m.configure(**GROWTH_INIT)
+    m.synchronize()
+    t_conf = time.time() - t0
+    m.step(steps)
+    m.synchronize()
+    total_ms, _ = m.time_steps(steps, per_sweep=False)
+    phi = m.get("phi")
+    m.close()
+    return {"workload": "go-or-grow 500x500 periodic, npc=1, FP64 (proliferation runcard)",
+            "mlups": round(L * L * steps / (total_ms / 1e3) / 1e6, 1),
+            "ms_per_step": round(total_ms / steps, 4), "configure_s": round(t_conf, 3),
+            "phi_sum_after": round(float(phi.sum()), 3)}
 
 
 def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warmup=200):
@@ -186,6 +223,9 @@ def main():
                     help="skip the FP64 run with one kernel per reference loop")
     ap.add_argument("--skip-walls", action="store_true",
                     help="skip the FP64 no-slip box (bc=4) leg")
+    ap.add_argument("--skip-growth", action="store_true",
+                    help="skip the go-or-grow leg (500^2 proliferation runcard)")
+    ap.add_argument("--growth-steps", type=int, default=300)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-surrogate", action="store_true")
@@ -262,6 +302,12 @@ def main():
         if t.get("size") == L and t.get("npc") == npc and dom in t.get("kernels", {}):
             traffic = t["kernels"][dom].get("bytes_per_launch")
             valu_ms = t["kernels"][dom].get("valu_issue_ms")
+
+    if not args.skip_growth:
+        log("go-or-grow 500x500: configure + %d + %d timed steps" % (args.growth_steps,
+                                                                    args.growth_steps))
+        extra["go_or_grow"] = growth_leg(lyo, args.growth_steps)
+        log("go-or-grow: %s" % extra["go_or_grow"])
 
     if not args.skip_surrogate:
         log("surrogate %dx%d: training on 512^2 frames, %d timed steps" % (L, L, args.surrogate_steps))
