@@ -5,6 +5,7 @@
 #include <sstream>
 
 #include "error_msg.hpp"
+#include "go_or_grow_hip.hpp"
 #include "lyotropic_hip.hpp"
 
 namespace {
@@ -42,7 +43,12 @@ void DeclareModels() {
   declare_model<LyotropicHip>(
       "lyotropic",
       "Biphasic, lyotropic, active nematic (D2Q9 lattice Boltzmann + Beris-Edwards Q-tensor + "
-      "Cahn-Hilliard phase field, predictor-corrector), run on an MI355X GPU. Periodic "
-      "boundaries (bc=0). precision = fp64 reproduces the reference arithmetic bit for bit; "
-      "precision = fp32 stores shifted populations and a compensated phase field.");
+      "Cahn-Hilliard phase field, predictor-corrector), run on an MI355X GPU. bc = 0 periodic, "
+      "1/2 free-/no-slip channel, 3/4 free-/no-slip box (fp64). precision = fp64 reproduces "
+      "the reference arithmetic bit for bit; precision = fp32 (bc = 0) stores shifted "
+      "populations and a compensated phase field.");
+  declare_model<GoOrGrowHip>(
+      "go-or-grow",
+      "Lyotropic with cell division and death and a go/grow phenotype field (chi = m/phi), "
+      "run on an MI355X GPU in fp64, bit for bit the reference arithmetic.");
 }

@@ -58,6 +58,15 @@ OptionGroup& OptionGroup::add_uint(const std::string& n, unsigned* t, const std:
   opts.push_back(o);
   return *this;
 }
+OptionGroup& OptionGroup::add_int(const std::string& n, int* t, const std::string& h) {
+  OptionSpec o;
+  o.name = n;
+  o.kind = OptionSpec::INT;
+  o.target = t;
+  o.help = h;
+  opts.push_back(o);
+  return *this;
+}
 OptionGroup& OptionGroup::add_double(const std::string& n, double* t, const std::string& h) {
   OptionSpec o;
   o.name = n;
@@ -220,6 +229,14 @@ void notify(const std::vector<const OptionGroup*>& gs, const VariablesMap& vm) {
           const unsigned long x = std::strtoul(v.c_str(), &end, 10);
           if (errno || *end || x > 0xffffffffUL) throw bad();
           *static_cast<unsigned*>(o.target) = (unsigned)x;
+          break;
+        }
+        case OptionSpec::INT: {
+          errno = 0;
+          char* end = nullptr;
+          const long x = std::strtol(v.c_str(), &end, 10);
+          if (v.empty() || errno || *end || x < -2147483647L - 1 || x > 2147483647L) throw bad();
+          *static_cast<int*>(o.target) = (int)x;
           break;
         }
         case OptionSpec::DOUBLE: {

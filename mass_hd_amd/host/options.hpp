@@ -20,7 +20,7 @@
 #include <vector>
 
 struct OptionSpec {
-  enum Kind { UINT, DOUBLE, STRING, SWITCH };
+  enum Kind { UINT, INT, DOUBLE, STRING, SWITCH };
   std::string name;
   char short_name = 0;
   Kind kind = STRING;
@@ -37,6 +37,7 @@ struct OptionGroup {
   std::vector<OptionSpec> opts;
 
   OptionGroup& add_uint(const std::string& name, unsigned* t, const std::string& help);
+  OptionGroup& add_int(const std::string& name, int* t, const std::string& help);
   OptionGroup& add_double(const std::string& name, double* t, const std::string& help);
   OptionGroup& add_string(const std::string& name, std::string* t, const std::string& help);
   OptionGroup& add_switch(const std::string& name, bool* t, const std::string& help);

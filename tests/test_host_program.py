@@ -44,6 +44,60 @@ npc = 1
 """
 
 
+# the reference's proliferation runcard (go-or-grow), at 64^2 and 20 steps
+GROWTH_RUNCARD = """model    = go-or-grow
+config   = circle
+nsteps   = 20
+ninfo    = 10
+LX       = 64
+LY       = 64
+bc       = 0
+seed     = 1001
+GammaQ   = 0.3
+GammaP   = 5e-2
+zeta     = 0
+zetaI    = 0
+friction = 0.0
+tauNem   = 1.0
+tauIso   = 1.0
+AA       = 0.03
+CC       = 0.1
+LL       = 0.002
+KK       = 0.06
+rho      = 40
+xi       = 0.3
+B        = 0.1
+Snem     = 1
+level    = 32
+radius   = 12
+conc     = 1
+noise    = 1
+angle    = 0
+relax-steps = 100
+relax-dt    = 1.0
+relax-phi   = 1
+relax-Q     = 1
+chi-config = noise
+chi0       = 1
+chi-noise  = 0
+chi-length = 0
+Dchi       = 0
+Achi      = 0
+Ochi      = 0
+phiswitch = 0
+growTogether = 0
+beta         = 0
+death-rate   = 0
+death-time   = 1
+death-radius = 0
+division-rate   = 1e-1
+division-time   = 2
+division-radius = 3
+alpha           = 0.001
+phi-critical    = 1.2
+"""
+
+
 def run(args, cwd=None):
     r = subprocess.run([EXE] + args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
                        cwd=cwd, timeout=120)
@@ -74,6 +128,24 @@ def test_help_shows_model_options():
     assert rc == 0
     for key in ("GammaQ", "npc", "config", "precision", "--LX", "--nsteps"):
         assert key in out
+
+
+def test_go_or_grow_options():
+    rc, out, _ = run(["-h", "-m", "go-or-grow"])
+    assert rc == 0
+    for key in ("division-rate", "phi-critical", "chi-config", "relax-steps", "growTogether",
+                "Snem", "GammaQ"):
+        assert key in out
+    rc, out, _ = run(["--list"])
+    assert "go-or-grow" in out
+
+
+def test_go_or_grow_bad_values(card):
+    rc, _, err = run([card(GROWTH_RUNCARD.replace("relax-phi   = 1", "relax-phi   = x")),
+                      "--no-write"])
+    assert rc == 1 and "relax-phi" in err
+    rc, _, err = run([card(GROWTH_RUNCARD + "precision = fp32\n"), "--no-write"])
+    assert rc == 1
 
 
 def test_missing_runcard(tmp_path):
@@ -218,6 +290,34 @@ def test_host_walled_run_matches_oracle(card, tmp_path, require_gpu, bc):
     for k in ("QQxx", "QQyx", "phi", "sigmaXX", "sigmaXY"):
         assert np.allclose(f20[k]["value"], o.get(k), rtol=1e-5, atol=1e-6), k
     assert np.allclose(np.array(f20["ff"]["value"]), o.get("ff"), rtol=1e-5)
+
+
+@pytest.mark.gpu
+def test_host_go_or_grow_run_matches_oracle(card, tmp_path, require_gpu):
+    """The reference's proliferation runcard (64^2): frames with the masks,
+    m and chi equal the oracle's to the archive's six digits."""
+    from oracle import pyoracle as po
+    outdir = tmp_path / "out"
+    rc, out, err = run([card(GROWTH_RUNCARD), "-o", str(outdir), "-f"])
+    assert rc == 0, err
+    params = json.load(open(outdir / "parameters.json"))["data"]
+    assert params["model_name"]["value"] == "go-or-grow"
+    assert params["division_rate"]["value"] == 0.1 and params["relax_steps"]["value"] == 100
+    o = po.Oracle(64, 64, npc=1, GammaQ=0.3, GammaP=5e-2, xi=0.3, tauNem=1.0, tauIso=1.0,
+                  rho=40.0, friction=0.0, AA=0.03, CC=0.1, LL=0.002, KK=0.06, zeta=0.0,
+                  zetaI=0.0, conc=1.0)
+    o.set_growth(B=0.1, Snem=1.0, relax_steps=100, relax_dt=1.0, relax_phi=1, relax_Q=1,
+                 chi_config="noise", chi0=1.0, alpha=0.001, beta=0.0, division_rate=0.1,
+                 division_time=2.0, division_radius=3.0, death_rate=0.0, death_time=1.0,
+                 death_radius=0.0, phi_critical=1.2)
+    o.configure(config="circle", seed=1001, radius=12.0, level=32.0, noise=1.0, angle=0.0)
+    o.step(20)
+    f = json.load(open(outdir / "frame20.json"))["data"]
+    for k in ("QQxx", "QQyx", "phi", "m", "chi", "sigmaXX"):
+        assert np.allclose(f[k]["value"], o.get(k), rtol=1e-5, atol=1e-6), k
+    div, death = o.masks()
+    assert np.array_equal(np.array(f["division_mask"]["value"]) != 0, div != 0)
+    assert not np.any(f["death_mask"]["value"])
 
 
 @pytest.mark.gpu
