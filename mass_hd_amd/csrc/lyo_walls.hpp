@@ -104,6 +104,24 @@ __device__ __forceinline__ bool wall_ghost(const WallGeo& g, int phase, int i, i
   return true;
 }
 
+// phase -1 (periodic ghosts only, whose copies read the domain alone): one
+// launch covers the three phases' ghost nodes.
+__device__ __forceinline__ bool any_ghost(const WallGeo& g, int phase, int i, int& p, int& id,
+                                          int& ph) {
+  ph = phase;
+  if (ph < 0) {
+    if (i < 2 * g.LY) {
+      ph = 0;
+    } else if ((i -= 2 * g.LY) < 2 * g.LX) {
+      ph = 1;
+    } else {
+      i -= 2 * g.LX;
+      ph = 2;
+    }
+  }
+  return wall_ghost(g, ph, i, p, id);
+}
+
 // The domain node a ghost node is the periodic image of.  With periodic
 // ghosts every condition copies it (ApplyPBC, fields.hpp:619-631,749-760);
 // the domain then sees exactly the periodic grid's neighbours and streams.
@@ -203,52 +221,57 @@ __device__ __forceinline__ void lb_ghost(const WallGeo& g, T* f, int phase, int 
   }
 }
 
-// BoundaryConditionsFields (lyotropic.cpp:422-449): Neumann on Qxx, Qyx, phi.
+// BoundaryConditionsFields (lyotropic.cpp:422-449): Neumann on Qxx, Qyx,
+// phi, and for go-or-grow on m and chi (go-or-grow.cpp:365-385).
 template <class T>
 __global__ void __launch_bounds__(256) wall_fields_kernel(WallGeo g, int phase, T* qxx, T* qyx,
-                                                          T* phi) {
-  int p, id;
-  if (!wall_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id)) return;
-  scalar_ghost(g, qxx, 0, phase, p, id);
-  scalar_ghost(g, qyx, 0, phase, p, id);
-  scalar_ghost(g, phi, 0, phase, p, id);
+                                                          T* phi, T* m, T* chi) {
+  int p, id, ph;
+  if (!any_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id, ph)) return;
+  scalar_ghost(g, qxx, 0, ph, p, id);
+  scalar_ghost(g, qyx, 0, ph, p, id);
+  scalar_ghost(g, phi, 0, ph, p, id);
+  if (m) {
+    scalar_ghost(g, m, 0, ph, p, id);
+    scalar_ghost(g, chi, 0, ph, p, id);
+  }
 }
 
 // BoundaryConditionsFields2 (lyotropic.cpp:451-498) on the derived planes:
 // copy derivative on u, Dirichlet 0 on u*phi, Neumann on MU and the stresses.
 template <class T>
 __global__ void __launch_bounds__(256) wall_fields2_kernel(WallGeo g, int phase, T* D) {
-  int p, id;
-  if (!wall_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id)) return;
+  int p, id, ph;
+  if (!any_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id, ph)) return;
   auto pl = [&](int i) { return D + (size_t)i * g.NP; };
-  scalar_ghost(g, pl(LP_UY), 2, phase, p, id);
-  scalar_ghost(g, pl(LP_UX), 2, phase, p, id);
-  scalar_ghost(g, pl(LP_UYP), 1, phase, p, id);
-  scalar_ghost(g, pl(LP_UXP), 1, phase, p, id);
-  scalar_ghost(g, pl(LP_MU), 0, phase, p, id);
-  scalar_ghost(g, pl(LP_SXX), 0, phase, p, id);
-  scalar_ghost(g, pl(LP_SYY), 0, phase, p, id);
-  scalar_ghost(g, pl(LP_SYX), 0, phase, p, id);
-  scalar_ghost(g, pl(LP_SXY), 0, phase, p, id);
+  scalar_ghost(g, pl(LP_UY), 2, ph, p, id);
+  scalar_ghost(g, pl(LP_UX), 2, ph, p, id);
+  scalar_ghost(g, pl(LP_UYP), 1, ph, p, id);
+  scalar_ghost(g, pl(LP_UXP), 1, ph, p, id);
+  scalar_ghost(g, pl(LP_MU), 0, ph, p, id);
+  scalar_ghost(g, pl(LP_SXX), 0, ph, p, id);
+  scalar_ghost(g, pl(LP_SYY), 0, ph, p, id);
+  scalar_ghost(g, pl(LP_SYX), 0, ph, p, id);
+  scalar_ghost(g, pl(LP_SXY), 0, ph, p, id);
 }
 
 // Neumann on up to two more scalar planes: m and chi (go-or-grow.cpp:
 // 365-385), or MU of the dry relaxation (:70-83).
 template <class T>
 __global__ void __launch_bounds__(256) wall_scalars_kernel(WallGeo g, int phase, T* a, T* b) {
-  int p, id;
-  if (!wall_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id)) return;
-  scalar_ghost(g, a, 0, phase, p, id);
-  if (b) scalar_ghost(g, b, 0, phase, p, id);
+  int p, id, ph;
+  if (!any_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id, ph)) return;
+  scalar_ghost(g, a, 0, ph, p, id);
+  if (b) scalar_ghost(g, b, 0, ph, p, id);
 }
 
 // BoundaryConditionsLB (lyotropic.cpp:388-420) on ff and fn.
 template <class T>
 __global__ void __launch_bounds__(256) wall_lb_kernel(WallGeo g, int phase, T* f, T* fn) {
-  int p, id;
-  if (!wall_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id)) return;
-  lb_ghost(g, f, phase, p, id);
-  lb_ghost(g, fn, phase, p, id);
+  int p, id, ph;
+  if (!any_ghost(g, phase, blockIdx.x * blockDim.x + threadIdx.x, p, id, ph)) return;
+  lb_ghost(g, f, ph, p, id);
+  lb_ghost(g, fn, ph, p, id);
 }
 
 // Move over every node of the torus (lyotropic.cpp:371-386), as a pull:
@@ -436,7 +459,7 @@ class WallEngine final : public Engine {
       update_quantities(s);
       fields2_bc(s);
       update_fields(true, s);
-      for (int ph = 0; ph < 3; ++ph)
+      for (int ph : phases())
         hipLaunchKernelGGL((wall_lb_kernel<T>), dim3(phase_blocks(ph)), dim3(bs), 0, s, geo(),
                            ph, F_[cur_], Fn_[cur_]);
       hipLaunchKernelGGL((wall_move_kernel<T>), dim3(nbt), dim3(bs), 0, s, geo(),
@@ -575,20 +598,23 @@ class WallEngine final : public Engine {
  private:
   WallGeo geo() const { return WallGeo{LX_, LY_, LY_ + 2, NPw_, bc_}; }
   const T* plane(int i) const { return D_ + (size_t)i * NPw_; }
+  // the launches of one boundary-condition call: the three phases in order,
+  // or one launch over every ghost node for periodic ghosts (which read the
+  // domain only)
+  std::vector<int> phases() const { return bc_ == 0 ? std::vector<int>{-1} : std::vector<int>{0, 1, 2}; }
   unsigned phase_blocks(int phase) const {
-    const int n = phase == 0 ? 2 * LY_ : (phase == 1 ? 2 * LX_ : 4);
+    const int n = phase < 0 ? 2 * LY_ + 2 * LX_ + 4
+                            : (phase == 0 ? 2 * LY_ : (phase == 1 ? 2 * LX_ : 4));
     return (unsigned)((n + 255) / 256);
   }
   // BoundaryConditionsFields (lyotropic.cpp:422-449; go-or-grow.cpp:365-385
   // adds m and chi)
   void fields_bc(hipStream_t s, bool with_growth = true) {
-    for (int ph = 0; ph < 3; ++ph) {
+    const bool g = growth_ && with_growth;
+    for (int ph : phases())
       hipLaunchKernelGGL((wall_fields_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s, geo(),
-                         ph, qxx_, qyx_, phi_[pc_]);
-      if (growth_ && with_growth)
-        hipLaunchKernelGGL((wall_scalars_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s,
-                           geo(), ph, m_[mc_], chi_);
-    }
+                         ph, qxx_, qyx_, phi_[pc_], g ? m_[mc_] : (T*)nullptr,
+                         g ? chi_ : (T*)nullptr);
   }
   unsigned node_blocks() const { return (unsigned)((LX_ * LY_ + 255) / 256); }
   // UpdatePhenotypeQuantities (go-or-grow.cpp:201-209)
@@ -678,7 +704,7 @@ class WallEngine final : public Engine {
     phenotype(s);
   }
   void fields2_bc(hipStream_t s) {
-    for (int ph = 0; ph < 3; ++ph)
+    for (int ph : phases())
       hipLaunchKernelGGL((wall_fields2_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s, geo(),
                          ph, D_);
   }
@@ -863,7 +889,7 @@ class WallEngine final : public Engine {
         hipLaunchKernelGGL((relax_mu_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
                            (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], P_, gd_, LX_, LY_,
                            hxx, hyx, mu);
-        for (int ph = 0; ph < 3; ++ph)
+        for (int ph : phases())
           hipLaunchKernelGGL((wall_scalars_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s,
                              geo(), ph, mu, (T*)nullptr);
         hipLaunchKernelGGL((relax_update_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
