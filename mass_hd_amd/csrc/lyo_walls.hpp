@@ -411,6 +411,7 @@ class WallEngine final : public Engine {
   }
 
   void release() override {
+    drop_graphs();
     for (T* p : ptrs_) (void)hipFree(p);
     ptrs_.clear();
     sizes_.clear();
@@ -428,6 +429,7 @@ class WallEngine final : public Engine {
   }
 
   void set_params(const double* v) override {
+    drop_graphs();
     P_.GammaQ = v[0];
     P_.GammaP = v[1];
     P_.xi = v[2];
@@ -446,15 +448,27 @@ class WallEngine final : public Engine {
 
   // Lyotropic::Step (lyotropic.cpp:500-525); go-or-grow redraws its masks
   // first (lyotropic-with-division.cpp:150-156)
+  // The device part of a step is replayed from a HIP graph (captured on first
+  // use for each parity of the ping-pong buffers; dropped whenever buffers,
+  // parameters or the state change): a step is ~25 launches, which at small
+  // grids take longer to issue than to run.  The graph holds kernels only.
   hipError_t steps(int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) override {
     (void)L;
     (void)tm;
-    const int bs = 256, nbt = (TP_ + bs - 1) / bs;
     for (int st = 0; st < nsteps; ++st) {
       if (growth_) {
         const hipError_t e = set_masks(s);
         if (e != hipSuccess) return e;
       }
+      const hipError_t e = graph_step(s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+
+  hipError_t step_body(hipStream_t s) {
+    const int bs = 256, nbt = (TP_ + bs - 1) / bs;
+    {
       fields_bc(s);
       update_quantities(s);
       fields2_bc(s);
@@ -471,16 +485,47 @@ class WallEngine final : public Engine {
         fields2_bc(s);
         update_fields(false, s);
       }
-      const hipError_t e = hipGetLastError();
-      if (e != hipSuccess) return e;
     }
-    return hipSuccess;
+    return hipGetLastError();
+  }
+
+  hipError_t graph_step(hipStream_t s) {
+    for (const StepGraph& g : graphs_) {
+      if (g.cur == cur_ && g.pc == pc_ && g.mc == mc_) {
+        cur_ = g.cur_after;
+        pc_ = g.pc_after;
+        mc_ = g.mc_after;
+        return hipGraphLaunch(g.exec, s);
+      }
+    }
+    StepGraph g{cur_, pc_, mc_, 0, 0, 0, nullptr};
+    const int cur0 = cur_, pc0 = pc_, mc0 = mc_;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const hipError_t e1 = step_body(s);
+    e = hipStreamEndCapture(s, &graph);
+    if (e == hipSuccess) e = e1;
+    if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      cur_ = cur0;
+      pc_ = pc0;
+      mc_ = mc0;
+      return e;
+    }
+    g.cur_after = cur_;
+    g.pc_after = pc_;
+    g.mc_after = mc_;
+    graphs_.push_back(g);
+    return hipGraphLaunch(g.exec, s);
   }
 
   // Every buffer restarts at zero (ghost layer included, as after
   // Field::SetSize), then the domain takes the given state.
   hipError_t pack(const double* d_ff, const double* d_qxx, const double* d_qyx,
                   const double* d_phi, hipStream_t s) override {
+    drop_graphs();
     hipError_t e = zero_all(s);
     if (e != hipSuccess) return e;
     cur_ = 0;
@@ -588,6 +633,7 @@ class WallEngine final : public Engine {
     return copy_state(host, nullptr, s);
   }
   hipError_t load_state(const void* host, hipStream_t s) override {
+    drop_graphs();
     hipError_t e = zero_all(s);
     if (e != hipSuccess) return e;
     cur_ = 0;
@@ -838,6 +884,7 @@ class WallEngine final : public Engine {
 
  public:
   hipError_t set_growth(const mass_lyo_growth& g, std::mt19937* gen) override {
+    drop_graphs();
     const size_t n1 = (size_t)NPw_;
     hipError_t e = hipSuccess;
     if (!growth_) {
@@ -875,6 +922,7 @@ class WallEngine final : public Engine {
   // GoOrGrow::Configure after Lyotropic::Configure (go-or-grow.cpp:34-40)
   hipError_t growth_configure(double* ftot, double* ptot, hipStream_t s) override {
     if (!growth_) return hipErrorNotSupported;
+    drop_graphs();  // relaxation and smoothing swap buffers; totalphi changes
     const int N = LX_ * LY_;
     const unsigned nb = node_blocks();
     hipError_t e = hipSuccess;
@@ -984,6 +1032,18 @@ class WallEngine final : public Engine {
   }
 
  private:
+  // a captured step: the parities it starts from and leaves
+  struct StepGraph {
+    int cur, pc, mc;
+    int cur_after, pc_after, mc_after;
+    hipGraphExec_t exec;
+  };
+  std::vector<StepGraph> graphs_;
+  void drop_graphs() {
+    for (StepGraph& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+  }
+
   bool growth_ = false, conserve_ = false;
   mass_lyo_growth gp_{};
   GrowthDev gd_{};
