@@ -133,7 +133,10 @@ typedef struct mass_lyo_growth {
   int32_t growTogether; /* m's growth source: 0 R, 1 chi*R */
   int32_t relax_phi, relax_Q;
   uint32_t relax_steps; /* dry free-energy relaxation before the dynamics */
-  int32_t reserved;     /* 0 */
+  int32_t division_only; /* 1: lyotropic-with-division instead (Lyotropic + the
+                            masks, phi += phi*(alpha*division - beta*death),
+                            lyotropic-with-division.cpp:110-126; only alpha ...
+                            phi_critical are read) */
   double B, Snem, Dchi, Achi, Ochi, phiswitch;
   double chi0, chi_noise, chi_length, relax_dt;
   double alpha, beta, division_rate, division_time, division_radius;
@@ -272,7 +275,9 @@ int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on);
 
 /* Makes h a go-or-grow run: replaces the GoOrGrow(LX,LY,BC) factory +
  * GoOrGrow::Initialize (go-or-grow.cpp:13-32, lyotropic-with-division.cpp:
- * 14-32).  Call after mass_lyo_create and before mass_lyo_configure; FP64
+ * 14-32); with division_only, a lyotropic-with-division run
+ * (LyotropicWithDivision, lyotropic-with-division.cpp:14-156: no m, chi,
+ * relaxation or phenotype).  Call after mass_lyo_create and before mass_lyo_configure; FP64
  * only, any bc.  mass_lyo_configure then also runs Configure's dry
  * relaxation, hydrodynamic reset and phenotype set-up (go-or-grow.cpp:34-40)
  * and returns ftot of the reset and ptot of the relaxed phi; every

@@ -121,8 +121,10 @@ struct GrowthUF {
   int conserve;
 };
 
-// GoOrGrow::UpdateFields (go-or-grow.cpp:322-362) for one domain node.
-template <bool FIRST, class G = LayerGrid>
+// GoOrGrow::UpdateFields (go-or-grow.cpp:322-362) for one domain node; with
+// DIV, LyotropicWithDivision::UpdateFields (lyotropic-with-division.cpp:
+// 110-126): Lyotropic's update and phi += phi*(alpha*division - beta*death).
+template <bool FIRST, class G = LayerGrid, bool DIV = false>
 __global__ void __launch_bounds__(256) growth_uf_kernel(GrowthUF a, Params<double> P,
                                                         GrowthDev g, int LX, int LY, int NP) {
   using T = double;
@@ -143,24 +145,27 @@ __global__ void __launch_bounds__(256) growth_uf_kernel(GrowthUF a, Params<doubl
   const T division_m = a.division[k] ? T(1.) : T(0.);
   const T death_m = a.death[k] ? T(1.) : T(0.);
   const T growth_rate = g.alpha * division_m - g.beta * death_m;
-  const T chi_eff = chi[k];
-  const T R = chi_eff * phi[k] * growth_rate;
-  const T mFlux = T(.5) * (ux[n.kxp] * m[n.kxp] - ux[n.kxm] * m[n.kxm]) +
-                  T(.5) * (uy[n.kyp] * m[n.kyp] - uy[n.kym] * m[n.kym]);
-  const T chi_px = chi[n.kxp], chi_mx = chi[n.kxm], chi_py = chi[n.kyp], chi_my = chi[n.kym];
-  const T diffusiveMFlux = T(.5) * (chi_px + chi_eff) * (MU[n.kxp] - MU[k]) -
-                           T(.5) * (chi_eff + chi_mx) * (MU[k] - MU[n.kxm]) +
-                           T(.5) * (chi_py + chi_eff) * (MU[n.kyp] - MU[k]) -
-                           T(.5) * (chi_eff + chi_my) * (MU[k] - MU[n.kym]);
-  const T phenotypeDiffusion = T(.5) * (phi[n.kxp] + phi[k]) * (chi_px - chi_eff) -
-                               T(.5) * (phi[k] + phi[n.kxm]) * (chi_eff - chi_mx) +
-                               T(.5) * (phi[n.kyp] + phi[k]) * (chi_py - chi_eff) -
-                               T(.5) * (phi[k] + phi[n.kym]) * (chi_eff - chi_my);
-  const T dVdChi = T(2) * g.Achi * chi_eff * (T(1) - chi_eff) * (T(1) - T(2) * chi_eff) +
-                   g.Ochi * (phi[k] - g.phiswitch);
-  const T Sswitch = -phi[k] * dVdChi;
-  const T mGrowth = g.growTogether ? chi_eff * R : R;
-  const T Dm = P.GammaP * diffusiveMFlux + g.Dchi * phenotypeDiffusion - mFlux + Sswitch + mGrowth;
+  const T chi_eff = DIV ? T(0) : chi[k];
+  const T R = DIV ? phi[k] * growth_rate : chi_eff * phi[k] * growth_rate;
+  T Dm = T(0);
+  if constexpr (!DIV) {
+    const T mFlux = T(.5) * (ux[n.kxp] * m[n.kxp] - ux[n.kxm] * m[n.kxm]) +
+                    T(.5) * (uy[n.kyp] * m[n.kyp] - uy[n.kym] * m[n.kym]);
+    const T chi_px = chi[n.kxp], chi_mx = chi[n.kxm], chi_py = chi[n.kyp], chi_my = chi[n.kym];
+    const T diffusiveMFlux = T(.5) * (chi_px + chi_eff) * (MU[n.kxp] - MU[k]) -
+                             T(.5) * (chi_eff + chi_mx) * (MU[k] - MU[n.kxm]) +
+                             T(.5) * (chi_py + chi_eff) * (MU[n.kyp] - MU[k]) -
+                             T(.5) * (chi_eff + chi_my) * (MU[k] - MU[n.kym]);
+    const T phenotypeDiffusion = T(.5) * (phi[n.kxp] + phi[k]) * (chi_px - chi_eff) -
+                                 T(.5) * (phi[k] + phi[n.kxm]) * (chi_eff - chi_mx) +
+                                 T(.5) * (phi[n.kyp] + phi[k]) * (chi_py - chi_eff) -
+                                 T(.5) * (phi[k] + phi[n.kym]) * (chi_eff - chi_my);
+    const T dVdChi = T(2) * g.Achi * chi_eff * (T(1) - chi_eff) * (T(1) - T(2) * chi_eff) +
+                     g.Ochi * (phi[k] - g.phiswitch);
+    const T Sswitch = -phi[k] * dVdChi;
+    const T mGrowth = g.growTogether ? chi_eff * R : R;
+    Dm = P.GammaP * diffusiveMFlux + g.Dchi * phenotypeDiffusion - mFlux + Sswitch + mGrowth;
+  }
 
   // Lyotropic::UpdateFieldsAtNode (lyotropic.cpp:249-341)
   CenterVals<T> c;
@@ -232,6 +237,7 @@ __global__ void __launch_bounds__(256) growth_uf_kernel(GrowthUF a, Params<doubl
   // division and death (go-or-grow.cpp:349-350), then the bounded m
   phi_new += R;
   u.phi_out[k] = phi_new;
+  if constexpr (DIV) return;
   T mt;
   if constexpr (FIRST) {
     a.mn_out[k] = m[k] + T(.5) * Dm;

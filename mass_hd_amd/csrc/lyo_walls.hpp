@@ -564,8 +564,8 @@ class WallEngine final : public Engine {
       case D_MU: a = plane(LP_MU); break;
       case D_HXX: a = plane(LP_HXX); break;
       case D_HYX: a = plane(LP_HYX); break;
-      case MASS_LYO_M: a = growth_ ? m_[mc_] : nullptr; break;
-      case MASS_LYO_CHI: a = chi_; break;
+      case MASS_LYO_M: a = growth_ && !gp_.division_only ? m_[mc_] : nullptr; break;
+      case MASS_LYO_CHI: a = growth_ && !gp_.division_only ? chi_ : nullptr; break;
       case MASS_LYO_DIVISION_MASK:
       case MASS_LYO_DEATH_MASK: {
         if (!growth_) return hipErrorInvalidValue;
@@ -656,7 +656,7 @@ class WallEngine final : public Engine {
   // BoundaryConditionsFields (lyotropic.cpp:422-449; go-or-grow.cpp:365-385
   // adds m and chi)
   void fields_bc(hipStream_t s, bool with_growth = true) {
-    const bool g = growth_ && with_growth;
+    const bool g = growth_ && !gp_.division_only && with_growth;
     for (int ph : phases())
       hipLaunchKernelGGL((wall_fields_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s, geo(),
                          ph, qxx_, qyx_, phi_[pc_], g ? m_[mc_] : (T*)nullptr,
@@ -672,15 +672,15 @@ class WallEngine final : public Engine {
   // UpdateQuantities (lyotropic.cpp:343-360; go-or-grow.cpp:305-320)
   void update_quantities(hipStream_t s) {
     const T* f = F_[cur_];
-    if (!growth_) {
+    const bool gog = growth_ && !gp_.division_only;
+    if (!gog)
       hipLaunchKernelGGL((loop_uq_kernel<T, true, LayerGrid>), dim3(node_blocks()), dim3(256), 0,
                          s, f, qxx_, qyx_, phi_[pc_], P_, LX_, LY_, NPw_, D_);
-      return;
-    }
-    hipLaunchKernelGGL((growth_uq_kernel<LayerGrid>), dim3(node_blocks()), dim3(256), 0, s, f,
-                       (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], (const T*)chi_, P_,
-                       gd_, LX_, LY_, NPw_, D_);
-    if (conserve_) {  // countphi: the sum of phi this UpdateQuantities sees
+    else
+      hipLaunchKernelGGL((growth_uq_kernel<LayerGrid>), dim3(node_blocks()), dim3(256), 0, s, f,
+                         (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], (const T*)chi_, P_,
+                         gd_, LX_, LY_, NPw_, D_);
+    if (growth_ && conserve_) {  // countphi: the sum of phi this UpdateQuantities sees
       const int N = LX_ * LY_;
       int nparts = (N + 4095) / 4096;
       if (nparts > kGrowthParts) nparts = kGrowthParts;
@@ -691,7 +691,7 @@ class WallEngine final : public Engine {
       hipLaunchKernelGGL(final_sums_kernel, dim3(1), dim3(256), 0, s, part_,
                          part_ + kGrowthParts, nparts, 0.0, part_ + 2 * kGrowthParts);
     }
-    phenotype(s);
+    if (gog) phenotype(s);
   }
   // UpdateFields (lyotropic.cpp:362-369; go-or-grow.cpp:322-362)
   void update_fields(bool first, hipStream_t s) {
@@ -739,6 +739,16 @@ class WallEngine final : public Engine {
     a.countphi = part_ + 2 * kGrowthParts + 1;
     a.totalphi = totalphi_;
     a.conserve = conserve_ ? 1 : 0;
+    if (gp_.division_only) {
+      if (first)
+        hipLaunchKernelGGL((growth_uf_kernel<true, LayerGrid, true>), dim3(node_blocks()),
+                           dim3(256), 0, s, a, P_, gd_, LX_, LY_, NPw_);
+      else
+        hipLaunchKernelGGL((growth_uf_kernel<false, LayerGrid, true>), dim3(node_blocks()),
+                           dim3(256), 0, s, a, P_, gd_, LX_, LY_, NPw_);
+      pc_ ^= 1;
+      return;
+    }
     if (first)
       hipLaunchKernelGGL((growth_uf_kernel<true, LayerGrid>), dim3(node_blocks()), dim3(256), 0,
                          s, a, P_, gd_, LX_, LY_, NPw_);
@@ -923,10 +933,11 @@ class WallEngine final : public Engine {
   hipError_t growth_configure(double* ftot, double* ptot, hipStream_t s) override {
     if (!growth_) return hipErrorNotSupported;
     drop_graphs();  // relaxation and smoothing swap buffers; totalphi changes
+    totalphi_ = *ptot;
+    if (gp_.division_only) return hipSuccess;  // Lyotropic's Configure only
     const int N = LX_ * LY_;
     const unsigned nb = node_blocks();
     hipError_t e = hipSuccess;
-    totalphi_ = *ptot;
     // RelaxFreeEnergy (:42-119)
     if (gp_.relax_steps != 0 && (gp_.relax_phi || gp_.relax_Q)) {
       T* hxx = D_ + (size_t)LP_HXX * NPw_;

@@ -1,15 +1,11 @@
 // go_or_grow_hip.cpp -- see go_or_grow_hip.hpp.
 #include "go_or_grow_hip.hpp"
 
-#include <cmath>
-#include <iostream>
-
 #include "error_msg.hpp"
 
-void GoOrGrowHip::Initialize() {
-  LyotropicHip::Initialize();
-  if (precision != "fp64") throw error_msg("go-or-grow runs in fp64 only.");
-  mass_lyo_growth g{};
+void GoOrGrowHip::growth_params(mass_lyo_growth& g) const {
+  LyotropicWithDivisionHip::growth_params(g);
+  g.division_only = 0;
   if (chi_config == "noise")
     g.chi_config = 0;
   else if (chi_config == "front")
@@ -30,54 +26,13 @@ void GoOrGrowHip::Initialize() {
   g.chi_noise = chi_noise;
   g.chi_length = chi_length;
   g.relax_dt = relax_dt;
-  g.alpha = alpha;
-  g.beta = beta;
-  g.division_rate = division_rate;
-  g.division_time = division_time;
-  g.division_radius = division_radius;
-  g.death_rate = death_rate;
-  g.death_time = death_time;
-  g.death_radius = death_radius;
-  g.phi_critical = phi_critical;
-  if (mass_lyo_set_growth(h_, &g) != MASS_LYO_OK) throw error_msg(mass_lyo_last_error(h_));
-  // lyotropic-with-division.cpp:31
-  conserve_phi = (alpha == 0 || division_rate == 0) && (beta == 0 || death_rate == 0);
-}
-
-void GoOrGrowHip::RuntimeChecks() {
-  // lyotropic.cpp:527-549: phi is checked when it is conserved
-  LyotropicHip::RuntimeChecks();
-  if (conserve_phi) {
-    double fcheck = 0, pcheck = 0;
-    if (mass_lyo_sums(h_, &fcheck, &pcheck) != MASS_LYO_OK)
-      throw error_msg(mass_lyo_last_error(h_));
-    if (std::abs(ptot - pcheck) > 1)
-      throw error_msg("phi is not conserved (", ptot, "/", pcheck, ")");
-  }
-}
-
-std::vector<bool> GoOrGrowHip::mask(int id) const {
-  const std::vector<double> m = field(id);
-  std::vector<bool> out(m.size());
-  for (size_t k = 0; k < m.size(); ++k) out[k] = m[k] != 0;
-  return out;
 }
 
 option_list GoOrGrowHip::GetOptions() {
-  option_list options = LyotropicHip::GetOptions();
-  OptionGroup& model = options[0];
-  // lyotropic-with-division.cpp:128-148
-  model.add_double("alpha", &alpha, "division strength")
-      .add_double("beta", &beta, "death strength")
-      .add_double("division-rate", &division_rate, "division rate")
-      .add_double("division-time", &division_time, "division life time")
-      .add_double("division-radius", &division_radius, "division radius")
-      .add_double("death-rate", &death_rate, "death rate")
-      .add_double("death-time", &death_time, "death life time")
-      .add_double("death-radius", &death_radius, "death radius")
-      .add_double("phi-critical", &phi_critical, "phi critical");
+  option_list options = LyotropicWithDivisionHip::GetOptions();
   // go-or-grow.cpp:387-423
-  model.add_double("B", &B, "crowding/compressibility penalty strength")
+  options[0]
+      .add_double("B", &B, "crowding/compressibility penalty strength")
       .add_double("Snem", &Snem, "preferred value of one half Tr(Q^2)")
       .add_double("Dchi", &Dchi, "phenotype diffusion coefficient")
       .add_double("Achi", &Achi, "phenotype switching double-well barrier")

@@ -7,6 +7,7 @@
 #include "error_msg.hpp"
 #include "go_or_grow_hip.hpp"
 #include "lyotropic_hip.hpp"
+#include "lyotropic_with_division_hip.hpp"
 
 namespace {
 struct Entry {
@@ -47,6 +48,11 @@ void DeclareModels() {
       "1/2 free-/no-slip channel, 3/4 free-/no-slip box (fp64). precision = fp64 reproduces "
       "the reference arithmetic bit for bit; precision = fp32 (bc = 0) stores shifted "
       "populations and a compensated phase field.");
+  declare_model<LyotropicWithDivisionHip>(
+      "lyotropic-with-division",
+      "The lyotropic model with cell division and death: random circular patches where phi "
+      "grows (alpha) or decays (beta), redrawn every division-time / death-time steps; on an "
+      "MI355X GPU in fp64, bit for bit the reference arithmetic.");
   declare_model<GoOrGrowHip>(
       "go-or-grow",
       "Lyotropic with cell division and death and a go/grow phenotype field (chi = m/phi), "

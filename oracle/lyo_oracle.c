@@ -566,7 +566,7 @@ void lyo_oracle_masks(const lyo_oracle* o, uint8_t* division, uint8_t* death) {
 /* go-or-grow.cpp:365-385: Lyotropic's field conditions, then m and chi */
 static void growth_boundary_fields(lyo_oracle* o) {
   lyo_oracle_boundary(o, 0);
-  if (o->growth && o->p.bc) {
+  if (o->growth && !o->g.division_only && o->p.bc) {
     s_apply(o, F(LYO_O_M), 0, 0., o->p.bc <= 2);
     s_apply(o, F(LYO_O_CHI), 0, 0., o->p.bc <= 2);
   }
@@ -708,6 +708,7 @@ static void configure_phenotype(lyo_oracle* o) {
 }
 
 static void growth_configure(lyo_oracle* o) {
+  if (o->g.division_only) return; /* LyotropicWithDivision keeps Lyotropic's Configure */
   relax_free_energy(o);
   reset_hydrodynamics(o);
   configure_phenotype(o);
@@ -899,13 +900,13 @@ void lyo_oracle_update_quantities(lyo_oracle* o) {
 #pragma omp parallel for reduction(+ : sum) num_threads(nt) if (nt)
   for (unsigned k = 0; k < N; ++k) {
     sum = sum + F(LYO_O_PHI)[k];
-    if (o->growth)
+    if (o->growth && !o->g.division_only)
       growth_uq_at_node(o, k);
     else
       update_quantities_at_node(o, k);
   }
   o->countphi = sum;
-  if (o->growth) update_phenotype(o); /* go-or-grow.cpp:305-320 */
+  if (o->growth && !o->g.division_only) update_phenotype(o); /* go-or-grow.cpp:305-320 */
 }
 
 /* src/models/lyotropic.cpp:362-369 */
@@ -913,7 +914,16 @@ void lyo_oracle_update_fields(lyo_oracle* o, int first) {
   const int nt = o->p.nthreads;
   const unsigned N = o->N;
   (void)nt;
-  if (o->growth) {
+  if (o->growth && o->g.division_only) {
+    /* lyotropic-with-division.cpp:110-126 */
+#pragma omp parallel for num_threads(nt) if (nt)
+    for (unsigned k = 0; k < N; ++k) {
+      const double division_m = o->div_mask[k] ? 1. : 0.;
+      const double death_m = o->death_mask[k] ? 1. : 0.;
+      update_fields_at_node(o, k, first);
+      o->phi_tmp[k] += F(LYO_O_PHI)[k] * (o->g.alpha * division_m - o->g.beta * death_m);
+    }
+  } else if (o->growth) {
 #pragma omp parallel for num_threads(nt) if (nt)
     for (unsigned k = 0; k < N; ++k) growth_uf_at_node(o, k, first);
   } else {
@@ -923,7 +933,7 @@ void lyo_oracle_update_fields(lyo_oracle* o, int first) {
   double* t = o->s[LYO_O_PHI];
   o->s[LYO_O_PHI] = o->phi_tmp;
   o->phi_tmp = t;
-  if (o->growth) { /* go-or-grow.cpp:358-362 */
+  if (o->growth && !o->g.division_only) { /* go-or-grow.cpp:358-362 */
     t = o->s[LYO_O_M];
     o->s[LYO_O_M] = o->m_tmp;
     o->m_tmp = t;

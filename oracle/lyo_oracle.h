@@ -66,7 +66,9 @@ typedef struct lyo_oracle_growth {
   int32_t chi_config;   /* 0 noise, 1 front */
   int32_t growTogether, relax_phi, relax_Q;
   uint32_t relax_steps;
-  int32_t pad_;
+  int32_t division_only; /* 1: lyotropic-with-division (Lyotropic + the masks'
+                            phi growth, lyotropic-with-division.cpp:110-126);
+                            only alpha ... phi_critical are used */
   double B, Snem, Dchi, Achi, Ochi, phiswitch;
   double chi0, chi_noise, chi_length, relax_dt;
   double alpha, beta, division_rate, division_time, division_radius;

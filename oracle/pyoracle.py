@@ -41,7 +41,8 @@ GROWTH_DOUBLES = ["B", "Snem", "Dchi", "Achi", "Ochi", "phiswitch", "chi0", "chi
                   "chi_length", "relax_dt", "alpha", "beta", "division_rate", "division_time",
                   "division_radius", "death_rate", "death_time", "death_radius", "phi_critical"]
 # go-or-grow defaults (go-or-grow.hpp:10-28)
-GROWTH_DEFAULTS = dict(chi_config="noise", growTogether=0, relax_phi=0, relax_Q=0, relax_steps=0,
+GROWTH_DEFAULTS = dict(division_only=0, chi_config="noise", growTogether=0, relax_phi=0,
+                       relax_Q=0, relax_steps=0,
                        B=0.0, Snem=1.0, Dchi=0.0, Achi=0.0, Ochi=0.0, phiswitch=0.0, chi0=0.0,
                        chi_noise=0.0, chi_length=0.0, relax_dt=1.0)
 
@@ -49,7 +50,7 @@ GROWTH_DEFAULTS = dict(chi_config="noise", growTogether=0, relax_phi=0, relax_Q=
 class _Growth(ctypes.Structure):
     _fields_ = ([("chi_config", ctypes.c_int32), ("growTogether", ctypes.c_int32),
                  ("relax_phi", ctypes.c_int32), ("relax_Q", ctypes.c_int32),
-                 ("relax_steps", ctypes.c_uint32), ("pad_", ctypes.c_int32)]
+                 ("relax_steps", ctypes.c_uint32), ("division_only", ctypes.c_int32)]
                 + [(k, ctypes.c_double) for k in GROWTH_DOUBLES])
 
 
@@ -185,7 +186,8 @@ class Oracle:
                         [x.value for x in v]))
 
     def set_growth(self, **kw):
-        """Makes this a go-or-grow run (before configure).  Keys: those of
+        """Makes this a go-or-grow run (before configure; division_only=1: a
+        lyotropic-with-division run).  Keys: those of
         GROWTH_DEFAULTS (chi_config "noise" or "front") and alpha, beta,
         division_rate/time/radius, death_rate/time/radius, phi_critical."""
         g = dict(GROWTH_DEFAULTS)
@@ -193,7 +195,7 @@ class Oracle:
         cfg = {"noise": 0, "front": 1}[g["chi_config"]]
         s = _Growth(chi_config=cfg, growTogether=int(g["growTogether"]),
                     relax_phi=int(g["relax_phi"]), relax_Q=int(g["relax_Q"]),
-                    relax_steps=int(g["relax_steps"]), pad_=0,
+                    relax_steps=int(g["relax_steps"]), division_only=int(g["division_only"]),
                     **{k: float(g[k]) for k in GROWTH_DOUBLES})
         if self.L.lyo_oracle_set_growth(self.o, ctypes.byref(s)) != 0:
             raise ValueError("oracle: bad growth parameters")

@@ -136,3 +136,21 @@ def test_growth_refusals(require_gpu):
             call()
     a.step(2)
     assert np.isfinite(a.get("m")).all()
+
+
+@pytest.mark.parametrize("bc", [0, 2])
+@pytest.mark.parametrize("npc", [1, 2])
+def test_division_only_bit_exact(require_gpu, bc, npc):
+    # lyotropic-with-division: Lyotropic with the masks' phi growth and death
+    div = dict(division_only=1, alpha=0.002, beta=0.001, division_rate=0.1, division_time=2.0,
+               division_radius=3.0, death_rate=0.05, death_time=1.0, death_radius=2.0,
+               phi_critical=1.2)
+    a, o, ta, to = pair(40, 36, bc, npc, phys=dict(PHYS, zeta=0.03), growth=div)
+    assert ta == to
+    for s in range(3):
+        a.step(3)
+        o.step(3)
+        check(a, o, ["ff", "QQxx", "QQyx", "phi"] + DERIVED, "after %d steps" % (3 * s + 3))
+    assert a.get("division_mask").any() and a.get("death_mask").any()
+    with pytest.raises(lyo.MassLyoError):
+        a.get("m")  # no phenotype in this model

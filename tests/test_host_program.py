@@ -140,6 +140,14 @@ def test_go_or_grow_options():
     assert "go-or-grow" in out
 
 
+def test_division_model_options():
+    rc, out, _ = run(["-h", "-m", "lyotropic-with-division"])
+    assert rc == 0
+    for key in ("division-rate", "death-radius", "phi-critical", "GammaQ"):
+        assert key in out
+    assert "chi-config" not in out
+
+
 def test_go_or_grow_bad_values(card):
     rc, _, err = run([card(GROWTH_RUNCARD.replace("relax-phi   = 1", "relax-phi   = x")),
                       "--no-write"])
@@ -318,6 +326,44 @@ def test_host_go_or_grow_run_matches_oracle(card, tmp_path, require_gpu):
     div, death = o.masks()
     assert np.array_equal(np.array(f["division_mask"]["value"]) != 0, div != 0)
     assert not np.any(f["death_mask"]["value"])
+
+
+@pytest.mark.gpu
+def test_host_division_run_matches_oracle(card, tmp_path, require_gpu):
+    """model = lyotropic-with-division: frames with the masks equal the
+    oracle's to the archive's six digits."""
+    from oracle import pyoracle as po
+    text = RUNCARD + """model = lyotropic-with-division
+alpha = 0.002
+beta = 0.001
+division-rate = 0.2
+division-time = 2
+division-radius = 2
+death-rate = 0.05
+death-time = 1
+death-radius = 1
+phi-critical = 1.2
+"""
+    outdir = tmp_path / "out"
+    rc, out, err = run([card(text.replace("model = lyotropic\n", "")), "-o", str(outdir), "-f"])
+    assert rc == 0, err
+    params = json.load(open(outdir / "parameters.json"))["data"]
+    assert params["model_name"]["value"] == "lyotropic-with-division"
+    o = po.Oracle(32, 32, npc=1, GammaQ=0.1, GammaP=0.1, xi=0.7, tauNem=2.0, tauIso=2.0,
+                  rho=40.0, friction=0.0, AA=0.01, CC=0.0, LL=0.04, KK=0.04, zeta=0.02,
+                  zetaI=0.0, conc=0.5)
+    o.set_growth(division_only=1, alpha=0.002, beta=0.001, division_rate=0.2, division_time=2.0,
+                 division_radius=2.0, death_rate=0.05, death_time=1.0, death_radius=1.0,
+                 phi_critical=1.2, chi0=0.0)
+    o.configure(config="random", seed=1, angle=90, noise=1, qamp=0.5, phinoise=0.05)
+    o.step(20)
+    f = json.load(open(outdir / "frame20.json"))["data"]
+    for k in ("QQxx", "QQyx", "phi", "sigmaXX"):
+        assert np.allclose(f[k]["value"], o.get(k), rtol=1e-5, atol=1e-6), k
+    div, death = o.masks()
+    assert np.array_equal(np.array(f["division_mask"]["value"]) != 0, div != 0)
+    assert np.array_equal(np.array(f["death_mask"]["value"]) != 0, death != 0)
+    assert "m" not in f
 
 
 @pytest.mark.gpu

@@ -258,3 +258,33 @@ def test_phenotype_front():
     # with the C library's tanh (math.tanh) bit for bit
     want = [.5 * (1. - math.tanh((k // 6 - 14.5) / 3.0)) for k in range(30 * 6)]
     assert np.array_equal(o.get("chi"), np.array(want))
+
+
+def test_division_only_is_lyotropic_without_patches():
+    # lyotropic-with-division (lyotropic-with-division.cpp:110-126) adds
+    # phi*(alpha*division - beta*death) to Lyotropic's phi update: with no
+    # patch drawn it is Lyotropic bit for bit
+    from tests.conftest import CONFIG0_INIT, CONFIG0_PHYS
+    a = po.Oracle(24, 20, npc=2, **CONFIG0_PHYS)
+    a.set_growth(**dict(NO_GROWTH, division_only=1, alpha=1e-3, division_rate=1e-300,
+                        division_time=2.0, division_radius=2.0))
+    a.configure(**CONFIG0_INIT)
+    b = po.Oracle(24, 20, npc=2, **CONFIG0_PHYS)
+    b.configure(**CONFIG0_INIT)
+    a.step(15)
+    b.step(15)
+    for n in ("ff", "QQxx", "QQyx", "phi", "sigmaXY", "MU"):
+        assert np.array_equal(a.get(n), b.get(n)), n
+    assert not a.masks()[0].any()
+
+
+def test_division_only_growth_recurrence():
+    # every node divides: each sweep adds alpha*phi of the phi it reads, so
+    # one npc = 1 step maps phi to phi (1 + alpha + alpha^2)
+    a = 0.01
+    o = po.Oracle(6, 6, npc=1, **PHYS)
+    o.set_growth(**dict(NO_GROWTH, division_only=1, alpha=a, division_rate=2.0,
+                        division_time=0.0, division_radius=0.0, phi_critical=1e9))
+    o.configure(config="random", seed=1, qamp=0.0, phinoise=0.0, noise=0.0)
+    o.step(5)
+    assert np.allclose(o.get("phi"), (1 + a + a * a) ** 5, rtol=1e-14, atol=0)
