@@ -154,11 +154,27 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warm
             "train_loss": [round(l0, 4), round(l1, 5)]}
 
 
-def cpu_baseline(L, steps, threads):
+def cpu_baseline(L, steps, threads, bc=0):
     from oracle import pyoracle as po
-    o = po.Oracle(L, L, npc=2, nthreads=threads, **PHYS)
+    o = po.Oracle(L, L, npc=2, nthreads=threads, bc=bc, **PHYS)
     o.configure(**INIT)
     o.step(1)  # warm-up (first touch of the work arrays)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return L * L * steps / dt / 1e6
+
+
+def cpu_growth(steps, threads, relax_steps=200, L=500):
+    """The oracle on the go-or-grow runcard (sample: 200 instead of 2,000
+    relaxation steps in configure, then `steps` timed steps; the mask draws
+    are serial as in the reference)."""
+    from oracle import pyoracle as po
+    o = po.Oracle(L, L, npc=1, nthreads=threads, **GROWTH_PHYS)
+    o.set_growth(**dict(GROWTH_PARAMS, relax_steps=relax_steps))
+    o.configure(**GROWTH_INIT)
+    o.step(1)
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
@@ -326,6 +342,20 @@ def main():
                "sample": "%dx%d FP64 npc=2, %d steps after 1 warm-up step, OpenMP" %
                          (L, L, args.cpu_steps)}
         log("cpu: %.2f MLUPS" % v)
+        # the reference's CPU path beside the walls and go-or-grow legs
+        if "fp64_walls_bc4_mlups" in extra:
+            vw = cpu_baseline(L, args.cpu_steps, threads, bc=4)
+            extra["cpu_walls_bc4"] = {"value": round(vw, 2), "unit": "MLUPS", "cores": threads,
+                                      "kind": "port", "sample": "%dx%d FP64 bc=4 npc=2, %d steps "
+                                      "after 1 warm-up step, OpenMP" % (L, L, args.cpu_steps)}
+            log("cpu walls: %.2f MLUPS" % vw)
+        if "go_or_grow" in extra:
+            vg = cpu_growth(20, threads)
+            extra["cpu_go_or_grow"] = {"value": round(vg, 2), "unit": "MLUPS", "cores": threads,
+                                       "kind": "port", "sample": "500x500 go-or-grow runcard with "
+                                       "200 relaxation steps, 20 steps after 1 warm-up step, "
+                                       "OpenMP loops, serial mask draws"}
+            log("cpu go-or-grow: %.2f MLUPS" % vg)
 
     line = {
         "metric": "MLUPS at %d^2 (FP32 plain, npc=%d)" % (L, npc),
