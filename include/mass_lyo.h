@@ -16,7 +16,9 @@
  *    crosses the boundary.  mass_lyo_last_error() gives the message.
  *  - One handle is used by one host thread.  Device work is queued on the
  *    handle's own HIP stream; calls that return data synchronise it.
- *  - Only periodic boundaries (bc=0) exist on the device path.
+ *  - Boundaries: bc 0 periodic on every path; bc 1-4 (channels and boxes on
+ *    a ghost layer) in FP64 on the reference's loop structure.  Node index
+ *    and layout at the boundary are those of the domain either way.
  */
 #ifndef MASS_LYO_H_
 #define MASS_LYO_H_
