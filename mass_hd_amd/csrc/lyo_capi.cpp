@@ -638,9 +638,8 @@ int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on) {
   if (on != 0 && on != 1) return fail(h, MASS_LYO_EINVAL, "on must be 0 or 1");
   if (on && h->prm.precision != MASS_LYO_FP64)
     return fail(h, MASS_LYO_EINVAL, "the loop kernels run the FP64 path only");
-  if (!on && (h->prm.bc != 0 || h->growth))
-    return fail(h, MASS_LYO_EINVAL,
-                "walled grids (bc 1-4) and go-or-grow run the loop kernels only");
+  if (!on && h->growth)
+    return fail(h, MASS_LYO_EINVAL, "go-or-grow runs the loop kernels only");
   if (on) {
     const hipError_t e = h->eng->alloc_loops();
     if (e != hipSuccess) return hip_fail(h, e, "loop kernel buffers");

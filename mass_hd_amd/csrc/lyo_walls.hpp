@@ -274,6 +274,230 @@ __global__ void __launch_bounds__(256) wall_lb_kernel(WallGeo g, int phase, T* f
   lb_ghost(g, fn, ph, p, id);
 }
 
+// One sweep of the walled step in one kernel: UpdateQuantities,
+// BoundaryConditionsFields2 and UpdateFields (lyotropic.cpp:168-341,451-498)
+// for a tile of domain nodes.  The block stages Q and phi of the tile
+// plus two nodes of halo in LDS (ghost nodes take their field condition --
+// the adjacent domain node, or the periodic image across a channel's open
+// ends -- as they are read), runs UpdateQuantities on the tile plus one node
+// of halo into LDS, fills the walled halo nodes' derived values by the
+// fields2 rules from the two nodes inside, and runs UpdateFields on the tile.
+// The same device functions as the node loops, so results are bitwise theirs.
+// Inputs and outputs are distinct buffers (a tile reads its neighbours'
+// state); the diagonal halo nodes are never read by the 5-point stencils.
+struct WallSweep {
+  const double* f;     // f (predictor) or streamed f (corrector)
+  const double* qxx;
+  const double* qyx;
+  const double* phi;
+  const double* fn;    // corrector: streamed fn
+  const double* qnxx;  // corrector
+  const double* qnyx;  // corrector
+  const double* phn;   // corrector
+  double* f_out;
+  double* fn_out;      // predictor
+  double* qxx_out;
+  double* qyx_out;
+  double* qnxx_out;    // predictor
+  double* qnyx_out;    // predictor
+  double* phi_out;
+  double* phn_out;     // predictor
+  double* D;           // derive: the 23 derived planes of the tile's nodes
+};
+
+// Tile of domain nodes per block: 6 rows (x) by 62 columns (y).  The block's
+// 512 threads are the 8 x 64 nodes of the tile plus one ring, one wave per
+// row (contiguous in y), and each runs UpdateQuantities on its node; the
+// tile's threads keep their own node's values in registers for UpdateFields.
+constexpr int kWallTX = 6, kWallTY = 62, kWallThreads = (kWallTX + 2) * (kWallTY + 2);
+
+// Domain coordinate a stencil reads for coordinate v of an axis of length L
+// (periodic: wrapped; walled: the ghost at -1 / L reads its Neumann source
+// 0 / L-1), or -1 if no stencil reads v.
+__device__ __forceinline__ int wall_axis(int v, int L, bool periodic) {
+  if (v >= 0 && v < L) return v;
+  if (periodic) return ((v % L) + L) % L;
+  if (v == -1) return 0;
+  if (v == L) return L - 1;
+  return -1;
+}
+
+enum WallSweepMode { kSweepPredictor = 0, kSweepCorrector = 1, kSweepDerive = 2 };
+
+// MODE kSweepDerive runs UpdateQuantities only and stores the 23 derived
+// planes of the tile (the frame fields of the last UpdateQuantities,
+// recomputed from its inputs when a frame asks for them).  PULL: f and fn are
+// the predictor's post-collision populations with the LB conditions applied,
+// and Move (lyotropic.cpp:371-386) happens as they are read -- node p takes
+// direction v from p - c_v on the torus; the streamed fn of the tile goes to
+// fn_out for the later correctors.
+template <int MODE, bool PULL = false>
+__global__ void __launch_bounds__(kWallThreads) wall_sweep_kernel(WallSweep a, Params<double> P,
+                                                                  WallGeo g) {
+  using T = double;
+  constexpr int TX = kWallTX, TY = kWallTY;
+  constexpr int PW = TY + 4, PN = (TX + 4) * PW;  // staged Q, phi: tile + 2
+  constexpr int DW = TY + 2, DN = (TX + 2) * DW;  // derived: tile + 1 (= threads)
+  __shared__ T sq[3][PN];
+  __shared__ T sd[9][DN];  // ux uy uxp uyp mu sxx syy sxy syx
+  const int x0 = blockIdx.y * TX, y0 = blockIdx.x * TY;
+  const int tid = threadIdx.x;
+  const bool perx = g.bc <= 2;  // channels are periodic in x
+  const int NP = g.NP;
+  auto node = [&](int x, int y) { return (y + 1) + g.W * (x + 1); };
+
+  // Q and phi of the tile + 2 (BoundaryConditionsFields, lyotropic.cpp:422-449)
+  for (int i = tid; i < PN; i += kWallThreads) {
+    const int r = i / PW, c = i - r * PW;
+    const int x = wall_axis(x0 - 2 + r, g.LX, perx);
+    const int y = wall_axis(y0 - 2 + c, g.LY, false);
+    T q0 = T(0), q1 = T(0), ph = T(0);
+    if (x >= 0 && y >= 0) {
+      const int k = node(x, y);
+      q0 = a.qxx[k];
+      q1 = a.qyx[k];
+      ph = a.phi[k];
+    }
+    sq[0][i] = q0;
+    sq[1][i] = q1;
+    sq[2][i] = ph;
+  }
+  __syncthreads();
+
+  // this thread's node of the tile + 1
+  const int r = tid / DW, c = tid - r * DW;
+  const int i = tid;
+  const bool tile = r >= 1 && r <= TX && c >= 1 && c <= TY;
+  const int xv = x0 - 1 + r, yv = y0 - 1 + c;
+
+  // UpdateQuantities (lyotropic.cpp:168-247) on the domain nodes of the tile + 1
+  CenterVals<T> cv{};
+  T fl[9];  // this node's populations, kept for UpdateFields
+  {
+    const bool xin = (xv >= 0 && xv < g.LX) || (perx && (xv == -1 || xv == g.LX));
+    const bool yin = yv >= 0 && yv < g.LY;
+    T vals[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (xin && yin) {
+      const int k = node(wall_axis(xv, g.LX, perx), yv);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) fl[v] = a.f[v * NP + (PULL ? wnext(g, k, wopp(v)) : k)];
+      const int q = (r + 1) * PW + (c + 1);
+      NbrVals<T> nv;
+      DiagVals<T> dg;
+      update_quantities<T, true, true>(
+          fl, sq[0][q], sq[0][q + PW], sq[0][q - PW], sq[0][q + 1], sq[0][q - 1], sq[1][q],
+          sq[1][q + PW], sq[1][q - PW], sq[1][q + 1], sq[1][q - 1], sq[2][q], sq[2][q + PW],
+          sq[2][q - PW], sq[2][q + 1], sq[2][q - 1], P, nv, cv, &dg);
+      vals[0] = nv.ux;
+      vals[1] = nv.uy;
+      vals[2] = nv.uxp;
+      vals[3] = nv.uyp;
+      vals[4] = nv.mu;
+      vals[5] = nv.sxx;
+      vals[6] = nv.syy;
+      vals[7] = nv.sxy;
+      vals[8] = nv.syx;
+      if constexpr (MODE == kSweepDerive) {
+        if (tile && xv < g.LX) {
+          const T out[kLoopPlanes] = {cv.nn,     nv.ux,     nv.uy,     nv.uxp,    nv.uyp,
+                                      cv.Hxx,    cv.Hyx,    nv.mu,     cv.dxQxx,  cv.dxQyx,
+                                      cv.dyQxx,  cv.dyQyx,  nv.sxx,    nv.syy,    nv.sxy,
+                                      nv.syx,    dg.bulk,   dg.el_xx,  dg.el_yx,  dg.pf_xx,
+                                      dg.pf_yx,  dg.act_xx, dg.act_yx};
+#pragma unroll
+          for (int pl = 0; pl < kLoopPlanes; ++pl) a.D[(size_t)pl * NP + k] = out[pl];
+        }
+      }
+    }
+#pragma unroll
+    for (int pl = 0; pl < 9; ++pl) sd[pl][i] = vals[pl];
+  }
+  if constexpr (MODE == kSweepDerive) return;
+  __syncthreads();
+
+  // BoundaryConditionsFields2 at the walled nodes of the tile + 1 (the ring,
+  // or past the domain's end inside a partial tile; fields.hpp:704-768): copy
+  // derivative on u, Dirichlet 0 on u*phi, Neumann on MU and sigma.  Ghosts
+  // are written and only domain nodes read.
+  {
+    const bool xdom = xv >= 0 && xv < g.LX, ydom = yv >= 0 && yv < g.LY;
+    int d = 0;  // offset towards the domain
+    if (xdom && (yv == -1 || yv == g.LY))
+      d = yv == -1 ? 1 : -1;
+    else if (!perx && ydom && (xv == -1 || xv == g.LX))
+      d = xv == -1 ? DW : -DW;
+    if (d != 0) {
+      const int j1 = i + d, j2 = i + 2 * d;
+      sd[0][i] = T(2) * sd[0][j1] - sd[0][j2];
+      sd[1][i] = T(2) * sd[1][j1] - sd[1][j2];
+      sd[2][i] = T(2) * T(0) - sd[2][j1];
+      sd[3][i] = T(2) * T(0) - sd[3][j1];
+#pragma unroll
+      for (int pl = 4; pl < 9; ++pl) sd[pl][i] = sd[pl][j1];
+    }
+  }
+  __syncthreads();
+
+  // UpdateFields (lyotropic.cpp:249-341) on the tile
+  if (!tile || xv >= g.LX || yv >= g.LY) return;
+  const int k = node(xv, yv);
+  const int p = i;
+  cv.dn = T(0);
+  UFStencil<T> st;
+  st.mu_c = sd[4][p];
+  st.ux_xp = sd[0][p + DW];
+  st.ux_xm = sd[0][p - DW];
+  st.uy_xp = sd[1][p + DW];
+  st.uy_xm = sd[1][p - DW];
+  st.uxp_xp = sd[2][p + DW];
+  st.uxp_xm = sd[2][p - DW];
+  st.sxx_xp = sd[5][p + DW];
+  st.sxx_xm = sd[5][p - DW];
+  st.syx_xp = sd[8][p + DW];
+  st.syx_xm = sd[8][p - DW];
+  st.mu_xp = sd[4][p + DW];
+  st.mu_xm = sd[4][p - DW];
+  st.ux_yp = sd[0][p + 1];
+  st.ux_ym = sd[0][p - 1];
+  st.uy_yp = sd[1][p + 1];
+  st.uy_ym = sd[1][p - 1];
+  st.uyp_yp = sd[3][p + 1];
+  st.uyp_ym = sd[3][p - 1];
+  st.sxy_yp = sd[7][p + 1];
+  st.sxy_ym = sd[7][p - 1];
+  st.syy_yp = sd[6][p + 1];
+  st.syy_ym = sd[6][p - 1];
+  st.mu_yp = sd[4][p + 1];
+  st.mu_ym = sd[4][p - 1];
+  const int q = (r + 1) * PW + (c + 1);
+  const T Qxx = sq[0][q], Qyx = sq[1][q], ph = sq[2][q];
+  T Dxx, Dyx, Dp, om[9];
+  update_fields<T, true>(cv, Qxx, Qyx, ph, fl, st, P, Dxx, Dyx, Dp, om);
+  if constexpr (MODE == kSweepPredictor) {
+    a.qnxx_out[k] = Qxx + T(.5) * Dxx;
+    a.qnyx_out[k] = Qyx + T(.5) * Dyx;
+    a.phn_out[k] = ph + T(.5) * Dp;
+    a.qxx_out[k] = Qxx + Dxx;
+    a.qyx_out[k] = Qyx + Dyx;
+    a.phi_out[k] = ph + Dp;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      a.fn_out[v * NP + k] = fl[v] + T(.5) * om[v];
+      a.f_out[v * NP + k] = fl[v] + om[v];
+    }
+  } else {
+    a.qxx_out[k] = a.qnxx[k] + T(.5) * Dxx;
+    a.qyx_out[k] = a.qnyx[k] + T(.5) * Dyx;
+    a.phi_out[k] = a.phn[k] + T(.5) * Dp;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      const T fnv = a.fn[v * NP + (PULL ? wnext(g, k, wopp(v)) : k)];
+      if constexpr (PULL) a.fn_out[v * NP + k] = fnv;
+      a.f_out[v * NP + k] = fnv + T(.5) * om[v];
+    }
+  }
+}
+
 // Move over every node of the torus (lyotropic.cpp:371-386), as a pull:
 // node p takes direction v from p - c_v, the node whose push lands on p.
 template <class T>
@@ -406,6 +630,8 @@ class WallEngine final : public Engine {
     if (e == hipSuccess) e = get_(n1, &qnxx_);
     if (e == hipSuccess) e = get_(n1, &qnyx_);
     if (e == hipSuccess) e = get_(n1, &phn_);
+    if (e == hipSuccess) e = get_(n1, &qxx2_);
+    if (e == hipSuccess) e = get_(n1, &qyx2_);
     if (e == hipSuccess) e = get_((size_t)kLoopPlanes * n1, &D_);
     return e;
   }
@@ -417,7 +643,7 @@ class WallEngine final : public Engine {
     sizes_.clear();
     bytes_ = 0;
     F_[0] = F_[1] = Fn_[0] = Fn_[1] = phi_[0] = phi_[1] = nullptr;
-    qxx_ = qyx_ = qnxx_ = qnyx_ = phn_ = D_ = nullptr;
+    qxx_ = qyx_ = qnxx_ = qnyx_ = phn_ = D_ = qxx2_ = qyx2_ = nullptr;
     m_[0] = m_[1] = mn_ = chi_ = part_ = stage_ = fe_ = nullptr;
     if (div_) (void)hipFree(div_);
     if (death_) (void)hipFree(death_);
@@ -453,8 +679,8 @@ class WallEngine final : public Engine {
   // parameters or the state change): a step is ~25 launches, which at small
   // grids take longer to issue than to run.  The graph holds kernels only.
   hipError_t steps(int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) override {
-    (void)L;
     (void)tm;
+    loops_ = growth_ || L.loops != 0;
     for (int st = 0; st < nsteps; ++st) {
       if (growth_) {
         const hipError_t e = set_masks(s);
@@ -462,11 +688,100 @@ class WallEngine final : public Engine {
       }
       const hipError_t e = graph_step(s);
       if (e != hipSuccess) return e;
+      d_stale_ = !loops_;  // the node loops store the derived planes as they go
     }
     return hipSuccess;
   }
 
+  // the walled step as three fused sweeps (wall_sweep_kernel) with the LB
+  // conditions and Move between the predictor and the correctors
+  hipError_t fused_step(hipStream_t s) {
+    const dim3 grid = sweep_grid();
+    const int bs = 256, nbt = (TP_ + bs - 1) / bs;
+    WallSweep a{};
+    a.f = F_[cur_];
+    a.qxx = qxx_;
+    a.qyx = qyx_;
+    a.phi = phi_[pc_];
+    a.f_out = F_[cur_ ^ 1];
+    a.fn_out = Fn_[cur_ ^ 1];
+    a.qxx_out = qxx2_;
+    a.qyx_out = qyx2_;
+    a.qnxx_out = qnxx_;
+    a.qnyx_out = qnyx_;
+    a.phi_out = phi_[pc_ ^ 1];
+    a.phn_out = phn_;
+    hipLaunchKernelGGL((wall_sweep_kernel<kSweepPredictor>), grid, dim3(kWallThreads), 0, s, a,
+                       P_, geo());
+    swap_q();
+    pc_ ^= 1;
+    // the LB conditions on the post-collision populations; the first
+    // corrector streams them as it reads (writing its f to F[cur], which no
+    // block pulls from), the later ones read plain planes
+    for (int ph : phases())
+      hipLaunchKernelGGL((wall_lb_kernel<T>), dim3(phase_blocks(ph)), dim3(bs), 0, s, geo(), ph,
+                         F_[cur_ ^ 1], Fn_[cur_ ^ 1]);
+    int fc = cur_ ^ 1;  // the corrector's f input
+    for (int c = 1; c <= npc_; ++c) {
+      WallSweep b{};
+      b.f = F_[fc];
+      b.qxx = qxx_;
+      b.qyx = qyx_;
+      b.phi = phi_[pc_];
+      b.fn = c == 1 ? Fn_[cur_ ^ 1] : Fn_[cur_];
+      b.fn_out = Fn_[cur_];
+      b.qnxx = qnxx_;
+      b.qnyx = qnyx_;
+      b.phn = phn_;
+      b.f_out = F_[fc ^ 1];
+      b.qxx_out = qxx2_;
+      b.qyx_out = qyx2_;
+      b.phi_out = phi_[pc_ ^ 1];
+      if (c == 1)
+        hipLaunchKernelGGL((wall_sweep_kernel<kSweepCorrector, true>), grid, dim3(kWallThreads), 0,
+                           s, b, P_, geo());
+      else
+        hipLaunchKernelGGL((wall_sweep_kernel<kSweepCorrector, false>), grid, dim3(kWallThreads),
+                           0, s, b, P_, geo());
+      swap_q();
+      pc_ ^= 1;
+      fc ^= 1;
+    }
+    cur_ = fc;
+    (void)nbt;
+    d_stale_ = true;  // the frame fields are recomputed from the last corrector's inputs
+    return hipGetLastError();
+  }
+  dim3 sweep_grid() const {
+    return dim3((unsigned)((LY_ + kWallTY - 1) / kWallTY), (unsigned)((LX_ + kWallTX - 1) / kWallTX));
+  }
+  // The derived planes of the last UpdateQuantities of a fused step, from its
+  // inputs (f, Q and phi before the last corrector, left intact by it)
+  hipError_t materialize(hipStream_t s) {
+    if (!d_stale_) return hipSuccess;
+    WallSweep a{};
+    a.f = F_[cur_ ^ 1];  // with npc = 1: the post-collision planes, streamed as read
+    a.qxx = qxx2_;
+    a.qyx = qyx2_;
+    a.phi = phi_[pc_ ^ 1];
+    a.D = D_;
+    if (npc_ == 1)
+      hipLaunchKernelGGL((wall_sweep_kernel<kSweepDerive, true>), sweep_grid(), dim3(kWallThreads),
+                         0, s, a, P_, geo());
+    else
+      hipLaunchKernelGGL((wall_sweep_kernel<kSweepDerive, false>), sweep_grid(),
+                         dim3(kWallThreads), 0, s, a, P_, geo());
+    d_stale_ = false;
+    return hipGetLastError();
+  }
+  void swap_q() {
+    std::swap(qxx_, qxx2_);
+    std::swap(qyx_, qyx2_);
+    qa_ ^= 1;
+  }
+
   hipError_t step_body(hipStream_t s) {
+    if (!loops_) return fused_step(s);
     const int bs = 256, nbt = (TP_ + bs - 1) / bs;
     {
       fields_bc(s);
@@ -491,15 +806,16 @@ class WallEngine final : public Engine {
 
   hipError_t graph_step(hipStream_t s) {
     for (const StepGraph& g : graphs_) {
-      if (g.cur == cur_ && g.pc == pc_ && g.mc == mc_) {
+      if (g.cur == cur_ && g.pc == pc_ && g.mc == mc_ && g.qa == qa_ && g.loops == loops_) {
         cur_ = g.cur_after;
         pc_ = g.pc_after;
         mc_ = g.mc_after;
+        if (g.qa_after != qa_) swap_q();
         return hipGraphLaunch(g.exec, s);
       }
     }
-    StepGraph g{cur_, pc_, mc_, 0, 0, 0, nullptr};
-    const int cur0 = cur_, pc0 = pc_, mc0 = mc_;
+    StepGraph g{cur_, pc_, mc_, qa_, loops_, 0, 0, 0, 0, nullptr};
+    const int cur0 = cur_, pc0 = pc_, mc0 = mc_, qa0 = qa_;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
@@ -512,11 +828,13 @@ class WallEngine final : public Engine {
       cur_ = cur0;
       pc_ = pc0;
       mc_ = mc0;
+      if (qa_ != qa0) swap_q();
       return e;
     }
     g.cur_after = cur_;
     g.pc_after = pc_;
     g.mc_after = mc_;
+    g.qa_after = qa_;
     graphs_.push_back(g);
     return hipGraphLaunch(g.exec, s);
   }
@@ -531,6 +849,7 @@ class WallEngine final : public Engine {
     cur_ = 0;
     pc_ = 0;
     mc_ = 0;
+    d_stale_ = false;
     const int N = LX_ * LY_, bs = 256;
     hipLaunchKernelGGL((wall_pack_kernel<T>), dim3((N + bs - 1) / bs), dim3(bs), 0, s, geo(), d_ff,
                        d_qxx, d_qyx, d_phi, F_[0], qxx_, qyx_, phi_[0]);
@@ -540,6 +859,10 @@ class WallEngine final : public Engine {
   // mass_lyo_field numbering; derived fields are the planes of the last
   // UpdateQuantities call (zero before the first step)
   hipError_t field(int id, double* d_out, hipStream_t s) override {
+    if (id >= D_N && id <= D_HYX) {
+      const hipError_t e = materialize(s);
+      if (e != hipSuccess) return e;
+    }
     const T* a = nullptr;
     int np = 1;
     switch (id) {
@@ -638,6 +961,7 @@ class WallEngine final : public Engine {
     if (e != hipSuccess) return e;
     cur_ = 0;
     pc_ = 0;
+    d_stale_ = false;
     return copy_state(nullptr, host, s);
   }
 
@@ -1045,8 +1369,9 @@ class WallEngine final : public Engine {
  private:
   // a captured step: the parities it starts from and leaves
   struct StepGraph {
-    int cur, pc, mc;
-    int cur_after, pc_after, mc_after;
+    int cur, pc, mc, qa;
+    bool loops;
+    int cur_after, pc_after, mc_after, qa_after;
     hipGraphExec_t exec;
   };
   std::vector<StepGraph> graphs_;
@@ -1056,6 +1381,10 @@ class WallEngine final : public Engine {
   }
 
   bool growth_ = false, conserve_ = false;
+  bool loops_ = false;  // the node loops instead of the fused sweeps
+  int qa_ = 0;          // which of the two Q buffers is current (fused sweeps)
+  bool d_stale_ = false;  // D_ predates the last fused step
+  T *qxx2_ = nullptr, *qyx2_ = nullptr;
   mass_lyo_growth gp_{};
   GrowthDev gd_{};
   std::mt19937* gen_ = nullptr;

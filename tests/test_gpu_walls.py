@@ -26,8 +26,10 @@ ACTIVE = dict(CONFIG0_PHYS, zeta=0.05)
 BCS = [1, 2, 3, 4]
 
 
-def pair(LX, LY, bc, npc=1, phys=ACTIVE, init=CONFIG0_INIT):
+def pair(LX, LY, bc, npc=1, phys=ACTIVE, init=CONFIG0_INIT, loops=False):
     g = lyo.Lyotropic(LX, LY, npc=npc, precision=lyo.FP64, bc=bc, **phys)
+    if loops:
+        g.set_loop_kernels(True)
     o = po.Oracle(LX, LY, npc=npc, bc=bc, **phys)
     fg = g.configure(**init)
     fo = o.configure(**init)
@@ -40,12 +42,15 @@ def assert_bitwise(g, o, names, what=""):
         assert np.array_equal(a, b), "%s %s differs: max |d| = %g" % (what, n, np.abs(a - b).max())
 
 
+@pytest.mark.parametrize("loops", [False, True])
 @pytest.mark.parametrize("bc", BCS)
 @pytest.mark.parametrize("npc", [1, 2, 3])
-@pytest.mark.parametrize("LX,LY", [(2, 2), (5, 3), (16, 12), (45, 67)])
-def test_walls_one_step_bit_exact(require_gpu, bc, npc, LX, LY):
-    print("walls one step bc=%d npc=%d %dx%d" % (bc, npc, LX, LY), flush=True)
-    g, o, fg, fo = pair(LX, LY, bc, npc)
+@pytest.mark.parametrize("LX,LY", [(2, 2), (5, 3), (16, 12), (45, 67), (17, 33)])
+def test_walls_one_step_bit_exact(require_gpu, bc, npc, LX, LY, loops):
+    # loops: the node loops with condition kernels between them; otherwise
+    # the fused sweeps (16 x 16 tiles: 17 x 33 leaves partial tiles)
+    print("walls one step bc=%d npc=%d %dx%d loops=%d" % (bc, npc, LX, LY, loops), flush=True)
+    g, o, fg, fo = pair(LX, LY, bc, npc, loops=loops)
     assert fg == fo
     assert_bitwise(g, o, STATE, "initial")
     g.step(1)
@@ -54,15 +59,27 @@ def test_walls_one_step_bit_exact(require_gpu, bc, npc, LX, LY):
     assert_bitwise(g, o, DERIVED, "step 1")
 
 
+@pytest.mark.parametrize("loops", [False, True])
 @pytest.mark.parametrize("bc", BCS)
-def test_walls_60_steps_bit_exact(require_gpu, bc):
-    print("walls 60 steps bc=%d" % bc, flush=True)
-    g, o, _, _ = pair(70, 52, bc, npc=2)
+def test_walls_60_steps_bit_exact(require_gpu, bc, loops):
+    print("walls 60 steps bc=%d loops=%d" % (bc, loops), flush=True)
+    g, o, _, _ = pair(70, 52, bc, npc=2, loops=loops)
     g.step(60)
     o.step(60)
     assert_bitwise(g, o, STATE + ["ux", "sigmaXY", "MU"])
     ux = g.get("ux")
     assert np.abs(ux).max() > 1e-5  # an active flow, not a rest state
+
+
+@pytest.mark.parametrize("bc", BCS)
+def test_walls_switching_kernel_sets_mid_run(require_gpu, bc):
+    # fused sweeps and node loops hand the state over at step boundaries
+    g, o, _, _ = pair(33, 40, bc, npc=2)
+    for s, loops in enumerate([False, True, True, False, True, False]):
+        g.set_loop_kernels(loops)
+        g.step(s % 3 + 1)
+        o.step(s % 3 + 1)
+    assert_bitwise(g, o, STATE + ["ux", "sigmaXY", "MU"])
 
 
 @pytest.mark.parametrize("config", ["boxatwall", "cuttingatwall", "circleatwall", "wettedwall"])
@@ -140,9 +157,8 @@ def test_walls_refuse_what_needs_a_periodic_grid(require_gpu):
     g.configure(**CONFIG0_INIT)
     with pytest.raises(lyo.MassLyoError):
         g.diagnostics()
-    with pytest.raises(lyo.MassLyoError):
-        g.set_loop_kernels(False)
     g.set_loop_kernels(True)
+    g.set_loop_kernels(False)
     run = dict(LX=16, LY=16, bc=2, **ACTIVE)
     with pytest.raises(lyo.MassLyoError):
         lyo.Batch([run], npc=2, precision=lyo.FP64)
