@@ -326,12 +326,12 @@ enum WallSweepMode { kSweepPredictor = 0, kSweepCorrector = 1, kSweepDerive = 2 
 
 // MODE kSweepDerive runs UpdateQuantities only and stores the 23 derived
 // planes of the tile (the frame fields of the last UpdateQuantities,
-// recomputed from its inputs when a frame asks for them).  PULL: f and fn are
-// the predictor's post-collision populations with the LB conditions applied,
-// and Move (lyotropic.cpp:371-386) happens as they are read -- node p takes
-// direction v from p - c_v on the torus; the streamed fn of the tile goes to
-// fn_out for the later correctors.
-template <int MODE, bool PULL = false>
+// recomputed from its inputs when a frame asks for them).  PULL_F / PULL_FN:
+// f / fn are the predictor's post-collision populations with the LB
+// conditions applied, and Move (lyotropic.cpp:371-386) happens as they are
+// read -- node p takes direction v from p - c_v on the torus.  Every
+// corrector pulls fn from those planes, which no sweep overwrites.
+template <int MODE, bool PULL_F = false, bool PULL_FN = false>
 __global__ void __launch_bounds__(kWallThreads) wall_sweep_kernel(WallSweep a, Params<double> P,
                                                                   WallGeo g) {
   using T = double;
@@ -340,7 +340,18 @@ __global__ void __launch_bounds__(kWallThreads) wall_sweep_kernel(WallSweep a, P
   constexpr int DW = TY + 2, DN = (TX + 2) * DW;  // derived: tile + 1 (= threads)
   __shared__ T sq[3][PN];
   __shared__ T sd[9][DN];  // ux uy uxp uyp mu sxx syy sxy syx
-  const int x0 = blockIdx.y * TX, y0 = blockIdx.x * TY;
+  // XCD-aware order: the GPU deals consecutive blocks to its 8 XCDs in turn,
+  // so block b runs on XCD b % 8; give each XCD a contiguous run of tiles
+  // (neighbours in y, then the next tile row) so the halo lines two tiles
+  // share are fetched into one L2
+  int x0, y0;
+  {
+    const int nb = gridDim.x * gridDim.y, b = blockIdx.x + gridDim.x * blockIdx.y;
+    const int q = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
+    const int t = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + idx;
+    x0 = (t / gridDim.x) * TX;
+    y0 = (t % gridDim.x) * TY;
+  }
   const int tid = threadIdx.x;
   const bool perx = g.bc <= 2;  // channels are periodic in x
   const int NP = g.NP;
@@ -380,7 +391,7 @@ __global__ void __launch_bounds__(kWallThreads) wall_sweep_kernel(WallSweep a, P
     if (xin && yin) {
       const int k = node(wall_axis(xv, g.LX, perx), yv);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) fl[v] = a.f[v * NP + (PULL ? wnext(g, k, wopp(v)) : k)];
+      for (int v = 0; v < 9; ++v) fl[v] = a.f[v * NP + (PULL_F ? wnext(g, k, wopp(v)) : k)];
       const int q = (r + 1) * PW + (c + 1);
       NbrVals<T> nv;
       DiagVals<T> dg;
@@ -490,11 +501,8 @@ __global__ void __launch_bounds__(kWallThreads) wall_sweep_kernel(WallSweep a, P
     a.qyx_out[k] = a.qnyx[k] + T(.5) * Dyx;
     a.phi_out[k] = a.phn[k] + T(.5) * Dp;
 #pragma unroll
-    for (int v = 0; v < 9; ++v) {
-      const T fnv = a.fn[v * NP + (PULL ? wnext(g, k, wopp(v)) : k)];
-      if constexpr (PULL) a.fn_out[v * NP + k] = fnv;
-      a.f_out[v * NP + k] = fnv + T(.5) * om[v];
-    }
+    for (int v = 0; v < 9; ++v)
+      a.f_out[v * NP + k] = a.fn[v * NP + (PULL_FN ? wnext(g, k, wopp(v)) : k)] + T(.5) * om[v];
   }
 }
 
@@ -697,7 +705,7 @@ class WallEngine final : public Engine {
   // conditions and Move between the predictor and the correctors
   hipError_t fused_step(hipStream_t s) {
     const dim3 grid = sweep_grid();
-    const int bs = 256, nbt = (TP_ + bs - 1) / bs;
+    const int bs = 256;
     WallSweep a{};
     a.f = F_[cur_];
     a.qxx = qxx_;
@@ -716,8 +724,8 @@ class WallEngine final : public Engine {
     swap_q();
     pc_ ^= 1;
     // the LB conditions on the post-collision populations; the first
-    // corrector streams them as it reads (writing its f to F[cur], which no
-    // block pulls from), the later ones read plain planes
+    // corrector streams f as it reads (writing its f to F[cur], which no
+    // block pulls from), the later ones read plain f; all stream fn
     for (int ph : phases())
       hipLaunchKernelGGL((wall_lb_kernel<T>), dim3(phase_blocks(ph)), dim3(bs), 0, s, geo(), ph,
                          F_[cur_ ^ 1], Fn_[cur_ ^ 1]);
@@ -728,8 +736,7 @@ class WallEngine final : public Engine {
       b.qxx = qxx_;
       b.qyx = qyx_;
       b.phi = phi_[pc_];
-      b.fn = c == 1 ? Fn_[cur_ ^ 1] : Fn_[cur_];
-      b.fn_out = Fn_[cur_];
+      b.fn = Fn_[cur_ ^ 1];
       b.qnxx = qnxx_;
       b.qnyx = qnyx_;
       b.phn = phn_;
@@ -738,17 +745,16 @@ class WallEngine final : public Engine {
       b.qyx_out = qyx2_;
       b.phi_out = phi_[pc_ ^ 1];
       if (c == 1)
-        hipLaunchKernelGGL((wall_sweep_kernel<kSweepCorrector, true>), grid, dim3(kWallThreads), 0,
-                           s, b, P_, geo());
+        hipLaunchKernelGGL((wall_sweep_kernel<kSweepCorrector, true, true>), grid,
+                           dim3(kWallThreads), 0, s, b, P_, geo());
       else
-        hipLaunchKernelGGL((wall_sweep_kernel<kSweepCorrector, false>), grid, dim3(kWallThreads),
-                           0, s, b, P_, geo());
+        hipLaunchKernelGGL((wall_sweep_kernel<kSweepCorrector, false, true>), grid,
+                           dim3(kWallThreads), 0, s, b, P_, geo());
       swap_q();
       pc_ ^= 1;
       fc ^= 1;
     }
     cur_ = fc;
-    (void)nbt;
     d_stale_ = true;  // the frame fields are recomputed from the last corrector's inputs
     return hipGetLastError();
   }
