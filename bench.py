@@ -302,8 +302,15 @@ def main():
             mlupsw = N * s64 / (tw / 1e3) / 1e6
             log("fp64 walls: %.3f ms/step, %.0f MLUPS" % (tw / s64, mlupsw))
             extra.update({"fp64_walls_bc4_mlups": round(mlupsw, 1),
-                          "fp64_walls_kernel_set": "ghost layer, boundary-condition kernels "
-                                                   "between the reference's node loops"})
+                          "fp64_walls_kernel_set": "ghost layer, three fused tile sweeps "
+                                                   "(conditions on Q, phi, u, MU, sigma "
+                                                   "in LDS; LB kernels; Move in the "
+                                                   "correctors' reads)"})
+            if not args.skip_fp64_loops:
+                twl, _ = gpu_run(lyo, L, lyo.FP64, npc, s64, w64, loops=True, bc=4)
+                mlupswl = N * s64 / (twl / 1e3) / 1e6
+                log("fp64 walls, loop kernels: %.3f ms/step, %.0f MLUPS" % (twl / s64, mlupswl))
+                extra["fp64_walls_bc4_loops_mlups"] = round(mlupswl, 1)
 
     dom = max((k for k in k32 if k32[k] > 0), key=lambda k: k32[k])
     # the one-kernel step performs all 1 + npc sweeps of a step

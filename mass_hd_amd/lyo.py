@@ -328,7 +328,8 @@ class Lyotropic:
     def set_loop_kernels(self, on=True):
         """FP64: run each step as the reference's node loops, one kernel per
         loop (UpdateQuantities storing its 23 derived fields, UpdateFields,
-        Move); False returns to the fused kernels.  Bitwise the same results."""
+        Move); False returns to the fused kernels (on walled runs the
+        ghost-layer tile sweeps).  Bitwise the same results."""
         self._check(self.lib.mass_lyo_set_loop_kernels(self.h, 1 if on else 0))
 
 
