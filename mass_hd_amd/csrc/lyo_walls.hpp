@@ -305,11 +305,13 @@ struct WallSweep {
   double* D;           // derive: the 23 derived planes of the tile's nodes
 };
 
-// Tile of domain nodes per block: 6 rows (x) by 62 columns (y).  The block's
-// 512 threads are the 8 x 64 nodes of the tile plus one ring, one wave per
+// Tile of domain nodes per block: 10 rows (x) by 62 columns (y).  The block's
+// 768 threads are the 12 x 64 nodes of the tile plus one ring, one wave per
 // row (contiguous in y), and each runs UpdateQuantities on its node; the
 // tile's threads keep their own node's values in registers for UpdateFields.
-constexpr int kWallTX = 6, kWallTY = 62, kWallThreads = (kWallTX + 2) * (kWallTY + 2);
+// 77.5 KB of LDS per block: two blocks per CU, held at 6 waves per SIMD by
+// an 80-register cap (the correctors spill a few values to scratch).
+constexpr int kWallTX = 10, kWallTY = 62, kWallThreads = (kWallTX + 2) * (kWallTY + 2);
 
 // Domain coordinate a stencil reads for coordinate v of an axis of length L
 // (periodic: wrapped; walled: the ghost at -1 / L reads its Neumann source
@@ -332,8 +334,8 @@ enum WallSweepMode { kSweepPredictor = 0, kSweepCorrector = 1, kSweepDerive = 2 
 // read -- node p takes direction v from p - c_v on the torus.  Every
 // corrector pulls fn from those planes, which no sweep overwrites.
 template <int MODE, bool PULL_F = false, bool PULL_FN = false>
-__global__ void __launch_bounds__(kWallThreads) wall_sweep_kernel(WallSweep a, Params<double> P,
-                                                                  WallGeo g) {
+__global__ void __launch_bounds__(kWallThreads) __attribute__((amdgpu_waves_per_eu(6)))
+wall_sweep_kernel(WallSweep a, Params<double> P, WallGeo g) {
   using T = double;
   constexpr int TX = kWallTX, TY = kWallTY;
   constexpr int PW = TY + 4, PN = (TX + 4) * PW;  // staged Q, phi: tile + 2
