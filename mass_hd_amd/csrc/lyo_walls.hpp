@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <random>
 #include <utility>
 #include <vector>
@@ -657,10 +658,19 @@ class WallEngine final : public Engine {
     m_[0] = m_[1] = mn_ = chi_ = part_ = stage_ = fe_ = nullptr;
     if (div_) (void)hipFree(div_);
     if (death_) (void)hipFree(death_);
-    if (cent_) (void)hipFree(cent_);
+    // the centre copies of the last draws may still be in flight
+    if (hcent_[0] || hcent_[1]) (void)hipDeviceSynchronize();
+    for (int i = 0; i < 2; ++i) {
+      if (cent_[i]) (void)hipFree(cent_[i]);
+      if (hcent_[i]) (void)hipHostFree(hcent_[i]);
+      cent_[i] = nullptr;
+      hcent_[i] = nullptr;
+      cent_cap_[i] = 0;
+    }
+    if (hphi_) (void)hipHostFree(hphi_);
+    hphi_ = nullptr;
+    hphi_n_ = 0;
     div_ = death_ = nullptr;
-    cent_ = nullptr;
-    cent_cap_ = 0;
     growth_ = false;
   }
 
@@ -1168,10 +1178,14 @@ class WallEngine final : public Engine {
   }
   // One mask of LyotropicWithDivision::SetMasks (lyotropic-with-division.cpp:
   // 34-108): due when its counter was 0; the centres are drawn on the host
-  // from phi in node order, the disks marked on the device.
-  hipError_t redraw(unsigned char* mask, const std::vector<double>& phi, double rate,
-                    double radius, bool division, hipStream_t s) {
-    std::vector<int> centres;
+  // from phi in node order, the disks marked on the device.  The centres go
+  // through a pinned buffer per mask, so nothing waits for the copy: the next
+  // draw into that buffer comes after set_masks' next synchronous download.
+  hipError_t redraw(unsigned char* mask, const double* phi, double rate, double radius,
+                    bool division, hipStream_t s) {
+    const int slot = division ? 0 : 1;
+    std::vector<int>& centres = centres_;
+    centres.clear();
     for (int k = 0; k < LX_ * LY_; ++k) {
       const double p = phi[k];
       if (p >= .5) {
@@ -1186,33 +1200,52 @@ class WallEngine final : public Engine {
     hipError_t e = hipMemsetAsync(mask, 0, (size_t)TP_, s);
     const int nc = (int)centres.size() / 2;
     if (e != hipSuccess || nc == 0) return e;
-    if ((size_t)nc * 2 > cent_cap_) {
-      if (cent_) (void)hipFree(cent_);
-      cent_ = nullptr;
-      cent_cap_ = 0;
-      e = hipMalloc((void**)&cent_, sizeof(int) * 2 * (size_t)nc);
+    const size_t n = 2 * (size_t)nc;
+    if (n > cent_cap_[slot]) {
+      e = hipStreamSynchronize(s);  // nothing queued still uses the old buffers
       if (e != hipSuccess) return e;
-      cent_cap_ = 2 * (size_t)nc;
+      if (cent_[slot]) (void)hipFree(cent_[slot]);
+      if (hcent_[slot]) (void)hipHostFree(hcent_[slot]);
+      cent_[slot] = nullptr;
+      hcent_[slot] = nullptr;
+      cent_cap_[slot] = 0;
+      e = hipMalloc((void**)&cent_[slot], sizeof(int) * n);
+      if (e == hipSuccess)
+        e = hipHostMalloc((void**)&hcent_[slot], sizeof(int) * n, hipHostMallocDefault);
+      if (e != hipSuccess) return e;
+      cent_cap_[slot] = n;
     }
-    e = hipMemcpyAsync(cent_, centres.data(), sizeof(int) * 2 * (size_t)nc,
-                       hipMemcpyHostToDevice, s);
+    std::memcpy(hcent_[slot], centres.data(), sizeof(int) * n);
+    e = hipMemcpyAsync(cent_[slot], hcent_[slot], sizeof(int) * n, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     const double r2 = std::ceil(radius * radius);
     int R = 0;
     while ((double)(R + 1) * (double)(R + 1) <= r2) ++R;
     const long long span = 2 * R + 1, threads = (long long)nc * span * span;
     hipLaunchKernelGGL(mask_disk_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
-                       s, (const int*)cent_, nc, R, r2, LX_, LY_, mask);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(s);  // `centres` goes away
-    return e;
+                       s, (const int*)cent_[slot], nc, R, r2, LX_, LY_, mask);
+    return hipGetLastError();
   }
   hipError_t set_masks(hipStream_t s) {
     const uint32_t div_old = div_count_--;
     const uint32_t death_old = death_count_--;
     if (div_old != 0 && death_old != 0) return hipSuccess;
-    std::vector<double> phi;
-    hipError_t e = download(phi_[pc_], phi, s);
+    // phi of the domain into a pinned host buffer (kept for the run)
+    const int N = LX_ * LY_;
+    hipError_t e = hipSuccess;
+    if ((size_t)N > hphi_n_) {
+      if (hphi_) (void)hipHostFree(hphi_);
+      hphi_ = nullptr;
+      hphi_n_ = 0;
+      e = hipHostMalloc((void**)&hphi_, sizeof(double) * N, hipHostMallocDefault);
+      if (e != hipSuccess) return e;
+      hphi_n_ = (size_t)N;
+    }
+    hipLaunchKernelGGL((wall_unpack_kernel<T>), dim3((N + 255) / 256), dim3(256), 0, s, geo(),
+                       (const T*)phi_[pc_], 1, stage_);
+    e = hipMemcpyAsync(hphi_, stage_, sizeof(double) * N, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const double* phi = hphi_;
     if (e == hipSuccess && div_old == 0) {
       div_count_ = (uint32_t)gp_.division_time;
       e = redraw(div_, phi, gp_.division_rate, gp_.division_radius, true, s);
@@ -1407,8 +1440,12 @@ class WallEngine final : public Engine {
   T* fe_ = nullptr;
   unsigned char* div_ = nullptr;
   unsigned char* death_ = nullptr;
-  int* cent_ = nullptr;
-  size_t cent_cap_ = 0;
+  int* cent_[2] = {};          // mask centres on the device: division, death
+  int* hcent_[2] = {};         // their pinned host copies
+  size_t cent_cap_[2] = {};
+  std::vector<int> centres_;   // the draw's centres, (x, y) pairs
+  double* hphi_ = nullptr;     // pinned host copy of phi for the draws
+  size_t hphi_n_ = 0;
 
   int bc_;
   Params<T> P_{};
