@@ -69,6 +69,22 @@ struct LayerGrid {
   }
 };
 
+// The ghost-layer layout with periodic neighbours taken inside the domain (a
+// domain edge's neighbour is the node across the grid, not a ghost): a
+// periodic run on that layout reads and writes no ghost node.
+struct LayerWrapGrid {
+  __device__ __forceinline__ static LoopNbr at(int t, int LX, int LY) {
+    const int x = t / LY, y = t - x * LY, W = LY + 2;
+    LoopNbr n;
+    n.k = (y + 1) + W * (x + 1);
+    n.kxp = x + 1 < LX ? n.k + W : (y + 1) + W;
+    n.kxm = x > 0 ? n.k - W : (y + 1) + W * LX;
+    n.kyp = y + 1 < LY ? n.k + 1 : 1 + W * (x + 1);
+    n.kym = y > 0 ? n.k - 1 : LY + W * (x + 1);
+    return n;
+  }
+};
+
 // UpdateQuantities (lyotropic.cpp:168-247, loop :343-360): all 23 derived
 // planes of every domain node, plane i at D + i*NP.
 template <class T, bool EXACT, class G = PeriodicGrid>

@@ -528,6 +528,30 @@ __global__ void __launch_bounds__(256) wall_move_kernel(WallGeo g, const T* __re
   }
 }
 
+// Move on a periodic grid over the ghost-layer layout (no ghosts): every
+// domain node pulls direction v from its periodic upstream neighbour.
+template <class T>
+__global__ void __launch_bounds__(256) wall_move_wrap_kernel(WallGeo g, const T* __restrict__ f,
+                                                             const T* __restrict__ fn,
+                                                             T* __restrict__ f_to,
+                                                             T* __restrict__ fn_to) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.LX * g.LY) return;
+  const int x = t / g.LY, y = t - x * g.LY;
+  const int p = wnode(g, x + 1, y + 1);
+  f_to[p] = f[p];
+  fn_to[p] = fn[p];
+#pragma unroll
+  for (int v = 1; v < 9; ++v) {
+    int xs = x - kWallCX[v], ys = y - kWallCY[v];
+    xs = xs < 0 ? xs + g.LX : (xs >= g.LX ? xs - g.LX : xs);
+    ys = ys < 0 ? ys + g.LY : (ys >= g.LY ? ys - g.LY : ys);
+    const int src = wnode(g, xs + 1, ys + 1);
+    f_to[v * g.NP + p] = f[v * g.NP + src];
+    fn_to[v * g.NP + p] = fn[v * g.NP + src];
+  }
+}
+
 // Host FP64 domain array -> the domain of a torus plane.
 static __global__ void wall_scatter_kernel(WallGeo g, const double* __restrict__ in,
                                            double* __restrict__ out) {
@@ -809,12 +833,23 @@ class WallEngine final : public Engine {
       for (int ph : phases())
         hipLaunchKernelGGL((wall_lb_kernel<T>), dim3(phase_blocks(ph)), dim3(bs), 0, s, geo(),
                            ph, F_[cur_], Fn_[cur_]);
-      hipLaunchKernelGGL((wall_move_kernel<T>), dim3(nbt), dim3(bs), 0, s, geo(),
-                         (const T*)F_[cur_], (const T*)Fn_[cur_], F_[cur_ ^ 1], Fn_[cur_ ^ 1]);
+      if (bc_ == 0)
+        hipLaunchKernelGGL((wall_move_wrap_kernel<T>), dim3(node_blocks()), dim3(bs), 0, s,
+                           geo(), (const T*)F_[cur_], (const T*)Fn_[cur_], F_[cur_ ^ 1],
+                           Fn_[cur_ ^ 1]);
+      else
+        hipLaunchKernelGGL((wall_move_kernel<T>), dim3(nbt), dim3(bs), 0, s, geo(),
+                           (const T*)F_[cur_], (const T*)Fn_[cur_], F_[cur_ ^ 1],
+                           Fn_[cur_ ^ 1]);
       cur_ ^= 1;
+      // go-or-grow: UpdateFields ended with UpdatePhenotypeQuantities (chi =
+      // m/phi, pointwise) and the field conditions, and Move touches only f,
+      // so a corrector's leading field conditions and its UpdateQuantities'
+      // phenotype pass would rewrite the values already there: skipped
+      const bool gog = growth_ && !gp_.division_only;
       for (int c = 1; c <= npc_; ++c) {
-        fields_bc(s);
-        update_quantities(s);
+        if (!gog) fields_bc(s);
+        update_quantities(s, !gog);
         fields2_bc(s);
         update_fields(false, s);
       }
@@ -986,10 +1021,18 @@ class WallEngine final : public Engine {
  private:
   WallGeo geo() const { return WallGeo{LX_, LY_, LY_ + 2, NPw_, bc_}; }
   const T* plane(int i) const { return D_ + (size_t)i * NPw_; }
-  // the launches of one boundary-condition call: the three phases in order,
-  // or one launch over every ghost node for periodic ghosts (which read the
-  // domain only)
-  std::vector<int> phases() const { return bc_ == 0 ? std::vector<int>{-1} : std::vector<int>{0, 1, 2}; }
+  // the launches of one boundary-condition call: the three phases in order;
+  // none on a periodic grid (bc 0), whose kernels take the periodic
+  // neighbours inside the domain (LayerWrapGrid) and leave the ghosts unused
+  std::vector<int> phases() const { return bc_ == 0 ? std::vector<int>{} : std::vector<int>{0, 1, 2}; }
+  // runs f with the node-index policy of this grid
+  template <class F>
+  void by_grid(F&& f) const {
+    if (bc_ == 0)
+      f(LayerWrapGrid{});
+    else
+      f(LayerGrid{});
+  }
   unsigned phase_blocks(int phase) const {
     const int n = phase < 0 ? 2 * LY_ + 2 * LX_ + 4
                             : (phase == 0 ? 2 * LY_ : (phase == 1 ? 2 * LX_ : 4));
@@ -1007,21 +1050,29 @@ class WallEngine final : public Engine {
   unsigned node_blocks() const { return (unsigned)((LX_ * LY_ + 255) / 256); }
   // UpdatePhenotypeQuantities (go-or-grow.cpp:201-209)
   void phenotype(hipStream_t s) {
-    hipLaunchKernelGGL((phenotype_kernel<LayerGrid>), dim3(node_blocks()), dim3(256), 0, s,
-                       (const T*)m_[mc_], (const T*)phi_[pc_], chi_, LX_, LY_);
+    by_grid([&](auto gg) {
+      using G = decltype(gg);
+      hipLaunchKernelGGL((phenotype_kernel<G>), dim3(node_blocks()), dim3(256), 0, s,
+                         (const T*)m_[mc_], (const T*)phi_[pc_], chi_, LX_, LY_);
+    });
     fields_bc(s);
   }
-  // UpdateQuantities (lyotropic.cpp:343-360; go-or-grow.cpp:305-320)
-  void update_quantities(hipStream_t s) {
+  // UpdateQuantities (lyotropic.cpp:343-360; go-or-grow.cpp:305-320);
+  // with_phenotype = false leaves out the closing phenotype pass where chi is
+  // already m/phi of the current fields
+  void update_quantities(hipStream_t s, bool with_phenotype = true) {
     const T* f = F_[cur_];
     const bool gog = growth_ && !gp_.division_only;
-    if (!gog)
-      hipLaunchKernelGGL((loop_uq_kernel<T, true, LayerGrid>), dim3(node_blocks()), dim3(256), 0,
-                         s, f, qxx_, qyx_, phi_[pc_], P_, LX_, LY_, NPw_, D_);
-    else
-      hipLaunchKernelGGL((growth_uq_kernel<LayerGrid>), dim3(node_blocks()), dim3(256), 0, s, f,
-                         (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], (const T*)chi_, P_,
-                         gd_, LX_, LY_, NPw_, D_);
+    by_grid([&](auto gg) {
+      using G = decltype(gg);
+      if (!gog)
+        hipLaunchKernelGGL((loop_uq_kernel<T, true, G>), dim3(node_blocks()), dim3(256), 0, s, f,
+                           qxx_, qyx_, phi_[pc_], P_, LX_, LY_, NPw_, D_);
+      else
+        hipLaunchKernelGGL((growth_uq_kernel<G>), dim3(node_blocks()), dim3(256), 0, s, f,
+                           (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], (const T*)chi_,
+                           P_, gd_, LX_, LY_, NPw_, D_);
+    });
     if (growth_ && conserve_) {  // countphi: the sum of phi this UpdateQuantities sees
       const int N = LX_ * LY_;
       int nparts = (N + 4095) / 4096;
@@ -1033,7 +1084,7 @@ class WallEngine final : public Engine {
       hipLaunchKernelGGL(final_sums_kernel, dim3(1), dim3(256), 0, s, part_,
                          part_ + kGrowthParts, nparts, 0.0, part_ + 2 * kGrowthParts);
     }
-    if (gog) phenotype(s);
+    if (gog && with_phenotype) phenotype(s);
   }
   // UpdateFields (lyotropic.cpp:362-369; go-or-grow.cpp:322-362)
   void update_fields(bool first, hipStream_t s) {
@@ -1060,12 +1111,15 @@ class WallEngine final : public Engine {
       u.phn = phn_;
     }
     if (!growth_) {
-      if (first)
-        hipLaunchKernelGGL((loop_uf_kernel<T, true, true, LayerGrid>), dim3(node_blocks()),
-                           dim3(256), 0, s, u, P_, LX_, LY_, NPw_);
-      else
-        hipLaunchKernelGGL((loop_uf_kernel<T, true, false, LayerGrid>), dim3(node_blocks()),
-                           dim3(256), 0, s, u, P_, LX_, LY_, NPw_);
+      by_grid([&](auto gg) {
+        using G = decltype(gg);
+        if (first)
+          hipLaunchKernelGGL((loop_uf_kernel<T, true, true, G>), dim3(node_blocks()), dim3(256),
+                             0, s, u, P_, LX_, LY_, NPw_);
+        else
+          hipLaunchKernelGGL((loop_uf_kernel<T, true, false, G>), dim3(node_blocks()),
+                             dim3(256), 0, s, u, P_, LX_, LY_, NPw_);
+      });
       pc_ ^= 1;
       return;
     }
@@ -1081,22 +1135,28 @@ class WallEngine final : public Engine {
     a.countphi = part_ + 2 * kGrowthParts + 1;
     a.totalphi = totalphi_;
     a.conserve = conserve_ ? 1 : 0;
-    if (gp_.division_only) {
-      if (first)
-        hipLaunchKernelGGL((growth_uf_kernel<true, LayerGrid, true>), dim3(node_blocks()),
-                           dim3(256), 0, s, a, P_, gd_, LX_, LY_, NPw_);
-      else
-        hipLaunchKernelGGL((growth_uf_kernel<false, LayerGrid, true>), dim3(node_blocks()),
-                           dim3(256), 0, s, a, P_, gd_, LX_, LY_, NPw_);
+    const bool div_only = gp_.division_only != 0;
+    by_grid([&](auto gg) {
+      using G = decltype(gg);
+      if (div_only) {
+        if (first)
+          hipLaunchKernelGGL((growth_uf_kernel<true, G, true>), dim3(node_blocks()), dim3(256), 0,
+                             s, a, P_, gd_, LX_, LY_, NPw_);
+        else
+          hipLaunchKernelGGL((growth_uf_kernel<false, G, true>), dim3(node_blocks()), dim3(256),
+                             0, s, a, P_, gd_, LX_, LY_, NPw_);
+      } else if (first) {
+        hipLaunchKernelGGL((growth_uf_kernel<true, G>), dim3(node_blocks()), dim3(256), 0, s, a,
+                           P_, gd_, LX_, LY_, NPw_);
+      } else {
+        hipLaunchKernelGGL((growth_uf_kernel<false, G>), dim3(node_blocks()), dim3(256), 0, s,
+                           a, P_, gd_, LX_, LY_, NPw_);
+      }
+    });
+    if (div_only) {
       pc_ ^= 1;
       return;
     }
-    if (first)
-      hipLaunchKernelGGL((growth_uf_kernel<true, LayerGrid>), dim3(node_blocks()), dim3(256), 0,
-                         s, a, P_, gd_, LX_, LY_, NPw_);
-    else
-      hipLaunchKernelGGL((growth_uf_kernel<false, LayerGrid>), dim3(node_blocks()), dim3(256),
-                         0, s, a, P_, gd_, LX_, LY_, NPw_);
     pc_ ^= 1;
     mc_ ^= 1;
     phenotype(s);
@@ -1310,16 +1370,21 @@ class WallEngine final : public Engine {
       T* mu = D_ + (size_t)LP_MU * NPw_;
       for (uint32_t st = 0; st < gp_.relax_steps; ++st) {
         fields_bc(s, false);
-        hipLaunchKernelGGL((relax_mu_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
-                           (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], P_, gd_, LX_, LY_,
-                           hxx, hyx, mu);
+        by_grid([&](auto gg) {
+          using G = decltype(gg);
+          hipLaunchKernelGGL((relax_mu_kernel<G>), dim3(nb), dim3(256), 0, s, (const T*)qxx_,
+                             (const T*)qyx_, (const T*)phi_[pc_], P_, gd_, LX_, LY_, hxx, hyx, mu);
+        });
         for (int ph : phases())
           hipLaunchKernelGGL((wall_scalars_kernel<T>), dim3(phase_blocks(ph)), dim3(256), 0, s,
                              geo(), ph, mu, (T*)nullptr);
-        hipLaunchKernelGGL((relax_update_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
-                           (const T*)qxx_, (const T*)qyx_, (const T*)phi_[pc_], (const T*)hxx,
-                           (const T*)hyx, (const T*)mu, P_, gp_.relax_dt, gp_.relax_phi,
-                           gp_.relax_Q, LX_, LY_, phi_[pc_ ^ 1], qnxx_, qnyx_);
+        by_grid([&](auto gg) {
+          using G = decltype(gg);
+          hipLaunchKernelGGL((relax_update_kernel<G>), dim3(nb), dim3(256), 0, s, (const T*)qxx_,
+                             (const T*)qyx_, (const T*)phi_[pc_], (const T*)hxx, (const T*)hyx,
+                             (const T*)mu, P_, gp_.relax_dt, gp_.relax_phi, gp_.relax_Q, LX_, LY_,
+                             phi_[pc_ ^ 1], qnxx_, qnyx_);
+        });
         pc_ ^= 1;
         std::swap(qxx_, qnxx_);
         std::swap(qyx_, qnyx_);
@@ -1349,8 +1414,11 @@ class WallEngine final : public Engine {
       fe[0] = nn - fe[1] - fe[2] - fe[3] - fe[4] - fe[5] - fe[6] - fe[7] - fe[8];
       e = hipMemcpy(fe_, fe, sizeof(fe), hipMemcpyHostToDevice);
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL((reset_hydro_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s, F_[0], F_[1],
-                         Fn_[0], Fn_[1], D_, P_.rho, (const T*)fe_, LX_, LY_, NPw_);
+      by_grid([&](auto gg) {
+        using G = decltype(gg);
+        hipLaunchKernelGGL((reset_hydro_kernel<G>), dim3(nb), dim3(256), 0, s, F_[0], F_[1],
+                           Fn_[0], Fn_[1], D_, P_.rho, (const T*)fe_, LX_, LY_, NPw_);
+      });
       double ftotal = 0.;
       for (int k = 0; k < N; ++k)
         for (int v = 0; v < 9; ++v) ftotal = ftotal + fe[v];
@@ -1370,8 +1438,11 @@ class WallEngine final : public Engine {
               : 0u;
       for (unsigned st = 0; st < smooth_steps; ++st) {
         T* tmp = m_[mc_ ^ 1];
-        hipLaunchKernelGGL((chi_smooth_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
-                           (const T*)chi_, tmp, LX_, LY_);
+        by_grid([&](auto gg) {
+          using G = decltype(gg);
+          hipLaunchKernelGGL((chi_smooth_kernel<G>), dim3(nb), dim3(256), 0, s, (const T*)chi_,
+                             tmp, LX_, LY_);
+        });
         m_[mc_ ^ 1] = chi_;  // swap(chi, m_tmp)
         chi_ = tmp;
         fields_bc(s);
@@ -1400,8 +1471,11 @@ class WallEngine final : public Engine {
     }
     e = upload(chi, chi_, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((phenotype_init_kernel<LayerGrid>), dim3(nb), dim3(256), 0, s,
-                       (const T*)phi_[pc_], (const T*)chi_, m_[mc_], LX_, LY_);
+    by_grid([&](auto gg) {
+      using G = decltype(gg);
+      hipLaunchKernelGGL((phenotype_init_kernel<G>), dim3(nb), dim3(256), 0, s,
+                         (const T*)phi_[pc_], (const T*)chi_, m_[mc_], LX_, LY_);
+    });
     phenotype(s);
     fields_bc(s);
     return hipGetLastError();
