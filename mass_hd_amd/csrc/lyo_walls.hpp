@@ -9,9 +9,13 @@
 // populations like any other node (lyotropic.cpp:371-386 runs over the whole
 // grid).  Here every plane is that torus, node (X, Y) at Y + (LY+2)*X with
 // X, Y in [0, LX+1] x [0, LY+2); domain node (x, y) is (x+1, y+1).  The step
-// runs the reference's loop structure -- the node loops of lyo_loops.hpp on
-// LayerGrid -- with the boundary conditions as kernels in between, so results
-// are bitwise those of the oracle's restatement.
+// runs either as three fused tile sweeps (wall_sweep_kernel: field conditions
+// applied in LDS, the LB conditions as kernels, Move in the correctors'
+// reads) or as the reference's loop structure -- the node loops of
+// lyo_loops.hpp on LayerGrid -- with the boundary conditions as kernels in
+// between; both are bitwise the oracle's restatement.  go-or-grow on a
+// periodic grid (bc 0) keeps this layout with LayerWrapGrid neighbours and
+// no ghost conditions.
 //
 // The boundary conditions run in three launches per call, in the
 // reference's order: Left/Right walls, then Front/Back walls, then the four
@@ -61,7 +65,8 @@ struct WallGeo {
   int W;   // LY + 2: stride of X
   int NP;  // plane stride (>= (LX+2)*(LY+2))
   int bc;  // 1 free-slip channel, 2 no-slip channel, 3 free-slip box, 4 no-slip box;
-           // 0: periodic ghosts (go-or-grow on a periodic grid)
+           // 0: periodic grid (go-or-grow): the node kernels wrap inside the
+           // domain (LayerWrapGrid) and no condition runs
 };
 
 __device__ __forceinline__ int wopp(int d) { return kRefl[0][d]; }
@@ -105,32 +110,11 @@ __device__ __forceinline__ bool wall_ghost(const WallGeo& g, int phase, int i, i
   return true;
 }
 
-// phase -1 (periodic ghosts only, whose copies read the domain alone): one
-// launch covers the three phases' ghost nodes.
+// the ghost node of thread i in a launch of the given phase, and its phase
 __device__ __forceinline__ bool any_ghost(const WallGeo& g, int phase, int i, int& p, int& id,
                                           int& ph) {
   ph = phase;
-  if (ph < 0) {
-    if (i < 2 * g.LY) {
-      ph = 0;
-    } else if ((i -= 2 * g.LY) < 2 * g.LX) {
-      ph = 1;
-    } else {
-      i -= 2 * g.LX;
-      ph = 2;
-    }
-  }
-  return wall_ghost(g, ph, i, p, id);
-}
-
-// The domain node a ghost node is the periodic image of.  With periodic
-// ghosts every condition copies it (ApplyPBC, fields.hpp:619-631,749-760);
-// the domain then sees exactly the periodic grid's neighbours and streams.
-__device__ __forceinline__ int periodic_image(const WallGeo& g, int p) {
-  int X = p / g.W, Y = p - X * g.W;
-  X = X == 0 ? g.LX : (X == g.LX + 1 ? 1 : X);
-  Y = Y == 0 ? g.LY : (Y == g.LY + 1 ? 1 : Y);
-  return wnode(g, X, Y);
+  return wall_ghost(g, phase, i, p, id);
 }
 
 // channel corners: the closed wall next to corner c (the other wall is
@@ -145,10 +129,6 @@ __device__ __forceinline__ int channel_wall(int c) {
 template <class T>
 __device__ __forceinline__ void scalar_ghost(const WallGeo& g, T* a, int kind, int phase, int p,
                                              int id) {
-  if (g.bc == 0) {
-    a[p] = a[periodic_image(g, p)];
-    return;
-  }
   const bool channel = g.bc <= 2;
   if (phase == 0 && channel) {  // ApplyPBC(Left), ApplyPBC(Right)
     const int Y = p % g.W;
@@ -181,12 +161,6 @@ __device__ __forceinline__ bool incoming(int v, int w) {
 template <class T>
 __device__ __forceinline__ void lb_ghost(const WallGeo& g, T* f, int phase, int p, int id) {
   const int NP = g.NP;
-  if (g.bc == 0) {
-    const int src = periodic_image(g, p);
-#pragma unroll
-    for (int v = 0; v < 9; ++v) f[v * NP + p] = f[v * NP + src];
-    return;
-  }
   const bool channel = g.bc <= 2, slip = g.bc == 1 || g.bc == 3;
   if (phase == 0 && channel) {
     const int Y = p % g.W;
@@ -1034,8 +1008,7 @@ class WallEngine final : public Engine {
       f(LayerGrid{});
   }
   unsigned phase_blocks(int phase) const {
-    const int n = phase < 0 ? 2 * LY_ + 2 * LX_ + 4
-                            : (phase == 0 ? 2 * LY_ : (phase == 1 ? 2 * LX_ : 4));
+    const int n = phase == 0 ? 2 * LY_ : (phase == 1 ? 2 * LX_ : 4);
     return (unsigned)((n + 255) / 256);
   }
   // BoundaryConditionsFields (lyotropic.cpp:422-449; go-or-grow.cpp:365-385
