@@ -1224,7 +1224,7 @@ class WallEngine final : public Engine {
       if (p >= .5) {
         const double adjusted =
             division ? rate * p * (1 - p / gp_.phi_critical) : rate * p;
-        if (std::uniform_real_distribution<>(0., 1.)(*gen_) < adjusted) {
+        if (next_uniform() < adjusted) {
           centres.push_back(k / LY_);
           centres.push_back(k % LY_);
         }
@@ -1259,10 +1259,34 @@ class WallEngine final : public Engine {
                        s, (const int*)cent_[slot], nc, R, r2, LX_, LY_, mask);
     return hipGetLastError();
   }
+  // The draws' uniform numbers, taken from gen_ ahead of use and consumed in
+  // order: the sequence each draw sees is the one drawing on demand gives.
+  double next_uniform() {
+    ++udraws_;
+    if (ufront_ < ufifo_.size()) return ufifo_[ufront_++];
+    return std::uniform_real_distribution<>(0., 1.)(*gen_);
+  }
+  void clear_uniforms() {
+    ufifo_.clear();
+    ufront_ = 0;
+    uexpect_ = 0;
+  }
   hipError_t set_masks(hipStream_t s) {
     const uint32_t div_old = div_count_--;
     const uint32_t death_old = death_count_--;
     if (div_old != 0 && death_old != 0) return hipSuccess;
+    // draw about as many numbers as the last redraw used while the queued
+    // steps still run, before waiting for phi
+    if (ufront_ > 0) {
+      ufifo_.erase(ufifo_.begin(), ufifo_.begin() + (std::ptrdiff_t)ufront_);
+      ufront_ = 0;
+    }
+    {
+      const size_t want = uexpect_ + uexpect_ / 4 + 256;
+      std::uniform_real_distribution<> u(0., 1.);
+      while (ufifo_.size() < want) ufifo_.push_back(u(*gen_));
+    }
+    const size_t used0 = udraws_;
     // phi of the domain into a pinned host buffer (kept for the run)
     const int N = LX_ * LY_;
     hipError_t e = hipSuccess;
@@ -1287,6 +1311,7 @@ class WallEngine final : public Engine {
       death_count_ = (uint32_t)gp_.death_time;
       e = redraw(death_, phi, gp_.death_rate, gp_.death_radius, false, s);
     }
+    uexpect_ = udraws_ - used0;
     return e;
   }
 
@@ -1331,6 +1356,10 @@ class WallEngine final : public Engine {
   hipError_t growth_configure(double* ftot, double* ptot, hipStream_t s) override {
     if (!growth_) return hipErrorNotSupported;
     drop_graphs();  // relaxation and smoothing swap buffers; totalphi changes
+    clear_uniforms();  // configure has just reseeded the generator
+    // a configured run starts afresh: the masks were cleared with the state,
+    // and both are due at the first step (lyotropic-with-division.cpp:23-31)
+    div_count_ = death_count_ = 0u;
     totalphi_ = *ptot;
     if (gp_.division_only) return hipSuccess;  // Lyotropic's Configure only
     const int N = LX_ * LY_;
@@ -1491,6 +1520,10 @@ class WallEngine final : public Engine {
   int* hcent_[2] = {};         // their pinned host copies
   size_t cent_cap_[2] = {};
   std::vector<int> centres_;   // the draw's centres, (x, y) pairs
+  std::vector<double> ufifo_;  // uniforms drawn ahead from gen_, in order
+  size_t ufront_ = 0;          // the next unused one
+  size_t udraws_ = 0;          // numbers the draws have consumed
+  size_t uexpect_ = 0;         // used by the last redraw
   double* hphi_ = nullptr;     // pinned host copy of phi for the draws
   size_t hphi_n_ = 0;
 

@@ -354,7 +354,15 @@ int lyo_oracle_configure(lyo_oracle* o, const lyo_oracle_init* in) {
     for (int v = 0; v < 9; ++v) o->ftot = o->ftot + o->ff[9 * (size_t)k + v];
     o->ptot += F(LYO_O_PHI)[k];
   }
-  if (o->growth) growth_configure(o);
+  if (o->growth) {
+    /* a configured run starts afresh: no masks, both due at the first step
+       (the reference configures once, after lyotropic-with-division.cpp:23-31
+       set the counters) */
+    o->div_count = o->death_count = 0u;
+    memset(o->div_mask, 0, o->N);
+    memset(o->death_mask, 0, o->N);
+    growth_configure(o);
+  }
   return 0;
 }
 

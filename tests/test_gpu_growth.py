@@ -154,3 +154,23 @@ def test_division_only_bit_exact(require_gpu, bc, npc):
     assert a.get("division_mask").any() and a.get("death_mask").any()
     with pytest.raises(lyo.MassLyoError):
         a.get("m")  # no phenotype in this model
+
+
+def test_growth_reconfigure_restarts_the_draws(require_gpu):
+    # configure starts the run afresh (generator reseeded, masks cleared and
+    # due at the first step): a run configured again after redrawing its
+    # masks must match the oracle after the same calls, and a fresh oracle
+    a, o, ta, to = pair(40, 36, 0, npc=1)
+    a.step(7)
+    o.step(7)
+    assert a.get("division_mask").any()
+    assert a.configure(**INIT) == o.configure(**INIT)
+    o2 = po.Oracle(40, 36, npc=1, bc=0, **PHYS)
+    o2.set_growth(**GROWTH)
+    o2.configure(**INIT)
+    for s in range(2):
+        a.step(5)
+        o.step(5)
+        o2.step(5)
+        check(a, o, STATE, "after reconfigure, %d steps" % (5 * s + 5))
+        check(a, o2, STATE, "against a fresh run, %d steps" % (5 * s + 5))
