@@ -28,6 +28,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -526,6 +527,59 @@ __global__ void __launch_bounds__(256) wall_move_wrap_kernel(WallGeo g, const T*
   }
 }
 
+// The domain nodes with phi >= 1/2, in node order (the nodes the mask draws
+// visit, lyotropic-with-division.cpp:42-50): select_count_kernel counts each
+// block's; select_write_kernel adds up the earlier blocks' counts and writes
+// its nodes' (domain index, phi) at that offset, in order within the block.
+constexpr int kSelBlock = 1024;
+__device__ __forceinline__ bool select_node(const WallGeo& g, const double* phi, int t) {
+  return t < g.LX * g.LY && phi[LayerGrid::at(t, g.LX, g.LY).k] >= .5;
+}
+static __global__ void __launch_bounds__(kSelBlock) select_count_kernel(WallGeo g,
+                                                                        const double* __restrict__ phi,
+                                                                        int* __restrict__ counts) {
+  __shared__ int ws[kSelBlock / 64];
+  const bool e = select_node(g, phi, blockIdx.x * kSelBlock + threadIdx.x);
+  const unsigned long long b = __ballot(e);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int w = 0; w < kSelBlock / 64; ++w) c += ws[w];
+    counts[blockIdx.x] = c;
+  }
+}
+static __global__ void __launch_bounds__(kSelBlock) select_write_kernel(
+    WallGeo g, const double* __restrict__ phi, const int* __restrict__ counts,
+    int* __restrict__ idx, double* __restrict__ val, int* __restrict__ total) {
+  __shared__ int ws[kSelBlock / 64];
+  __shared__ int part[kSelBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // offset of this block: the counts of blocks 0 .. blockIdx.x - 1
+  int o = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kSelBlock) o += counts[b];
+  for (int d = 32; d > 0; d >>= 1) o += __shfl_xor(o, d);
+  if (lane == 0) part[w] = o;
+  const int t = blockIdx.x * kSelBlock + threadIdx.x;
+  const bool e = select_node(g, phi, t);
+  const unsigned long long b = __ballot(e);
+  if (lane == 0) ws[w] = __popcll(b);
+  __syncthreads();
+  int base = 0, before = 0, all = 0;
+  for (int i = 0; i < kSelBlock / 64; ++i) {
+    base += part[i];
+    before += i < w ? ws[i] : 0;
+    all += ws[i];
+  }
+  if (e) {
+    const unsigned long long lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int pos = base + before + __popcll(b & lower);
+    idx[pos] = t;
+    val[pos] = phi[LayerGrid::at(t, g.LX, g.LY).k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = base + all;
+}
+
 // Host FP64 domain array -> the domain of a torus plane.
 static __global__ void wall_scatter_kernel(WallGeo g, const double* __restrict__ in,
                                            double* __restrict__ out) {
@@ -665,9 +719,13 @@ class WallEngine final : public Engine {
       hcent_[i] = nullptr;
       cent_cap_[i] = 0;
     }
-    if (hphi_) (void)hipHostFree(hphi_);
-    hphi_ = nullptr;
-    hphi_n_ = 0;
+    if (sel_idx_) (void)hipFree(sel_idx_);
+    if (sel_val_) (void)hipFree(sel_val_);
+    if (hsel_idx_) (void)hipHostFree(hsel_idx_);
+    if (hsel_val_) (void)hipHostFree(hsel_val_);
+    sel_idx_ = hsel_idx_ = nullptr;
+    sel_val_ = hsel_val_ = nullptr;
+    sel_last_ = 0;
     div_ = death_ = nullptr;
     growth_ = false;
   }
@@ -1211,23 +1269,22 @@ class WallEngine final : public Engine {
   }
   // One mask of LyotropicWithDivision::SetMasks (lyotropic-with-division.cpp:
   // 34-108): due when its counter was 0; the centres are drawn on the host
-  // from phi in node order, the disks marked on the device.  The centres go
-  // through a pinned buffer per mask, so nothing waits for the copy: the next
-  // draw into that buffer comes after set_masks' next synchronous download.
-  hipError_t redraw(unsigned char* mask, const double* phi, double rate, double radius,
-                    bool division, hipStream_t s) {
+  // over the nodes with phi >= 1/2 in node order (the device's list), the
+  // disks marked on the device.  The centres go through a pinned buffer per
+  // mask, so nothing waits for the copy: the next draw into that buffer comes
+  // after set_masks' next synchronous download.
+  hipError_t redraw(unsigned char* mask, const int* sel_idx, const double* sel_phi, int nsel,
+                    double rate, double radius, bool division, hipStream_t s) {
     const int slot = division ? 0 : 1;
     std::vector<int>& centres = centres_;
     centres.clear();
-    for (int k = 0; k < LX_ * LY_; ++k) {
-      const double p = phi[k];
-      if (p >= .5) {
-        const double adjusted =
-            division ? rate * p * (1 - p / gp_.phi_critical) : rate * p;
-        if (next_uniform() < adjusted) {
-          centres.push_back(k / LY_);
-          centres.push_back(k % LY_);
-        }
+    for (int j = 0; j < nsel; ++j) {
+      const int k = sel_idx[j];
+      const double p = sel_phi[j];
+      const double adjusted = division ? rate * p * (1 - p / gp_.phi_critical) : rate * p;
+      if (next_uniform() < adjusted) {
+        centres.push_back(k / LY_);
+        centres.push_back(k % LY_);
       }
     }
     hipError_t e = hipMemsetAsync(mask, 0, (size_t)TP_, s);
@@ -1287,29 +1344,57 @@ class WallEngine final : public Engine {
       while (ufifo_.size() < want) ufifo_.push_back(u(*gen_));
     }
     const size_t used0 = udraws_;
-    // phi of the domain into a pinned host buffer (kept for the run)
-    const int N = LX_ * LY_;
+    // the nodes with phi >= 1/2 listed on the device and read back into
+    // pinned buffers (kept for the run): the count and as many entries as
+    // the last list held plus a margin, the rest only if the list grew more
+    const int N = LX_ * LY_, nsb = (N + kSelBlock - 1) / kSelBlock;
     hipError_t e = hipSuccess;
-    if ((size_t)N > hphi_n_) {
-      if (hphi_) (void)hipHostFree(hphi_);
-      hphi_ = nullptr;
-      hphi_n_ = 0;
-      e = hipHostMalloc((void**)&hphi_, sizeof(double) * N, hipHostMallocDefault);
+    if (!sel_idx_) {
+      e = hipMalloc((void**)&sel_idx_, sizeof(int) * ((size_t)N + nsb + 1));
+      if (e == hipSuccess) e = hipMalloc((void**)&sel_val_, sizeof(double) * N);
+      if (e == hipSuccess)
+        e = hipHostMalloc((void**)&hsel_idx_, sizeof(int) * ((size_t)N + 1), hipHostMallocDefault);
+      if (e == hipSuccess)
+        e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * N, hipHostMallocDefault);
       if (e != hipSuccess) return e;
-      hphi_n_ = (size_t)N;
     }
-    hipLaunchKernelGGL((wall_unpack_kernel<T>), dim3((N + 255) / 256), dim3(256), 0, s, geo(),
-                       (const T*)phi_[pc_], 1, stage_);
-    e = hipMemcpyAsync(hphi_, stage_, sizeof(double) * N, hipMemcpyDeviceToHost, s);
+    int* counts = sel_idx_ + N;
+    int* total = counts + nsb;
+    hipLaunchKernelGGL(select_count_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
+                       (const T*)phi_[pc_], counts);
+    hipLaunchKernelGGL(select_write_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
+                       (const T*)phi_[pc_], (const int*)counts, sel_idx_, sel_val_, total);
+    e = hipGetLastError();
+    const int guess = std::min(N, sel_last_ + sel_last_ / 4 + 1024);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hsel_idx_ + N, total, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hsel_idx_, sel_idx_, sizeof(int) * guess, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hsel_val_, sel_val_, sizeof(double) * guess, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    const double* phi = hphi_;
-    if (e == hipSuccess && div_old == 0) {
+    if (e != hipSuccess) return e;
+    const int nsel = hsel_idx_[N];
+    if (nsel < 0 || nsel > N) return hipErrorUnknown;
+    if (nsel > guess) {
+      const size_t rest = (size_t)(nsel - guess);
+      e = hipMemcpyAsync(hsel_idx_ + guess, sel_idx_ + guess, sizeof(int) * rest,
+                         hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(hsel_val_ + guess, sel_val_ + guess, sizeof(double) * rest,
+                           hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return e;
+    }
+    sel_last_ = nsel;
+    if (div_old == 0) {
       div_count_ = (uint32_t)gp_.division_time;
-      e = redraw(div_, phi, gp_.division_rate, gp_.division_radius, true, s);
+      e = redraw(div_, hsel_idx_, hsel_val_, nsel, gp_.division_rate, gp_.division_radius, true,
+                 s);
     }
     if (e == hipSuccess && death_old == 0) {
       death_count_ = (uint32_t)gp_.death_time;
-      e = redraw(death_, phi, gp_.death_rate, gp_.death_radius, false, s);
+      e = redraw(death_, hsel_idx_, hsel_val_, nsel, gp_.death_rate, gp_.death_radius, false, s);
     }
     uexpect_ = udraws_ - used0;
     return e;
@@ -1524,8 +1609,12 @@ class WallEngine final : public Engine {
   size_t ufront_ = 0;          // the next unused one
   size_t udraws_ = 0;          // numbers the draws have consumed
   size_t uexpect_ = 0;         // used by the last redraw
-  double* hphi_ = nullptr;     // pinned host copy of phi for the draws
-  size_t hphi_n_ = 0;
+  int* sel_idx_ = nullptr;     // nodes with phi >= 1/2: domain index (N), block
+                               // counts, the count
+  double* sel_val_ = nullptr;  // and their phi (N)
+  int* hsel_idx_ = nullptr;    // pinned host copies (the count at [N])
+  double* hsel_val_ = nullptr;
+  int sel_last_ = 0;           // the last list's length
 
   int bc_;
   Params<T> P_{};
