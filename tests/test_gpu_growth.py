@@ -174,3 +174,15 @@ def test_growth_reconfigure_restarts_the_draws(require_gpu):
         o2.step(5)
         check(a, o, STATE, "after reconfigure, %d steps" % (5 * s + 5))
         check(a, o2, STATE, "against a fresh run, %d steps" % (5 * s + 5))
+
+
+def test_growth_long_node_lists_bit_exact(require_gpu):
+    # about 2,000 nodes with phi >= 1/2 over four selection blocks: the first
+    # redraw reads back more of the device's node list than its first guess
+    a, o, ta, to = pair(64, 64, 0, npc=1, init=dict(INIT, radius=25.0))
+    assert ta == to
+    assert (o.get("phi") >= 0.5).sum() > 1024
+    for s in range(2):
+        a.step(4)
+        o.step(4)
+        check(a, o, STATE, "after %d steps" % (4 * s + 4))
