@@ -265,6 +265,15 @@ def main():
 
     fuse = 0 if args.no_fuse else -1
     alt = args.alt
+    extra = {}
+    # go-or-grow first: after a sustained 4096^2 leg the next few hundred ms of
+    # this launch-bound run are 25-30% slower
+    if not args.skip_growth:
+        log("go-or-grow 500x500: configure + %d + %d timed steps" % (args.growth_steps,
+                                                                    args.growth_steps))
+        extra["go_or_grow"] = growth_leg(lyo, args.growth_steps)
+        log("go-or-grow: %s" % extra["go_or_grow"])
+
     log("fp32 %dx%d npc=%d: %d warmup + %d timed steps" % (L, L, npc, args.warmup, args.steps))
     t32, k32 = gpu_run(lyo, L, lyo.FP32, npc, args.steps, args.warmup, args.chunk,
                        args.pair_chunk, fuse, alt, 0 if args.no_full else -1, args.step_chunk,
@@ -273,14 +282,6 @@ def main():
     log("fp32: %.3f ms/step, %.0f MLUPS, mean launch ms: %s"
         % (t32 / args.steps, mlups32, {k: round(v, 4) for k, v in k32.items() if v}))
 
-    extra = {}
-    # go-or-grow before the FP64 legs: their sustained FP64 load leaves the
-    # next few hundred ms of a launch-bound run about 30% slower
-    if not args.skip_growth:
-        log("go-or-grow 500x500: configure + %d + %d timed steps" % (args.growth_steps,
-                                                                    args.growth_steps))
-        extra["go_or_grow"] = growth_leg(lyo, args.growth_steps)
-        log("go-or-grow: %s" % extra["go_or_grow"])
     if not args.skip_fp64:
         s64 = max(2, args.steps // 4)
         log("fp64 %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
