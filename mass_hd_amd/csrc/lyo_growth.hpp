@@ -116,6 +116,7 @@ struct GrowthUF {
   const unsigned char* death;
   double* m_out;
   double* mn_out;          // predictor
+  double* chi_out;         // if set: chi = m/phi of the outputs (the phenotype pass)
   const double* countphi;  // conserve_phi: device sum of phi at the last UpdateQuantities
   double totalphi;
   int conserve;
@@ -251,6 +252,8 @@ __global__ void __launch_bounds__(256) growth_uf_kernel(GrowthUF a, Params<doubl
   else if (mt > upper)
     mt = upper;
   a.m_out[k] = mt;
+  // UpdatePhenotypeQuantities (go-or-grow.cpp:201-209) on the values just stored
+  if (a.chi_out) a.chi_out[k] = phi_new > 1e-12 ? mt / phi_new : 0.;
 }
 
 // UpdatePhenotypeQuantities (go-or-grow.cpp:201-209): chi = m/phi on the

@@ -707,7 +707,7 @@ class WallEngine final : public Engine {
     bytes_ = 0;
     F_[0] = F_[1] = Fn_[0] = Fn_[1] = phi_[0] = phi_[1] = nullptr;
     qxx_ = qyx_ = qnxx_ = qnyx_ = phn_ = D_ = qxx2_ = qyx2_ = nullptr;
-    m_[0] = m_[1] = mn_ = chi_ = part_ = stage_ = fe_ = nullptr;
+    m_[0] = m_[1] = mn_ = chi_ = chi2_ = part_ = stage_ = fe_ = nullptr;
     if (div_) (void)hipFree(div_);
     if (death_) (void)hipFree(death_);
     // the centre copies of the last draws may still be in flight
@@ -853,6 +853,10 @@ class WallEngine final : public Engine {
     std::swap(qyx_, qyx2_);
     qa_ ^= 1;
   }
+  void swap_chi() {
+    std::swap(chi_, chi2_);
+    ca_ ^= 1;
+  }
 
   hipError_t step_body(hipStream_t s) {
     if (!loops_) return fused_step(s);
@@ -891,16 +895,18 @@ class WallEngine final : public Engine {
 
   hipError_t graph_step(hipStream_t s) {
     for (const StepGraph& g : graphs_) {
-      if (g.cur == cur_ && g.pc == pc_ && g.mc == mc_ && g.qa == qa_ && g.loops == loops_) {
+      if (g.cur == cur_ && g.pc == pc_ && g.mc == mc_ && g.qa == qa_ && g.ca == ca_ &&
+          g.loops == loops_) {
         cur_ = g.cur_after;
         pc_ = g.pc_after;
         mc_ = g.mc_after;
         if (g.qa_after != qa_) swap_q();
+        if (g.ca_after != ca_) swap_chi();
         return hipGraphLaunch(g.exec, s);
       }
     }
-    StepGraph g{cur_, pc_, mc_, qa_, loops_, 0, 0, 0, 0, nullptr};
-    const int cur0 = cur_, pc0 = pc_, mc0 = mc_, qa0 = qa_;
+    StepGraph g{cur_, pc_, mc_, qa_, ca_, loops_, 0, 0, 0, 0, 0, nullptr};
+    const int cur0 = cur_, pc0 = pc_, mc0 = mc_, qa0 = qa_, ca0 = ca_;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
@@ -914,12 +920,14 @@ class WallEngine final : public Engine {
       pc_ = pc0;
       mc_ = mc0;
       if (qa_ != qa0) swap_q();
+      if (ca_ != ca0) swap_chi();
       return e;
     }
     g.cur_after = cur_;
     g.pc_after = pc_;
     g.mc_after = mc_;
     g.qa_after = qa_;
+    g.ca_after = ca_;
     graphs_.push_back(g);
     return hipGraphLaunch(g.exec, s);
   }
@@ -1167,6 +1175,11 @@ class WallEngine final : public Engine {
     a.totalphi = totalphi_;
     a.conserve = conserve_ ? 1 : 0;
     const bool div_only = gp_.division_only != 0;
+    // on a periodic grid no ghost condition follows the phenotype pass, so
+    // the kernel writes chi = m/phi itself, into the other chi buffer (it
+    // reads chi at the neighbours)
+    const bool fold = bc_ == 0 && !div_only;
+    a.chi_out = fold ? chi2_ : nullptr;
     by_grid([&](auto gg) {
       using G = decltype(gg);
       if (div_only) {
@@ -1190,7 +1203,10 @@ class WallEngine final : public Engine {
     }
     pc_ ^= 1;
     mc_ ^= 1;
-    phenotype(s);
+    if (fold)
+      swap_chi();
+    else
+      phenotype(s);
   }
   void fields2_bc(hipStream_t s) {
     for (int ph : phases())
@@ -1409,6 +1425,7 @@ class WallEngine final : public Engine {
       for (int i = 0; i < 2 && e == hipSuccess; ++i) e = get_(n1, &m_[i]);
       if (e == hipSuccess) e = get_(n1, &mn_);
       if (e == hipSuccess) e = get_(n1, &chi_);
+      if (e == hipSuccess) e = get_(n1, &chi2_);
       if (e == hipSuccess) e = get_(2 * kGrowthParts + 2, &part_);
       if (e == hipSuccess) e = get_((size_t)LX_ * LY_, &stage_);
       if (e == hipSuccess) e = get_(9, &fe_);
@@ -1571,9 +1588,9 @@ class WallEngine final : public Engine {
  private:
   // a captured step: the parities it starts from and leaves
   struct StepGraph {
-    int cur, pc, mc, qa;
+    int cur, pc, mc, qa, ca;
     bool loops;
-    int cur_after, pc_after, mc_after, qa_after;
+    int cur_after, pc_after, mc_after, qa_after, ca_after;
     hipGraphExec_t exec;
   };
   std::vector<StepGraph> graphs_;
@@ -1596,6 +1613,8 @@ class WallEngine final : public Engine {
   T* m_[2] = {};
   T* mn_ = nullptr;
   T* chi_ = nullptr;
+  T* chi2_ = nullptr;  // periodic go-or-grow: UpdateFields writes chi here
+  int ca_ = 0;         // which of chi_/chi2_ holds chi (graph key)
   T* part_ = nullptr;   // fixed-tree sum partials; [2*parts+1]: sum of phi
   T* stage_ = nullptr;  // N doubles: host <-> domain staging
   T* fe_ = nullptr;
