@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblyo_oracle.so")
+# LYO_ORACLE_LIB points the tests at another build of the oracle (the
+# planted-mistake checks build altered copies of lyo_oracle.c)
+LIB_PATH = os.environ.get("LYO_ORACLE_LIB") or os.path.join(_HERE, "liblyo_oracle.so")
 
 PHYS_KEYS = ["GammaQ", "GammaP", "xi", "tauNem", "tauIso", "rho", "friction",
              "AA", "CC", "LL", "KK", "zeta", "zetaI", "conc"]
