@@ -15,23 +15,31 @@ MLUPS = LX*LY*steps / seconds / 1e6; one step = predictor + npc correctors
 by HIP events on the library's stream; the fields are resident in HBM
 throughout.
 
-roofline: the kernel with the longest mean launch.  Its algorithmic bytes are
-the per-sweep figure of the step's data flow -- one sweep reads and writes the
-model state and the predictor's half-step state once, 36 values per node
-(predictor: read f, Q, phi = 12, write f*, fn*, Q+D, Q+D/2, phi terms = 24;
-corrector: read 24, write 12) -- times the sweeps one launch performs: 1 for
-a predictor or corrector sweep, 2 for the fused last corrector pair, 1 + npc
-for the one-kernel step.  In FP32 that is 144 B per node per sweep against
-8 TB/s.  The fused kernels keep intermediate results in registers and LDS, so
-they move fewer distinct bytes than their sweeps' worth:
-"distinct_bytes_per_launch" counts every plane a launch touches once (one-
-kernel step: read and write f', Q, phi, phic = 26 values; pair: 38;
-predictor: 24), "traffic" is the HBM bytes the counters measured per launch
-and "valu_busy_frac" the counted VALU instructions' issue time over the
-launch time (profiles/).
+roofline: the kernel with the longest mean launch (the one-kernel step by
+default).  Its algorithmic bytes are the compulsory HBM traffic of one launch:
+the model state read once and written once, 12 values per node each way
+(f' 9, Qxx, Qyx, phi), 96 B per node in FP32.  The compensation term of phi
+(4 B each way) is overhead, not algorithm.  "per_sweep_equivalent" keeps the
+per-sweep figure of the step's data flow (36 values per node per sweep,
+3 sweeps at npc = 2: 432 B per node), which counts the predictor's
+half-step state that the fused kernel keeps in registers and LDS.
+"traffic" is the HBM bytes per launch that the rocprofv3 counters measured
+(profiles/traffic_fp32_4096_npc2.json).  The kernel is bound by VALU issue,
+not HBM: "valu" gives its counted VALU instructions (SQ_INSTS_VALU) at
+4 cycles per wave64 instruction on 256 CUs x 4 SIMDs at the 2.4 GHz peak
+engine clock, against the measured launch time.
 
-cpu_baseline: the CPU oracle (the reference's algorithm and layout, OpenMP)
-on the host cores.
+Leg order: the FP32 headline run is configured first; the FP64 leg then runs
+on its own handle (its warm-up long enough for the engine clock to settle
+after the load step: a rocprofv3 GRBM_GUI_ACTIVE trace shows the clock
+dipping to ~1.6 GHz for ~30 ms after work starts on an idle GPU), and the
+headline's warm-up and timed steps follow with no idle gap in between.
+
+cpu_baseline: the CPU oracle (the reference's algorithm and AoS layout,
+built with the reference's -O3 -fopenmp) in a child process on the host: one
+thread (the reference's default) at 1024^2 and the GPU's share of the host
+cores (16, or fewer if fewer are available) at the headline size, each
+5 warm-up + 20 timed steps.
 """
 import argparse
 import json
@@ -46,36 +54,55 @@ PHYS = dict(GammaQ=0.1, GammaP=0.1, xi=0.7, tauNem=2.0, tauIso=2.0, rho=40.0, fr
             AA=0.01, CC=0.0, LL=0.04, KK=0.04, zeta=0.03, zetaI=0.0, conc=0.5)
 INIT = dict(config="random", seed=7, angle=90.0, noise=1.0, qamp=0.5, phinoise=0.05)
 HBM_PEAK = 8.0e12
+VALU_PEAK_INSTR_PER_S = 256 * 4 * 2.4e9 / 4     # wave64 VALU instructions per second
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fp32_4096_npc2.json")
-# algorithmic values per node per launch (x4 bytes in FP32)
+LAUNCHES = ("predictor", "corrector", "pair", "step")
+# compulsory values per node a launch must read and write (FP32 x4 bytes):
+# the one-kernel step reads and writes the state (f' 9, Q 2, phi); the
+# predictor reads the state and writes its outputs (f* 9, Q* 2, dphi); a
+# corrector or the fused pair reads both and writes the next state
+COMPULSORY = {"predictor": 24, "corrector": 36, "pair": 36, "step": 24}
 SWEEPS = {"predictor": 1, "corrector": 1, "pair": 2, "step": 3}   # sweeps per launch
 SWEEP_VALUES = 36                                        # values per node per sweep
-DISTINCT = {"predictor": 24, "corrector": 37, "pair": 38, "step": 26}  # FP32, npc=2 data flow
 KERNEL = {"predictor": "sweep_kernel<float, false, 0, false>",
           "corrector": "sweep_kernel<float, false, 1, true>",
           "pair": "pair_kernel<float, false, true>",
-          "step": "full_kernel<float>"}
+          "step": "full_kernel<float, false, 2>"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_run(lyo, L, precision, npc, steps, warmup, chunk=0, pair_chunk=0, fuse=-1, alt=-1,
-            full=-1, step_chunk=0, wpb=0, loops=False, bc=0):
+def make_run(lyo, L, precision, npc, chunk=0, pair_chunk=0, fuse=-1, alt=-1, full=-1,
+             step_chunk=0, wpb=0, loops=False, bc=0):
     m = lyo.Lyotropic(L, L, npc=npc, precision=precision, bc=bc, **PHYS)
     m.set_launch(chunk, wpb)
     m.set_schedule(fuse, pair_chunk, alt, full, step_chunk)
     if loops:
         m.set_loop_kernels(True)
     m.configure(**INIT)
+    m.synchronize()
+    return m
+
+
+def timed(m, steps, warmup):
+    """warmup untimed steps, then `steps` steps between HIP events on the
+    run's stream; returns (total ms, mean launch ms per launch kind)"""
     if warmup:
         m.step(warmup)
     m.synchronize()
     total_ms, sweeps = m.time_steps(steps, per_sweep=True)
     m.runtime_checks()
-    m.close()
-    return total_ms, dict(zip(("predictor", "corrector", "pair", "step"), sweeps))
+    return total_ms, dict(zip(LAUNCHES, sweeps))
+
+
+def gpu_run(lyo, L, precision, npc, steps, warmup, **kw):
+    m = make_run(lyo, L, precision, npc, **kw)
+    try:
+        return timed(m, steps, warmup)
+    finally:
+        m.close()
 
 
 # the reference's go-or-grow proliferation runcard
@@ -154,32 +181,58 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warm
             "train_loss": [round(l0, 4), round(l1, 5)]}
 
 
-def cpu_baseline(L, steps, threads, bc=0):
-    from oracle import pyoracle as po
-    o = po.Oracle(L, L, npc=2, nthreads=threads, bc=bc, **PHYS)
-    o.configure(**INIT)
-    o.step(1)  # warm-up (first touch of the work arrays)
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
-    o.close()
-    return L * L * steps / dt / 1e6
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail}
 
 
-def cpu_growth(steps, threads, relax_steps=200, L=500):
-    """The oracle on the go-or-grow runcard (sample: 200 instead of 2,000
-    relaxation steps in configure, then `steps` timed steps; the mask draws
-    are serial as in the reference)."""
+def cpu_child(spec):
+    """Runs in a child process (so the OpenMP placement variables take
+    effect): times the oracle on one workload and prints its MLUPS."""
     from oracle import pyoracle as po
-    o = po.Oracle(L, L, npc=1, nthreads=threads, **GROWTH_PHYS)
-    o.set_growth(**dict(GROWTH_PARAMS, relax_steps=relax_steps))
-    o.configure(**GROWTH_INIT)
-    o.step(1)
+    L, threads = spec["L"], spec["threads"]
+    if spec["kind"] == "growth":
+        o = po.Oracle(L, L, npc=1, nthreads=threads, **GROWTH_PHYS)
+        o.set_growth(**dict(GROWTH_PARAMS, relax_steps=spec["relax_steps"]))
+        o.configure(**GROWTH_INIT)
+    else:
+        o = po.Oracle(L, L, npc=spec["npc"], nthreads=threads, bc=spec["bc"], **PHYS)
+        o.configure(**INIT)
+    o.step(spec["warmup"])
     t0 = time.perf_counter()
-    o.step(steps)
+    o.step(spec["steps"])
     dt = time.perf_counter() - t0
     o.close()
-    return L * L * steps / dt / 1e6
+    print(json.dumps({"mlups": L * L * spec["steps"] / dt / 1e6, "seconds": dt}), flush=True)
+
+
+def cpu_time(kind, L, steps, warmup, threads, npc=2, bc=0, relax_steps=0):
+    """The oracle in a child process: threads = 0 is the reference's serial
+    default (OpenMP off), otherwise OpenMP with OMP_PLACES=cores and
+    OMP_PROC_BIND=close."""
+    import subprocess
+    spec = dict(kind=kind, L=L, steps=steps, warmup=warmup, threads=threads, npc=npc, bc=bc,
+                relax_steps=relax_steps)
+    env = dict(os.environ, OMP_PLACES="cores", OMP_PROC_BIND="close",
+               OMP_NUM_THREADS=str(max(1, threads)))
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", json.dumps(spec)],
+                       env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline child failed: %s" % r.stderr[-2000:])
+    return json.loads(r.stdout.strip().splitlines()[-1])["mlups"]
 
 
 def calibrate(gib=1, reps=10):
@@ -218,9 +271,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate", action="store_true",
                     help="time plain 1 GiB device copies (HBM ceiling, PMC scale)")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--gpus", type=int, default=1)
-    # 200 + 500 steps: the throughput of a long run (the first few dozen steps
-    # after start-up run ~8% slower at 4096^2)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=4096)
@@ -242,15 +294,24 @@ def main():
     ap.add_argument("--skip-growth", action="store_true",
                     help="skip the go-or-grow leg (500^2 proliferation runcard)")
     ap.add_argument("--growth-steps", type=int, default=300)
+    ap.add_argument("--fp64-preload-ms", type=float, default=80.0,
+                    help="minimum untimed FP64 work before its timed steps (clock settling)")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-warmup", type=int, default=5)
+    ap.add_argument("--cpu-size-1t", type=int, default=1024,
+                    help="grid of the single-thread CPU sample")
     ap.add_argument("--skip-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=50,
                     help="timed steps of the surrogate leg (50: one judged step among them)")
     ap.add_argument("--surrogate-train", type=int, default=500,
                     help="Adam iterations of the surrogate leg's training")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the multi-core CPU sample (0: min(16, available))")
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(json.loads(args.cpu_child))
+        return
     if args.calibrate:
         calibrate()
         return
@@ -262,39 +323,48 @@ def main():
     from mass_hd_amd import lyo
     L, npc = args.size, args.npc
     N = L * L
-
-    fuse = 0 if args.no_fuse else -1
-    alt = args.alt
+    sched = dict(chunk=args.chunk, pair_chunk=args.pair_chunk, fuse=0 if args.no_fuse else -1,
+                 alt=args.alt, full=0 if args.no_full else -1, step_chunk=args.step_chunk,
+                 wpb=args.wpb)
     extra = {}
-    # go-or-grow first: after a sustained 4096^2 leg the next few hundred ms of
-    # this launch-bound run are 25-30% slower
+    # go-or-grow first: it is launch-bound, and right after a sustained 4096^2
+    # leg its first few hundred ms run 25-30% slower
     if not args.skip_growth:
         log("go-or-grow 500x500: configure + %d + %d timed steps" % (args.growth_steps,
                                                                     args.growth_steps))
         extra["go_or_grow"] = growth_leg(lyo, args.growth_steps)
         log("go-or-grow: %s" % extra["go_or_grow"])
 
+    log("fp32 %dx%d npc=%d: configure" % (L, L, npc))
+    m32 = make_run(lyo, L, lyo.FP32, npc, **sched)
+    m64 = None
+    if not args.skip_fp64:
+        s64 = max(2, args.steps // 4)
+        m64 = make_run(lyo, L, lyo.FP64, npc, **sched)
+        # untimed FP64 steps until the engine clock has settled, then the
+        # timed FP64 steps; the FP32 steps follow right after
+        t_one, _ = m64.time_steps(1, per_sweep=False)
+        w64 = max(2, args.warmup // 4, int(args.fp64_preload_ms / max(t_one, 1e-3)) + 1)
+        log("fp64 %dx%d npc=%d: %d warm-up + %d timed steps" % (L, L, npc, w64, s64))
+        t64, k64 = timed(m64, s64, w64)
+        mlups64 = N * s64 / (t64 / 1e3) / 1e6
+        log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
     log("fp32 %dx%d npc=%d: %d warmup + %d timed steps" % (L, L, npc, args.warmup, args.steps))
-    t32, k32 = gpu_run(lyo, L, lyo.FP32, npc, args.steps, args.warmup, args.chunk,
-                       args.pair_chunk, fuse, alt, 0 if args.no_full else -1, args.step_chunk,
-                       args.wpb)
+    t32, k32 = timed(m32, args.steps, args.warmup)
+    m32.close()
+    if m64 is not None:
+        m64.close()
     mlups32 = N * args.steps / (t32 / 1e3) / 1e6
     log("fp32: %.3f ms/step, %.0f MLUPS, mean launch ms: %s"
         % (t32 / args.steps, mlups32, {k: round(v, 4) for k, v in k32.items() if v}))
 
     if not args.skip_fp64:
-        s64 = max(2, args.steps // 4)
-        log("fp64 %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
-        w64 = max(2, args.warmup // 4)
-        t64, k64 = gpu_run(lyo, L, lyo.FP64, npc, s64, w64, args.chunk, 0, fuse, alt,
-                           0 if args.no_full else -1, args.step_chunk, args.wpb)
-        mlups64 = N * s64 / (t64 / 1e3) / 1e6
-        log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
         extra.update({"fp64_mlups": round(mlups64, 1),
                       "fp32_over_fp64": round(mlups32 / mlups64, 3),
                       "fp64_kernel_set": "fused strip-march kernels (one-kernel step for "
                                          "npc=2), reference expression order",
-                      "fp64_launch_ms": {k: round(v, 4) for k, v in k64.items() if v}})
+                      "fp64_launch_ms": {k: round(v, 4) for k, v in k64.items() if v},
+                      "fp64_steps": {"warmup": w64, "timed": s64}})
         if not args.skip_fp64_loops:
             log("fp64 loop kernels %dx%d npc=%d: %d timed steps" % (L, L, npc, s64))
             tl, _ = gpu_run(lyo, L, lyo.FP64, npc, s64, w64, loops=True)
@@ -322,18 +392,42 @@ def main():
                 extra["fp64_walls_bc4_loops_mlups"] = round(mlupswl, 1)
 
     dom = max((k for k in k32 if k32[k] > 0), key=lambda k: k32[k])
-    # the one-kernel step performs all 1 + npc sweeps of a step
+    kms = k32[dom]
+    alg_bytes = COMPULSORY[dom] * 4 * N
+    achieved = alg_bytes / (kms / 1e3)
     sweeps = (1 + npc) if dom == "step" else SWEEPS[dom]
-    alg_bytes = sweeps * SWEEP_VALUES * 4 * N
-    achieved = alg_bytes / (k32[dom] / 1e3)
-    distinct = DISTINCT[dom] * 4 * N
-    traffic = valu_ms = None
+    per_sweep_bytes = sweeps * SWEEP_VALUES * 4 * N
+    prof = {}
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        if t.get("size") == L and t.get("npc") == npc and dom in t.get("kernels", {}):
-            traffic = t["kernels"][dom].get("bytes_per_launch")
-            valu_ms = t["kernels"][dom].get("valu_issue_ms")
+        if t.get("size") == L and t.get("npc") == npc:
+            prof = t.get("kernels", {}).get(dom, {})
+    traffic = prof.get("bytes_per_launch")
+    valu_instr = prof.get("valu_instructions")
+    roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK, 4),
+                "traffic": traffic,
+                "kernel": KERNEL[dom], "kernel_ms": round(kms, 4),
+                "alg_bytes_per_launch": alg_bytes,
+                "alg_bytes_per_node": COMPULSORY[dom] * 4,
+                "traffic_GBps": (round(traffic / (kms / 1e3) / 1e9, 1) if traffic else None),
+                "traffic_frac": (round(traffic / (kms / 1e3) / HBM_PEAK, 4) if traffic else None),
+                "per_sweep_equivalent": {
+                    "bytes_per_node": sweeps * SWEEP_VALUES * 4, "sweeps": sweeps,
+                    "GBps": round(per_sweep_bytes / (kms / 1e3) / 1e9, 1),
+                    "note": "the step's data flow counted per sweep, including the predictor's "
+                            "half-step state that the fused kernel keeps on chip"},
+                "launch_ms": {k: round(v, 4) for k, v in k32.items() if v}}
+    assert roofline["frac"] <= 1.0, roofline
+    if valu_instr:
+        issue_ms = valu_instr / VALU_PEAK_INSTR_PER_S * 1e3
+        roofline["valu"] = {"instructions_per_launch": valu_instr,
+                            "issue_ms_at_peak_clock": round(issue_ms, 4),
+                            "frac": round(issue_ms / kms, 3),
+                            "note": "SQ_INSTS_VALU x 4 cycles / (256 CUs x 4 SIMDs x 2.4 GHz) "
+                                    "over the measured launch time: the bound of this kernel"}
 
     if not args.skip_surrogate:
         log("surrogate %dx%d: training on 512^2 frames, %d timed steps" % (L, L, args.surrogate_steps))
@@ -344,24 +438,37 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("cpu baseline: oracle %dx%d npc=2, %d threads, 1 warmup + %d steps"
-            % (L, L, threads, args.cpu_steps))
-        v = cpu_baseline(L, args.cpu_steps, threads)
-        cpu = {"value": round(v, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-               "sample": "%dx%d FP64 npc=2, %d steps after 1 warm-up step, OpenMP" %
-                         (L, L, args.cpu_steps)}
-        log("cpu: %.2f MLUPS" % v)
+        info = cpu_info()
+        mt = args.cpu_threads or min(16, info["cpus_available"])
+        cw, cs = args.cpu_warmup, args.cpu_steps
+        L1 = min(args.cpu_size_1t, L)
+        log("cpu baseline: oracle %dx%d npc=%d, %d threads, %d + %d steps"
+            % (L, L, npc, mt, cw, cs))
+        vm = cpu_time("lyo", L, cs, cw, mt, npc=npc)
+        log("cpu: %.2f MLUPS (%d threads)" % (vm, mt))
+        log("cpu baseline: oracle %dx%d npc=%d, 1 thread, %d + %d steps" % (L1, L1, npc, cw, cs))
+        v1 = cpu_time("lyo", L1, cs, cw, 0, npc=npc)
+        log("cpu: %.2f MLUPS (1 thread)" % v1)
+        cpu = {"value": round(vm, 2), "unit": "MLUPS", "cores": mt, "kind": "port",
+               "sample": "%dx%d FP64 npc=%d, %d warm-up + %d timed steps, OpenMP %d threads "
+                         "(OMP_PLACES=cores, OMP_PROC_BIND=close)" % (L, L, npc, cw, cs, mt),
+               "single_thread": {"value": round(v1, 2), "unit": "MLUPS", "cores": 1,
+                                 "sample": "%dx%d FP64 npc=%d, %d warm-up + %d timed steps, "
+                                           "serial (the reference's default)" % (L1, L1, npc,
+                                                                                   cw, cs)},
+               "build": "gcc -O3 -fopenmp -ffp-contract=off (oracle/Makefile)",
+               "host": dict(info, cores_note="a GPU box allots 16 host CPUs per GPU"),
+               "gpu_over_cpu": round(mlups32 / vm, 1)}
         # the reference's CPU path beside the walls and go-or-grow legs
         if "fp64_walls_bc4_mlups" in extra:
-            vw = cpu_baseline(L, args.cpu_steps, threads, bc=4)
-            extra["cpu_walls_bc4"] = {"value": round(vw, 2), "unit": "MLUPS", "cores": threads,
-                                      "kind": "port", "sample": "%dx%d FP64 bc=4 npc=2, %d steps "
-                                      "after 1 warm-up step, OpenMP" % (L, L, args.cpu_steps)}
+            vw = cpu_time("lyo", L, cs, cw, mt, npc=npc, bc=4)
+            extra["cpu_walls_bc4"] = {"value": round(vw, 2), "unit": "MLUPS", "cores": mt,
+                                      "kind": "port", "sample": "%dx%d FP64 bc=4 npc=%d, %d "
+                                      "warm-up + %d timed steps, OpenMP" % (L, L, npc, cw, cs)}
             log("cpu walls: %.2f MLUPS" % vw)
         if "go_or_grow" in extra:
-            vg = cpu_growth(20, threads)
-            extra["cpu_go_or_grow"] = {"value": round(vg, 2), "unit": "MLUPS", "cores": threads,
+            vg = cpu_time("growth", 500, 20, 1, mt, relax_steps=200)
+            extra["cpu_go_or_grow"] = {"value": round(vg, 2), "unit": "MLUPS", "cores": mt,
                                        "kind": "port", "sample": "500x500 go-or-grow runcard with "
                                        "200 relaxation steps, 20 steps after 1 warm-up step, "
                                        "OpenMP loops, serial mask draws"}
@@ -383,21 +490,7 @@ def main():
         "config": {"workload": "lyotropic active nematic %dx%d periodic D2Q9, FP32 shifted "
                                "populations, npc=%d, zeta=0.03" % (L, L, npc),
                    "LX": L, "LY": L, "npc": npc},
-        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": traffic,
-                     "kernel": KERNEL[dom], "kernel_ms": round(k32[dom], 4),
-                     "alg_bytes_per_launch": alg_bytes,
-                     "distinct_bytes_per_launch": distinct,
-                     "step_alg_GBps": round((1 + npc) * SWEEP_VALUES * 4 * N * args.steps
-                                            / (t32 / 1e3) / 1e9, 1),
-                     "launch_ms": {k: round(v, 4) for k, v in k32.items() if v},
-                     "traffic_GBps": (round(traffic / (k32[dom] / 1e3) / 1e9, 1)
-                                      if traffic else None),
-                     # the fused step is bound by VALU issue: profiled VALU
-                     # instructions x 4 cycles over the measured launch time
-                     "valu_busy_frac": (round(valu_ms / k32[dom], 3) if valu_ms else None)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -169,7 +169,8 @@ int mass_lyo_configure(mass_lyo_t h, const mass_lyo_init* init, double* ftot,
                        double* ptot);
 
 /* Uploads an arbitrary state: ff (N*9 AoS), Qxx, Qyx, phi (N each).  Derived
- * fields are reset to zero. */
+ * fields are reset to zero.  Refused (MASS_LYO_EINVAL) on a go-or-grow run,
+ * whose state also holds m, chi and the division/death masks. */
 int mass_lyo_set_state(mass_lyo_t h, const double* ff, const double* Qxx,
                        const double* Qyx, const double* phi);
 

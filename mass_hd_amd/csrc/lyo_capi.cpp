@@ -196,6 +196,9 @@ int mass_lyo_set_state(mass_lyo_t h, const double* ff, const double* qxx, const 
                        const double* phi) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (!ff || !qxx || !qyx || !phi) return fail(h, MASS_LYO_EINVAL, "null field pointer");
+  // a go-or-grow run's state also holds m, chi, the masks, their counters and
+  // the uniforms drawn ahead, which this call cannot set consistently
+  if (h->growth) return fail(h, MASS_LYO_EINVAL, "go-or-grow runs take their state from configure");
   return upload_state(h, ff, qxx, qyx, phi);
 }
 

@@ -1,9 +1,12 @@
 """Precision and statistics reports of the GPU path.
 
-precision_report: the same run in FP64 (reference arithmetic) and FP32
-(shifted populations, compensated phi) on the GPU, started from the same
-initial fields; per-field max and rms relative error of FP32 against FP64,
-the drift of sum(phi) in both, and the diagnostics (u_rms, defects) of both.
+precision_report: the same run in FP64 (reference arithmetic, bitwise the
+CPU oracle's) and FP32 (shifted populations, compensated phi) on the GPU,
+started from the same initial fields; for every field of the run (the state
+f, Qxx, Qyx, phi and every derived frame field: n, u, the stresses, MU, H)
+the max and rms error of FP32 against FP64 relative to the field's largest
+FP64 magnitude, the drift of sum(phi) in both, and the diagnostics (u_rms,
+defects, phi histogram) of both.
 
     python -m mass_hd_amd.reports precision --size 1024 --steps 10000 --npc 2
 """
@@ -21,7 +24,8 @@ CONFIG0_PHYS = dict(GammaQ=0.1, GammaP=0.1, xi=0.7, tauNem=2.0, tauIso=2.0, rho=
                     friction=0.0, AA=0.01, CC=0.0, LL=0.04, KK=0.04, zeta=0.02, zetaI=0.0,
                     conc=0.5)
 CONFIG0_INIT = dict(config="random", seed=1, angle=90.0, noise=1.0, qamp=0.5, phinoise=0.05)
-FIELDS = ["ff", "QQxx", "QQyx", "phi", "ux", "uy", "sigmaXX", "sigmaXY"]
+# every field of a plain lyotropic run (state and derived frame fields)
+FIELDS = [k for k, v in sorted(lyo.FIELDS.items(), key=lambda kv: kv[1]) if v < lyo.FIELDS["m"]]
 
 
 def _rel_errors(a, b):
@@ -60,6 +64,8 @@ def precision_report(size=1024, steps=10000, npc=2, chunks=10, phys=None, init=N
         rec["u_rms"] = [d64["u_rms"], d32["u_rms"]]
         rec["defects"] = [d64["defects_pos"] + d64["defects_neg"],
                           d32["defects_pos"] + d32["defects_neg"]]
+        h64, h32 = (np.asarray(d["phi_hist"], dtype=float) for d in (d64, d32))
+        rec["phi_hist_l1"] = float(np.abs(h64 / h64.sum() - h32 / h32.sum()).sum())
         out["series"].append(rec)
         if progress:
             progress("step %d: max_rel ff %.2e Q %.2e phi %.2e u %.2e  (%.0fs)" % (

@@ -1,0 +1,206 @@
+"""The benchmark configurations on the GPU, shortened, through the C ABI.
+
+config 1: 1024^2 npc=2 -- FP64 bitwise the CPU oracle, FP32 against FP64 for
+          every field within the FP32 error budget, sum(phi) drift
+config 2: the 64-run sweep (L in {128..512}, zeta log-uniform, xi in
+          {0.5, 0.7, 1.0}) in one autotuned batch -- every run bitwise its
+          solo run; an FP64 batch of one run per size bitwise the oracle
+config 3: 512^2 surrogate corrector, briefly trained -- every judged
+          velocity error within 1e-3 or a recorded fallback
+config 4: 4096^2 FP32 npc=2 with the benchmark's launch plan (one-kernel
+          step, its wave-round chunk length) -- bitwise the separate sweeps,
+          within the FP32 budget of the FP64 run, which is bitwise the oracle
+long run: 512^2 FP32 vs FP64 over 6,000 steps -- defect density, u_rms and
+          the phi histogram within 2%
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mass_hd_amd import lyo, reports, workloads
+from oracle import pyoracle as po
+from tests.conftest import CONFIG0_INIT, CONFIG0_PHYS
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("ff", "QQxx", "QQyx", "phi")
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def run(L, precision, npc=2, steps=0, phys=CONFIG0_PHYS, init=CONFIG0_INIT, **sched):
+    m = lyo.Lyotropic(L, L, npc=npc, precision=precision, **phys)
+    if sched:
+        m.set_schedule(**sched)
+    m.configure(**init)
+    if steps:
+        m.step(steps)
+    return m
+
+
+def oracle_run(L, npc, steps, phys=CONFIG0_PHYS, init=CONFIG0_INIT):
+    o = po.Oracle(L, L, npc=npc, nthreads=cpu_threads(), **phys)
+    o.configure(**init)
+    o.step(steps)
+    return o
+
+
+def rel_err(a, b):
+    return reports._rel_errors(a, b)
+
+
+# FP32 budget after a few hundred steps: max / rms error relative to the
+# field's largest FP64 magnitude, about ten times what 1024^2 after 300
+# steps shows on an MI355X (f 1.3e-9 / 9e-11, Q 2.0e-6 / 1.2e-7, phi
+# 4.3e-8 / 7.3e-9; the derived fields at most 3.9e-5 / 8.2e-6, MU)
+BUDGET = {"ff": (1e-8, 1e-9), "QQxx": (2e-5, 2e-6), "QQyx": (2e-5, 2e-6),
+          "phi": (5e-7, 1e-7)}
+DERIVED_BUDGET = (4e-4, 8e-5)
+
+
+def test_config1_fp64_bitwise_and_fp32_every_field(require_gpu):
+    L, steps = 1024, 300
+    print("config1: %d^2 npc=2, %d steps, FP64 / FP32 / oracle" % (L, steps), flush=True)
+    m64 = run(L, lyo.FP64, steps=steps)
+    m32 = run(L, lyo.FP32, steps=steps)
+    o = oracle_run(L, 2, steps)
+    for f in STATE + ("sigmaXY", "MU"):
+        assert np.array_equal(m64.get(f), o.get(f)), f
+    p0 = o.sums()["ptot"]
+    errs = {f: rel_err(m32.get(f), m64.get(f)) for f in reports.FIELDS}
+    print("config1 FP32 vs FP64 (max, rms):", errs, flush=True)
+    bad = {f: e for f, e in errs.items()
+           if not (e[0] < BUDGET.get(f, DERIVED_BUDGET)[0]
+                   and e[1] < BUDGET.get(f, DERIVED_BUDGET)[1])}
+    assert not bad, (bad, errs)
+    # compensated phi: the FP32 total drifts far below FP32 rounding
+    drift32 = abs(m32.sums()[1] - p0) / p0
+    drift64 = abs(m64.sums()[1] - p0) / p0
+    assert drift32 < 1e-12 and drift64 < 1e-13, (drift32, drift64)
+    m64.close()
+    m32.close()
+
+
+def test_config2_sweep_batch_equals_solo_runs(require_gpu):
+    runs = workloads.config2_runs()
+    assert len(runs) == 64 and {r["LX"] for r in runs} == set(workloads.SIZES)
+    steps = 6
+    print("config2: 64 runs, %d nodes, %d steps" % (sum(r["LX"] ** 2 for r in runs), steps),
+          flush=True)
+    b, tuned = workloads.make_batch(runs)
+    b.step(steps)
+    for i, r in enumerate(runs):
+        phys = {k: r[k] for k in lyo.PHYS_KEYS}
+        m = run(r["LX"], lyo.FP32, steps=steps, phys=phys,
+                init=dict(CONFIG0_INIT, seed=r["seed"]))
+        for f in STATE:
+            assert np.array_equal(b.get(i, f), m.get(f)), (i, r["LX"], f, tuned)
+        m.close()
+    b.close()
+
+
+def test_config2_fp64_batch_equals_oracle(require_gpu):
+    runs = workloads.config2_runs()
+    subset = []
+    for L in workloads.SIZES:  # the first run of each size
+        subset.append(next(r for r in runs if r["LX"] == L))
+    steps = 4
+    b, _ = workloads.make_batch(subset, precision="fp64", autotune=False)
+    b.step(steps)
+    for i, r in enumerate(subset):
+        phys = {k: r[k] for k in lyo.PHYS_KEYS}
+        o = oracle_run(r["LX"], 2, steps, phys=phys, init=dict(CONFIG0_INIT, seed=r["seed"]))
+        for f in STATE:
+            assert np.array_equal(b.get(i, f), o.get(f)), (r["LX"], f)
+        o.close()
+    b.close()
+
+
+def test_config3_surrogate_judged_error_or_fallback(require_gpu):
+    L, K, tol = 512, 50, 1e-3
+    net = lyo.Net(seed=0)
+    src = run(L, lyo.FP32, steps=200)
+    for _ in range(6):
+        src.step(25)
+        net.collect(src)
+    src.close()
+    l0, l1 = net.train(300, lr=2e-3)
+    assert l1 < 0.1 * l0
+    m = run(L, lyo.FP32, steps=200)
+    lyo.enable_surrogate(m, net, K, tol)
+    m.step(6 * K)
+    st = lyo.surrogate_stats(m)
+    assert st["steps"] == 6 * K and st["judged"] >= 5
+    assert st["surrogate_steps"] + st["plain_steps"] == st["steps"]
+    # every judgement above tolerance hands the next K - 1 steps to the
+    # plain corrector; without one, every judged error was within tolerance
+    if st["fallbacks"] == 0:
+        assert st["max_error"] <= tol, st
+    else:
+        # (the last window may run past the end of the run)
+        assert st["plain_steps"] >= st["judged"] + (st["fallbacks"] - 1) * (K - 1), st
+    f, _ = m.sums()
+    assert abs(f - m.ftot) < 1.0
+    m.close()
+    net.close()
+
+
+def test_config4_headline_geometry(require_gpu):
+    """4096^2 FP32 npc=2 with the benchmark's default launch plan."""
+    L, steps = 4096, 5
+    print("config4: %d^2 FP32 one-kernel step vs separate sweeps, %d steps" % (L, steps),
+          flush=True)
+    phys = dict(CONFIG0_PHYS, zeta=0.03)
+    init = dict(CONFIG0_INIT, seed=7)
+    a = run(L, lyo.FP32, steps=steps, phys=phys, init=init)
+    sep = run(L, lyo.FP32, steps=steps, phys=phys, init=init, one_kernel_step=0,
+              fuse_last_pair=0)
+    for f in STATE:
+        assert np.array_equal(a.get(f), sep.get(f)), f
+    sep.close()
+    print("config4: FP64 and oracle, %d steps" % steps, flush=True)
+    m64 = run(L, lyo.FP64, steps=steps, phys=phys, init=init)
+    errs = {f: rel_err(a.get(f), m64.get(f)) for f in STATE}
+    print("config4 FP32 vs FP64 (max, rms):", errs, flush=True)
+    bad = {f: e for f, e in errs.items() if not (e[0] < BUDGET[f][0] and e[1] < BUDGET[f][1])}
+    assert not bad, (bad, errs)
+    o = oracle_run(L, 2, steps, phys=phys, init=init)
+    for f in STATE:
+        assert np.array_equal(m64.get(f), o.get(f)), f
+    fs, ps = a.sums()
+    assert abs(fs - a.ftot) < 1.0
+    assert abs(ps - a.ptot) < 1e-12 * a.ptot
+    o.close()
+    m64.close()
+    a.close()
+
+
+def test_long_run_statistics_fp32_vs_fp64(require_gpu):
+    """Defect density, u_rms and the phi histogram of FP32 against FP64 over
+    a long run at 512^2: within 2% at every checkpoint (histogram: L1
+    distance of the normalised 64-bin histograms over [0, 1))."""
+    L, every, n = 512, 1000, 6
+    m64 = run(L, lyo.FP64)
+    m32 = run(L, lyo.FP32)
+    for c in range(n):
+        m64.step(every)
+        m32.step(every)
+        d64, d32 = m64.diagnostics(), m32.diagnostics()
+        h64, h32 = (np.asarray(d["phi_hist"], dtype=float) for d in (d64, d32))
+        l1 = np.abs(h64 / h64.sum() - h32 / h32.sum()).sum()
+        print("long run step %d: defects %d / %d, u_rms %.6g / %.6g, hist L1 %.3g" % (
+            (c + 1) * every, d64["defects_pos"] + d64["defects_neg"],
+            d32["defects_pos"] + d32["defects_neg"], d64["u_rms"], d32["u_rms"], l1), flush=True)
+        assert d64["defects_pos"] + d64["defects_neg"] > 20
+        for k in ("defect_density", "u_rms"):
+            assert abs(d32[k] - d64[k]) <= 0.02 * d64[k], ((c + 1) * every, k, d32[k], d64[k])
+        assert l1 <= 0.02, ((c + 1) * every, l1)
+    m64.close()
+    m32.close()

@@ -186,3 +186,13 @@ def test_growth_long_node_lists_bit_exact(require_gpu):
         a.step(4)
         o.step(4)
         check(a, o, STATE, "after %d steps" % (4 * s + 4))
+
+
+def test_growth_set_state_is_refused(require_gpu):
+    # m, chi, the masks and the draws ahead cannot be set through set_state
+    a = lyo.Lyotropic(12, 10, npc=1, precision=lyo.FP64, **PHYS)
+    a.set_growth(**GROWTH)
+    a.configure(**INIT)
+    with pytest.raises(lyo.MassLyoError):
+        a.set_state(a.get("ff"), a.get("QQxx"), a.get("QQyx"), a.get("phi"))
+    a.close()

@@ -45,10 +45,11 @@ def assert_bitwise(g, o, names, what=""):
 @pytest.mark.parametrize("loops", [False, True])
 @pytest.mark.parametrize("bc", BCS)
 @pytest.mark.parametrize("npc", [1, 2, 3])
-@pytest.mark.parametrize("LX,LY", [(2, 2), (5, 3), (16, 12), (45, 67), (17, 33)])
+@pytest.mark.parametrize("LX,LY", [(2, 2), (5, 3), (16, 12), (45, 67), (17, 33), (23, 130)])
 def test_walls_one_step_bit_exact(require_gpu, bc, npc, LX, LY, loops):
     # loops: the node loops with condition kernels between them; otherwise
-    # the fused sweeps (16 x 16 tiles: 17 x 33 leaves partial tiles)
+    # the fused sweeps (10 x 62 tiles: 17 x 33 leaves partial tiles, 45 x 67
+    # spans two tiles in y, 23 x 130 three, with an interior y-tile)
     print("walls one step bc=%d npc=%d %dx%d loops=%d" % (bc, npc, LX, LY, loops), flush=True)
     g, o, fg, fo = pair(LX, LY, bc, npc, loops=loops)
     assert fg == fo
@@ -61,9 +62,10 @@ def test_walls_one_step_bit_exact(require_gpu, bc, npc, LX, LY, loops):
 
 @pytest.mark.parametrize("loops", [False, True])
 @pytest.mark.parametrize("bc", BCS)
-def test_walls_60_steps_bit_exact(require_gpu, bc, loops):
-    print("walls 60 steps bc=%d loops=%d" % (bc, loops), flush=True)
-    g, o, _, _ = pair(70, 52, bc, npc=2, loops=loops)
+@pytest.mark.parametrize("LX,LY", [(70, 52), (23, 130)])
+def test_walls_60_steps_bit_exact(require_gpu, bc, loops, LX, LY):
+    print("walls 60 steps bc=%d loops=%d %dx%d" % (bc, loops, LX, LY), flush=True)
+    g, o, _, _ = pair(LX, LY, bc, npc=2, loops=loops)
     g.step(60)
     o.step(60)
     assert_bitwise(g, o, STATE + ["ux", "sigmaXY", "MU"])
@@ -72,9 +74,10 @@ def test_walls_60_steps_bit_exact(require_gpu, bc, loops):
 
 
 @pytest.mark.parametrize("bc", BCS)
-def test_walls_switching_kernel_sets_mid_run(require_gpu, bc):
+@pytest.mark.parametrize("LX,LY", [(33, 40), (23, 130)])
+def test_walls_switching_kernel_sets_mid_run(require_gpu, bc, LX, LY):
     # fused sweeps and node loops hand the state over at step boundaries
-    g, o, _, _ = pair(33, 40, bc, npc=2)
+    g, o, _, _ = pair(LX, LY, bc, npc=2)
     for s, loops in enumerate([False, True, True, False, True, False]):
         g.set_loop_kernels(loops)
         g.step(s % 3 + 1)
