@@ -190,6 +190,29 @@ int mass_lyo_get_field(mass_lyo_t h, int32_t field, double* out);
  * Synchronises. */
 int mass_lyo_sums(mass_lyo_t h, double* fsum, double* phisum);
 
+/* The sums of mass_lyo_sums over a state held in the caller's device memory,
+ * computed without allocating anything.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).
+ *   precision   MASS_LYO_FP64: f (9 planes, populations) and phi are double;
+ *               MASS_LYO_FP32: float, f holds the shifted populations
+ *               f' = f - w_i rho (the library's FP32 layout), and
+ *               sum f = sum f' + n rho; phic (may be NULL) is the
+ *               compensation plane added to phi.  phic must be NULL in FP64.
+ *   f           9 planes of plane_stride elements (plane v at f + v*stride)
+ *   n           nodes per plane (1 <= n <= plane_stride <= (2^31-1)/9)
+ *   out         device memory for 2 doubles: sum f, sum phi
+ *   stream      a hipStream_t (NULL: the null stream); the call is
+ *               asynchronous on it and enqueues two kernels, nothing else
+ * The result is bitwise that of mass_lyo_sums on the same state (the same
+ * fixed summation tree for the same n).  The partial sums live in one
+ * 64 KiB device array per GPU, kept for the life of the process; calls that
+ * overlap on one GPU (other threads, other streams) race on it.  Errors are
+ * reported through mass_lyo_last_error(NULL). */
+int mass_lyo_state_sums_noalloc(int32_t precision, const void* f, int64_t plane_stride,
+                                const void* phi, const void* phic, int64_t n, double rho,
+                                double* out, void* stream);
+
 /* Run diagnostics of the current state, computed on the device with the
  * definitions of the reference's post-processing (POST/massPy):
  *   velocity u = sum_v c_v f_v / sum_v f_v (base_modules/flow.py:7-11);

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,8 @@
 
 namespace {
 thread_local std::string g_create_error;
+// largest plane stride of mass_lyo_state_sums_noalloc (int indexing of nine planes)
+constexpr int64_t kNoallocMaxStride = 2147483647LL / 9;
 }  // namespace
 
 static int fail(mass_lyo_t h, int code, const std::string& msg) {
@@ -381,6 +384,79 @@ int mass_lyo_sums(mass_lyo_t h, double* fsum, double* phisum) {
   if (e != hipSuccess) return hip_fail(h, e, "sums");
   if (fsum) *fsum = r[0];
   if (phisum) *phisum = r[1];
+  return MASS_LYO_OK;
+}
+
+}  // extern "C"
+
+// Scratch of mass_lyo_state_sums_noalloc: the per-block partial sums of Σf and
+// Σφ (kMaxPart each).  One copy per GPU, held until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress gives for the
+// current device (the array's name is a host-side stand-in).
+static __device__ __align__(256) double sums_scratch[2 * kMaxPart];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's copy of sums_scratch (looked up once per device)
+hipError_t scratch_of_current_device(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(sums_scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+}  // namespace
+
+extern "C" {
+
+int mass_lyo_state_sums_noalloc(int32_t precision, const void* f, int64_t plane_stride,
+                                const void* phi, const void* phic, int64_t n, double rho,
+                                double* out, void* stream) {
+  if (precision != MASS_LYO_FP64 && precision != MASS_LYO_FP32)
+    return fail(nullptr, MASS_LYO_EINVAL, "precision must be MASS_LYO_FP64 or MASS_LYO_FP32");
+  if (!f || !phi || !out) return fail(nullptr, MASS_LYO_EINVAL, "null device pointer");
+  if (precision == MASS_LYO_FP64 && phic)
+    return fail(nullptr, MASS_LYO_EINVAL, "FP64 states have no phi compensation plane");
+  if (n < 1) return fail(nullptr, MASS_LYO_EINVAL, "n must be at least 1");
+  if (plane_stride < n) return fail(nullptr, MASS_LYO_EINVAL, "plane_stride must be at least n");
+  if (plane_stride > kNoallocMaxStride)
+    return fail(nullptr, MASS_LYO_EINVAL,
+                "plane_stride too large: the nine population planes must span at most "
+                "2^31-1 elements (plane_stride <= " + std::to_string(kNoallocMaxStride) + ")");
+  double* scratch = nullptr;
+  hipError_t e = scratch_of_current_device(&scratch);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "device scratch address");
+  const int N = (int)n, NP = (int)plane_stride;
+  // the partition of state_sums (lyo_engine.hpp): the same tree for the same n
+  int nparts = (N + 4095) / 4096;
+  if (nparts > kMaxPart) nparts = kMaxPart;
+  const int per = (N + nparts - 1) / nparts;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == MASS_LYO_FP64)
+    hipLaunchKernelGGL((lyo::partial_sums_kernel<double>), dim3(nparts), dim3(256), 0, s,
+                       static_cast<const double*>(f), static_cast<const double*>(phi),
+                       (const double*)nullptr, N, NP, per, scratch, scratch + kMaxPart);
+  else
+    hipLaunchKernelGGL((lyo::partial_sums_kernel<float>), dim3(nparts), dim3(256), 0, s,
+                       static_cast<const float*>(f), static_cast<const float*>(phi),
+                       static_cast<const float*>(phic), N, NP, per, scratch,
+                       scratch + kMaxPart);
+  const double add = precision == MASS_LYO_FP64 ? 0.0 : (double)N * rho;
+  hipLaunchKernelGGL(lyo::final_sums_kernel, dim3(1), dim3(256), 0, s, scratch,
+                     scratch + kMaxPart, nparts, add, out);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(nullptr, e, "state sums");
   return MASS_LYO_OK;
 }
 

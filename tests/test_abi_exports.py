@@ -47,3 +47,28 @@ def test_header_compiles_as_c():
                         "-x", "c", "-", "-fsyntax-only"], input=src, text=True,
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     assert r.returncode == 0, r.stdout
+
+
+def test_noalloc_sums_reject_bad_arguments_on_the_host():
+    """The argument checks of mass_lyo_state_sums_noalloc run before any
+    device call, so they are exercised here without a GPU."""
+    lib = ctypes.CDLL(LIB)
+    fn = lib.mass_lyo_state_sums_noalloc
+    fn.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p,
+                   ctypes.c_void_p]
+    lib.mass_lyo_last_error.argtypes = [ctypes.c_void_p]
+    lib.mass_lyo_last_error.restype = ctypes.c_char_p
+    p = ctypes.c_void_p(4096)  # never dereferenced: every call below is refused first
+    max_stride = (2 ** 31 - 1) // 9
+    cases = [
+        (7, p, 10, p, None, 10, "precision"),                    # unknown precision
+        (0, None, 10, p, None, 10, "null"),                      # no populations
+        (0, p, 10, p, p, 10, "compensation"),                    # FP64 with phic
+        (1, p, 10, p, None, 0, "at least 1"),                    # empty state
+        (1, p, 10, p, None, 11, "plane_stride"),                 # stride below n
+        (1, p, max_stride + 1, p, None, 100, "too large"),       # beyond the scratch's limit
+    ]
+    for prec, f, stride, phi, phic, n, msg in cases:
+        assert fn(prec, f, stride, phi, phic, n, 40.0, p, None) == 1
+        assert msg in lib.mass_lyo_last_error(None).decode()
