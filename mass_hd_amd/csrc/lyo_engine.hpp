@@ -710,6 +710,7 @@ class EngineT final : public Engine {
     P_.zeta = (T)v[11];
     P_.zetaI = (T)v[12];
     P_.conc = (T)v[13];
+    finish_params(P_);
     rho_ = v[5];
   }
 

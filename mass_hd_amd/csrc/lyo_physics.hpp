@@ -21,7 +21,19 @@ template <class T>
 struct Params {
   T GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
   T AA, CC, LL, KK, zeta, zetaI, conc;
+  // FP32 stress coefficients for raw differences (a(+1) - a(-1), twice the
+  // central derivative): KK/8, KK/4, LL/4, LL/2 (exact power-of-two scalings,
+  // set by finish_params; the FP64 path does not read them)
+  T KK8, KK4, LL4, LL2;
 };
+
+template <class T>
+inline void finish_params(Params<T>& P) {
+  P.KK8 = P.KK * T(.125);
+  P.KK4 = P.KK * T(.25);
+  P.LL4 = P.LL * T(.25);
+  P.LL2 = P.LL * T(.5);
+}
 
 // Rest-equilibrium weights fe(u=0, n=1) of src/lb.hpp:24-41: these define the
 // shift of the FP32 populations (they are not the forcing weights w[]).
@@ -66,6 +78,8 @@ struct NbrVals {
   T ux, uy, uxp, uyp, mu, sxx, syy, sxy, syx;
 };
 // Values UpdateFields reads at the node itself (lyotropic.cpp:255-266).
+// FP32: the Q derivatives are kept as raw differences (twice the central
+// derivative); UpdateFields applies the factor one half once per increment.
 template <class T>
 struct CenterVals {
   T nn, dn, vx, vy, Hxx, Hyx, dxQxx, dyQxx, dxQyx, dyQyx;
@@ -103,15 +117,8 @@ __device__ __forceinline__ void update_quantities(
   }
 
   const T del2Qxx = lap5<T, EXACT>(qxx_c, qxx_xp, qxx_xm, qxx_yp, qxx_ym);
-  const T dxQxx = dcen<T, EXACT>(qxx_xp, qxx_xm);
-  const T dyQxx = dcen<T, EXACT>(qxx_yp, qxx_ym);
   const T del2Qyx = lap5<T, EXACT>(qyx_c, qyx_xp, qyx_xm, qyx_yp, qyx_ym);
-  const T dxQyx = dcen<T, EXACT>(qyx_xp, qyx_xm);
-  const T dyQyx = dcen<T, EXACT>(qyx_yp, qyx_ym);
   const T del2p = lap5<T, EXACT>(p_c, p_xp, p_xm, p_yp, p_ym);
-  const T dxPhi = dcen<T, EXACT>(p_xp, p_xm);
-  const T dyPhi = dcen<T, EXACT>(p_yp, p_ym);
-
   const T term = T(0) - Qxx * Qxx - Qyx * Qyx;
   const T Hxx = P.CC * term * Qxx + P.LL * del2Qxx;
   const T Hyx = P.CC * term * Qyx + P.LL * del2Qyx;
@@ -121,16 +128,48 @@ __device__ __forceinline__ void update_quantities(
   } else {
     mu = P.AA * p * (T(1) - p) * (T(1) - T(2) * p) - P.KK * del2p;
   }
-
   const T sigmaB = T(.5) * P.AA * p * p * (T(1) - p) * (T(1) - p) +
                    T(.5) * P.CC * term * term - mu * p;
-  const T sigmaF = T(2) * P.xi * ((Qxx * Qxx - T(1)) * Hxx + Qxx * Qyx * Hyx) -
-                   P.zeta * Qxx * (T(1) - p) +
-                   T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) +
-                   P.LL * (dyQxx * dyQxx + dyQyx * dyQyx - dxQxx * dxQxx - dxQyx * dxQyx);
-  const T sigmaS = T(2) * P.xi * (Qyx * Qxx * Hxx + (Qyx * Qyx - T(1)) * Hyx) -
-                   P.zeta * Qyx * (T(1) - p) - P.KK * dxPhi * dyPhi -
-                   T(2) * P.LL * (dxQxx * dyQxx + dxQyx * dyQyx);
+
+  // derivatives and the gradient (Ericksen) stresses; FP32 works on raw
+  // differences r = 2 d with the coefficients scaled by 1/4 (KK/2 -> KK/8,
+  // KK -> KK/4, LL -> LL/4, 2 LL -> LL/2), the same values in exact arithmetic
+  T dxQxx, dyQxx, dxQyx, dyQyx, grad_f, grad_s, pf_xx, pf_yx;
+  if constexpr (EXACT) {
+    dxQxx = dcen<T, EXACT>(qxx_xp, qxx_xm);
+    dyQxx = dcen<T, EXACT>(qxx_yp, qxx_ym);
+    dxQyx = dcen<T, EXACT>(qyx_xp, qyx_xm);
+    dyQyx = dcen<T, EXACT>(qyx_yp, qyx_ym);
+    const T dxPhi = dcen<T, EXACT>(p_xp, p_xm);
+    const T dyPhi = dcen<T, EXACT>(p_yp, p_ym);
+    pf_xx = T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi);
+    pf_yx = -P.KK * dxPhi * dyPhi;
+    // the reference's expressions as written (lyotropic.cpp:212-219)
+    const T sigmaF = T(2) * P.xi * ((Qxx * Qxx - T(1)) * Hxx + Qxx * Qyx * Hyx) -
+                     P.zeta * Qxx * (T(1) - p) +
+                     T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) +
+                     P.LL * (dyQxx * dyQxx + dyQyx * dyQyx - dxQxx * dxQxx - dxQyx * dxQyx);
+    const T sigmaS = T(2) * P.xi * (Qyx * Qxx * Hxx + (Qyx * Qyx - T(1)) * Hyx) -
+                     P.zeta * Qyx * (T(1) - p) - P.KK * dxPhi * dyPhi -
+                     T(2) * P.LL * (dxQxx * dyQxx + dxQyx * dyQyx);
+    grad_f = sigmaF;
+    grad_s = sigmaS;
+  } else {
+    dxQxx = qxx_xp - qxx_xm;
+    dyQxx = qxx_yp - qxx_ym;
+    dxQyx = qyx_xp - qyx_xm;
+    dyQyx = qyx_yp - qyx_ym;
+    const T rxP = p_xp - p_xm, ryP = p_yp - p_ym;
+    pf_xx = P.KK8 * (ryP * ryP - rxP * rxP);
+    pf_yx = -P.KK4 * rxP * ryP;
+    grad_f = T(2) * P.xi * ((Qxx * Qxx - T(1)) * Hxx + Qxx * Qyx * Hyx) -
+             P.zeta * Qxx * (T(1) - p) + pf_xx +
+             P.LL4 * (dyQxx * dyQxx + dyQyx * dyQyx - dxQxx * dxQxx - dxQyx * dxQyx);
+    grad_s = T(2) * P.xi * (Qyx * Qxx * Hxx + (Qyx * Qyx - T(1)) * Hyx) -
+             P.zeta * Qyx * (T(1) - p) + pf_yx -
+             P.LL2 * (dxQxx * dyQxx + dxQyx * dyQyx);
+  }
+  const T sigmaF = grad_f, sigmaS = grad_s;
   const T sigmaA = T(2) * (Qxx * Hyx - Qyx * Hxx);
 
   o.ux = vx;
@@ -156,10 +195,10 @@ __device__ __forceinline__ void update_quantities(
 
   if constexpr (DIAG) {
     dg->bulk = sigmaB + P.zetaI * (P.conc - p);
-    dg->el_xx = sigmaF - T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) + P.zeta * Qxx * (T(1) - p);
-    dg->el_yx = sigmaS + P.KK * dxPhi * dyPhi + P.zeta * Qxx * (T(1) - p);
-    dg->pf_xx = T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi);
-    dg->pf_yx = -P.KK * dxPhi * dyPhi;
+    dg->el_xx = sigmaF - pf_xx + P.zeta * Qxx * (T(1) - p);
+    dg->el_yx = sigmaS - pf_yx + P.zeta * Qxx * (T(1) - p);
+    dg->pf_xx = pf_xx;
+    dg->pf_yx = pf_yx;
     dg->act_xx = -P.zeta * Qxx * (T(1) - p);
     dg->act_yx = -P.zeta * Qyx * (T(1) - p);
   }
@@ -188,34 +227,55 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
   const T dxQxx = c.dxQxx, dyQxx = c.dyQxx, dxQyx = c.dxQyx, dyQyx = c.dyQyx;
 
   const T del2mu = lap5<T, EXACT>(s.mu_c, s.mu_xp, s.mu_xm, s.mu_yp, s.mu_ym);
-  const T dxux = dcen<T, EXACT>(s.ux_xp, s.ux_xm);
-  const T dyux = dcen<T, EXACT>(s.ux_yp, s.ux_ym);
-  const T dxuy = dcen<T, EXACT>(s.uy_xp, s.uy_xm);
-  const T dyuy = dcen<T, EXACT>(s.uy_yp, s.uy_ym);
-  const T dxSxx = dcen<T, EXACT>(s.sxx_xp, s.sxx_xm);
-  const T dySxy = dcen<T, EXACT>(s.sxy_yp, s.sxy_ym);
-  const T dxSyx = dcen<T, EXACT>(s.syx_xp, s.syx_xm);
-  const T dySyy = dcen<T, EXACT>(s.syy_yp, s.syy_ym);
-  const T pFlux = dcen<T, EXACT>(s.uxp_xp, s.uxp_xm) + dcen<T, EXACT>(s.uyp_yp, s.uyp_ym);
-
-  const T expansion = dxux + dyuy;
-  const T shear = T(.5) * (dxuy + dyux);
-  const T vorticity = T(.5) * (dxuy - dyux);
-  const T traceQL = Qxx * (dxux - dyuy) + T(2) * Qyx * shear;
-
-  Dxx = P.GammaQ * Hxx - vx * dxQxx - vy * dyQxx - T(2) * vorticity * Qyx +
-        P.xi * ((Qxx + T(1)) * (T(2) * dxux - traceQL) + T(2) * Qyx * shear - expansion);
-  Dyx = P.GammaQ * Hyx - vx * dxQyx - vy * dyQyx + T(2) * vorticity * Qxx +
-        P.xi * (Qyx * (expansion - traceQL) + T(2) * shear);
+  T Fx, Fy;
   if constexpr (EXACT) {
+    const T dxux = dcen<T, EXACT>(s.ux_xp, s.ux_xm);
+    const T dyux = dcen<T, EXACT>(s.ux_yp, s.ux_ym);
+    const T dxuy = dcen<T, EXACT>(s.uy_xp, s.uy_xm);
+    const T dyuy = dcen<T, EXACT>(s.uy_yp, s.uy_ym);
+    const T dxSxx = dcen<T, EXACT>(s.sxx_xp, s.sxx_xm);
+    const T dySxy = dcen<T, EXACT>(s.sxy_yp, s.sxy_ym);
+    const T dxSyx = dcen<T, EXACT>(s.syx_xp, s.syx_xm);
+    const T dySyy = dcen<T, EXACT>(s.syy_yp, s.syy_ym);
+    const T pFlux = dcen<T, EXACT>(s.uxp_xp, s.uxp_xm) + dcen<T, EXACT>(s.uyp_yp, s.uyp_ym);
+
+    const T expansion = dxux + dyuy;
+    const T shear = T(.5) * (dxuy + dyux);
+    const T vorticity = T(.5) * (dxuy - dyux);
+    const T traceQL = Qxx * (dxux - dyuy) + T(2) * Qyx * shear;
+
+    Dxx = P.GammaQ * Hxx - vx * dxQxx - vy * dyQxx - T(2) * vorticity * Qyx +
+          P.xi * ((Qxx + T(1)) * (T(2) * dxux - traceQL) + T(2) * Qyx * shear - expansion);
+    Dyx = P.GammaQ * Hyx - vx * dxQyx - vy * dyQyx + T(2) * vorticity * Qxx +
+          P.xi * (Qyx * (expansion - traceQL) + T(2) * shear);
     // conserve_phi is false for plain Lyotropic: the correction term is 0
     Dp = P.GammaP * del2mu - pFlux - T(0);
+    Fx = dxSxx + dySxy - P.friction * vx;
+    Fy = dxSyx + dySyy - P.friction * vy;
   } else {
-    Dp = P.GammaP * del2mu - pFlux;
+    // raw differences r = 2 d (the Q derivatives of CenterVals are raw too):
+    // E2 = 2 expansion, S4 = 4 shear, W4 = 4 vorticity, TQ2 = 2 traceQL; the
+    // flow and advection terms are formed doubled and halved once
+    const T rxux = s.ux_xp - s.ux_xm, ryux = s.ux_yp - s.ux_ym;
+    const T rxuy = s.uy_xp - s.uy_xm, ryuy = s.uy_yp - s.uy_ym;
+    const T E2 = rxux + ryuy;
+    const T S4 = rxuy + ryux;
+    const T W4 = rxuy - ryux;
+    const T TQ2 = Qxx * (rxux - ryuy) + Qyx * S4;
+    const T Vxx = P.xi * ((Qxx + T(1)) * (T(2) * rxux - TQ2) + Qyx * S4 - E2) - W4 * Qyx;
+    const T Vyx = P.xi * (Qyx * (E2 - TQ2) + S4) + W4 * Qxx;
+    const T axx = vx * dxQxx + vy * dyQxx;
+    const T ayx = vx * dxQyx + vy * dyQyx;
+    Dxx = P.GammaQ * Hxx + T(.5) * (Vxx - axx);
+    Dyx = P.GammaQ * Hyx + T(.5) * (Vyx - ayx);
+    const T rflux = (s.uxp_xp - s.uxp_xm) + (s.uyp_yp - s.uyp_ym);
+    Dp = P.GammaP * del2mu - T(.5) * rflux;
+    const T gx = (s.sxx_xp - s.sxx_xm) + (s.sxy_yp - s.sxy_ym);
+    const T gy = (s.syx_xp - s.syx_xm) + (s.syy_yp - s.syy_ym);
+    Fx = T(.5) * gx - P.friction * vx;
+    Fy = T(.5) * gy - P.friction * vy;
   }
 
-  const T Fx = dxSxx + dySxy - P.friction * vx;
-  const T Fy = dxSyx + dySyy - P.friction * vy;
   const T tau = P.tauNem * p + P.tauIso * (T(1) - p);
 
   if constexpr (EXACT) {

@@ -746,6 +746,7 @@ class WallEngine final : public Engine {
     P_.zeta = v[11];
     P_.zetaI = v[12];
     P_.conc = v[13];
+    finish_params(P_);
   }
 
   // Lyotropic::Step (lyotropic.cpp:500-525); go-or-grow redraws its masks
