@@ -262,8 +262,10 @@ int mass_lyo_set_launch(mass_lyo_t h, int32_t chunk_rows, int32_t waves_per_bloc
  * in opposite directions so that they reach their shared rows together, 1 on
  * (the default) or 0 off (every chunk upwards); one_kernel_step: with npc = 1,
  * or npc = 2 and the pair fused (LX >= 8), the whole step -- predictor and
- * correctors -- runs in one kernel, 1 on (the default) or 0 off; step_chunk_rows:
- * rows per wave chunk of that kernel (0 keeps).  -1 keeps a setting.  Results
+ * correctors -- runs in one kernel, 1 on (the default) or 0 off; 2 (npc = 2):
+ * the block-cooperative form of that kernel, whose four waves per block hand
+ * their edge values to each other through LDS so that only the block's outer
+ * columns are halo; step_chunk_rows: rows per chunk of that kernel (0 keeps).  -1 keeps a setting.  Results
  * are bitwise the same either way; derived fields of a frame are recomputed on
  * demand after a fused step.  No reference counterpart (Lyotropic::Step,
  * src/models/lyotropic.cpp:500-525, runs its sweeps one after another). */

@@ -48,12 +48,11 @@ const char* mass_lyo_last_error(mass_lyo_t h) {
   return h ? h->err.c_str() : g_create_error.c_str();
 }
 
-// Rows per chunk of the one-kernel step for an LX x LY grid on a GPU with
-// `slots` resident waves of that kernel: minimises ceil(waves / slots) x
-// (rows + repeated rows); the shortest such chunk.
-static int full_chunk_rows(int LX, int LY, int npc, int slots) {
-  const int strip = npc == 1 ? lyo::kFull1Strip : lyo::kFullStrip;
-  const int repeat = npc == 1 ? 4 : 8;
+// Rows per chunk of a one-kernel step for an LX x LY grid on a GPU with
+// `slots` resident work items (waves, or blocks of the block-cooperative
+// step) of `strip` columns that repeat `repeat` rows per chunk: minimises
+// ceil(items / slots) x (rows + repeated rows); the shortest such chunk.
+static int chunk_rows_for(int LX, int LY, int strip, int repeat, int slots) {
   const long long nstrips = (LY + strip - 1) / strip;
   if (slots < 1) slots = 1;
   int best = 64;
@@ -156,9 +155,13 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
   // at 4096^2 FP32 npc=2, rounds 0.96 / 1.93 / 2.47 / 2.2 / 1.58 / 1.23 for
   // 164 / 82 / 64 / 72 / 100 / 128 rows: 30.8k / 30.6k / 30.0k / 27.8k /
   // 26.5k / 25.0k MLUPS)
-  h->launch.xc3 = full_chunk_rows(p->LX, p->LY, p->npc,
-                                  prop.multiProcessorCount * 4 *
-                                      (p->precision == MASS_LYO_FP64 ? 1 : 2));
+  const int per_simd = p->precision == MASS_LYO_FP64 ? 1 : 2;  // resident waves per SIMD
+  h->launch.xc3 = chunk_rows_for(p->LX, p->LY, p->npc == 1 ? lyo::kFull1Strip : lyo::kFullStrip,
+                                 p->npc == 1 ? 4 : 8, prop.multiProcessorCount * 4 * per_simd);
+  // the block-cooperative variant: one block of four waves per CU and wave
+  // slot per SIMD (its LDS allows two FP32 blocks per CU, one FP64 block)
+  h->launch.xc4 = chunk_rows_for(p->LX, p->LY, lyo::kXStrip, 8,
+                                 prop.multiProcessorCount * per_simd);
   *out = h.release();
   return MASS_LYO_OK;
 }
@@ -591,15 +594,15 @@ int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chu
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (fuse_last_pair < -1 || fuse_last_pair > 1 || pair_chunk_rows < 0 ||
       pair_chunk_rows > 4096 || alternate < -1 || alternate > 1 || one_kernel_step < -1 ||
-      one_kernel_step > 1 || step_chunk_rows < 0 || step_chunk_rows > 4096)
+      one_kernel_step > 2 || step_chunk_rows < 0 || step_chunk_rows > 4096)
     return fail(h, MASS_LYO_EINVAL,
-                "fuse_last_pair, alternate and one_kernel_step in {-1,0,1}, chunk rows in "
-                "[0,4096]");
+                "fuse_last_pair and alternate in {-1,0,1}, one_kernel_step in {-1,0,1,2}, "
+                "chunk rows in [0,4096]");
   if (fuse_last_pair >= 0) h->launch.fuse = fuse_last_pair;
   if (pair_chunk_rows) h->launch.xc2 = pair_chunk_rows;
   if (alternate >= 0) h->launch.alt = alternate;
   if (one_kernel_step >= 0) h->launch.full = one_kernel_step;
-  if (step_chunk_rows) h->launch.xc3 = step_chunk_rows;
+  if (step_chunk_rows) h->launch.xc3 = h->launch.xc4 = step_chunk_rows;
   return MASS_LYO_OK;
 }
 

@@ -69,8 +69,10 @@ struct LaunchCfg {
   int fuse = 1;  // last two correctors in one kernel (npc >= 2)
   int xc2 = 16;  // rows per wave chunk of the fused pair
   int alt = 1;   // alternate the march direction between neighbouring chunks
-  int full = 1;  // FP32, npc = 2: the whole step in one kernel
+  int full = 1;  // npc <= 2: the whole step in one kernel (2: the block-cooperative
+                 // variant for npc = 2, single runs)
   int xc3 = 32;  // rows per wave chunk of the one-kernel step
+  int xc4 = 32;  // rows per block chunk of the block-cooperative one-kernel step
   int loops = 0; // FP64: one kernel per loop of the reference's step instead
 };
 
@@ -82,8 +84,9 @@ struct SweepTimer {
 };
 
 // kind 0: sweep, 1: fused pair, 2: one-kernel step, 3: one-kernel step with
-// one corrector (strip width and chunk rows)
-constexpr int kGeoKinds = 4;
+// one corrector, 4: block-cooperative one-kernel step (strip width and chunk
+// rows; kind 4 counts blocks in nwaves)
+constexpr int kGeoKinds = 5;
 inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, int kind = 0) {
   Geo g;
   g.LX = LX;
@@ -91,9 +94,11 @@ inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, int kind = 0) {
   g.N = LX * LY;
   g.NP = NP;
   const int strip =
-      kind == 3 ? kFull1Strip : (kind == 2 ? kFullStrip : (kind == 1 ? kPairStrip : kStrip));
+      kind == 4 ? kXStrip
+                : (kind == 3 ? kFull1Strip
+                             : (kind == 2 ? kFullStrip : (kind == 1 ? kPairStrip : kStrip)));
   g.nstrips = (LY + strip - 1) / strip;
-  g.xc = kind >= 2 ? L.xc3 : (kind == 1 ? L.xc2 : L.xc);
+  g.xc = kind == 4 ? L.xc4 : (kind >= 2 ? L.xc3 : (kind == 1 ? L.xc2 : L.xc));
   g.nchunks = (LX + g.xc - 1) / g.xc;
   g.nwaves = g.nstrips * g.nchunks;
   g.xcd = L.xcd;
@@ -133,7 +138,7 @@ inline hipError_t timed_pair(const SweepArgs<T>& a, const Params<T>& p, const Ge
   return err;
 }
 
-template <class T, bool EXACT, int NPC>
+template <class T, bool EXACT, int NPC, bool XCH = false>
 inline hipError_t timed_full(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
                              const LaunchCfg& L, hipStream_t s, SweepTimer* tm) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -144,7 +149,8 @@ inline hipError_t timed_full(const SweepArgs<T>& a, const Params<T>& p, const Ge
     tm->used += 2;
     (void)hipEventRecord(e0, s);
   }
-  hipError_t err = launch_full<T, EXACT, NPC>(a, p, g, L.wpb, s);
+  hipError_t err = XCH ? launch_full_x<T, EXACT>(a, p, g, s)
+                       : launch_full<T, EXACT, NPC>(a, p, g, L.wpb, s);
   if (e1) (void)hipEventRecord(e1, s);
   return err;
 }
@@ -254,13 +260,14 @@ SweepArgs<T> pair_args(const Bufs<T>& B, int npc, int cur) {
 constexpr int kStagePair = 100;
 constexpr int kStageFull = 200;
 constexpr int kStageFull1 = 201;  // the whole step with npc = 1
+constexpr int kStageFullX = 202;  // the whole step, npc = 2, block-cooperative
 
 inline std::vector<int> step_plan(bool exact, int npc, const LaunchCfg& L, int LX) {
   const bool fuse = L.fuse && npc >= 2 && LX >= 8;  // the fused kernels wrap rows once
   std::vector<int> p;
   (void)exact;
   if (fuse && L.full && npc == 2) {
-    p.push_back(kStageFull);
+    p.push_back(L.full == 2 ? kStageFullX : kStageFull);
     return p;
   }
   if (L.full && npc == 1 && LX >= 8) {
@@ -288,10 +295,14 @@ inline bool plan_stores_half(const std::vector<int>& p, int npc) {
 }
 
 inline int stage_geo_kind(int sc) {
-  return sc == kStageFull1 ? 3 : (sc == kStageFull ? 2 : (sc == kStagePair ? 1 : 0));
+  return sc == kStageFullX
+             ? 4
+             : (sc == kStageFull1 ? 3 : (sc == kStageFull ? 2 : (sc == kStagePair ? 1 : 0)));
 }
-static_assert(kGeoKinds == 4, "stage_geo_kind returns kinds 0..3");
-inline bool is_full_stage(int sc) { return sc == kStageFull || sc == kStageFull1; }
+static_assert(kGeoKinds == 5, "stage_geo_kind returns kinds 0..4");
+inline bool is_full_stage(int sc) {
+  return sc == kStageFull || sc == kStageFull1 || sc == kStageFullX;
+}
 
 // Arguments of the one-kernel step: state S[cur] in, S[next] out.
 template <class T, bool EXACT>
@@ -337,6 +348,7 @@ hipError_t launch_stage(int sc, const SweepArgs<T>& a, const Params<T>& P, const
                         const LaunchCfg& L, int npc, bool store_half, hipStream_t s,
                         SweepTimer* tm) {
   if (sc == kStageFull) return timed_full<T, EXACT, 2>(a, P, g, L, s, tm);
+  if (sc == kStageFullX) return timed_full<T, EXACT, 2, true>(a, P, g, L, s, tm);
   if (sc == kStageFull1) return timed_full<T, EXACT, 1>(a, P, g, L, s, tm);
   if (sc == kStagePair) {
     if constexpr (!EXACT) {

@@ -38,6 +38,37 @@ constexpr int kFullStrip = kWave - 2 * kFullHalo;
 constexpr int kFull1Halo = 4;  // npc = 1
 constexpr int kFull1Strip = kWave - 2 * kFull1Halo;
 
+// Block-cooperative one-kernel step (npc = 2): the kXWaves waves of a block
+// cover kXWaves*64 contiguous columns and hand the values their stencils
+// need across the wave boundaries through LDS, so only the block's outer
+// edges carry the six-lane halo (244 interior columns per block instead of
+// 4 x 52).  Exchange area per block: [wave][side: lane 0 / lane 63][slot]
+// [quantity] values, then a scratch region the other lanes write into (every
+// publication is one all-lane ds_write: lane 0 and lane 63 to their slots,
+// lanes 1..62 to distinct scratch words that nobody reads, two words apart so
+// that the compiler's paired writes of adjacent ids stay free of bank
+// conflicts).
+constexpr int kXWaves = 4;
+constexpr int kXStrip = kXWaves * kWave - 2 * kFullHalo;
+constexpr int kXQ = 40;     // quantities per slot
+constexpr int kXSlots = 3;  // slot period = the rows-in-threes period
+constexpr int kXTrash = kXWaves * 2 * kXSlots * kXQ;
+constexpr int kXElems = kXTrash + 2 * kWave + kXSlots * kXQ;
+// Quantity ids, published in the iteration's slot: at the iteration's start
+// (P's stencil row of Q/phi, the UpdateQuantities values of the UpdateFields
+// centre rows of P, A and B, A's stencil row, P's post-collision populations
+// and half-step values of the two previous rows in the directions the pull
+// shifts), then a barrier; after P's UpdateFields (its populations of the
+// current row), then a second barrier; after A's UpdateFields (A's Q/phi
+// outputs, read by B one iteration later).  Every read comes after the
+// barrier that follows its write; a slot is rewritten three iterations later,
+// after every wave has passed the barriers that end its reads.
+enum XId {
+  kXPQ = 0, kXPU = 3, kXAQ = 9, kXAU = 12, kXBU = 18, kXPCP = 24, kXHNP = 28,
+  kXPC = 32, kXQA = 36, kXEnd = 39
+};
+static_assert(kXEnd <= kXQ, "exchange quantities fit a slot");
+
 // Per-row outputs of P that A needs two rows later: half-step Q and the phi
 // terms (FP32: Dp and old phi; FP64: phi + Dp/2).
 template <class T>
@@ -75,10 +106,10 @@ constexpr size_t ring_bytes_per_wave() {
 // Arguments: f_in, qxx_in, qyx_in, phi_in (and FP32 phic_in) = the state
 // S[cur]; f_out, qxx_out, qyx_out, phi_out (and FP32 phic_out) = S[next].
 // NPC: correctors per step, 1 or 2.
-template <class T, bool EXACT, int NPC>
+template <class T, bool EXACT, int NPC, bool XCH = false>
 __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>& P,
                                           const Geo& g, const int wid,
-                                          typename Ring<T>::V* ring) {
+                                          typename Ring<T>::V* ring, T* xe = nullptr) {
   using RV = typename Ring<T>::V;
   constexpr int W = Ring<T>::W, G = Ring<T>::G;
   const int lane = threadIdx.x & 63;
@@ -102,16 +133,49 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     }
   };
   static_assert(NPC == 1 || NPC == 2, "one or two correctors");
+  static_assert(!XCH || NPC == 2, "the block-cooperative step has two correctors");
   constexpr int HALO = NPC == 2 ? kFullHalo : kFull1Halo;
   const int strip = wid % g.nstrips;
   const int chunk = wid / g.nstrips;
   const int LX = g.LX, LY = g.LY, N = g.NP;
 
-  const int ycol = strip * (kWave - 2 * HALO) - HALO + lane;
+  // column of the lane within its strip (XCH: within the block's columns)
+  const int wv = XCH ? ((int)threadIdx.x >> 6) : 0;
+  const int cl = XCH ? wv * kWave + lane : lane;
+  constexpr int SW = XCH ? kXWaves * kWave : kWave;
+  const int ycol = strip * (SW - 2 * HALO) - HALO + cl;
   int yi = ycol % LY;
   if (yi < 0) yi += LY;
   LaneOff<T> ly{(unsigned)yi * (unsigned)sizeof(T)};
-  const bool active = lane >= HALO && lane < kWave - HALO && ycol < LY;
+  const bool active = cl >= HALO && cl < SW - HALO && ycol < LY;
+
+  // ---- exchange between the waves of a block (XCH)
+  // write index of the lane (lane 0: its side-0 slots, lane 63: side 1,
+  // others: scratch); read bases: side 1 of the left wave (the y-1
+  // neighbour of lane 0), side 0 of the right wave (the y+1 neighbour of
+  // lane 63).  The block's outer waves read whatever their wrapped
+  // neighbours left there: those lanes are halo.
+  [[maybe_unused]] const int xw =
+      lane == 0 ? (wv * 2) * kXSlots * kXQ
+                : (lane == kWave - 1 ? (wv * 2 + 1) * kXSlots * kXQ : kXTrash + 2 * lane);
+  [[maybe_unused]] const int xr_m = (((wv + kXWaves - 1) % kXWaves) * 2 + 1) * kXSlots * kXQ;
+  [[maybe_unused]] const int xr_p = (((wv + 1) % kXWaves) * 2) * kXSlots * kXQ;
+  auto xpub = [&](int slot, int id, T v) {
+    if constexpr (XCH) xe[xw + slot * kXQ + id] = v;
+  };
+  // value of the y-1 (dir < 0) or y+1 (dir > 0) neighbour; XCH: across the
+  // wave boundary from the slot the neighbour published
+  auto sh = [&](T v, int dir, int slot, int id) -> T {
+    if constexpr (XCH) {
+      const T old = xe[(dir < 0 ? xr_m : xr_p) + slot * kXQ + id];
+      return dir < 0 ? from_ym_or(v, old) : from_yp_or(v, old);
+    } else {
+      return dir < 0 ? from_ym(v) : from_yp(v);
+    }
+  };
+  auto xsync = [&]() {
+    if constexpr (XCH) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
   const int x0 = chunk * g.xc;
   const int x1 = min(x0 + g.xc, LX);
 
@@ -167,12 +231,49 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     }
   };
   // UpdateQuantities on the centre row of three stencil rows (x-1, x, x+1)
+  // (slot, id: where the wave boundary values of q0 were published)
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
-                NbrVals<T>& nv, CenterVals<T>& cv) {
-    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, from_yp(q0.xx), from_ym(q0.xx),
-                                q0.yx, qp.yx, qm.yx, from_yp(q0.yx), from_ym(q0.yx),
-                                q0.p, qp.p, qm.p, from_yp(q0.p), from_ym(q0.p), P, nv,
-                                cv);
+                NbrVals<T>& nv, CenterVals<T>& cv, int slot, int id) {
+    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, sh(q0.xx, 1, slot, id),
+                                sh(q0.xx, -1, slot, id), q0.yx, qp.yx, qm.yx,
+                                sh(q0.yx, 1, slot, id + 1), sh(q0.yx, -1, slot, id + 1), q0.p,
+                                qp.p, qm.p, sh(q0.p, 1, slot, id + 2), sh(q0.p, -1, slot, id + 2),
+                                P, nv, cv);
+  };
+  auto pub_q = [&](int slot, int id, const QP<T>& q) {
+    xpub(slot, id, q.xx);
+    xpub(slot, id + 1, q.yx);
+    xpub(slot, id + 2, q.p);
+  };
+  // UpdateFields stencil of the centre row u0 (the x neighbours um, un)
+  auto ufs = [&](const NbrVals<T>& um, const NbrVals<T>& u0, const NbrVals<T>& un, int slot,
+                 int id) {
+    if constexpr (!XCH) {
+      return uf_stencil(um, u0, un);
+    } else {
+      UFStencil<T> s = uf_stencil_x(um, u0, un);
+      s.ux_yp = sh(u0.ux, 1, slot, id);
+      s.ux_ym = sh(u0.ux, -1, slot, id);
+      s.uy_yp = sh(u0.uy, 1, slot, id + 1);
+      s.uy_ym = sh(u0.uy, -1, slot, id + 1);
+      s.uyp_yp = sh(u0.uyp, 1, slot, id + 2);
+      s.uyp_ym = sh(u0.uyp, -1, slot, id + 2);
+      s.sxy_yp = sh(u0.sxy, 1, slot, id + 3);
+      s.sxy_ym = sh(u0.sxy, -1, slot, id + 3);
+      s.syy_yp = sh(u0.syy, 1, slot, id + 4);
+      s.syy_ym = sh(u0.syy, -1, slot, id + 4);
+      s.mu_yp = sh(u0.mu, 1, slot, id + 5);
+      s.mu_ym = sh(u0.mu, -1, slot, id + 5);
+      return s;
+    }
+  };
+  auto pub_u = [&](int slot, int id, const NbrVals<T>& u) {
+    xpub(slot, id, u.ux);
+    xpub(slot, id + 1, u.uy);
+    xpub(slot, id + 2, u.uyp);
+    xpub(slot, id + 3, u.sxy);
+    xpub(slot, id + 4, u.syy);
+    xpub(slot, id + 5, u.mu);
   };
   // x components of the directions (src/geometry.hpp:17-27)
   constexpr int cxv[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
@@ -202,9 +303,14 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     cq = load_qp(xs + 2);
     load_f(xs + 1, cf);
     load_old(xs - 2 * NPC, bold);
+    // (XCH: slot 2 is next written in the third iteration, after every wave
+    // has passed four more barriers)
+    pub_q(2, kXPQ, pqb);
+    pub_q(2, kXAQ, pqc);
+    xsync();
     CenterVals<T> ct;
-    uq(qa, pqb, pqc, ft, pum, ct);
-    uq(pqb, pqc, qd, pfc, pu0, pc0);
+    uq(qa, pqb, pqc, ft, pum, ct, 2, kXPQ);
+    uq(pqb, pqc, qd, pfc, pu0, pc0, 2, kXAQ);
     pqb = pqc;
     pqc = qd;
   }
@@ -232,15 +338,33 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     constexpr bool WARM = decltype(warm)::value;
     const int s1 = (s0 == 0) ? 2 : s0 - 1;  // slot of iteration i-1
     const int s2 = (s1 == 0) ? 2 : s1 - 1;  // slot of iteration i-2
+    if constexpr (XCH) {
+      // the boundary values this iteration's stencils and pull need that
+      // earlier iterations produced (read after the barrier)
+      pub_q(s0, kXPQ, pqc);
+      pub_u(s0, kXPU, pu0);
+      pub_q(s0, kXAQ, aqc);
+      pub_u(s0, kXAU, au0);
+      pub_u(s0, kXBU, bu0);
+      xpub(s0, kXPCP, pcP[3]);
+      xpub(s0, kXPCP + 1, pcP[4]);
+      xpub(s0, kXPCP + 2, pcPP[5]);
+      xpub(s0, kXPCP + 3, pcPP[8]);
+      xpub(s0, kXHNP, hnP[3]);
+      xpub(s0, kXHNP + 1, hnP[4]);
+      xpub(s0, kXHNP + 2, hnPP[5]);
+      xpub(s0, kXHNP + 3, hnPP[8]);
+      xsync();
+    }
     // ---- P: UpdateQuantities on row x+1, UpdateFields on row x
     NbrVals<T> pun;
     CenterVals<T> pcn;
-    uq(pqb, pqc, cq, cf, pun, pcn);
+    uq(pqb, pqc, cq, cf, pun, pcn, s0, kXPQ);
     T pc[9], hn[9];
     QP<T> aq_new;  // A's stencil values of row x
     POut<T> po;
     {
-      const UFStencil<T> s = uf_stencil(pum, pu0, pun);
+      const UFStencil<T> s = ufs(pum, pu0, pun, s0, kXPU);
       T Dxx, Dyx, Dp, om[9];
       update_fields<T, EXACT>(pc0, pqb.xx, pqb.yx, pqb.p, pfc, s, P, Dxx, Dyx, Dp, om);
       const T Qsx = pqb.xx + Dxx, Qsy = pqb.yx + Dyx;
@@ -283,6 +407,17 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     load_f(x + 2, cf);
 
     // ---- pull (Move): streamed populations of row x-1
+    if constexpr (XCH) {
+      xpub(s0, kXPC, pc[6]);
+      xpub(s0, kXPC + 1, pc[7]);
+      xpub(s0, kXPC + 2, hn[6]);
+      xpub(s0, kXPC + 3, hn[7]);
+      xsync();
+    }
+    // exchange ids of the shifted directions: 3, 4 (row x-1), 5, 8 (row
+    // x-2) published at the iteration's start, 6, 7 (row x) just now
+    constexpr int xpc[9] = {0, 0, 0, kXPCP, kXPCP + 1, kXPCP + 2, kXPC, kXPC + 1, kXPCP + 3};
+    constexpr int xhn[9] = {0, 0, 0, kXHNP, kXHNP + 1, kXHNP + 2, kXPC + 2, kXPC + 3, kXHNP + 3};
     T fst[9], hst[9];
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
@@ -297,12 +432,9 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         a = pcPP[v];
         h = hnPP[v];
       }
-      if (cyv[v] > 0) {
-        a = from_ym(a);
-        h = from_ym(h);
-      } else if (cyv[v] < 0) {
-        a = from_yp(a);
-        h = from_yp(h);
+      if (cyv[v] != 0) {
+        a = sh(a, -cyv[v], s0, xpc[v]);
+        h = sh(h, -cyv[v], s0, xhn[v]);
       }
       fst[v] = a;
       hst[v] = h;
@@ -320,7 +452,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     if (!WARM || i >= 2) {  // wave-uniform
       NbrVals<T> aun;
       CenterVals<T> acn;
-      uq(aqb, aqc, aq_new, fst, aun, acn);
+      uq(aqb, aqc, aq_new, fst, aun, acn, s0, kXAQ);
       if (!WARM || i >= 4) {
         // half-step values of row x-2 (NPC = 2: kept for B two rows later)
         T cen[12];
@@ -330,7 +462,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         cen[10] = po2.qny;
         cen[11] = po2.dp;
         if constexpr (NPC == 2) ring_put(1, s0, cen);
-        const UFStencil<T> s = uf_stencil(aum, au0, aun);
+        const UFStencil<T> s = ufs(aum, au0, aun, s0, kXAU);
         T Dxx, Dyx, Dp, om[9];
         update_fields<T, EXACT>(ac0, aqb.xx, aqb.yx, aqb.p, afc, s, P, Dxx, Dyx, Dp, om);
         if constexpr (NPC == 2) {
@@ -351,6 +483,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
           ao[10] = qa_out.yx;
           ao[11] = qa_out.p;
           ring_put(0, s0, ao);
+          pub_q(s0, kXQA, qa_out);  // B's stencil row one iteration later
         } else if (active) {
           // the last corrector: the new state of row x-2
           store_state(x - 2, cen, Dxx, Dyx, Dp, om, bold);
@@ -385,9 +518,9 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         }
         NbrVals<T> bun;
         CenterVals<T> bcn;
-        uq(bqb, bqc, qa_out, bfp, bun, bcn);
+        uq(bqb, bqc, qa_out, bfp, bun, bcn, s1, kXQA);
         if (!WARM || i >= 8) {
-          const UFStencil<T> s = uf_stencil(bum, bu0, bun);
+          const UFStencil<T> s = ufs(bum, bu0, bun, s0, kXBU);
           T Dxx, Dyx, Dp, om[9];
           update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
           T cmv[12];  // half-step values of row x-4
@@ -448,6 +581,43 @@ full_batch_kernel(const SweepJob<T>* __restrict__ jobs, const int njobs, const i
     const SweepJob<T>& J = find_job(jobs, njobs, item);
     full_item<T, EXACT, NPC>(J.A, J.P, J.g, item - J.first, wave_ring<T>());
   }
+}
+
+// The block-cooperative one-kernel step (npc = 2): one block of kXWaves waves
+// per item (block-uniform, so every wave of a block passes the same
+// barriers).
+template <class T>
+__device__ __forceinline__ T* block_exchange() {
+  extern __shared__ __attribute__((aligned(16))) unsigned char full_lds[];
+  return reinterpret_cast<T*>(full_lds + kXWaves * ring_bytes_per_wave<T>());
+}
+
+template <class T, bool EXACT>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(EXACT ? 1 : LYO_FULL_WAVES_PER_EU)))
+full_x_kernel(const SweepArgs<T> A, const Params<T> P, const Geo g) {
+  int b = blockIdx.x;
+  if (g.xcd) {
+    const int nb8 = (int)gridDim.x >> 3;
+    if (b < (nb8 << 3)) b = (b & 7) * nb8 + (b >> 3);
+  }
+  if (b >= g.nwaves) return;  // the whole block
+  full_item<T, EXACT, 2, true>(A, P, g, __builtin_amdgcn_readfirstlane(b), wave_ring<T>(),
+                               block_exchange<T>());
+}
+
+template <class T>
+constexpr size_t full_x_lds_bytes() {
+  return (size_t)kXWaves * ring_bytes_per_wave<T>() + (size_t)kXElems * sizeof(T);
+}
+
+template <class T, bool EXACT>
+inline hipError_t launch_full_x(const SweepArgs<T>& a, const Params<T>& p, const Geo& g,
+                                hipStream_t s) {
+  if (!geo_ok(g, kXStrip, 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((full_x_kernel<T, EXACT>), dim3(g.nwaves), dim3(kXWaves * 64),
+                     full_x_lds_bytes<T>(), s, a, p, g);
+  return hipGetLastError();
 }
 
 // LDS of a block: the rings of its waves (NPC = 2 only)

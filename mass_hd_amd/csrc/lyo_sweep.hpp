@@ -34,6 +34,28 @@ __device__ __forceinline__ float from_ym(float v) {
 __device__ __forceinline__ float from_yp(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
 }
+// The same shifts where the lane without a source (lane 0 for from_ym, lane
+// 63 for from_yp) keeps `old` instead of receiving 0 (bound_ctrl off).
+__device__ __forceinline__ float from_ym_or(float v, float old) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_yp_or(float v, float old) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_ym_or(double v, double old) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), 0x138, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double from_yp_or(double v, double old) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), 0x130, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 __device__ __forceinline__ double from_ym(double v) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x138, 0xF, 0xF, true);
@@ -165,6 +187,25 @@ __device__ __forceinline__ T half_of(T full, T src) {
 // Neighbour values UpdateFields needs at the centre row: x neighbours from the
 // rows before (um) and after (un), y neighbours from the adjacent lanes of the
 // centre row (u0).
+template <class T>
+__device__ __forceinline__ UFStencil<T> uf_stencil_x(const NbrVals<T>& um, const NbrVals<T>& u0,
+                                                     const NbrVals<T>& un) {
+  UFStencil<T> s;
+  s.mu_c = u0.mu;
+  s.ux_xp = un.ux;
+  s.ux_xm = um.ux;
+  s.uy_xp = un.uy;
+  s.uy_xm = um.uy;
+  s.uxp_xp = un.uxp;
+  s.uxp_xm = um.uxp;
+  s.sxx_xp = un.sxx;
+  s.sxx_xm = um.sxx;
+  s.syx_xp = un.syx;
+  s.syx_xm = um.syx;
+  s.mu_xp = un.mu;
+  s.mu_xm = um.mu;
+  return s;  // the y neighbours are the caller's
+}
 template <class T>
 __device__ __forceinline__ UFStencil<T> uf_stencil(const NbrVals<T>& um, const NbrVals<T>& u0,
                                                    const NbrVals<T>& un) {

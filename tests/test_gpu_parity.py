@@ -106,7 +106,7 @@ def test_fused_pair_equals_separate_sweeps(require_gpu, precision, npc, LX, LY, 
     # (fused pair, march order, one-kernel step): separate sweeps, all chunks
     # upwards or alternating; fused pair, alternating; whole step in one kernel
     # (FP32, npc = 2; otherwise the fused pair)
-    for fuse, alt, full in ((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 1, 1)):
+    for fuse, alt, full in ((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 1, 1), (1, 1, 2)):
         g = lyo.Lyotropic(LX, LY, npc=npc, precision=precision, **CONFIG0_PHYS)
         g.set_schedule(fuse, xc2, alt, full, xc2)
         g.configure(**CONFIG0_INIT)
@@ -178,10 +178,36 @@ def test_one_kernel_step_npc1_equals_sweeps(require_gpu, precision, LX, LY, xc):
         assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
 
 
+@pytest.mark.parametrize("precision", [lyo.FP64, lyo.FP32])
+@pytest.mark.parametrize("LX,LY,xc", [(24, 700, 9), (30, 500, 37), (16, 250, 8), (40, 244, 8),
+                                      (40, 245, 13), (16, 20, 8), (9, 489, 30)])
+def test_block_cooperative_step_equals_sweeps(require_gpu, precision, LX, LY, xc):
+    """The block-cooperative one-kernel step (four waves per block trading
+    their edge values through LDS) gives the separate sweeps' state and frame
+    fields bit for bit: grids narrower than a block, exactly one and just over
+    one block wide, several blocks with a ragged last one, short chunks."""
+    print("block step %dx%d xc=%d precision %d" % (LX, LY, xc, precision), flush=True)
+    runs = []
+    for fuse, full in ((0, 0), (1, 2)):
+        g = lyo.Lyotropic(LX, LY, npc=2, precision=precision, **CONFIG0_PHYS)
+        g.set_schedule(fuse_last_pair=fuse, one_kernel_step=full, step_chunk_rows=xc)
+        g.configure(**CONFIG0_INIT)
+        g.step(4)
+        mid = [g.get(n) for n in ("ux", "sigmaXY", "MU")]
+        g.step(3)
+        runs.append((g, mid))
+    (a, ma), (b, mb) = runs
+    for x, y in zip(ma, mb):
+        assert np.array_equal(x, y)
+    for n in STATE + DERIVED:
+        x, y = a.get(n), b.get(n)
+        assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
+
+
 def test_fused_pair_timing_kinds(require_gpu):
     # (predictor, separate corrector, fused pair, one-kernel step)
     for npc, full, kinds in ((1, 0, (1, 1, 0, 0)), (1, 1, (0, 0, 0, 1)), (2, 0, (1, 0, 1, 0)),
-                             (2, 1, (0, 0, 0, 1)), (3, 1, (1, 1, 1, 0))):
+                             (2, 1, (0, 0, 0, 1)), (2, 2, (0, 0, 0, 1)), (3, 1, (1, 1, 1, 0))):
         g = lyo.Lyotropic(64, 64, npc=npc, precision=lyo.FP32, **CONFIG0_PHYS)
         g.set_schedule(one_kernel_step=full)
         g.configure(**CONFIG0_INIT)
