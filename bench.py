@@ -54,6 +54,7 @@ PHYS = dict(GammaQ=0.1, GammaP=0.1, xi=0.7, tauNem=2.0, tauIso=2.0, rho=40.0, fr
             AA=0.01, CC=0.0, LL=0.04, KK=0.04, zeta=0.03, zetaI=0.0, conc=0.5)
 INIT = dict(config="random", seed=7, angle=90.0, noise=1.0, qamp=0.5, phinoise=0.05)
 HBM_PEAK = 8.0e12
+MFMA_BF16_PEAK = 2.5e15                          # dense BF16 FLOP/s
 VALU_PEAK_INSTR_PER_S = 256 * 4 * 2.4e9 / 4     # wave64 VALU instructions per second
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fp32_4096_npc2.json")
 LAUNCHES = ("predictor", "corrector", "pair", "step")
@@ -169,7 +170,19 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warm
     st = lyo.surrogate_stats(m)
     m.close()
     net.close()
+    # the structural bound of a surrogate step: the predictor + one corrector
+    # it still runs (the npc = 1 one-kernel step, timed on its own) plus the
+    # network's FLOPs at the dense BF16 MFMA peak
+    m1 = lyo.Lyotropic(L, L, npc=1, precision=lyo.FP32, **PHYS)
+    m1.configure(**INIT)
+    m1.step(warmup)
+    m1.synchronize()
+    t_one, _ = m1.time_steps(steps, per_sweep=False)
+    m1.close()
     N = L * L
+    flop = 2 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 8)
+    net_ms = flop * N / MFMA_BF16_PEAK * 1e3
+    bound_ms = t_one / steps + net_ms
     return {"surrogate_mlups": round(N * steps / (t_sur / 1e3) / 1e6, 1),
             "plain_mlups_same_run": round(N * steps / (t_plain / 1e3) / 1e6, 1),
             "surrogate_over_plain": round(t_plain / t_sur, 3),
@@ -177,7 +190,16 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warm
             "judged_velocity_error_last": st["last_error"],
             "judged_velocity_error_max": st["max_error"],
             "surrogate_steps": st["surrogate_steps"], "fallbacks": st["fallbacks"],
-            "net_flop_per_lu": 2 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 8),
+            "net_flop_per_lu": flop,
+            "ms_per_step": round(t_sur / steps, 4),
+            "plain_ms_per_step": round(t_plain / steps, 4),
+            "structural_bound": {
+                "npc1_step_ms": round(t_one / steps, 4),
+                "network_ms_at_bf16_peak": round(net_ms, 4),
+                "bound_ms": round(bound_ms, 4),
+                "best_possible_over_plain": round(t_plain / steps / bound_ms, 3),
+                "note": "predictor + one corrector (still run) + the network at 2.5 PFLOP/s "
+                        "dense BF16: the surrogate cannot beat the fused plain step on this GPU"},
             "train_loss": [round(l0, 4), round(l1, 5)]}
 
 

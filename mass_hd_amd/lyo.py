@@ -285,8 +285,9 @@ class Lyotropic:
         the fused kernel (0 keeps); alternate 1: neighbouring chunks march in
         opposite directions (default), 0: all upwards; one_kernel_step 1:
         npc=1 or 2 runs predictor and correctors in one kernel (default), 0:
-        predictor + fused pair or corrector sweep; step_chunk_rows: its rows
-        per wave chunk.  -1
+        predictor + fused pair or corrector sweep, 2 (npc = 2): its
+        block-cooperative form (waves trade edge values through LDS);
+        step_chunk_rows: its rows per chunk.  -1
         keeps a setting.  Results are bitwise identical either way."""
         self._check(self.lib.mass_lyo_set_schedule(self.h, int(fuse_last_pair),
                                                    int(pair_chunk_rows), int(alternate),

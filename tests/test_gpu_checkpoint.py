@@ -71,3 +71,38 @@ def test_checkpoint_refuses_other_runs(require_gpu):
     fresh = _run(16, 16, 1, lyo.FP32)
     with pytest.raises(lyo.MassLyoError):  # nothing to save before configure/set_state
         fresh.checkpoint()
+
+
+def test_fp64_restart_matches_the_oracle(require_gpu):
+    """A restarted FP64 run against the CPU oracle's uninterrupted run: the
+    checkpoint image holds the oracle's state bit for bit (SoA planes f v,
+    Qxx, Qyx, phi after the header), and two restarts later the state and the
+    frame fields still equal the oracle's."""
+    from oracle import pyoracle as po
+    LX, LY, npc = 52, 70, 2
+    o = po.Oracle(LX, LY, npc=npc, **CONFIG0_PHYS)
+    o.configure(**CONFIG0_INIT)
+    a = _run(LX, LY, npc, lyo.FP64)
+    a.configure(**CONFIG0_INIT)
+    a.step(5)
+    o.step(5)
+    img = a.checkpoint()
+    N = LX * LY
+    planes = np.frombuffer(img, dtype=np.float64, offset=64, count=12 * N).reshape(12, N)
+    assert np.array_equal(planes[:9].T, o.get("ff"))
+    assert np.array_equal(planes[9], o.get("QQxx"))
+    assert np.array_equal(planes[10], o.get("QQyx"))
+    assert np.array_equal(planes[11], o.get("phi"))
+    a.close()
+    b = _run(LX, LY, npc, lyo.FP64)
+    b.restore(img)
+    b.step(4)
+    img2 = b.checkpoint()
+    b.close()
+    c = _run(LX, LY, npc, lyo.FP64)
+    c.restore(img2)
+    c.step(3)
+    o.step(7)
+    for n in STATE + DERIVED:
+        x, y = c.get(n), o.get(n)
+        assert np.array_equal(x, y), "%s differs: max |d| = %g" % (n, np.abs(x - y).max())
