@@ -69,6 +69,14 @@ enum XId {
 };
 static_assert(kXEnd <= kXQ, "exchange quantities fit a slot");
 
+// s_waitcnt vmcnt(0) (gfx9 encoding; expcnt and lgkmcnt at their maxima): wait
+// for every vector memory access in flight.  Issued before a row's stores: the
+// next row's loads (issued after stage P) have arrived by then, so the waits
+// the compiler places after the divergent store block -- where it must assume
+// the stores may still be in flight ahead of those loads -- find nothing left
+// to wait for, and the next row starts without waiting for this row's stores.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Per-row outputs of P that A needs two rows later: half-step Q and the phi
 // terms (FP32: Dp and old phi; FP64: phi + Dp/2).
 template <class T>
@@ -484,9 +492,10 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
           ao[11] = qa_out.p;
           ring_put(0, s0, ao);
           pub_q(s0, kXQA, qa_out);  // B's stencil row one iteration later
-        } else if (active) {
+        } else {
           // the last corrector: the new state of row x-2
-          store_state(x - 2, cen, Dxx, Dyx, Dp, om, bold);
+          wait_vmem();
+          if (active) store_state(x - 2, cen, Dxx, Dyx, Dp, om, bold);
         }
       }
       aum = au0;
@@ -525,6 +534,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
           update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
           T cmv[12];  // half-step values of row x-4
           ring_get(1, s2, cmv);
+          wait_vmem();
           if (active) store_state(x - 4, cmv, Dxx, Dyx, Dp, om, bold);
         }
         bum = bu0;
@@ -538,6 +548,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
   constexpr int kWarm = 4 * NPC;  // nI >= kWarm + 1: a chunk has a row
   int i = 0;
   for (; i < kWarm; ++i) row(i, xs + i, i % 3, std::true_type{});
+  wait_vmem();  // the loop's entry then has no loads in flight either
   for (; i + 3 <= nI; i += 3) {
     row(i, xs + i, kWarm % 3, std::false_type{});
     row(i + 1, xs + i + 1, (kWarm + 1) % 3, std::false_type{});
