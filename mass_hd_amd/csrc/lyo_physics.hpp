@@ -302,10 +302,11 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     // shifted equilibrium fe'_v = fe_v - rest_weight(v)*rho
     //   = dn*rest_weight(v) + nn*(u-dependent part of fe_v/nn)
     const T it = fast_rcp(tau);
-    const T a = T(1. / 3.) * (vx * vx + vy * vy);
+    const T v2 = vx * vx + vy * vy;
+    const T a = T(1. / 3.) * v2;
     const T bx = T(.5) * vx * vx, by = T(.5) * vy * vy;
     const T sx = T(1. / 3.) * vx, sy = T(1. / 3.) * vy;
-    const T v2q = T(1. / 12.) * (vx * vx + vy * vy);
+    const T v2q = T(1. / 12.) * v2;
     const T xy = T(.25) * vx * vy;
     const T pp = T(1. / 12.) * (vx + vy), pm = T(1. / 12.) * (vx - vy);
     const T dn9 = c.dn * T(1. / 9.), dn36 = c.dn * T(1. / 36.);
@@ -318,7 +319,9 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     fe[6] = dn36 + nn * (v2q + xy - pp);
     fe[7] = dn36 + nn * (v2q - xy - pm);
     fe[8] = dn36 + nn * (v2q - xy + pm);
-    fe[0] = c.dn - (((fe[1] + fe[2]) + (fe[3] + fe[4])) + ((fe[5] + fe[6]) + (fe[7] + fe[8])));
+    // the rest direction in closed form, fe_0 = nn (4/9 - (2/3) u^2): the
+    // reference's nn - (sum of the other eight) in exact arithmetic
+    fe[0] = T(4. / 9.) * c.dn - nn * (T(2. / 3.) * v2);
     const T F3x = T(1. / 3.) * Fx, F3y = T(1. / 3.) * Fy;
     const T F12p = T(1. / 12.) * (Fx + Fy), F12m = T(1. / 12.) * (Fx - Fy);
     const T force[9] = {T(0), F3x, -F3x, F3y, -F3y, F12p, -F12p, -F12m, F12m};
