@@ -77,9 +77,10 @@ __global__ void derived_kernel(const T* __restrict__ f, const T* __restrict__ qx
   NbrVals<T> nv;
   CenterVals<T> cv;
   DiagVals<T> dg;
-  update_quantities<T, EXACT, true>(fl, qxx[k], qxx[kxp], qxx[kxm], qxx[kyp], qxx[kym], qyx[k],
-                                    qyx[kxp], qyx[kxm], qyx[kyp], qyx[kym], ph(k), ph(kxp),
-                                    ph(kxm), ph(kyp), ph(kym), P, nv, cv, &dg);
+  update_quantities<T, EXACT, true>(
+      fl, qxx[k], qxx[kxp], qxx[kxm], ynb<T, EXACT>(qxx[kyp], qxx[kym]), qyx[k], qyx[kxp],
+      qyx[kxm], ynb<T, EXACT>(qyx[kyp], qyx[kym]), ph(k), ph(kxp), ph(kxm),
+      ynb<T, EXACT>(ph(kyp), ph(kym)), P, nv, cv, &dg);
   T r = T(0);
   switch (field) {
     case D_N: r = cv.nn; break;

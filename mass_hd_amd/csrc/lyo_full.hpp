@@ -240,13 +240,16 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
   };
   // UpdateQuantities on the centre row of three stencil rows (x-1, x, x+1)
   // (slot, id: where the wave boundary values of q0 were published)
+  // y neighbours of v (XCH: the wave boundary values from the slot and id
+  // where the neighbour waves published v)
+  auto yn = [&](T v, int slot, int id) {
+    return ynb<T, EXACT>(sh(v, 1, slot, id), sh(v, -1, slot, id));
+  };
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
                 NbrVals<T>& nv, CenterVals<T>& cv, int slot, int id) {
-    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, sh(q0.xx, 1, slot, id),
-                                sh(q0.xx, -1, slot, id), q0.yx, qp.yx, qm.yx,
-                                sh(q0.yx, 1, slot, id + 1), sh(q0.yx, -1, slot, id + 1), q0.p,
-                                qp.p, qm.p, sh(q0.p, 1, slot, id + 2), sh(q0.p, -1, slot, id + 2),
-                                P, nv, cv);
+    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, yn(q0.xx, slot, id), q0.yx, qp.yx, qm.yx,
+                                yn(q0.yx, slot, id + 1), q0.p, qp.p, qm.p,
+                                yn(q0.p, slot, id + 2), P, nv, cv);
   };
   auto pub_q = [&](int slot, int id, const QP<T>& q) {
     xpub(slot, id, q.xx);
@@ -257,21 +260,15 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
   auto ufs = [&](const NbrVals<T>& um, const NbrVals<T>& u0, const NbrVals<T>& un, int slot,
                  int id) {
     if constexpr (!XCH) {
-      return uf_stencil(um, u0, un);
+      return uf_stencil<T, EXACT>(um, u0, un);
     } else {
-      UFStencil<T> s = uf_stencil_x(um, u0, un);
-      s.ux_yp = sh(u0.ux, 1, slot, id);
-      s.ux_ym = sh(u0.ux, -1, slot, id);
-      s.uy_yp = sh(u0.uy, 1, slot, id + 1);
-      s.uy_ym = sh(u0.uy, -1, slot, id + 1);
-      s.uyp_yp = sh(u0.uyp, 1, slot, id + 2);
-      s.uyp_ym = sh(u0.uyp, -1, slot, id + 2);
-      s.sxy_yp = sh(u0.sxy, 1, slot, id + 3);
-      s.sxy_ym = sh(u0.sxy, -1, slot, id + 3);
-      s.syy_yp = sh(u0.syy, 1, slot, id + 4);
-      s.syy_ym = sh(u0.syy, -1, slot, id + 4);
-      s.mu_yp = sh(u0.mu, 1, slot, id + 5);
-      s.mu_ym = sh(u0.mu, -1, slot, id + 5);
+      UFStencil<T, EXACT> s = uf_stencil_x<T, EXACT>(um, u0, un);
+      s.ux_y = yn(u0.ux, slot, id);
+      s.uy_y = yn(u0.uy, slot, id + 1);
+      s.uyp_y = yn(u0.uyp, slot, id + 2);
+      s.sxy_y = yn(u0.sxy, slot, id + 3);
+      s.syy_y = yn(u0.syy, slot, id + 4);
+      s.mu_y = yn(u0.mu, slot, id + 5);
       return s;
     }
   };
@@ -372,7 +369,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     QP<T> aq_new;  // A's stencil values of row x
     POut<T> po;
     {
-      const UFStencil<T> s = ufs(pum, pu0, pun, s0, kXPU);
+      const UFStencil<T, EXACT> s = ufs(pum, pu0, pun, s0, kXPU);
       T Dxx, Dyx, Dp, om[9];
       update_fields<T, EXACT>(pc0, pqb.xx, pqb.yx, pqb.p, pfc, s, P, Dxx, Dyx, Dp, om);
       const T Qsx = pqb.xx + Dxx, Qsy = pqb.yx + Dyx;
@@ -470,7 +467,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         cen[10] = po2.qny;
         cen[11] = po2.dp;
         if constexpr (NPC == 2) ring_put(1, s0, cen);
-        const UFStencil<T> s = ufs(aum, au0, aun, s0, kXAU);
+        const UFStencil<T, EXACT> s = ufs(aum, au0, aun, s0, kXAU);
         T Dxx, Dyx, Dp, om[9];
         update_fields<T, EXACT>(ac0, aqb.xx, aqb.yx, aqb.p, afc, s, P, Dxx, Dyx, Dp, om);
         if constexpr (NPC == 2) {
@@ -529,7 +526,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         CenterVals<T> bcn;
         uq(bqb, bqc, qa_out, bfp, bun, bcn, s1, kXQA);
         if (!WARM || i >= 8) {
-          const UFStencil<T> s = ufs(bum, bu0, bun, s0, kXBU);
+          const UFStencil<T, EXACT> s = ufs(bum, bu0, bun, s0, kXBU);
           T Dxx, Dyx, Dp, om[9];
           update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
           T cmv[12];  // half-step values of row x-4

@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(256) growth_uf_kernel(GrowthUF a, Params<doubl
   c.dyQxx = pl(LP_DYQXX)[k];
   c.dxQyx = pl(LP_DXQYX)[k];
   c.dyQyx = pl(LP_DYQYX)[k];
-  UFStencil<T> s;
+  UFStencil<T, true> s;
   s.mu_c = MU[k];
   s.ux_xp = ux[n.kxp];
   s.ux_xm = ux[n.kxm];
@@ -194,18 +194,12 @@ __global__ void __launch_bounds__(256) growth_uf_kernel(GrowthUF a, Params<doubl
   s.syx_xm = pl(LP_SYX)[n.kxm];
   s.mu_xp = MU[n.kxp];
   s.mu_xm = MU[n.kxm];
-  s.ux_yp = ux[n.kyp];
-  s.ux_ym = ux[n.kym];
-  s.uy_yp = uy[n.kyp];
-  s.uy_ym = uy[n.kym];
-  s.uyp_yp = pl(LP_UYP)[n.kyp];
-  s.uyp_ym = pl(LP_UYP)[n.kym];
-  s.sxy_yp = pl(LP_SXY)[n.kyp];
-  s.sxy_ym = pl(LP_SXY)[n.kym];
-  s.syy_yp = pl(LP_SYY)[n.kyp];
-  s.syy_ym = pl(LP_SYY)[n.kym];
-  s.mu_yp = MU[n.kyp];
-  s.mu_ym = MU[n.kym];
+  s.ux_y = ynb<T, true>(ux[n.kyp], ux[n.kym]);
+  s.uy_y = ynb<T, true>(uy[n.kyp], uy[n.kym]);
+  s.uyp_y = ynb<T, true>(pl(LP_UYP)[n.kyp], pl(LP_UYP)[n.kym]);
+  s.sxy_y = ynb<T, true>(pl(LP_SXY)[n.kyp], pl(LP_SXY)[n.kym]);
+  s.syy_y = ynb<T, true>(pl(LP_SYY)[n.kyp], pl(LP_SYY)[n.kym]);
+  s.mu_y = ynb<T, true>(MU[n.kyp], MU[n.kym]);
   T fl[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) fl[v] = u.f[v * NP + k];

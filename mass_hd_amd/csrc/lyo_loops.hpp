@@ -104,10 +104,10 @@ __global__ void __launch_bounds__(256) loop_uq_kernel(const T* __restrict__ f,
   NbrVals<T> nv;
   CenterVals<T> cv;
   DiagVals<T> dg;
-  update_quantities<T, EXACT, true>(fl, qxx[k], qxx[n.kxp], qxx[n.kxm], qxx[n.kyp], qxx[n.kym],
-                                    qyx[k], qyx[n.kxp], qyx[n.kxm], qyx[n.kyp], qyx[n.kym],
-                                    phi[k], phi[n.kxp], phi[n.kxm], phi[n.kyp], phi[n.kym], P,
-                                    nv, cv, &dg);
+  update_quantities<T, EXACT, true>(
+      fl, qxx[k], qxx[n.kxp], qxx[n.kxm], ynb<T, EXACT>(qxx[n.kyp], qxx[n.kym]), qyx[k],
+      qyx[n.kxp], qyx[n.kxm], ynb<T, EXACT>(qyx[n.kyp], qyx[n.kym]), phi[k], phi[n.kxp],
+      phi[n.kxm], ynb<T, EXACT>(phi[n.kyp], phi[n.kym]), P, nv, cv, &dg);
   const T out[kLoopPlanes] = {cv.nn,     nv.ux,     nv.uy,     nv.uxp,    nv.uyp,    cv.Hxx,
                               cv.Hyx,    nv.mu,     cv.dxQxx,  cv.dxQyx,  cv.dyQxx,  cv.dyQyx,
                               nv.sxx,    nv.syy,    nv.sxy,    nv.syx,    dg.bulk,   dg.el_xx,
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) loop_uf_kernel(LoopUF<T> a, Params<T> P, 
   c.dyQxx = pl(LP_DYQXX)[k];
   c.dxQyx = pl(LP_DXQYX)[k];
   c.dyQyx = pl(LP_DYQYX)[k];
-  UFStencil<T> s;
+  UFStencil<T, EXACT> s;
   s.mu_c = pl(LP_MU)[k];
   s.ux_xp = pl(LP_UX)[n.kxp];
   s.ux_xm = pl(LP_UX)[n.kxm];
@@ -176,18 +176,12 @@ __global__ void __launch_bounds__(256) loop_uf_kernel(LoopUF<T> a, Params<T> P, 
   s.syx_xm = pl(LP_SYX)[n.kxm];
   s.mu_xp = pl(LP_MU)[n.kxp];
   s.mu_xm = pl(LP_MU)[n.kxm];
-  s.ux_yp = pl(LP_UX)[n.kyp];
-  s.ux_ym = pl(LP_UX)[n.kym];
-  s.uy_yp = pl(LP_UY)[n.kyp];
-  s.uy_ym = pl(LP_UY)[n.kym];
-  s.uyp_yp = pl(LP_UYP)[n.kyp];
-  s.uyp_ym = pl(LP_UYP)[n.kym];
-  s.sxy_yp = pl(LP_SXY)[n.kyp];
-  s.sxy_ym = pl(LP_SXY)[n.kym];
-  s.syy_yp = pl(LP_SYY)[n.kyp];
-  s.syy_ym = pl(LP_SYY)[n.kym];
-  s.mu_yp = pl(LP_MU)[n.kyp];
-  s.mu_ym = pl(LP_MU)[n.kym];
+  s.ux_y = ynb<T, EXACT>(pl(LP_UX)[n.kyp], pl(LP_UX)[n.kym]);
+  s.uy_y = ynb<T, EXACT>(pl(LP_UY)[n.kyp], pl(LP_UY)[n.kym]);
+  s.uyp_y = ynb<T, EXACT>(pl(LP_UYP)[n.kyp], pl(LP_UYP)[n.kym]);
+  s.sxy_y = ynb<T, EXACT>(pl(LP_SXY)[n.kyp], pl(LP_SXY)[n.kym]);
+  s.syy_y = ynb<T, EXACT>(pl(LP_SYY)[n.kyp], pl(LP_SYY)[n.kym]);
+  s.mu_y = ynb<T, EXACT>(pl(LP_MU)[n.kyp], pl(LP_MU)[n.kym]);
   T fl[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) fl[v] = a.f[v * NP + k];

@@ -32,9 +32,10 @@ __device__ void uq_at(const T* f, const T* qxx, const T* qyx, const T* phi, cons
   T fl[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) fl[v] = f[v * NP + k];
-  update_quantities<T, EXACT>(fl, qxx[k], qxx[kxp], qxx[kxm], qxx[kyp], qxx[kym], qyx[k],
-                              qyx[kxp], qyx[kxm], qyx[kyp], qyx[kym], ph(k), ph(kxp), ph(kxm),
-                              ph(kyp), ph(kym), P, nv, cv);
+  update_quantities<T, EXACT>(fl, qxx[k], qxx[kxp], qxx[kxm], ynb<T, EXACT>(qxx[kyp], qxx[kym]),
+                              qyx[k], qyx[kxp], qyx[kxm], ynb<T, EXACT>(qyx[kyp], qyx[kym]),
+                              ph(k), ph(kxp), ph(kxm), ynb<T, EXACT>(ph(kyp), ph(kym)), P, nv,
+                              cv);
 }
 
 // Features of every node: FP32 [N][8] (feat), or -- surrogate inference --

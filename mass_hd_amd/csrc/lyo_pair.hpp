@@ -132,10 +132,9 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
   };
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
                 NbrVals<T>& nv, CenterVals<T>& cv) {
-    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, from_yp(q0.xx), from_ym(q0.xx),
-                                q0.yx, qp.yx, qm.yx, from_yp(q0.yx), from_ym(q0.yx),
-                                q0.p, qp.p, qm.p, from_yp(q0.p), from_ym(q0.p), P, nv,
-                                cv);
+    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, ynb_lanes<T, EXACT>(q0.xx), q0.yx,
+                                qp.yx, qm.yx, ynb_lanes<T, EXACT>(q0.yx), q0.p, qp.p, qm.p,
+                                ynb_lanes<T, EXACT>(q0.p), P, nv, cv);
   };
 
   // The chunk is marched upwards (D = +1) or downwards (D = -1), alternating
@@ -151,7 +150,7 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
       }
     };
     auto ufs = [&](const NbrVals<T>& prev, const NbrVals<T>& u0, const NbrVals<T>& next) {
-      return (D > 0) ? uf_stencil(prev, u0, next) : uf_stencil(next, u0, prev);
+      return (D > 0) ? uf_stencil<T, EXACT>(prev, u0, next) : uf_stencil<T, EXACT>(next, u0, prev);
     };
     // stage A visits rows xs, xs+D, ...: two rows before the chunk, the
     // chunk, two rows after it; stage B trails A by two rows
@@ -218,7 +217,7 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
         cen = cur.c;
       }
       {
-        const UFStencil<T> s = ufs(um, u0, un);
+        const UFStencil<T, EXACT> s = ufs(um, u0, un);
         T Dxx, Dyx, Dp, om[9];
         update_fields<T, EXACT>(c0, qb.xx, qb.yx, qb.p, fc, s, P, Dxx, Dyx, Dp, om);
         qa_out.xx = cen.qnxx + T(.5) * Dxx;
@@ -250,7 +249,7 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
         CenterVals<T> bcn;
         uqd(bqb, bqc, qa_out, bfp, bun, bcn);
         if (i >= 4) {
-          const UFStencil<T> s = ufs(bum, bu0, bun);
+          const UFStencil<T, EXACT> s = ufs(bum, bu0, bun);
           T Dxx, Dyx, Dp, om[9];
           update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
           if (active) {

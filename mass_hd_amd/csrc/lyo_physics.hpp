@@ -62,6 +62,38 @@ __device__ __forceinline__ T lap5(T c, T xp, T xm, T yp, T ym) {
   }
 }
 
+// The y+1 and y-1 values of a quantity at a node, as the node arithmetic
+// reads them.  FP64 (EXACT) keeps both: the reference's sums run left to
+// right over them.  FP32 keeps their sum and difference, all its stencils use,
+// formed where the values are taken (from the adjacent lanes of a wave), so
+// that the lane shift folds into the add or subtract that consumes it.
+template <class T, bool EXACT>
+struct YNb;
+template <class T>
+struct YNb<T, true> {
+  T p, m;
+};
+template <class T>
+struct YNb<T, false> {
+  T s, d;  // (y+1) + (y-1), (y+1) - (y-1)
+};
+template <class T, bool EXACT>
+__device__ __forceinline__ YNb<T, EXACT> ynb(T yp, T ym) {
+  if constexpr (EXACT) {
+    return {yp, ym};
+  } else {
+    return {yp + ym, yp - ym};
+  }
+}
+template <class T, bool EXACT>
+__device__ __forceinline__ T lap5(T c, T xp, T xm, const YNb<T, EXACT>& y) {
+  if constexpr (EXACT) {
+    return lap5<T, true>(c, xp, xm, y.p, y.m);
+  } else {
+    return (xp + xm) + y.s - T(4) * c;
+  }
+}
+
 template <class T>
 __device__ __forceinline__ T fast_rcp(T x) {
   if constexpr (sizeof(T) == 4) {
@@ -91,13 +123,13 @@ struct DiagVals {
 };
 
 // UpdateQuantitiesAtNode (src/models/lyotropic.cpp:168-247) for one node.
-// f: populations at the node (EXACT: f; else f').  q*/ph: 5-point stencil
-// values (centre, x+1, x-1, y+1, y-1) of Qxx, Qyx, phi.
+// f: populations at the node (EXACT: f; else f').  q*/p*: 5-point stencil
+// values (centre, x+1, x-1, y neighbours) of Qxx, Qyx, phi.
 template <class T, bool EXACT, bool DIAG = false>
 __device__ __forceinline__ void update_quantities(
-    const T (&f)[9], T qxx_c, T qxx_xp, T qxx_xm, T qxx_yp, T qxx_ym, T qyx_c,
-    T qyx_xp, T qyx_xm, T qyx_yp, T qyx_ym, T p_c, T p_xp, T p_xm, T p_yp,
-    T p_ym, const Params<T>& P, NbrVals<T>& o, CenterVals<T>& c,
+    const T (&f)[9], T qxx_c, T qxx_xp, T qxx_xm, const YNb<T, EXACT>& qxx_y, T qyx_c,
+    T qyx_xp, T qyx_xm, const YNb<T, EXACT>& qyx_y, T p_c, T p_xp, T p_xm,
+    const YNb<T, EXACT>& p_y, const Params<T>& P, NbrVals<T>& o, CenterVals<T>& c,
     DiagVals<T>* dg = nullptr) {
   const T Qxx = qxx_c, Qyx = qyx_c, p = p_c;
   T nn, dn, vx, vy;
@@ -116,9 +148,9 @@ __device__ __forceinline__ void update_quantities(
     vy = ((f[3] - f[4]) + (f[5] - f[6]) + (f[7] - f[8])) * inv;
   }
 
-  const T del2Qxx = lap5<T, EXACT>(qxx_c, qxx_xp, qxx_xm, qxx_yp, qxx_ym);
-  const T del2Qyx = lap5<T, EXACT>(qyx_c, qyx_xp, qyx_xm, qyx_yp, qyx_ym);
-  const T del2p = lap5<T, EXACT>(p_c, p_xp, p_xm, p_yp, p_ym);
+  const T del2Qxx = lap5<T, EXACT>(qxx_c, qxx_xp, qxx_xm, qxx_y);
+  const T del2Qyx = lap5<T, EXACT>(qyx_c, qyx_xp, qyx_xm, qyx_y);
+  const T del2p = lap5<T, EXACT>(p_c, p_xp, p_xm, p_y);
   const T term = T(0) - Qxx * Qxx - Qyx * Qyx;
   const T Hxx = P.CC * term * Qxx + P.LL * del2Qxx;
   const T Hyx = P.CC * term * Qyx + P.LL * del2Qyx;
@@ -137,11 +169,11 @@ __device__ __forceinline__ void update_quantities(
   T dxQxx, dyQxx, dxQyx, dyQyx, grad_f, grad_s, pf_xx, pf_yx;
   if constexpr (EXACT) {
     dxQxx = dcen<T, EXACT>(qxx_xp, qxx_xm);
-    dyQxx = dcen<T, EXACT>(qxx_yp, qxx_ym);
+    dyQxx = dcen<T, EXACT>(qxx_y.p, qxx_y.m);
     dxQyx = dcen<T, EXACT>(qyx_xp, qyx_xm);
-    dyQyx = dcen<T, EXACT>(qyx_yp, qyx_ym);
+    dyQyx = dcen<T, EXACT>(qyx_y.p, qyx_y.m);
     const T dxPhi = dcen<T, EXACT>(p_xp, p_xm);
-    const T dyPhi = dcen<T, EXACT>(p_yp, p_ym);
+    const T dyPhi = dcen<T, EXACT>(p_y.p, p_y.m);
     pf_xx = T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi);
     pf_yx = -P.KK * dxPhi * dyPhi;
     // the reference's expressions as written (lyotropic.cpp:212-219)
@@ -156,10 +188,10 @@ __device__ __forceinline__ void update_quantities(
     grad_s = sigmaS;
   } else {
     dxQxx = qxx_xp - qxx_xm;
-    dyQxx = qxx_yp - qxx_ym;
+    dyQxx = qxx_y.d;
     dxQyx = qyx_xp - qyx_xm;
-    dyQyx = qyx_yp - qyx_ym;
-    const T rxP = p_xp - p_xm, ryP = p_yp - p_ym;
+    dyQyx = qyx_y.d;
+    const T rxP = p_xp - p_xm, ryP = p_y.d;
     pf_xx = P.KK8 * (ryP * ryP - rxP * rxP);
     pf_yx = -P.KK4 * rxP * ryP;
     grad_f = T(2) * P.xi * ((Qxx * Qxx - T(1)) * Hxx + Qxx * Qyx * Hyx) -
@@ -208,36 +240,36 @@ __device__ __forceinline__ void update_quantities(
 // Beris-Edwards D, the Cahn-Hilliard Dp, the force and the collision operator
 // Omega_v = (fe_v - f_v)/tau + w_v (F . c_v).
 // Stencil inputs: x+1 / x-1 values of ux, uy, ux_phi, sigmaXX, sigmaYX, MU and
-// y+1 / y-1 values of ux, uy, uy_phi, sigmaXY, sigmaYY, MU.
-template <class T>
+// the y neighbours of ux, uy, uy_phi, sigmaXY, sigmaYY, MU.
+template <class T, bool EXACT>
 struct UFStencil {
   T mu_c;
   T ux_xp, ux_xm, uy_xp, uy_xm, uxp_xp, uxp_xm, sxx_xp, sxx_xm, syx_xp, syx_xm, mu_xp, mu_xm;
-  T ux_yp, ux_ym, uy_yp, uy_ym, uyp_yp, uyp_ym, sxy_yp, sxy_ym, syy_yp, syy_ym, mu_yp, mu_ym;
+  YNb<T, EXACT> ux_y, uy_y, uyp_y, sxy_y, syy_y, mu_y;
 };
 
 template <class T, bool EXACT>
 __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Qyx,
                                               T p, const T (&f)[9],
-                                              const UFStencil<T>& s,
+                                              const UFStencil<T, EXACT>& s,
                                               const Params<T>& P, T& Dxx,
                                               T& Dyx, T& Dp, T (&om)[9]) {
   const T nn = c.nn, vx = c.vx, vy = c.vy;
   const T Hxx = c.Hxx, Hyx = c.Hyx;
   const T dxQxx = c.dxQxx, dyQxx = c.dyQxx, dxQyx = c.dxQyx, dyQyx = c.dyQyx;
 
-  const T del2mu = lap5<T, EXACT>(s.mu_c, s.mu_xp, s.mu_xm, s.mu_yp, s.mu_ym);
+  const T del2mu = lap5<T, EXACT>(s.mu_c, s.mu_xp, s.mu_xm, s.mu_y);
   T Fx, Fy;
   if constexpr (EXACT) {
     const T dxux = dcen<T, EXACT>(s.ux_xp, s.ux_xm);
-    const T dyux = dcen<T, EXACT>(s.ux_yp, s.ux_ym);
+    const T dyux = dcen<T, EXACT>(s.ux_y.p, s.ux_y.m);
     const T dxuy = dcen<T, EXACT>(s.uy_xp, s.uy_xm);
-    const T dyuy = dcen<T, EXACT>(s.uy_yp, s.uy_ym);
+    const T dyuy = dcen<T, EXACT>(s.uy_y.p, s.uy_y.m);
     const T dxSxx = dcen<T, EXACT>(s.sxx_xp, s.sxx_xm);
-    const T dySxy = dcen<T, EXACT>(s.sxy_yp, s.sxy_ym);
+    const T dySxy = dcen<T, EXACT>(s.sxy_y.p, s.sxy_y.m);
     const T dxSyx = dcen<T, EXACT>(s.syx_xp, s.syx_xm);
-    const T dySyy = dcen<T, EXACT>(s.syy_yp, s.syy_ym);
-    const T pFlux = dcen<T, EXACT>(s.uxp_xp, s.uxp_xm) + dcen<T, EXACT>(s.uyp_yp, s.uyp_ym);
+    const T dySyy = dcen<T, EXACT>(s.syy_y.p, s.syy_y.m);
+    const T pFlux = dcen<T, EXACT>(s.uxp_xp, s.uxp_xm) + dcen<T, EXACT>(s.uyp_y.p, s.uyp_y.m);
 
     const T expansion = dxux + dyuy;
     const T shear = T(.5) * (dxuy + dyux);
@@ -256,8 +288,8 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     // raw differences r = 2 d (the Q derivatives of CenterVals are raw too):
     // E2 = 2 expansion, S4 = 4 shear, W4 = 4 vorticity, TQ2 = 2 traceQL; the
     // flow and advection terms are formed doubled and halved once
-    const T rxux = s.ux_xp - s.ux_xm, ryux = s.ux_yp - s.ux_ym;
-    const T rxuy = s.uy_xp - s.uy_xm, ryuy = s.uy_yp - s.uy_ym;
+    const T rxux = s.ux_xp - s.ux_xm, ryux = s.ux_y.d;
+    const T rxuy = s.uy_xp - s.uy_xm, ryuy = s.uy_y.d;
     const T E2 = rxux + ryuy;
     const T S4 = rxuy + ryux;
     const T W4 = rxuy - ryux;
@@ -268,10 +300,10 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     const T ayx = vx * dxQyx + vy * dyQyx;
     Dxx = P.GammaQ * Hxx + T(.5) * (Vxx - axx);
     Dyx = P.GammaQ * Hyx + T(.5) * (Vyx - ayx);
-    const T rflux = (s.uxp_xp - s.uxp_xm) + (s.uyp_yp - s.uyp_ym);
+    const T rflux = (s.uxp_xp - s.uxp_xm) + s.uyp_y.d;
     Dp = P.GammaP * del2mu - T(.5) * rflux;
-    const T gx = (s.sxx_xp - s.sxx_xm) + (s.sxy_yp - s.sxy_ym);
-    const T gy = (s.syx_xp - s.syx_xm) + (s.syy_yp - s.syy_ym);
+    const T gx = (s.sxx_xp - s.sxx_xm) + s.sxy_y.d;
+    const T gy = (s.syx_xp - s.syx_xm) + s.syy_y.d;
     Fx = T(.5) * gx - P.friction * vx;
     Fy = T(.5) * gy - P.friction * vy;
   }

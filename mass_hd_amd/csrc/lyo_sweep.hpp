@@ -184,13 +184,20 @@ __device__ __forceinline__ T half_of(T full, T src) {
 #define LYO_SWEEP_WAVES_PER_EU 3
 #endif
 
+// y neighbours of a value from the adjacent lanes of the wave
+template <class T, bool EXACT>
+__device__ __forceinline__ YNb<T, EXACT> ynb_lanes(T v) {
+  return ynb<T, EXACT>(from_yp(v), from_ym(v));
+}
+
 // Neighbour values UpdateFields needs at the centre row: x neighbours from the
 // rows before (um) and after (un), y neighbours from the adjacent lanes of the
-// centre row (u0).
-template <class T>
-__device__ __forceinline__ UFStencil<T> uf_stencil_x(const NbrVals<T>& um, const NbrVals<T>& u0,
-                                                     const NbrVals<T>& un) {
-  UFStencil<T> s;
+// centre row (u0; uf_stencil_x leaves them to the caller).
+template <class T, bool EXACT>
+__device__ __forceinline__ UFStencil<T, EXACT> uf_stencil_x(const NbrVals<T>& um,
+                                                            const NbrVals<T>& u0,
+                                                            const NbrVals<T>& un) {
+  UFStencil<T, EXACT> s;
   s.mu_c = u0.mu;
   s.ux_xp = un.ux;
   s.ux_xm = um.ux;
@@ -204,37 +211,19 @@ __device__ __forceinline__ UFStencil<T> uf_stencil_x(const NbrVals<T>& um, const
   s.syx_xm = um.syx;
   s.mu_xp = un.mu;
   s.mu_xm = um.mu;
-  return s;  // the y neighbours are the caller's
+  return s;
 }
-template <class T>
-__device__ __forceinline__ UFStencil<T> uf_stencil(const NbrVals<T>& um, const NbrVals<T>& u0,
-                                                   const NbrVals<T>& un) {
-  UFStencil<T> s;
-  s.mu_c = u0.mu;
-  s.ux_xp = un.ux;
-  s.ux_xm = um.ux;
-  s.uy_xp = un.uy;
-  s.uy_xm = um.uy;
-  s.uxp_xp = un.uxp;
-  s.uxp_xm = um.uxp;
-  s.sxx_xp = un.sxx;
-  s.sxx_xm = um.sxx;
-  s.syx_xp = un.syx;
-  s.syx_xm = um.syx;
-  s.mu_xp = un.mu;
-  s.mu_xm = um.mu;
-  s.ux_yp = from_yp(u0.ux);
-  s.ux_ym = from_ym(u0.ux);
-  s.uy_yp = from_yp(u0.uy);
-  s.uy_ym = from_ym(u0.uy);
-  s.uyp_yp = from_yp(u0.uyp);
-  s.uyp_ym = from_ym(u0.uyp);
-  s.sxy_yp = from_yp(u0.sxy);
-  s.sxy_ym = from_ym(u0.sxy);
-  s.syy_yp = from_yp(u0.syy);
-  s.syy_ym = from_ym(u0.syy);
-  s.mu_yp = from_yp(u0.mu);
-  s.mu_ym = from_ym(u0.mu);
+template <class T, bool EXACT>
+__device__ __forceinline__ UFStencil<T, EXACT> uf_stencil(const NbrVals<T>& um,
+                                                          const NbrVals<T>& u0,
+                                                          const NbrVals<T>& un) {
+  UFStencil<T, EXACT> s = uf_stencil_x<T, EXACT>(um, u0, un);
+  s.ux_y = ynb_lanes<T, EXACT>(u0.ux);
+  s.uy_y = ynb_lanes<T, EXACT>(u0.uy);
+  s.uyp_y = ynb_lanes<T, EXACT>(u0.uyp);
+  s.sxy_y = ynb_lanes<T, EXACT>(u0.sxy);
+  s.syy_y = ynb_lanes<T, EXACT>(u0.syy);
+  s.mu_y = ynb_lanes<T, EXACT>(u0.mu);
   return s;
 }
 
@@ -315,10 +304,9 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
   };
   auto uq = [&](const QP<T>& qm, const QP<T>& q0, const QP<T>& qp, const T(&f)[9],
                 NbrVals<T>& nv, CenterVals<T>& cv) {
-    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, from_yp(q0.xx), from_ym(q0.xx),
-                                q0.yx, qp.yx, qm.yx, from_yp(q0.yx), from_ym(q0.yx),
-                                q0.p, qp.p, qm.p, from_yp(q0.p), from_ym(q0.p), P, nv,
-                                cv);
+    update_quantities<T, EXACT>(f, q0.xx, qp.xx, qm.xx, ynb_lanes<T, EXACT>(q0.xx), q0.yx,
+                                qp.yx, qm.yx, ynb_lanes<T, EXACT>(q0.yx), q0.p, qp.p, qm.p,
+                                ynb_lanes<T, EXACT>(q0.p), P, nv, cv);
   };
 
   // The chunk is marched upwards (D = +1, rows x0 .. x1-1) or downwards
@@ -376,7 +364,8 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
     uqd(qb, qc, cur.q, cur.f, un, cn);
 
     // UpdateFields on row x
-    const UFStencil<T> s = (D > 0) ? uf_stencil(um, u0, un) : uf_stencil(un, u0, um);
+    const UFStencil<T, EXACT> s =
+        (D > 0) ? uf_stencil<T, EXACT>(um, u0, un) : uf_stencil<T, EXACT>(un, u0, um);
     T Dxx, Dyx, Dp, om[9];
     const T Qxx = qb.xx, Qyx = qb.yx, p = qb.p;
     update_fields<T, EXACT>(c0, Qxx, Qyx, p, fc, s, P, Dxx, Dyx, Dp, om);

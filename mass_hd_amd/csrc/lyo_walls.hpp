@@ -374,9 +374,10 @@ wall_sweep_kernel(WallSweep a, Params<double> P, WallGeo g) {
       NbrVals<T> nv;
       DiagVals<T> dg;
       update_quantities<T, true, true>(
-          fl, sq[0][q], sq[0][q + PW], sq[0][q - PW], sq[0][q + 1], sq[0][q - 1], sq[1][q],
-          sq[1][q + PW], sq[1][q - PW], sq[1][q + 1], sq[1][q - 1], sq[2][q], sq[2][q + PW],
-          sq[2][q - PW], sq[2][q + 1], sq[2][q - 1], P, nv, cv, &dg);
+          fl, sq[0][q], sq[0][q + PW], sq[0][q - PW], ynb<T, true>(sq[0][q + 1], sq[0][q - 1]),
+          sq[1][q], sq[1][q + PW], sq[1][q - PW], ynb<T, true>(sq[1][q + 1], sq[1][q - 1]),
+          sq[2][q], sq[2][q + PW], sq[2][q - PW], ynb<T, true>(sq[2][q + 1], sq[2][q - 1]), P, nv,
+          cv, &dg);
       vals[0] = nv.ux;
       vals[1] = nv.uy;
       vals[2] = nv.uxp;
@@ -432,7 +433,7 @@ wall_sweep_kernel(WallSweep a, Params<double> P, WallGeo g) {
   const int k = node(xv, yv);
   const int p = i;
   cv.dn = T(0);
-  UFStencil<T> st;
+  UFStencil<T, true> st;
   st.mu_c = sd[4][p];
   st.ux_xp = sd[0][p + DW];
   st.ux_xm = sd[0][p - DW];
@@ -446,18 +447,12 @@ wall_sweep_kernel(WallSweep a, Params<double> P, WallGeo g) {
   st.syx_xm = sd[8][p - DW];
   st.mu_xp = sd[4][p + DW];
   st.mu_xm = sd[4][p - DW];
-  st.ux_yp = sd[0][p + 1];
-  st.ux_ym = sd[0][p - 1];
-  st.uy_yp = sd[1][p + 1];
-  st.uy_ym = sd[1][p - 1];
-  st.uyp_yp = sd[3][p + 1];
-  st.uyp_ym = sd[3][p - 1];
-  st.sxy_yp = sd[7][p + 1];
-  st.sxy_ym = sd[7][p - 1];
-  st.syy_yp = sd[6][p + 1];
-  st.syy_ym = sd[6][p - 1];
-  st.mu_yp = sd[4][p + 1];
-  st.mu_ym = sd[4][p - 1];
+  st.ux_y = ynb<T, true>(sd[0][p + 1], sd[0][p - 1]);
+  st.uy_y = ynb<T, true>(sd[1][p + 1], sd[1][p - 1]);
+  st.uyp_y = ynb<T, true>(sd[3][p + 1], sd[3][p - 1]);
+  st.sxy_y = ynb<T, true>(sd[7][p + 1], sd[7][p - 1]);
+  st.syy_y = ynb<T, true>(sd[6][p + 1], sd[6][p - 1]);
+  st.mu_y = ynb<T, true>(sd[4][p + 1], sd[4][p - 1]);
   const int q = (r + 1) * PW + (c + 1);
   const T Qxx = sq[0][q], Qyx = sq[1][q], ph = sq[2][q];
   T Dxx, Dyx, Dp, om[9];

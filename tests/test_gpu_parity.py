@@ -96,7 +96,7 @@ def test_launch_shape_does_not_change_results(require_gpu, chunk, wpb):
 @pytest.mark.parametrize("precision", [lyo.FP64, lyo.FP32])
 @pytest.mark.parametrize("npc", [2, 3])
 @pytest.mark.parametrize("LX,LY,xc2", [(128, 128, 32), (45, 67, 7), (5, 3, 1), (2, 60, 4),
-                                       (70, 113, 64)])
+                                       (70, 113, 64), (40, 109, 16), (24, 54, 5)])
 def test_fused_pair_equals_separate_sweeps(require_gpu, precision, npc, LX, LY, xc2):
     """The last two correctors in one kernel give the separate sweeps' results
     bit for bit, for the state and for the frame's derived fields (which the
@@ -157,7 +157,7 @@ def test_loop_kernels_are_fp64_only(require_gpu):
 
 @pytest.mark.parametrize("precision", [lyo.FP64, lyo.FP32])
 @pytest.mark.parametrize("LX,LY,xc", [(128, 128, 32), (45, 67, 7), (5, 3, 1), (8, 60, 3),
-                                      (70, 113, 64)])
+                                      (70, 113, 64), (40, 117, 16), (24, 58, 5)])
 def test_one_kernel_step_npc1_equals_sweeps(require_gpu, precision, LX, LY, xc):
     """npc = 1: predictor and corrector in one kernel give the two sweeps'
     state and frame fields bit for bit (LX < 8 runs the sweeps)."""
