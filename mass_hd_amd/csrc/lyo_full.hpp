@@ -371,14 +371,16 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     {
       const UFStencil<T, EXACT> s = ufs(pum, pu0, pun, s0, kXPU);
       T Dxx, Dyx, Dp, om[9];
-      update_fields<T, EXACT>(pc0, pqb.xx, pqb.yx, pqb.p, pfc, s, P, Dxx, Dyx, Dp, om);
+      // (FP32: om receives the post-collision populations f + Omega)
+      update_fields<T, EXACT, !EXACT>(pc0, pqb.xx, pqb.yx, pqb.p, pfc, s, P, Dxx, Dyx, Dp, om);
       const T Qsx = pqb.xx + Dxx, Qsy = pqb.yx + Dyx;
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
-        pc[v] = pfc[v] + om[v];
         if constexpr (EXACT) {
+          pc[v] = pfc[v] + om[v];
           hn[v] = pfc[v] + T(.5) * om[v];
         } else {
+          pc[v] = om[v];
           hn[v] = half_of(pc[v], pfc[v]);
         }
       }

@@ -21,10 +21,12 @@ template <class T>
 struct Params {
   T GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
   T AA, CC, LL, KK, zeta, zetaI, conc;
-  // FP32 stress coefficients for raw differences (a(+1) - a(-1), twice the
-  // central derivative): KK/8, KK/4, LL/4, LL/2 (exact power-of-two scalings,
-  // set by finish_params; the FP64 path does not read them)
-  T KK8, KK4, LL4, LL2;
+  // Per-run constants of the FP32 arithmetic, set by finish_params (the FP64
+  // path does not read them): stress coefficients for raw differences
+  // (a(+1) - a(-1), twice the central derivative) KK/8, KK/4, LL/4, LL/2
+  // (exact power-of-two scalings); zetaI*conc; tauNem - tauIso; friction/3,
+  // friction/12
+  T KK8, KK4, LL4, LL2, zIc, dtau, fr3, fr12;
 };
 
 template <class T>
@@ -33,6 +35,10 @@ inline void finish_params(Params<T>& P) {
   P.KK4 = P.KK * T(.25);
   P.LL4 = P.LL * T(.25);
   P.LL2 = P.LL * T(.5);
+  P.zIc = P.zetaI * P.conc;
+  P.dtau = P.tauNem - P.tauIso;
+  P.fr3 = P.friction * T(1. / 3.);
+  P.fr12 = P.friction * T(1. / 12.);
 }
 
 // Rest-equilibrium weights fe(u=0, n=1) of src/lb.hpp:24-41: these define the
@@ -144,8 +150,9 @@ __device__ __forceinline__ void update_quantities(
     dn = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
     nn = P.rho + dn;
     const T inv = fast_rcp(nn);
-    vx = ((f[1] - f[2]) + (f[5] - f[6]) + (f[8] - f[7])) * inv;
-    vy = ((f[3] - f[4]) + (f[5] - f[6]) + (f[7] - f[8])) * inv;
+    const T d56 = f[5] - f[6], d87 = f[8] - f[7];
+    vx = ((f[1] - f[2]) + d56 + d87) * inv;
+    vy = ((f[3] - f[4]) + d56 - d87) * inv;
   }
 
   const T del2Qxx = lap5<T, EXACT>(qxx_c, qxx_xp, qxx_xm, qxx_y);
@@ -154,14 +161,18 @@ __device__ __forceinline__ void update_quantities(
   const T term = T(0) - Qxx * Qxx - Qyx * Qyx;
   const T Hxx = P.CC * term * Qxx + P.LL * del2Qxx;
   const T Hyx = P.CC * term * Qyx + P.LL * del2Qyx;
-  T mu;
+  T mu, sigmaB;
   if constexpr (EXACT) {
     mu = P.AA * p * (T(1) - p) * (T(1) - T(2) * p) + T(0) * P.CC * term - P.KK * del2p;
+    sigmaB = T(.5) * P.AA * p * p * (T(1) - p) * (T(1) - p) + T(.5) * P.CC * term * term -
+             mu * p;
   } else {
-    mu = P.AA * p * (T(1) - p) * (T(1) - T(2) * p) - P.KK * del2p;
+    // with g = p (1 - p): mu = AA g (1 - 2p) - KK lap(phi),
+    // sigma_bulk = (AA g^2 + CC term^2)/2 - mu p
+    const T AAg = P.AA * (p * (T(1) - p));
+    mu = AAg * (T(1) - T(2) * p) - P.KK * del2p;
+    sigmaB = T(.5) * (AAg * (p * (T(1) - p)) + (P.CC * term) * term) - mu * p;
   }
-  const T sigmaB = T(.5) * P.AA * p * p * (T(1) - p) * (T(1) - p) +
-                   T(.5) * P.CC * term * term - mu * p;
 
   // derivatives and the gradient (Ericksen) stresses; FP32 works on raw
   // differences r = 2 d with the coefficients scaled by 1/4 (KK/2 -> KK/8,
@@ -209,8 +220,17 @@ __device__ __forceinline__ void update_quantities(
   o.uxp = vx * p;
   o.uyp = vy * p;
   o.mu = mu;
-  o.sxx = sigmaF + sigmaB + P.zetaI * (P.conc - p);
-  o.syy = -sigmaF + sigmaB + P.zetaI * (P.conc - p);
+  // isotropic part: sigma_bulk + zetaI (conc - phi)
+  T iso;
+  if constexpr (EXACT) {
+    o.sxx = sigmaF + sigmaB + P.zetaI * (P.conc - p);
+    o.syy = -sigmaF + sigmaB + P.zetaI * (P.conc - p);
+    iso = sigmaB + P.zetaI * (P.conc - p);
+  } else {
+    iso = sigmaB + (P.zIc - P.zetaI * p);
+    o.sxx = iso + sigmaF;
+    o.syy = iso - sigmaF;
+  }
   o.sxy = sigmaS + sigmaA;
   o.syx = sigmaS - sigmaA;
 
@@ -226,7 +246,7 @@ __device__ __forceinline__ void update_quantities(
   c.dyQyx = dyQyx;
 
   if constexpr (DIAG) {
-    dg->bulk = sigmaB + P.zetaI * (P.conc - p);
+    dg->bulk = iso;
     dg->el_xx = sigmaF - pf_xx + P.zeta * Qxx * (T(1) - p);
     dg->el_yx = sigmaS - pf_yx + P.zeta * Qxx * (T(1) - p);
     dg->pf_xx = pf_xx;
@@ -248,18 +268,22 @@ struct UFStencil {
   YNb<T, EXACT> ux_y, uy_y, uyp_y, sxy_y, syy_y, mu_y;
 };
 
-template <class T, bool EXACT>
+// POST (FP32 only): om receives the post-collision populations f + Omega
+// instead of Omega.
+template <class T, bool EXACT, bool POST = false>
 __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Qyx,
                                               T p, const T (&f)[9],
                                               const UFStencil<T, EXACT>& s,
                                               const Params<T>& P, T& Dxx,
                                               T& Dyx, T& Dp, T (&om)[9]) {
+  static_assert(!(EXACT && POST), "the FP64 path forms f + Omega itself");
   const T nn = c.nn, vx = c.vx, vy = c.vy;
   const T Hxx = c.Hxx, Hyx = c.Hyx;
   const T dxQxx = c.dxQxx, dyQxx = c.dyQxx, dxQyx = c.dxQyx, dyQyx = c.dyQyx;
 
   const T del2mu = lap5<T, EXACT>(s.mu_c, s.mu_xp, s.mu_xm, s.mu_y);
-  T Fx, Fy;
+  // EXACT: the force; FP32: twice the stress divergence (raw differences)
+  T Fx, Fy, gx, gy;
   if constexpr (EXACT) {
     const T dxux = dcen<T, EXACT>(s.ux_xp, s.ux_xm);
     const T dyux = dcen<T, EXACT>(s.ux_y.p, s.ux_y.m);
@@ -302,15 +326,12 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     Dyx = P.GammaQ * Hyx + T(.5) * (Vyx - ayx);
     const T rflux = (s.uxp_xp - s.uxp_xm) + s.uyp_y.d;
     Dp = P.GammaP * del2mu - T(.5) * rflux;
-    const T gx = (s.sxx_xp - s.sxx_xm) + s.sxy_y.d;
-    const T gy = (s.syx_xp - s.syx_xm) + s.syy_y.d;
-    Fx = T(.5) * gx - P.friction * vx;
-    Fy = T(.5) * gy - P.friction * vy;
+    gx = (s.sxx_xp - s.sxx_xm) + s.sxy_y.d;
+    gy = (s.syx_xp - s.syx_xm) + s.syy_y.d;
   }
 
-  const T tau = P.tauNem * p + P.tauIso * (T(1) - p);
-
   if constexpr (EXACT) {
+    const T tau = P.tauNem * p + P.tauIso * (T(1) - p);
     // src/lb.hpp:24-41 in the reference order
     T fe[9];
     const T v2 = vx * vx + vy * vy;
@@ -331,34 +352,48 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     for (int v = 0; v < 9; ++v)
       om[v] = (fe[v] - f[v]) / tau + wf[v] * (Fx * T(cx[v]) + Fy * T(cy[v]));
   } else {
-    // shifted equilibrium fe'_v = fe_v - rest_weight(v)*rho
-    //   = dn*rest_weight(v) + nn*(u-dependent part of fe_v/nn)
+    // Omega_v = g_v - f'_v / tau with g_v = fe'_v / tau + force_v and the
+    // shifted equilibrium fe'_v = fe_v - rest_weight(v) rho
+    //   = dn rest_weight(v) + nn E_v(u)   (src/lb.hpp:24-41 regrouped):
+    //   3 E_{1,2} = cx +- ux, 3 E_{3,4} = cy +- uy, cx = ux^2 - uy^2/2,
+    //   cy = uy^2 - ux^2/2; 12 E_{5,6} = s+ +- (ux + uy), 12 E_{7,8} =
+    //   s- -+ (ux - uy), s+- = u^2 +- 3 ux uy; E_0 = 4/9 - (2/3) u^2.
+    // The force w_v (F . c_v) with F = (stress divergence) - friction u is
+    // odd in c_v like the terms linear in u, so each opposite pair shares an
+    // even part a and an odd part l: g = a +- l.
+    const T tau = P.tauIso + P.dtau * p;
     const T it = fast_rcp(tau);
-    const T v2 = vx * vx + vy * vy;
-    const T a = T(1. / 3.) * v2;
-    const T bx = T(.5) * vx * vx, by = T(.5) * vy * vy;
-    const T sx = T(1. / 3.) * vx, sy = T(1. / 3.) * vy;
-    const T v2q = T(1. / 12.) * v2;
-    const T xy = T(.25) * vx * vy;
-    const T pp = T(1. / 12.) * (vx + vy), pm = T(1. / 12.) * (vx - vy);
-    const T dn9 = c.dn * T(1. / 9.), dn36 = c.dn * T(1. / 36.);
-    T fe[9];
-    fe[1] = dn9 + nn * (a - by + sx);
-    fe[2] = dn9 + nn * (a - by - sx);
-    fe[3] = dn9 + nn * (a - bx + sy);
-    fe[4] = dn9 + nn * (a - bx - sy);
-    fe[5] = dn36 + nn * (v2q + xy + pp);
-    fe[6] = dn36 + nn * (v2q + xy - pp);
-    fe[7] = dn36 + nn * (v2q - xy - pm);
-    fe[8] = dn36 + nn * (v2q - xy + pm);
-    // the rest direction in closed form, fe_0 = nn (4/9 - (2/3) u^2): the
-    // reference's nn - (sum of the other eight) in exact arithmetic
-    fe[0] = T(4. / 9.) * c.dn - nn * (T(2. / 3.) * v2);
-    const T F3x = T(1. / 3.) * Fx, F3y = T(1. / 3.) * Fy;
-    const T F12p = T(1. / 12.) * (Fx + Fy), F12m = T(1. / 12.) * (Fx - Fy);
-    const T force[9] = {T(0), F3x, -F3x, F3y, -F3y, F12p, -F12p, -F12m, F12m};
+    const T vx2 = vx * vx, vy2 = vy * vy, v2 = vx2 + vy2;
+    const T itnn = it * nn, itdn = it * c.dn;
+    const T n3 = itnn * T(1. / 3.), n12 = itnn * T(1. / 12.);
+    const T d9 = itdn * T(1. / 9.), d36 = itdn * T(1. / 36.);
+    const T tx = T(3) * vx;
+    const T ax = n3 * (vx2 - T(.5) * vy2) + d9;
+    const T ay = n3 * (vy2 - T(.5) * vx2) + d9;
+    const T ap = n12 * (tx * vy + v2) + d36;
+    const T am = n12 * (v2 - tx * vy) + d36;
+    const T lx = (n3 - P.fr3) * vx + T(1. / 6.) * gx;
+    const T ly = (n3 - P.fr3) * vy + T(1. / 6.) * gy;
+    const T lp = (n12 - P.fr12) * (vx + vy) + T(1. / 24.) * (gx + gy);
+    const T lm = (n12 - P.fr12) * (vx - vy) + T(1. / 24.) * (gx - gy);
+    T g[9];
+    g[0] = T(4. / 9.) * itdn - itnn * (T(2. / 3.) * v2);
+    g[1] = ax + lx;
+    g[2] = ax - lx;
+    g[3] = ay + ly;
+    g[4] = ay - ly;
+    g[5] = ap + lp;
+    g[6] = ap - lp;
+    g[7] = am - lm;
+    g[8] = am + lm;
+    if constexpr (POST) {
+      const T keep = T(1) - it;
 #pragma unroll
-    for (int v = 0; v < 9; ++v) om[v] = (fe[v] - f[v]) * it + force[v];
+      for (int v = 0; v < 9; ++v) om[v] = f[v] * keep + g[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < 9; ++v) om[v] = g[v] - f[v] * it;
+    }
   }
 }
 

@@ -368,7 +368,9 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
         (D > 0) ? uf_stencil<T, EXACT>(um, u0, un) : uf_stencil<T, EXACT>(un, u0, um);
     T Dxx, Dyx, Dp, om[9];
     const T Qxx = qb.xx, Qyx = qb.yx, p = qb.p;
-    update_fields<T, EXACT>(c0, Qxx, Qyx, p, fc, s, P, Dxx, Dyx, Dp, om);
+    // FP32 predictor: om receives the post-collision populations f + Omega
+    constexpr bool kPost = !EXACT && STAGE == PRED;
+    update_fields<T, EXACT, kPost>(c0, Qxx, Qyx, p, fc, s, P, Dxx, Dyx, Dp, om);
 
     if (active) {
       if constexpr (STAGE == PRED) {
@@ -394,7 +396,7 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
           const int d = v * N + rowv[v];
-          const T fs = fc[v] + om[v];
+          const T fs = kPost ? om[v] : fc[v] + om[v];
           if constexpr (EXACT) {
             colv[v]->st(A.fn_out + d, fc[v] + T(.5) * om[v]);
           } else if constexpr (kStoreHalf) {
