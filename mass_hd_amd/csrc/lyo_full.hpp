@@ -84,7 +84,8 @@ struct POut {
   T qnx, qny, dp, raw;
 };
 
-// Old phi and its compensation of B's output row (FP32), one iteration ahead.
+// Old phi and its compensation of the output row (FP32): phic is loaded one
+// iteration ahead; old phi is P's input of that row, kept in registers.
 template <class T>
 struct OldPhi {
   T pold, phic;
@@ -208,11 +209,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     for (int v = 0; v < 9; ++v) f[v] = ly.ld(A.f_in + (v * N + o));
   };
   auto load_old = [&](int r, OldPhi<T>& R) {
-    if constexpr (!EXACT) {
-      const int o = roff(r);
-      R.pold = ly.ld(A.phi_in + o);
-      R.phic = ly.ld(A.phic_in + o);
-    }
+    if constexpr (!EXACT) R.phic = ly.ld(A.phic_in + roff(r));
   };
   // The last corrector's outputs of row r (Lyotropic::UpdateFieldsAtNode with
   // first = false, lyotropic.cpp:331-340) from its half-step values hv (fn 9,
@@ -323,6 +320,8 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
   // x-1 (pcP, hnP) and x-2 (pcPP, hnPP); P outputs of rows x-1, x-2
   T pcP[9] = {}, hnP[9] = {}, pcPP[9] = {}, hnPP[9] = {};
   POut<T> po1{}, po2{};
+  // FP32, NPC = 2: old phi (P's input) of rows x-3, x-4
+  [[maybe_unused]] T raw3 = T(0), raw4 = T(0);
   // ---- stage A: stencil rows x-2 (aqb), x-1 (aqc) of (Q*, phi*);
   // UpdateQuantities of rows x-3 (aum), x-2 (au0); streamed populations and
   // half-step values of row x-2 (afc, ahn)
@@ -494,7 +493,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         } else {
           // the last corrector: the new state of row x-2
           wait_vmem();
-          if (active) store_state(x - 2, cen, Dxx, Dyx, Dp, om, bold);
+          if (active) store_state(x - 2, cen, Dxx, Dyx, Dp, om, {po2.raw, bold.phic});
         }
       }
       aum = au0;
@@ -508,6 +507,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
       afc[v] = fst[v];
       ahn[v] = hst[v];
     }
+    [[maybe_unused]] const T raw2 = po2.raw;
     po2 = po1;
     po1 = po;
 
@@ -534,7 +534,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
           T cmv[12];  // half-step values of row x-4
           ring_get(1, s2, cmv);
           wait_vmem();
-          if (active) store_state(x - 4, cmv, Dxx, Dyx, Dp, om, bold);
+          if (active) store_state(x - 4, cmv, Dxx, Dyx, Dp, om, {raw4, bold.phic});
         }
         bum = bu0;
         bu0 = bun;
@@ -543,6 +543,10 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
     }
     // old phi of the next iteration's output row
     load_old(x + 1 - 2 * NPC, bold);
+    if constexpr (NPC == 2 && !EXACT) {
+      raw4 = raw3;
+      raw3 = raw2;
+    }
   };
   constexpr int kWarm = 4 * NPC;  // nI >= kWarm + 1: a chunk has a row
   int i = 0;

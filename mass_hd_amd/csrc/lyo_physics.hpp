@@ -213,7 +213,8 @@ __device__ __forceinline__ void update_quantities(
              P.LL2 * (dxQxx * dyQxx + dxQyx * dyQyx);
   }
   const T sigmaF = grad_f, sigmaS = grad_s;
-  const T sigmaA = T(2) * (Qxx * Hyx - Qyx * Hxx);
+  // antisymmetric stress sigmaA = 2 (Qxx Hyx - Qyx Hxx)
+  const T rotA = Qxx * Hyx - Qyx * Hxx;
 
   o.ux = vx;
   o.uy = vy;
@@ -231,8 +232,14 @@ __device__ __forceinline__ void update_quantities(
     o.sxx = iso + sigmaF;
     o.syy = iso - sigmaF;
   }
-  o.sxy = sigmaS + sigmaA;
-  o.syx = sigmaS - sigmaA;
+  if constexpr (EXACT) {
+    const T sigmaA = T(2) * rotA;
+    o.sxy = sigmaS + sigmaA;
+    o.syx = sigmaS - sigmaA;
+  } else {
+    o.sxy = sigmaS + T(2) * rotA;
+    o.syx = sigmaS - T(2) * rotA;
+  }
 
   c.nn = nn;
   c.dn = dn;
