@@ -24,10 +24,13 @@ per-sweep figure of the step's data flow (36 values per node per sweep,
 3 sweeps at npc = 2: 432 B per node), which counts the predictor's
 half-step state that the fused kernel keeps in registers and LDS.
 "traffic" is the HBM bytes per launch that the rocprofv3 counters measured
-(profiles/traffic_fp32_4096_npc2.json).  The kernel is bound by VALU issue,
-not HBM: "valu" gives its counted VALU instructions (SQ_INSTS_VALU) at
-4 cycles per wave64 instruction on 256 CUs x 4 SIMDs at the 2.4 GHz peak
-engine clock, against the measured launch time.
+(profiles/traffic_fp32_4096_npc2.json).
+"device_copy_GBps" is the rate of plain 1 GiB device-to-device copies measured
+in the same run, the practical streaming ceiling for read + write traffic.
+"valu" gives the kernel's counted VALU instructions (SQ_INSTS_VALU) at 4
+cycles per wave64 instruction on 256 CUs x 4 SIMDs at the 2.4 GHz peak engine
+clock, against the measured launch time: the one-kernel step runs close to
+both its VALU issue time and the copy rate.
 
 Leg order: the FP32 headline run is configured first; the FP64 leg then runs
 on its own handle (its warm-up long enough for the engine clock to settle
@@ -257,10 +260,11 @@ def cpu_time(kind, L, steps, warmup, threads, npc=2, bc=0, relax_steps=0):
     return json.loads(r.stdout.strip().splitlines()[-1])["mlups"]
 
 
-def calibrate(gib=1, reps=10):
+def calibrate(gib=1, reps=10, quiet=False):
     """Plain device-to-device copies of a known size through the HIP runtime:
     the practical HBM ceiling, and the reference workload for scaling the
-    FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc."""
+    FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc.  Returns the copy
+    rate in GB/s (bytes read + written)."""
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     n = gib << 30
@@ -284,9 +288,11 @@ def calibrate(gib=1, reps=10):
     gbps = 2.0 * n * reps / (ms.value / 1e3) / 1e9
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"calibration": "device-to-device copy", "bytes_read_per_copy": n,
-                      "bytes_written_per_copy": n, "copies": reps + 1,
-                      "copy_GBps_read_plus_write": round(gbps, 1)}), flush=True)
+    if not quiet:
+        print(json.dumps({"calibration": "device-to-device copy", "bytes_read_per_copy": n,
+                          "bytes_written_per_copy": n, "copies": reps + 1,
+                          "copy_GBps_read_plus_write": round(gbps, 1)}), flush=True)
+    return gbps
 
 
 def main():
@@ -443,13 +449,22 @@ def main():
                             "half-step state that the fused kernel keeps on chip"},
                 "launch_ms": {k: round(v, 4) for k, v in k32.items() if v}}
     assert roofline["frac"] <= 1.0, roofline
+    # the practical streaming rate of this GPU, measured now: 1 GiB
+    # device-to-device copies (read + write), against the launch's counted
+    # traffic rate
+    copy = calibrate(quiet=True)
+    roofline["device_copy_GBps"] = round(copy, 1)
+    if traffic:
+        roofline["traffic_over_device_copy"] = round(roofline["traffic_GBps"] /
+                                                     roofline["device_copy_GBps"], 3)
     if valu_instr:
         issue_ms = valu_instr / VALU_PEAK_INSTR_PER_S * 1e3
         roofline["valu"] = {"instructions_per_launch": valu_instr,
                             "issue_ms_at_peak_clock": round(issue_ms, 4),
                             "frac": round(issue_ms / kms, 3),
                             "note": "SQ_INSTS_VALU x 4 cycles / (256 CUs x 4 SIMDs x 2.4 GHz) "
-                                    "over the measured launch time: the bound of this kernel"}
+                                    "over the measured launch time; the step is near both "
+                                    "this and the device copy rate"}
 
     if not args.skip_surrogate:
         log("surrogate %dx%d: training on 512^2 frames, %d timed steps" % (L, L, args.surrogate_steps))
