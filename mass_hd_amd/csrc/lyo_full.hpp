@@ -69,14 +69,6 @@ enum XId {
 };
 static_assert(kXEnd <= kXQ, "exchange quantities fit a slot");
 
-// s_waitcnt vmcnt(0) (gfx9 encoding; expcnt and lgkmcnt at their maxima): wait
-// for every vector memory access in flight.  Issued before a row's stores: the
-// next row's loads (issued after stage P) have arrived by then, so the waits
-// the compiler places after the divergent store block -- where it must assume
-// the stores may still be in flight ahead of those loads -- find nothing left
-// to wait for, and the next row starts without waiting for this row's stores.
-__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
 // Per-row outputs of P that A needs two rows later: half-step Q and the phi
 // terms (FP32: Dp and old phi; FP64: phi + Dp/2).
 template <class T>

@@ -69,6 +69,15 @@ __device__ __forceinline__ double from_yp(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// s_waitcnt vmcnt(0) (gfx9 encoding; expcnt and lgkmcnt at their maxima): wait
+// for every vector memory access in flight.  Issued before a row's stores: the
+// next row's loads (issued earlier in the row) have arrived by then, so the
+// waits the compiler places after the divergent store block -- where it must
+// assume the stores may still be in flight ahead of those loads -- find nothing
+// left to wait for, and the next row starts without waiting for this row's
+// stores.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 struct Geo {
   int LX, LY, N;
   int NP;  // plane stride (>= N) of every SoA field
@@ -372,6 +381,7 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
     constexpr bool kPost = !EXACT && STAGE == PRED;
     update_fields<T, EXACT, kPost>(c0, Qxx, Qyx, p, fc, s, P, Dxx, Dyx, Dp, om);
 
+    wait_vmem();
     if (active) {
       if constexpr (STAGE == PRED) {
         const T Qsx = Qxx + Dxx, Qsy = Qyx + Dyx;
