@@ -484,7 +484,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
           pub_q(s0, kXQA, qa_out);  // B's stencil row one iteration later
         } else {
           // the last corrector: the new state of row x-2
-          wait_vmem();
+          wait_vmem<EXACT>();
           if (active) store_state(x - 2, cen, Dxx, Dyx, Dp, om, {po2.raw, bold.phic});
         }
       }
@@ -525,7 +525,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
           update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
           T cmv[12];  // half-step values of row x-4
           ring_get(1, s2, cmv);
-          wait_vmem();
+          wait_vmem<EXACT>();
           if (active) store_state(x - 4, cmv, Dxx, Dyx, Dp, om, {raw4, bold.phic});
         }
         bum = bu0;
@@ -543,7 +543,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
   constexpr int kWarm = 4 * NPC;  // nI >= kWarm + 1: a chunk has a row
   int i = 0;
   for (; i < kWarm; ++i) row(i, xs + i, i % 3, std::true_type{});
-  wait_vmem();  // the loop's entry then has no loads in flight either
+  wait_vmem<EXACT>();  // the loop's entry then has no loads in flight either
   for (; i + 3 <= nI; i += 3) {
     row(i, xs + i, kWarm % 3, std::false_type{});
     row(i + 1, xs + i + 1, (kWarm + 1) % 3, std::false_type{});

@@ -252,7 +252,7 @@ __device__ __forceinline__ void pair_item(const SweepArgs<T>& A, const Params<T>
           const UFStencil<T, EXACT> s = ufs(bum, bu0, bun);
           T Dxx, Dyx, Dp, om[9];
           update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
-          wait_vmem();
+          wait_vmem<EXACT>();
           if (active) {
             const int ox = roff(x - 2 * D);
             ly.st(A.qxx_out + ox, cm2.qnxx + T(.5) * Dxx);

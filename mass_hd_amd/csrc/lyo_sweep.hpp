@@ -75,8 +75,13 @@ __device__ __forceinline__ double from_yp(double v) {
 // waits the compiler places after the divergent store block -- where it must
 // assume the stores may still be in flight ahead of those loads -- find nothing
 // left to wait for, and the next row starts without waiting for this row's
-// stores.
-__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// stores.  FP32 only (two waves per SIMD): the FP64 kernels run one wave per
+// SIMD, whose store block holds much of the row's remaining arithmetic, and an
+// early wait there stalls the SIMD (measured 10% slower at npc = 1).
+template <bool EXACT>
+__device__ __forceinline__ void wait_vmem() {
+  if constexpr (!EXACT) __builtin_amdgcn_s_waitcnt(0x0F70);
+}
 
 struct Geo {
   int LX, LY, N;
@@ -381,7 +386,7 @@ __device__ __forceinline__ void sweep_item(const SweepArgs<T>& A, const Params<T
     constexpr bool kPost = !EXACT && STAGE == PRED;
     update_fields<T, EXACT, kPost>(c0, Qxx, Qyx, p, fc, s, P, Dxx, Dyx, Dp, om);
 
-    wait_vmem();
+    wait_vmem<EXACT>();
     if (active) {
       if constexpr (STAGE == PRED) {
         const T Qsx = Qxx + Dxx, Qsy = Qyx + Dyx;
