@@ -14,9 +14,11 @@
 // so neither the predictor's outputs nor the first corrector's ever reach HBM:
 // per node the kernel reads the old state and writes the new one.  Every stage
 // calls the same device functions in the same order as the separate sweeps
-// (lyo_sweep.hpp), with the half-step values formed as there (FP32: half_of;
-// FP64: the reference's f + om/2, Q + D/2, phi + Dp/2), so the results are
-// bitwise those of the predictor sweep + fused pair.  Each stage needs its
+// (lyo_sweep.hpp); FP64 forms the half-step values as there (the reference's
+// f + om/2, Q + D/2, phi + Dp/2), FP32 keeps twice them (f* + f, Q* + Q: one
+// addition where half_of takes two operations) and has the correctors return
+// halved increments, the same values since halving is exact; so the results
+// are bitwise those of the predictor sweep + fused pair.  Each stage needs its
 // inputs one lane further in, so the halo is six lanes per side (52 interior
 // columns), and a chunk repeats eight rows.
 //
@@ -207,13 +209,24 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
   // first = false, lyotropic.cpp:331-340) from its half-step values hv (fn 9,
   // QN 2, phi term) and increments; FP32 phi: phi_new = phi_old + phic +
   // (Dp_pred + Dp)/2 with a TwoSum, as the separate last corrector.
+  // A corrector's output from its half-step value and increment.  FP64: the
+  // reference's hv + inc/2.  FP32: hv holds twice the half-step value and the
+  // corrector's increments come halved (update_fields HALF): s/2 + inc/2,
+  // bitwise the separate sweeps' half_of(...) + inc/2, since halving is exact.
+  auto hstep = [](T hv, T inc) -> T {
+    if constexpr (EXACT) {
+      return hv + T(.5) * inc;
+    } else {
+      return T(.5) * hv + inc;
+    }
+  };
   auto store_state = [&](int r, const T(&hv)[12], T Dxx, T Dyx, T Dp, const T(&om)[9],
                          const OldPhi<T>& old) {
     const int ox = roff(r);
-    ly.st(A.qxx_out + ox, hv[9] + T(.5) * Dxx);
-    ly.st(A.qyx_out + ox, hv[10] + T(.5) * Dyx);
+    ly.st(A.qxx_out + ox, hstep(hv[9], Dxx));
+    ly.st(A.qyx_out + ox, hstep(hv[10], Dyx));
 #pragma unroll
-    for (int v = 0; v < 9; ++v) ly.st(A.f_out + (v * N + ox), hv[v] + T(.5) * om[v]);
+    for (int v = 0; v < 9; ++v) ly.st(A.f_out + (v * N + ox), hstep(hv[v], om[v]));
     if constexpr (EXACT) {
       ly.st(A.phi_out + ox, hv[11] + T(.5) * Dp);
     } else {
@@ -372,7 +385,7 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
           hn[v] = pfc[v] + T(.5) * om[v];
         } else {
           pc[v] = om[v];
-          hn[v] = half_of(pc[v], pfc[v]);
+          hn[v] = pc[v] + pfc[v];  // twice the half-step value
         }
       }
       aq_new.xx = Qsx;
@@ -386,8 +399,8 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         po.dp = pqb.p + T(.5) * Dp;
         po.raw = T(0);
       } else {
-        po.qnx = half_of(Qsx, pqb.xx);
-        po.qny = half_of(Qsy, pqb.yx);
+        po.qnx = Qsx + pqb.xx;
+        po.qny = Qsy + pqb.yx;
         po.dp = Dp;
         po.raw = pqb.p;
       }
@@ -462,13 +475,14 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         if constexpr (NPC == 2) ring_put(1, s0, cen);
         const UFStencil<T, EXACT> s = ufs(aum, au0, aun, s0, kXAU);
         T Dxx, Dyx, Dp, om[9];
-        update_fields<T, EXACT>(ac0, aqb.xx, aqb.yx, aqb.p, afc, s, P, Dxx, Dyx, Dp, om);
+        update_fields<T, EXACT, false, !EXACT>(ac0, aqb.xx, aqb.yx, aqb.p, afc, s, P, Dxx, Dyx,
+                                               Dp, om);
         if constexpr (NPC == 2) {
           T ao[12];
-          qa_out.xx = cen[9] + T(.5) * Dxx;
-          qa_out.yx = cen[10] + T(.5) * Dyx;
+          qa_out.xx = hstep(cen[9], Dxx);
+          qa_out.yx = hstep(cen[10], Dyx);
 #pragma unroll
-          for (int v = 0; v < 9; ++v) ao[v] = cen[v] + T(.5) * om[v];
+          for (int v = 0; v < 9; ++v) ao[v] = hstep(cen[v], om[v]);
           if constexpr (EXACT) {
             qa_out.p = cen[11] + T(.5) * Dp;
           } else {
@@ -522,7 +536,8 @@ __device__ __forceinline__ void full_item(const SweepArgs<T>& A, const Params<T>
         if (!WARM || i >= 8) {
           const UFStencil<T, EXACT> s = ufs(bum, bu0, bun, s0, kXBU);
           T Dxx, Dyx, Dp, om[9];
-          update_fields<T, EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx, Dp, om);
+          update_fields<T, EXACT, false, !EXACT>(bc0, bqb.xx, bqb.yx, bqb.p, bfc, s, P, Dxx, Dyx,
+                                                 Dp, om);
           T cmv[12];  // half-step values of row x-4
           ring_get(1, s2, cmv);
           wait_vmem<EXACT>();

@@ -25,8 +25,8 @@ struct Params {
   // path does not read them): stress coefficients for raw differences
   // (a(+1) - a(-1), twice the central derivative) KK/8, KK/4, LL/4, LL/2
   // (exact power-of-two scalings); zetaI*conc; tauNem - tauIso; friction/3,
-  // friction/12
-  T KK8, KK4, LL4, LL2, zIc, dtau, fr3, fr12;
+  // friction/12; GammaQ/2, friction/6, friction/24 (the halved increments)
+  T KK8, KK4, LL4, LL2, zIc, dtau, fr3, fr12, GammaQh, fr3h, fr12h;
 };
 
 template <class T>
@@ -39,6 +39,9 @@ inline void finish_params(Params<T>& P) {
   P.dtau = P.tauNem - P.tauIso;
   P.fr3 = P.friction * T(1. / 3.);
   P.fr12 = P.friction * T(1. / 12.);
+  P.GammaQh = P.GammaQ * T(.5);
+  P.fr3h = P.fr3 * T(.5);
+  P.fr12h = P.fr12 * T(.5);
 }
 
 // Rest-equilibrium weights fe(u=0, n=1) of src/lb.hpp:24-41: these define the
@@ -276,14 +279,17 @@ struct UFStencil {
 };
 
 // POST (FP32 only): om receives the post-collision populations f + Omega
-// instead of Omega.
-template <class T, bool EXACT, bool POST = false>
+// instead of Omega.  HALF (FP32 only): om, Dxx and Dyx receive Omega/2, D/2
+// (every constant of their expressions halved, so the results are exactly
+// half of the full ones); Dp stays whole.
+template <class T, bool EXACT, bool POST = false, bool HALF = false>
 __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Qyx,
                                               T p, const T (&f)[9],
                                               const UFStencil<T, EXACT>& s,
                                               const Params<T>& P, T& Dxx,
                                               T& Dyx, T& Dp, T (&om)[9]) {
-  static_assert(!(EXACT && POST), "the FP64 path forms f + Omega itself");
+  static_assert(!(EXACT && (POST || HALF)), "the FP64 path keeps the reference's forms");
+  static_assert(!(POST && HALF), "one output form");
   const T nn = c.nn, vx = c.vx, vy = c.vy;
   const T Hxx = c.Hxx, Hyx = c.Hyx;
   const T dxQxx = c.dxQxx, dyQxx = c.dyQxx, dxQyx = c.dxQyx, dyQyx = c.dyQyx;
@@ -329,8 +335,13 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     const T Vyx = P.xi * (Qyx * (E2 - TQ2) + S4) + W4 * Qxx;
     const T axx = vx * dxQxx + vy * dyQxx;
     const T ayx = vx * dxQyx + vy * dyQyx;
-    Dxx = P.GammaQ * Hxx + T(.5) * (Vxx - axx);
-    Dyx = P.GammaQ * Hyx + T(.5) * (Vyx - ayx);
+    if constexpr (HALF) {
+      Dxx = P.GammaQh * Hxx + T(.25) * (Vxx - axx);
+      Dyx = P.GammaQh * Hyx + T(.25) * (Vyx - ayx);
+    } else {
+      Dxx = P.GammaQ * Hxx + T(.5) * (Vxx - axx);
+      Dyx = P.GammaQ * Hyx + T(.5) * (Vyx - ayx);
+    }
     const T rflux = (s.uxp_xp - s.uxp_xm) + s.uyp_y.d;
     Dp = P.GammaP * del2mu - T(.5) * rflux;
     gx = (s.sxx_xp - s.sxx_xm) + s.sxy_y.d;
@@ -370,8 +381,9 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     // even part a and an odd part l: g = a +- l.
     const T tau = P.tauIso + P.dtau * p;
     const T it = fast_rcp(tau);
+    const T itx = HALF ? T(.5) * it : it;  // (HALF: every term below halved)
     const T vx2 = vx * vx, vy2 = vy * vy, v2 = vx2 + vy2;
-    const T itnn = it * nn, itdn = it * c.dn;
+    const T itnn = itx * nn, itdn = itx * c.dn;
     const T n3 = itnn * T(1. / 3.), n12 = itnn * T(1. / 12.);
     const T d9 = itdn * T(1. / 9.), d36 = itdn * T(1. / 36.);
     const T tx = T(3) * vx;
@@ -379,10 +391,12 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     const T ay = n3 * (vy2 - T(.5) * vx2) + d9;
     const T ap = n12 * (tx * vy + v2) + d36;
     const T am = n12 * (v2 - tx * vy) + d36;
-    const T lx = (n3 - P.fr3) * vx + T(1. / 6.) * gx;
-    const T ly = (n3 - P.fr3) * vy + T(1. / 6.) * gy;
-    const T lp = (n12 - P.fr12) * (vx + vy) + T(1. / 24.) * (gx + gy);
-    const T lm = (n12 - P.fr12) * (vx - vy) + T(1. / 24.) * (gx - gy);
+    const T fr3 = HALF ? P.fr3h : P.fr3, fr12 = HALF ? P.fr12h : P.fr12;
+    const T g6 = HALF ? T(1. / 12.) : T(1. / 6.), g24 = HALF ? T(1. / 48.) : T(1. / 24.);
+    const T lx = (n3 - fr3) * vx + g6 * gx;
+    const T ly = (n3 - fr3) * vy + g6 * gy;
+    const T lp = (n12 - fr12) * (vx + vy) + g24 * (gx + gy);
+    const T lm = (n12 - fr12) * (vx - vy) + g24 * (gx - gy);
     T g[9];
     g[0] = T(4. / 9.) * itdn - itnn * (T(2. / 3.) * v2);
     g[1] = ax + lx;
@@ -399,7 +413,7 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
       for (int v = 0; v < 9; ++v) om[v] = f[v] * keep + g[v];
     } else {
 #pragma unroll
-      for (int v = 0; v < 9; ++v) om[v] = g[v] - f[v] * it;
+      for (int v = 0; v < 9; ++v) om[v] = g[v] - f[v] * itx;
     }
   }
 }
