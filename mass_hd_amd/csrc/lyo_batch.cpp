@@ -59,6 +59,23 @@ void drop_graph(mass_lyo_batch_t b) {
 }
 
 // (re)builds and uploads the job table for the current launch configuration
+// A run's launch: the batch's chunk lengths evened out over the run's rows
+// (LX = 192 with 80-row chunks: three chunks of 64 rows, not 80 + 80 + 32),
+// so every work item costs about the same and no short item pays the rows a
+// chunk repeats for little work.
+static lyo::LaunchCfg run_launch(const lyo::LaunchCfg& L, int LX) {
+  lyo::LaunchCfg r = L;
+  auto even = [LX](int xc) {
+    if (xc < 1) return xc;
+    const int n = (LX + xc - 1) / xc;
+    return (LX + n - 1) / n;
+  };
+  r.xc = even(L.xc);
+  r.xc2 = even(L.xc2);
+  r.xc3 = even(L.xc3);
+  return r;
+}
+
 int build_jobs(mass_lyo_batch_t b) {
   if (b->jobs_valid) return MASS_LYO_OK;
   drop_graph(b);
@@ -76,12 +93,13 @@ int build_jobs(mass_lyo_batch_t b) {
     int items = 0;
     for (int r = 0; r < R; ++r) {
       first[r] = items;
-      items += b->runs[r]->eng->job_items(b->plan[c], b->launch);
+      items += b->runs[r]->eng->job_items(b->plan[c], run_launch(b->launch, b->runs[r]->prm.LX));
     }
     b->nitems[c] = items;
     for (int p = 0; p < 2; ++p)
       for (int r = 0; r < R; ++r)
-        b->runs[r]->eng->fill_job(b->plan[c], p, first[r], b->launch,
+        b->runs[r]->eng->fill_job(b->plan[c], p, first[r],
+                                  run_launch(b->launch, b->runs[r]->prm.LX),
                                   &b->jobs[(((size_t)p * S + c) * R + r) * b->job_bytes]);
   }
   hipError_t e = hipSuccess;
