@@ -8,7 +8,13 @@
 //                           x86-64 build does.
 //   EXACT=false, T=float  : populations stored as deviations f' = f - w_i*rho
 //                           from the rest equilibrium, reciprocals instead of
-//                           divisions, FMA contraction allowed.
+//                           divisions, FMA contraction allowed, and the same
+//                           quantities regrouped for fewer operations: raw
+//                           central differences with the one-half factors in
+//                           per-run constants, y neighbours as sum and
+//                           difference, and the collision as even and odd
+//                           parts of each opposite population pair (equal to
+//                           the reference's expressions in exact arithmetic).
 //
 // Reference: src/models/lyotropic.cpp:168-341, src/lb.hpp:15-41,73-90,127-136.
 #pragma once
