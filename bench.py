@@ -196,6 +196,15 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warm
             "net_flop_per_lu": flop,
             "ms_per_step": round(t_sur / steps, 4),
             "plain_ms_per_step": round(t_plain / steps, 4),
+            # the network's FLOPs over the time the surrogate step spends
+            # beyond the predictor + one corrector (features, the four
+            # layers, the increment, the amortised judge): a lower bound
+            # on the layers' MFMA rate; the per-layer counters are in
+            # profiles/surrogate_fp32_4096.md
+            "network_tflops_beyond_step": round(
+                flop * N / max(1e-9, (t_sur - t_one) / steps / 1e3) / 1e12, 1),
+            "network_frac_of_bf16_peak_beyond_step": round(
+                flop * N / max(1e-9, (t_sur - t_one) / steps / 1e3) / MFMA_BF16_PEAK, 4),
             "structural_bound": {
                 "npc1_step_ms": round(t_one / steps, 4),
                 "network_ms_at_bf16_peak": round(net_ms, 4),
