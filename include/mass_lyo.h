@@ -417,6 +417,13 @@ typedef struct mass_lyo_surrogate_stats {
 int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_every, double tol);
 int mass_lyo_surrogate_disable(mass_lyo_t h);
 int mass_lyo_surrogate_get_stats(mass_lyo_t h, mass_lyo_surrogate_stats* out);
+/* Kernels of the network part of a surrogate step: 1 (the default for FP32
+ * runs with LX, LY >= 16) one kernel that computes the features, runs the
+ * four layers with the activations on chip and applies the increment; 0
+ * separate feature, convolution and increment kernels (the judged steps
+ * always use these).  Both apply the same increments up to the BF16 rounding
+ * of the activations.  No reference counterpart. */
+int mass_lyo_surrogate_set_kernels(mass_lyo_t h, int32_t fused);
 
 /* Version of this ABI (MASS_LYO_ABI_VERSION). */
 int mass_lyo_abi_version(void);

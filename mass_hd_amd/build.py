@@ -37,6 +37,8 @@ UNITS = [
     # FMA only within one source expression: the same arithmetic in every
     # kernel that inlines the step, so batched and single runs agree bitwise
     ("lyo_fp32.hip", ["-ffp-contract=on"]),
+    # the one-kernel network step computes the features as lyo_fp32.hip does
+    ("lyo_net_fused.hip", ["-ffp-contract=on"]),
     ("lyo_capi.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("lyo_batch.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("lyo_net.cpp", ["-ffp-contract=fast", "-x", "hip"]),

@@ -68,6 +68,13 @@ static int chunk_rows_for(int LX, int LY, int strip, int repeat, int slots) {
   return best;
 }
 
+// the network part of surrogate steps can run as one kernel (FP32 state,
+// LX and LY >= 16)
+static bool fused_net_ok(const mass_lyo_t h) {
+  return h->prm.precision == MASS_LYO_FP32 && h->prm.LX >= lyo::kNetMinSide &&
+         h->prm.LY >= lyo::kNetMinSide;
+}
+
 int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
   if (!p || !out) return fail(nullptr, MASS_LYO_EINVAL, "null argument");
   *out = nullptr;
@@ -162,6 +169,10 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
   // slot per SIMD (its LDS allows two FP32 blocks per CU, one FP64 block)
   h->launch.xc4 = chunk_rows_for(p->LX, p->LY, lyo::kXStrip, 8,
                                  prop.multiProcessorCount * per_simd);
+  // the one-kernel network step of surrogate runs: two waves per SIMD
+  h->launch.xc5 = chunk_rows_for(p->LX, p->LY, lyo::kNetStrip, lyo::kNetRepeat,
+                                 prop.multiProcessorCount * 4 * 2);
+  h->sur.fused = fused_net_ok(h.get()) ? 1 : 0;
   *out = h.release();
   return MASS_LYO_OK;
 }
@@ -286,6 +297,11 @@ static int surrogate_step(mass_lyo_t h) {
     e = h->eng->steps(1, h->launch, h->stream, nullptr);
     S.plain_left--;
     S.stats.plain_steps++;
+  } else if (S.fused) {
+    e = h->eng->steps_with(1, 1, h->launch, h->stream);
+    if (e == hipSuccess)
+      e = h->eng->net_fused(lyo::net_fused_weights(S.net), h->launch.xc5, nullptr, h->stream);
+    S.stats.surrogate_steps++;
   } else {
     e = h->eng->steps_with(1, 1, h->launch, h->stream);
     __bf16* in = nullptr;
@@ -344,6 +360,15 @@ int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_ev
 int mass_lyo_surrogate_disable(mass_lyo_t h) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   h->sur.net = nullptr;
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_surrogate_set_kernels(mass_lyo_t h, int32_t fused) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (fused != 0 && fused != 1) return fail(h, MASS_LYO_EINVAL, "fused is 0 or 1");
+  if (fused && !fused_net_ok(h))
+    return fail(h, MASS_LYO_EINVAL, "the one-kernel network step needs FP32 and LX, LY >= 16");
+  h->sur.fused = fused;
   return MASS_LYO_OK;
 }
 

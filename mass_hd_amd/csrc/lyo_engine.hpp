@@ -6,9 +6,12 @@
 //   S[2]  model state (f, Q, phi [, phic]) -- ping-pong across steps
 //   P     predictor outputs: streamed f*, fn*, Q+D, Q+D/2, phi terms
 //   C[2]  intermediate corrector outputs (npc >= 2) -- ping-pong
+//   X     FP32 surrogate steps: the one-kernel network step's output, which
+//         then trades places with S[cur] (allocated on first use)
 #pragma once
 
 #include <random>
+#include <utility>
 #include <vector>
 
 #include "../../include/mass_lyo.h"
@@ -17,6 +20,7 @@
 #include "lyo_full.hpp"
 #include "lyo_loops.hpp"
 #include "lyo_net_engine.hpp"
+#include "lyo_net_fused_api.hpp"
 #include "lyo_pair.hpp"
 #include "lyo_sweep.hpp"
 
@@ -41,6 +45,11 @@ struct Bufs {
   T* Cqxx[2] = {};
   T* Cqyx[2] = {};
   T* Cphi[2] = {};  // EXACT: phi; FP32: increment of phi
+  T* Xf = nullptr;
+  T* Xqxx = nullptr;
+  T* Xqyx = nullptr;
+  T* Xphi = nullptr;
+  T* Xphic = nullptr;
   int cur = 0;
   // inputs of the last UpdateQuantities (start of the last corrector sweep)
   bool have_uq = false;
@@ -74,6 +83,7 @@ struct LaunchCfg {
   int xc3 = 32;  // rows per wave chunk of the one-kernel step
   int xc4 = 32;  // rows per block chunk of the block-cooperative one-kernel step
   int loops = 0; // FP64: one kernel per loop of the reference's step instead
+  int xc5 = 32;  // rows per wave chunk of the one-kernel network step (surrogate)
 };
 
 // Optional per-sweep event timing.
@@ -634,6 +644,12 @@ class Engine {
   virtual hipError_t net_targets(float* tgt, hipStream_t s) = 0;
   virtual hipError_t net_apply(const float* delta, hipStream_t s) = 0;
   virtual hipError_t net_judge(const float* delta, double* part, int nparts, hipStream_t s) = 0;
+  // the network part of a surrogate step in one kernel (FP32, lyo_net_fused.hpp):
+  // features, the network and the increment of the current state, xc rows
+  // per wave chunk; delta (optional): the increments applied
+  virtual hipError_t net_fused(const NetFusedW&, int, float*, hipStream_t) {
+    return hipErrorNotSupported;
+  }
   // nsteps steps with npc_run correctors (npc_run <= npc of the engine)
   virtual hipError_t steps_with(int npc_run, int nsteps, const LaunchCfg& L, hipStream_t s) = 0;
   virtual int npc() const = 0;
@@ -832,6 +848,34 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL((judge_kernel<T, !EXACT>), dim3(nparts), dim3(256), 0, s, B_.Cf[0],
                        B_.Sf[B_.cur], delta, LX_ * LY_, NP_, rho_, part);
     return hipGetLastError();
+  }
+  hipError_t net_fused(const NetFusedW& w, int xc, float* delta, hipStream_t s) override {
+    if constexpr (EXACT) {
+      return hipErrorNotSupported;
+    } else {
+      if (!B_.Xf) {
+        const size_t n1 = (size_t)NP_;
+        hipError_t e = get_(9 * n1, &B_.Xf);
+        if (e == hipSuccess) e = get_(n1, &B_.Xqxx);
+        if (e == hipSuccess) e = get_(n1, &B_.Xqyx);
+        if (e == hipSuccess) e = get_(n1, &B_.Xphi);
+        if (e == hipSuccess) e = get_(n1, &B_.Xphic);
+        if (e != hipSuccess) return e;
+      }
+      const int cur = B_.cur;
+      const StatePlanes in{B_.Sf[cur], B_.Sqxx[cur], B_.Sqyx[cur], B_.Sphi[cur], B_.Sphic[cur]};
+      const StatePlanes out{B_.Xf, B_.Xqxx, B_.Xqyx, B_.Xphi, B_.Xphic};
+      const hipError_t e = launch_net_fused(in, out, LX_, LY_, NP_, xc, P_, rho_, w, delta, s);
+      if (e != hipSuccess) return e;
+      // the corrected state becomes S[cur]; the old state S[cur ^ 1] and the
+      // frame fields' bookkeeping (lazy_cur, uq_*) do not refer to it
+      std::swap(B_.Sf[cur], B_.Xf);
+      std::swap(B_.Sqxx[cur], B_.Xqxx);
+      std::swap(B_.Sqyx[cur], B_.Xqyx);
+      std::swap(B_.Sphi[cur], B_.Xphi);
+      std::swap(B_.Sphic[cur], B_.Xphic);
+      return hipSuccess;
+    }
   }
   hipError_t steps_with(int npc_run, int nsteps, const LaunchCfg& L, hipStream_t s) override {
     if (npc_run < 1 || npc_run > npc_) return hipErrorInvalidValue;

@@ -34,6 +34,7 @@ struct mass_lyo_s {
     double tol = 1e-3;
     long long count = 0;      // steps taken in surrogate mode
     int plain_left = 0;       // plain steps still owed after a failed judgement
+    int fused = 1;            // network part in one kernel (FP32, LX and LY >= 16)
     float* delta = nullptr;   // [N][8]
     double* part = nullptr;   // judge partial sums
     mass_lyo_surrogate_stats stats{};

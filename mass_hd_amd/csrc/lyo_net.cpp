@@ -69,6 +69,47 @@ __global__ void pack_frag_kernel(const float* __restrict__ W, int cin_f, int cou
   out[e] = (__bf16)v;
 }
 
+// Weights of the one-kernel network step (lyo_net_fused.hpp), A operands of
+// v_mfma_f32_32x32x16_bf16 (lane l = 32g + i holds A[i][8g..8g+7]):
+//   fragments 0-4 (layer 1): slice s, K = 8g + e -> tap 2s + g, input e
+//   fragments 5-40 (layers 2, 3): 2 tap + h, K = 8g + e -> input channel
+//     8(2h + e/4) + 4g + e%4 (the order of the layer below's accumulators)
+//   fragments 41-46 (layer 4): 2 dxi + h, M row 8s + c -> output c of tap
+//     3 dxi + s (rows 24..31 zero), K as layers 2, 3
+// then the 104 layer biases.  W_l = params + woff[l], W_l[t][cout][cin].
+constexpr int kFusedFrags = 47;
+constexpr int kFusedBias = 104;
+__global__ void pack_fused_kernel(const float* __restrict__ params, int woff1, int woff2,
+                                  int woff3, __bf16* __restrict__ frag, float* __restrict__ bias) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nf = kFusedFrags * 64 * 8;
+  if (e < nf) {
+    const int fr = e >> 9, lane = (e >> 3) & 63, el = e & 7;
+    const int g = lane >> 5, i = lane & 31;
+    auto perm = [&](int h) { return 8 * (2 * h + (el >> 2)) + 4 * g + (el & 3); };
+    float v = 0.f;
+    if (fr < 5) {
+      const int tap = 2 * fr + g;
+      if (tap < 9) v = params[(tap * 32 + i) * 8 + el];
+    } else if (fr < 41) {
+      const int f = fr < 23 ? fr - 5 : fr - 23;
+      const float* W = params + (fr < 23 ? woff1 : woff2);
+      const int tap = f >> 1, h = f & 1;
+      v = W[(tap * 32 + i) * 32 + perm(h)];
+    } else {
+      const int f = fr - 41, dxi = f >> 1, h = f & 1;
+      if (i < 24) v = params[woff3 + ((3 * dxi + (i >> 3)) * 8 + (i & 7)) * 32 + perm(h)];
+    }
+    frag[e] = (__bf16)v;
+  } else if (e < nf + kFusedBias) {
+    const int b = e - nf;
+    const int l = b < 32 ? 0 : (b < 64 ? 1 : (b < 96 ? 2 : 3));
+    const int woff[4] = {0, woff1, woff2, woff3};
+    const int cin = l == 0 ? 8 : 32, cout = l == 3 ? 8 : 32;
+    bias[b] = params[woff[l] + 9 * cin * cout + (b - 32 * l)];
+  }
+}
+
 __global__ void normalize_kernel(const float* __restrict__ feat, const float* __restrict__ mean,
                                  const float* __restrict__ inv, int n, __bf16* __restrict__ out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,6 +161,8 @@ struct mass_lyo_net_s {
   float *d_params = nullptr, *d_grads = nullptr, *d_m = nullptr, *d_v = nullptr;
   __bf16* d_wf[4] = {};  // forward fragments per layer
   __bf16* d_wb[4] = {};  // data-gradient fragments (layers 1..3 used)
+  __bf16* d_wfused = nullptr;  // the one-kernel network step's fragments and biases
+  float* d_fbias = nullptr;
   int adam_t = 0;
   // normalisation (host + device: mean[8], inv_std[8], out_scale[8], inv_out_scale[8])
   float norm[32];
@@ -201,6 +244,10 @@ hipError_t pack_all(mass_lyo_net_t n) {
                          n->d_params + layer_woff(l), cin, cout, 1, n->d_wb[l]);
     }
   }
+  const int nfu = kFusedFrags * 64 * 8 + kFusedBias;
+  hipLaunchKernelGGL(pack_fused_kernel, dim3((nfu + 255) / 256), dim3(256), 0, n->stream,
+                     n->d_params, layer_woff(1), layer_woff(2), layer_woff(3), n->d_wfused,
+                     n->d_fbias);
   return hipGetLastError();
 }
 
@@ -356,6 +403,14 @@ hipError_t net_predict_input(mass_lyo_net_t n, int LX, int LY, float* delta, hip
 
 bool net_ready(mass_lyo_net_t n) { return n && n->have_norm; }
 
+NetFusedW net_fused_weights(mass_lyo_net_t n) {
+  NetFusedW w;
+  w.wfrag = n->d_wfused;
+  w.bias = n->d_fbias;
+  w.norm = n->d_norm;
+  return w;
+}
+
 }  // namespace lyo
 
 // ------------------------------------------------------------------ C ABI
@@ -398,6 +453,8 @@ int mass_lyo_net_create(uint32_t seed, mass_lyo_net_t* out) {
     get((size_t)frag_count(kCh[l], kCh[l + 1]) * 16, (void**)&n->d_wf[l]);
     get((size_t)frag_count(kCh[l + 1], kCh[l]) * 16, (void**)&n->d_wb[l]);
   }
+  get((size_t)kFusedFrags * 64 * 16, (void**)&n->d_wfused);
+  get(sizeof(float) * kFusedBias, (void**)&n->d_fbias);
   if (e == hipSuccess)
     e = hipMemcpy(n->d_params, n->host_params.data(), sizeof(float) * n->np, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(n->d_norm, n->norm, sizeof(n->norm), hipMemcpyHostToDevice);
@@ -419,7 +476,7 @@ void mass_lyo_net_destroy(mass_lyo_net_t n) {
   for (float* p : n->feats) (void)hipFree(p);
   for (float* p : n->tgts) (void)hipFree(p);
   for (void* p : {(void*)n->d_params, (void*)n->d_grads, (void*)n->d_m, (void*)n->d_v,
-                  (void*)n->d_norm, (void*)n->part})
+                  (void*)n->d_norm, (void*)n->part, (void*)n->d_wfused, (void*)n->d_fbias})
     if (p) (void)hipFree(p);
   for (int l = 0; l < 4; ++l) {
     if (n->d_wf[l]) (void)hipFree(n->d_wf[l]);

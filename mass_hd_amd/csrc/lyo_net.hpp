@@ -24,29 +24,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include "lyo_net_act.hpp"
+
 namespace lyo {
 namespace net {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int kTaps = 9;
-constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2/pi)
-
-// tanh(u) = 1 - 2/(exp(2u) + 1), with a hardware reciprocal; gelu(x) =
-// x/2 (1 + tanh(u)) = x (1 - r) with r = 1/(exp(2u) + 1).  exp overflowing
-// to +inf gives r = 0 (gelu = x), exp underflowing to 0 gives r = 1.
-__device__ __forceinline__ float gelu_r(float x) {
-  const float u = kGeluC * (x + 0.044715f * x * x * x);
-  return __builtin_amdgcn_rcpf(__expf(2.f * u) + 1.f);
-}
-__device__ __forceinline__ float gelu(float x) { return x - x * gelu_r(x); }
-__device__ __forceinline__ float gelu_grad(float x) {
-  const float r = gelu_r(x);
-  const float t = 1.f - 2.f * r;  // tanh(u)
-  return (1.f - r) + 0.5f * x * (1.f - t * t) * kGeluC * (1.f + 3.f * 0.044715f * x * x);
-}
 
 // tap t = 3*(dx+1) + (dy+1)
 __host__ __device__ constexpr int tap_dx(int t) { return t / 3 - 1; }

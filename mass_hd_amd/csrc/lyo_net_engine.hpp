@@ -38,6 +38,24 @@ __device__ void uq_at(const T* f, const T* qxx, const T* qyx, const T* phi, cons
                               cv);
 }
 
+// The eight features of one node from its UpdateQuantities values c, the
+// stresses of its x+1 / x-1 rows and y+1 / y-1 columns, its Q and its
+// compensated phi p: ux, uy, Qxx, Qyx, phi, mu, Fx, Fy with F = div(sigma)
+// (central differences, lyotropic.cpp:268-299).
+template <class T>
+__device__ __forceinline__ void feature_values(const NbrVals<T>& c, T sxx_xp, T sxx_xm, T syx_xp,
+                                               T syx_xm, T sxy_yp, T sxy_ym, T syy_yp, T syy_ym,
+                                               T qxx, T qyx, double p, float (&o)[kFeat]) {
+  o[0] = (float)c.ux;
+  o[1] = (float)c.uy;
+  o[2] = (float)qxx;
+  o[3] = (float)qyx;
+  o[4] = (float)p;
+  o[5] = (float)c.mu;
+  o[6] = (float)(T(.5) * (sxx_xp - sxx_xm) + T(.5) * (sxy_yp - sxy_ym));
+  o[7] = (float)(T(.5) * (syx_xp - syx_xm) + T(.5) * (syy_yp - syy_ym));
+}
+
 // Features of every node: FP32 [N][8] (feat), or -- surrogate inference --
 // normalised bf16 [N][8] (feat_bf, (x - mean[c]) * inv_std[c] with norm =
 // mean[8], inv_std[8]), the network's first layer input.
@@ -61,14 +79,8 @@ __global__ void features_kernel(const T* __restrict__ f, const T* __restrict__ q
   if (dphi) p += (double)dphi[k];
   if (phic) p += (double)phic[k];
   float o[kFeat];
-  o[0] = (float)c.ux;
-  o[1] = (float)c.uy;
-  o[2] = (float)qxx[k];
-  o[3] = (float)qyx[k];
-  o[4] = (float)p;
-  o[5] = (float)c.mu;
-  o[6] = (float)(T(.5) * (xp.sxx - xm.sxx) + T(.5) * (yp.sxy - ym.sxy));
-  o[7] = (float)(T(.5) * (xp.syx - xm.syx) + T(.5) * (yp.syy - ym.syy));
+  feature_values<T>(c, xp.sxx, xm.sxx, xp.syx, xm.syx, yp.sxy, ym.sxy, yp.syy, ym.syy, qxx[k],
+                    qyx[k], p, o);
   if (feat_bf) {
     typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
     bf16x8_t v;
@@ -117,9 +129,105 @@ __global__ void targets_kernel(const T* __restrict__ f2, const T* __restrict__ q
   o[5] = o[6] = o[7] = 0.f;
 }
 
-// Applies (du, dQ, dphi) to a state in place: Q += dQ, phi += dphi
-// (compensated), and f += fe(n, u + du) - fe(n, u), which keeps the mass of
-// every node and changes its momentum by n du.
+// Applies the increment d = (du_x, du_y, dQxx, dQyx, dphi) to the state of one
+// node: Q += dQ, phi += dphi (FP32: compensated, phic the running error), and
+// f += fe(n, u + du) - fe(n, u) (src/lb.hpp:24-41), which keeps the node's
+// mass and changes its momentum by n du.  SHIFT: f holds deviations from the
+// rest equilibrium.
+template <class T, bool SHIFT>
+__device__ __forceinline__ void apply_node(T (&f)[9], T& qxx, T& qyx, T& phi, T* phic,
+                                           const float (&d)[5], double rho) {
+  if constexpr (sizeof(T) == 4) {
+    // FP32: the difference itself, from fe_v = n w_v (1 + 3 c.u + 4.5 (c.u)^2
+    // - 1.5 u.u): with s = u + (u + du) and D = du.s,
+    //   fe_v(u + du) - fe_v(u) = n w_v (3 c.du + 4.5 (c.du)(c.s) - 1.5 D),
+    // an even part a and an odd part b for each opposite pair (no
+    // cancellation between two equilibria); the rest direction takes
+    // -(2/3) n D, so the nine changes sum to zero
+    T n;
+    if constexpr (SHIFT) {
+      n = (T)rho + ((((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7]))) + f[8]);
+    } else {
+      n = (((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7]))) + f[8];
+    }
+    const T inv = fast_rcp(n);
+    const T d56 = f[5] - f[6], d87 = f[8] - f[7];
+    const T ux = ((f[1] - f[2]) + d56 + d87) * inv;
+    const T uy = ((f[3] - f[4]) + d56 - d87) * inv;
+    const T dx = d[0], dy = d[1];
+    const T sx = T(2) * ux + dx, sy = T(2) * uy + dy;
+    const T D = dx * sx + dy * sy;
+    const T n9 = n * T(1. / 9.), n36 = n * T(1. / 36.);
+    const T hD = T(1.5) * D;
+    const T a1 = n9 * (T(4.5) * dx * sx - hD), b1 = n9 * (T(3) * dx);
+    const T a3 = n9 * (T(4.5) * dy * sy - hD), b3 = n9 * (T(3) * dy);
+    const T ep = dx + dy, em = dx - dy;
+    const T a5 = n36 * (T(4.5) * ep * (sx + sy) - hD), b5 = n36 * (T(3) * ep);
+    const T a8 = n36 * (T(4.5) * em * (sx - sy) - hD), b8 = n36 * (T(3) * em);
+    f[0] += T(-2. / 3.) * n * D;
+    f[1] += a1 + b1;
+    f[2] += a1 - b1;
+    f[3] += a3 + b3;
+    f[4] += a3 - b3;
+    f[5] += a5 + b5;
+    f[6] += a5 - b5;
+    f[7] += a8 - b8;
+    f[8] += a8 + b8;
+    qxx += (T)d[2];
+    qyx += (T)d[3];
+    if (phic) {
+      const T yv = (T)d[4] + *phic;
+      const T p0 = phi;
+      const T t = p0 + yv;
+      const T bp = t - p0;
+      const T ap = t - bp;
+      *phic = (p0 - ap) + (yv - bp);
+      phi = t;
+    } else {
+      phi += (T)d[4];
+    }
+  } else {
+    double fv[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) fv[v] = (double)f[v] + (SHIFT ? rest_weight(v) * rho : 0.0);
+    const double n = fv[0] + fv[1] + fv[2] + fv[3] + fv[4] + fv[5] + fv[6] + fv[7] + fv[8];
+    const double ux = (fv[1] - fv[2] + fv[5] - fv[6] - fv[7] + fv[8]) / n;
+    const double uy = (fv[3] - fv[4] + fv[5] - fv[6] + fv[7] - fv[8]) / n;
+    auto feq = [&](double vx, double vy, double* fe) {
+      const double v2 = vx * vx + vy * vy;
+      fe[1] = n * (1. / 9. + 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
+      fe[2] = n * (1. / 9. - 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
+      fe[3] = n * (1. / 9. + 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
+      fe[4] = n * (1. / 9. - 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
+      fe[5] = n * (1. / 36. + 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
+      fe[6] = n * (1. / 36. - 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
+      fe[7] = n * (1. / 36. - 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
+      fe[8] = n * (1. / 36. + 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
+      fe[0] = n - fe[1] - fe[2] - fe[3] - fe[4] - fe[5] - fe[6] - fe[7] - fe[8];
+    };
+    double a[9], b[9];
+    feq(ux, uy, a);
+    feq(ux + (double)d[0], uy + (double)d[1], b);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) f[v] = (T)((double)f[v] + (b[v] - a[v]));
+    qxx = (T)((double)qxx + (double)d[2]);
+    qyx = (T)((double)qyx + (double)d[3]);
+    if (phic) {
+      const T yv = (T)d[4] + *phic;
+      const T p0 = phi;
+      const T t = p0 + yv;
+      const T bp = t - p0;
+      const T ap = t - bp;
+      *phic = (p0 - ap) + (yv - bp);
+      phi = t;
+    } else {
+      phi = (T)((double)phi + (double)d[4]);
+    }
+  }
+}
+
+// Applies the network's increments (FP32 [N][8], physical units) to a state
+// in place (apply_node per node).
 template <class T, bool SHIFT>
 __global__ void apply_increment_kernel(T* __restrict__ f, T* __restrict__ qxx, T* __restrict__ qyx,
                                        T* __restrict__ phi, T* __restrict__ phic,
@@ -127,39 +235,19 @@ __global__ void apply_increment_kernel(T* __restrict__ f, T* __restrict__ qxx, T
                                        double rho) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= N) return;
-  const float* d = delta + (size_t)k * kFeat;
-  double n, ux, uy;
-  velocity_at<T, SHIFT>(f, NP, k, rho, n, ux, uy);
-  auto feq = [&](double vx, double vy, double* fe) {
-    const double v2 = vx * vx + vy * vy;
-    fe[1] = n * (1. / 9. + 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
-    fe[2] = n * (1. / 9. - 1. / 3. * vx + 1. / 3. * v2 - 1. / 2. * vy * vy);
-    fe[3] = n * (1. / 9. + 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
-    fe[4] = n * (1. / 9. - 1. / 3. * vy - 1. / 2. * vx * vx + 1. / 3. * v2);
-    fe[5] = n * (1. / 36. + 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
-    fe[6] = n * (1. / 36. - 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 + 1. / 4. * vx * vy);
-    fe[7] = n * (1. / 36. - 1. / 12. * vx + 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
-    fe[8] = n * (1. / 36. + 1. / 12. * vx - 1. / 12. * vy + 1. / 12. * v2 - 1. / 4. * vx * vy);
-    fe[0] = n - fe[1] - fe[2] - fe[3] - fe[4] - fe[5] - fe[6] - fe[7] - fe[8];
-  };
-  double a[9], b[9];
-  feq(ux, uy, a);
-  feq(ux + (double)d[0], uy + (double)d[1], b);
+  const float* dp = delta + (size_t)k * kFeat;
+  const float d[5] = {dp[0], dp[1], dp[2], dp[3], dp[4]};
+  T fl[9];
 #pragma unroll
-  for (int v = 0; v < 9; ++v) f[v * NP + k] = (T)((double)f[v * NP + k] + (b[v] - a[v]));
-  qxx[k] = (T)((double)qxx[k] + (double)d[2]);
-  qyx[k] = (T)((double)qyx[k] + (double)d[3]);
-  if (phic) {
-    const T yv = (T)d[4] + phic[k];
-    const T p0 = phi[k];
-    const T t = p0 + yv;
-    const T bp = t - p0;
-    const T ap = t - bp;
-    phic[k] = (p0 - ap) + (yv - bp);
-    phi[k] = t;
-  } else {
-    phi[k] = (T)((double)phi[k] + (double)d[4]);
-  }
+  for (int v = 0; v < 9; ++v) fl[v] = f[v * NP + k];
+  T qx = qxx[k], qy = qyx[k], p = phi[k], pc = phic ? phic[k] : T(0);
+  apply_node<T, SHIFT>(fl, qx, qy, p, phic ? &pc : nullptr, d, rho);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) f[v * NP + k] = fl[v];
+  qxx[k] = qx;
+  qyx[k] = qy;
+  phi[k] = p;
+  if (phic) phic[k] = pc;
 }
 
 // Velocity error of a judged step: sum |u1 + du - u2|^2 and sum |u2|^2

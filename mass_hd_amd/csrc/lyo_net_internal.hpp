@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/mass_lyo.h"
+#include "lyo_net_fused_api.hpp"
 
 namespace lyo {
 // delta (FP32 [N][8], physical units) = network(features FP32 [N][8]) on s
@@ -17,4 +18,7 @@ hipError_t net_input(mass_lyo_net_t n, size_t N, __bf16** in, const float** norm
 hipError_t net_predict_input(mass_lyo_net_t n, int LX, int LY, float* delta, hipStream_t s);
 // trained (normalisation known)
 bool net_ready(mass_lyo_net_t n);
+// the weights in the layout of the one-kernel network step (device memory,
+// repacked with the other fragments after every parameter update)
+NetFusedW net_fused_weights(mass_lyo_net_t n);
 }  // namespace lyo

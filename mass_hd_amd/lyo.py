@@ -461,6 +461,7 @@ def _net_api(lib):
     lib.mass_lyo_surrogate_enable.argtypes = [h, h, i32, ctypes.c_double]
     lib.mass_lyo_surrogate_disable.argtypes = [h]
     lib.mass_lyo_surrogate_get_stats.argtypes = [h, P(SurrogateStats)]
+    lib.mass_lyo_surrogate_set_kernels.argtypes = [h, i32]
     lib._net_api = True
     return lib
 
@@ -572,6 +573,13 @@ class Net:
 def enable_surrogate(model, net, judge_every=50, tol=1e-3):
     lib = _net_api(model.lib)
     model._check(lib.mass_lyo_surrogate_enable(model.h, net.h, int(judge_every), float(tol)))
+
+
+def set_surrogate_kernels(model, fused):
+    """1: the network part of surrogate steps in one kernel (the FP32
+    default); 0: separate feature, convolution and increment kernels."""
+    lib = _net_api(model.lib)
+    model._check(lib.mass_lyo_surrogate_set_kernels(model.h, int(fused)))
 
 
 def disable_surrogate(model):
