@@ -171,6 +171,12 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warm
     m.synchronize()
     t_sur, _ = m.time_steps(steps, per_sweep=False)
     st = lyo.surrogate_stats(m)
+    # the same run with the network part as separate feature, convolution and
+    # increment kernels (features and activations through HBM)
+    lyo.set_surrogate_kernels(m, 0)
+    m.step(2)
+    m.synchronize()
+    t_sep, _ = m.time_steps(steps, per_sweep=False)
     m.close()
     net.close()
     # the structural bound of a surrogate step: the predictor + one corrector
@@ -196,11 +202,13 @@ def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warm
             "net_flop_per_lu": flop,
             "ms_per_step": round(t_sur / steps, 4),
             "plain_ms_per_step": round(t_plain / steps, 4),
+            "network_kernels": "one kernel: features, the four layers (activations in "
+                               "registers), the increment",
+            "separate_kernels_ms_per_step": round(t_sep / steps, 4),
             # the network's FLOPs over the time the surrogate step spends
-            # beyond the predictor + one corrector (features, the four
-            # layers, the increment, the amortised judge): a lower bound
-            # on the layers' MFMA rate; the per-layer counters are in
-            # profiles/surrogate_fp32_4096.md
+            # beyond the predictor + one corrector (the one-kernel network
+            # step and the amortised judge): a lower bound on its MFMA rate;
+            # counters in profiles/surrogate_fp32_4096.md
             "network_tflops_beyond_step": round(
                 flop * N / max(1e-9, (t_sur - t_one) / steps / 1e3) / 1e12, 1),
             "network_frac_of_bf16_peak_beyond_step": round(

@@ -48,6 +48,9 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 constexpr int kHalo = 6;                  // 2 (features) + 4 (convolutions)
 constexpr int kStrip = kWave - 2 * kHalo;  // 52 interior columns
 constexpr int kWarm = 10;                  // head rows before the first output row
+// first iteration whose features / layer 1 / 2 / 3 rows an output row needs
+// (output row x0 needs layer-3 rows from x0-1, layer-2 rows from x0-2, ...)
+constexpr int kFeatFrom = 2, kL1From = 4, kL2From = 6, kL3From = 8;
 // weight fragments (64 lanes x 16 bytes each) per layer, in this order
 constexpr int kFr1 = 0, kFr2 = 5, kFr3 = 23, kFr4 = 41, kFrags = 47;
 constexpr int kNBias = 32 + 32 + 32 + 8;
@@ -239,7 +242,7 @@ __device__ __forceinline__ void item(const Args& a, const Params<float>& P, cons
           qa.yx, ynb<float, false>(from_yp(qb.yx), from_ym(qb.yx)), qb.p, qc.p, qa.p,
           ynb<float, false>(from_yp(qb.p), from_ym(qb.p)), P, un, cv);
     }
-    {
+    if (i >= kFeatFrom) {
       float o[kFeat];
       feature_values<float>(uc, un.sxx, um.sxx, un.syx, um.syx, from_yp(uc.sxy), from_ym(uc.sxy),
                             from_yp(uc.syy), from_ym(uc.syy), qa.xx, qa.yx,
@@ -251,7 +254,7 @@ __device__ __forceinline__ void item(const Args& a, const Params<float>& P, cons
     }
 
     // ---- layer 1: row h-3 from feature rows h-4 (slot U+2), h-3 (U), h-2 (U+1)
-    {
+    if (i >= kL1From) {
       const int Sm = ((U + 2) % 3) * kFeatRow, S0 = U * kFeatRow, Sp = ((U + 1) % 3) * kFeatRow;
       const f32x16 bv = bias_acc(b1, gq);
       f32x16 acc[2];
@@ -266,10 +269,13 @@ __device__ __forceinline__ void item(const Args& a, const Params<float>& P, cons
     }
     // ---- layers 2 and 3: rows h-4, h-5; inputs rows out-1 (slot U+1),
     // out (U+2), out+1 (U, written this iteration)
-    {
+    if (i >= kL2From) {
       f32x16 acc[2];
       conv32(r2, (U + 1) % 3, (U + 2) % 3, U, wl + kFr2 * 64, lane, bias_acc(b2, gq), acc);
       gelu_out(acc, r3[U]);
+    }
+    if (i >= kL3From) {
+      f32x16 acc[2];
       conv32(r3, (U + 1) % 3, (U + 2) % 3, U, wl + kFr3 * 64, lane, bias_acc(b3, gq), acc);
       gelu_out(acc, r4[U]);
     }
