@@ -424,6 +424,15 @@ int mass_lyo_surrogate_get_stats(mass_lyo_t h, mass_lyo_surrogate_stats* out);
  * always use these).  Both apply the same increments up to the BF16 rounding
  * of the activations.  No reference counterpart. */
 int mass_lyo_surrogate_set_kernels(mass_lyo_t h, int32_t fused);
+/* Output calibration (on = 1; off by default; enabling or disabling resets
+ * it): every judged step also fits, per output c (du_x, du_y, dQxx, dQyx,
+ * dphi), the factor a_c that best maps the network's increments p onto the
+ * plain corrector's increments t in least squares, over the judged steps
+ * with weight 0.9^age (a_c = sum t.p / sum p.p, kept in [0.25, 4]); the
+ * following surrogate steps apply a_c p.  No reference counterpart. */
+int mass_lyo_surrogate_set_calibration(mass_lyo_t h, int32_t on);
+/* the current factors a_c (five values; 1 while calibration is off) */
+int mass_lyo_surrogate_get_calibration(mass_lyo_t h, double* factors5);
 
 /* Version of this ABI (MASS_LYO_ABI_VERSION). */
 int mass_lyo_abi_version(void);

@@ -185,7 +185,7 @@ void mass_lyo_destroy(mass_lyo_t h) {
   if (h->d_stage) (void)hipFree(h->d_stage);
   if (h->d_part) (void)hipFree(h->d_part);
   for (void* p : h->diag_allocs) (void)hipFree(p);
-  for (void* p : {(void*)h->sur.delta, (void*)h->sur.part})
+  for (void* p : {(void*)h->sur.delta, (void*)h->sur.part, (void*)h->sur.d_oscale})
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -269,20 +269,43 @@ static int surrogate_step(mass_lyo_t h) {
     const float* nrm = nullptr;
     if (e == hipSuccess) e = lyo::net_input(S.net, (size_t)LX * LY, &in, &nrm);
     if (e == hipSuccess) e = h->eng->net_features(1, nullptr, nrm, in, h->stream);
+    // the network's own output scales here: the judge applies the
+    // calibration itself and measures the uncalibrated increments
     if (e == hipSuccess) e = lyo::net_predict_input(S.net, LX, LY, S.delta, h->stream);
-    const int nparts = 256;
-    if (e == hipSuccess) e = h->eng->net_judge(S.delta, S.part, nparts, h->stream);
-    double part[2 * 256];
+    constexpr int nparts = 256;
+    lyo::NetCalib cal;
+    for (int c = 0; c < 5; ++c) cal.a[c] = S.calib ? S.alpha[c] : 1.f;
+    if (e == hipSuccess) e = h->eng->net_judge(S.delta, cal, S.part, nparts, h->stream);
+    std::vector<double> part((size_t)lyo::kJudgeSums * nparts);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(part, S.part, sizeof(part), hipMemcpyDeviceToHost, h->stream);
+      e = hipMemcpyAsync(part.data(), S.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                         h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "judged step");
-    double num = 0, den = 0;
-    for (int i = 0; i < nparts; ++i) {
-      num += part[2 * i];
-      den += part[2 * i + 1];
+    double sum[lyo::kJudgeSums] = {};
+    for (int i = 0; i < nparts; ++i)
+      for (int j = 0; j < lyo::kJudgeSums; ++j) sum[j] += part[(size_t)lyo::kJudgeSums * i + j];
+    const double err = sum[1] > 0 ? std::sqrt(sum[0] / sum[1]) : 0.0;
+    if (S.calib) {
+      // least squares of the plain corrector's increments t on the network's
+      // p, per output, over the judged steps with weight kDecay^age
+      constexpr double kDecay = 0.9;
+      float sc[8];
+      lyo::net_out_scale(S.net, sc);
+      for (int c = 0; c < 5; ++c) {
+        S.ctp[c] = kDecay * S.ctp[c] + sum[2 + 2 * c];
+        S.cpp[c] = kDecay * S.cpp[c] + sum[3 + 2 * c];
+        double a = S.cpp[c] > 0 ? S.ctp[c] / S.cpp[c] : 1.0;
+        a = a < 0.25 ? 0.25 : (a > 4.0 ? 4.0 : a);
+        S.alpha[c] = (float)a;
+        S.h_oscale[c] = sc[c] * S.alpha[c];
+      }
+      for (int c = 5; c < 8; ++c) S.h_oscale[c] = sc[c];
+      e = hipMemcpyAsync(S.d_oscale, S.h_oscale, sizeof(S.h_oscale), hipMemcpyHostToDevice,
+                         h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) return hip_fail(h, e, "calibration");
     }
-    const double err = den > 0 ? std::sqrt(num / den) : 0.0;
     S.stats.judged++;
     S.stats.plain_steps++;
     S.stats.last_error = err;
@@ -300,7 +323,8 @@ static int surrogate_step(mass_lyo_t h) {
   } else if (S.fused) {
     e = h->eng->steps_with(1, 1, h->launch, h->stream);
     if (e == hipSuccess)
-      e = h->eng->net_fused(lyo::net_fused_weights(S.net), h->launch.xc5, nullptr, h->stream);
+      e = h->eng->net_fused(lyo::net_fused_weights(S.net, S.calib ? S.d_oscale : nullptr),
+                            h->launch.xc5, nullptr, h->stream);
     S.stats.surrogate_steps++;
   } else {
     e = h->eng->steps_with(1, 1, h->launch, h->stream);
@@ -308,7 +332,9 @@ static int surrogate_step(mass_lyo_t h) {
     const float* nrm = nullptr;
     if (e == hipSuccess) e = lyo::net_input(S.net, (size_t)LX * LY, &in, &nrm);
     if (e == hipSuccess) e = h->eng->net_features(0, nullptr, nrm, in, h->stream);
-    if (e == hipSuccess) e = lyo::net_predict_input(S.net, LX, LY, S.delta, h->stream);
+    if (e == hipSuccess)
+      e = lyo::net_predict_input(S.net, LX, LY, S.delta, h->stream,
+                                 S.calib ? S.d_oscale : nullptr);
     if (e == hipSuccess) e = h->eng->net_apply(S.delta, h->stream);
     S.stats.surrogate_steps++;
   }
@@ -333,6 +359,16 @@ int mass_lyo_step(mass_lyo_t h, int32_t nsteps) {
   return MASS_LYO_OK;
 }
 
+// calibration factors back to one, the network's own output scales
+static void reset_calibration(mass_lyo_t h) {
+  auto& S = h->sur;
+  for (int c = 0; c < 5; ++c) {
+    S.ctp[c] = S.cpp[c] = 0;
+    S.alpha[c] = 1.f;
+  }
+  lyo::net_out_scale(S.net, S.h_oscale);
+}
+
 int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_every, double tol) {
   if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
   if (!net || !lyo::net_ready(net))
@@ -346,9 +382,15 @@ int mass_lyo_surrogate_enable(mass_lyo_t h, mass_lyo_net_t net, int32_t judge_ev
   hipError_t e = hipSuccess;
   const size_t N = (size_t)h->N;
   if (!S.delta) e = hipMalloc((void**)&S.delta, N * 8 * sizeof(float));
-  if (e == hipSuccess && !S.part) e = hipMalloc((void**)&S.part, 2 * 256 * sizeof(double));
+  if (e == hipSuccess && !S.part)
+    e = hipMalloc((void**)&S.part, (size_t)lyo::kJudgeSums * 256 * sizeof(double));
+  if (e == hipSuccess && !S.d_oscale) e = hipMalloc((void**)&S.d_oscale, 8 * sizeof(float));
   if (e != hipSuccess) return hip_fail(h, e, "surrogate buffers");
   S.net = net;
+  reset_calibration(h);
+  e = hipMemcpyAsync(S.d_oscale, S.h_oscale, sizeof(S.h_oscale), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "surrogate buffers");
   S.judge_every = judge_every;
   S.tol = tol;
   S.count = 0;
@@ -369,6 +411,25 @@ int mass_lyo_surrogate_set_kernels(mass_lyo_t h, int32_t fused) {
   if (fused && !fused_net_ok(h))
     return fail(h, MASS_LYO_EINVAL, "the one-kernel network step needs FP32 and LX, LY >= 16");
   h->sur.fused = fused;
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_surrogate_set_calibration(mass_lyo_t h, int32_t on) {
+  if (!h) return fail(nullptr, MASS_LYO_EINVAL, "null handle");
+  if (on != 0 && on != 1) return fail(h, MASS_LYO_EINVAL, "on is 0 or 1");
+  if (!h->sur.net) return fail(h, MASS_LYO_EINVAL, "enable the surrogate first");
+  h->sur.calib = on != 0;
+  reset_calibration(h);
+  hipError_t e = hipMemcpyAsync(h->sur.d_oscale, h->sur.h_oscale, sizeof(h->sur.h_oscale),
+                                hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "calibration");
+  return MASS_LYO_OK;
+}
+
+int mass_lyo_surrogate_get_calibration(mass_lyo_t h, double* factors5) {
+  if (!h || !factors5) return fail(h, MASS_LYO_EINVAL, "null argument");
+  for (int c = 0; c < 5; ++c) factors5[c] = h->sur.alpha[c];
   return MASS_LYO_OK;
 }
 

@@ -643,7 +643,9 @@ class Engine {
                                   hipStream_t s) = 0;
   virtual hipError_t net_targets(float* tgt, hipStream_t s) = 0;
   virtual hipError_t net_apply(const float* delta, hipStream_t s) = 0;
-  virtual hipError_t net_judge(const float* delta, double* part, int nparts, hipStream_t s) = 0;
+  // part: nparts x kJudgeSums partial sums (judge_kernel)
+  virtual hipError_t net_judge(const float* delta, const NetCalib& cal, double* part, int nparts,
+                               hipStream_t s) = 0;
   // the network part of a surrogate step in one kernel (FP32, lyo_net_fused.hpp):
   // features, the network and the increment of the current state, xc rows
   // per wave chunk; delta (optional): the increments applied
@@ -840,13 +842,17 @@ class EngineT final : public Engine {
                        rho_);
     return hipGetLastError();
   }
-  hipError_t net_judge(const float* delta, double* part, int nparts,
+  hipError_t net_judge(const float* delta, const NetCalib& cal, double* part, int nparts,
                        hipStream_t s) override {
     if (npc_ < 2) return hipErrorInvalidValue;
     const hipError_t e = materialize_uq<T, EXACT>(B_, P_, LX_, LY_, NP_, npc_, s);
     if (e != hipSuccess) return e;
+    const int cur = B_.cur;
     hipLaunchKernelGGL((judge_kernel<T, !EXACT>), dim3(nparts), dim3(256), 0, s, B_.Cf[0],
-                       B_.Sf[B_.cur], delta, LX_ * LY_, NP_, rho_, part);
+                       B_.Cqxx[0], B_.Cqyx[0], EXACT ? B_.Cphi[0] : B_.Sphi[cur ^ 1],
+                       EXACT ? (const T*)nullptr : (const T*)B_.Cphi[0], B_.Sf[cur], B_.Sqxx[cur],
+                       B_.Sqyx[cur], B_.Sphi[cur], (const T*)B_.Sphic[cur], delta, cal,
+                       LX_ * LY_, NP_, rho_, part);
     return hipGetLastError();
   }
   hipError_t net_fused(const NetFusedW& w, int xc, float* delta, hipStream_t s) override {

@@ -37,6 +37,14 @@ struct mass_lyo_s {
     int fused = 1;            // network part in one kernel (FP32, LX and LY >= 16)
     float* delta = nullptr;   // [N][8]
     double* part = nullptr;   // judge partial sums
+    // output calibration (mass_lyo_surrogate_set_calibration): decayed sums
+    // of t.p and p.p per output over the judged steps, the factors, and the
+    // calibrated output scales (host copy and device)
+    bool calib = false;
+    double ctp[5] = {}, cpp[5] = {};
+    float alpha[5] = {1.f, 1.f, 1.f, 1.f, 1.f};
+    float h_oscale[8] = {};
+    float* d_oscale = nullptr;
     mass_lyo_surrogate_stats stats{};
   } sur;
   std::string err;

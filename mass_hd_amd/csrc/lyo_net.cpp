@@ -396,18 +396,24 @@ hipError_t net_input(mass_lyo_net_t n, size_t N, __bf16** in, const float** norm
   return hipSuccess;
 }
 
-hipError_t net_predict_input(mass_lyo_net_t n, int LX, int LY, float* delta, hipStream_t s) {
+hipError_t net_predict_input(mass_lyo_net_t n, int LX, int LY, float* delta, hipStream_t s,
+                             const float* out_scale) {
   if ((size_t)LX * LY > n->wsN) return hipErrorInvalidValue;
-  return forward(n, LX, LY, false, s, nullptr, delta, n->d_norm + 16);
+  return forward(n, LX, LY, false, s, nullptr, delta, out_scale ? out_scale : n->d_norm + 16);
+}
+
+void net_out_scale(mass_lyo_net_t n, float* out8) {
+  for (int c = 0; c < 8; ++c) out8[c] = n->norm[16 + c];
 }
 
 bool net_ready(mass_lyo_net_t n) { return n && n->have_norm; }
 
-NetFusedW net_fused_weights(mass_lyo_net_t n) {
+NetFusedW net_fused_weights(mass_lyo_net_t n, const float* out_scale) {
   NetFusedW w;
   w.wfrag = n->d_wfused;
   w.bias = n->d_fbias;
   w.norm = n->d_norm;
+  w.out_scale = out_scale ? out_scale : n->d_norm + 16;
   return w;
 }
 

@@ -250,34 +250,60 @@ __global__ void apply_increment_kernel(T* __restrict__ f, T* __restrict__ qxx, T
   if (phic) phic[k] = pc;
 }
 
-// Velocity error of a judged step: sum |u1 + du - u2|^2 and sum |u2|^2
-// (per-block partials), where 1 is the one-corrector state, du the predicted
-// increment and 2 the plain two-corrector state.
+// Output calibration of the surrogate: per output c (du_x, du_y, dQxx, dQyx,
+// dphi) a factor a_c on the network's increments, least-squares over judged
+// steps.
+struct NetCalib {
+  float a[5];
+};
+constexpr int kJudgeSums = 12;  // error, norm, then (t.p, p.p) per output
+
+// A judged step: per-block partial sums (part[b][kJudgeSums]) of
+//   |u1 + a du - u2|^2, |u2|^2, and per output c: t_c p_c, p_c p_c
+// where 1 is the one-corrector state, 2 the plain two-corrector state, p the
+// network's increments (delta, before calibration), a the current
+// calibration and t = (state 2) - (state 1) the increments the plain
+// corrector made (as targets_kernel forms them).
 template <class T, bool SHIFT>
 __global__ void __launch_bounds__(256)
-judge_kernel(const T* __restrict__ f1, const T* __restrict__ f2, const float* __restrict__ delta,
-             int N, int NP, double rho, double* __restrict__ part) {
-  __shared__ double r0[4], r1[4];
-  double e = 0, s = 0;
+judge_kernel(const T* __restrict__ f1, const T* __restrict__ q1xx, const T* __restrict__ q1yx,
+             const T* __restrict__ phi1, const T* __restrict__ dphi1, const T* __restrict__ f2,
+             const T* __restrict__ q2xx, const T* __restrict__ q2yx, const T* __restrict__ phi2,
+             const T* __restrict__ phic2, const float* __restrict__ delta, NetCalib cal, int N,
+             int NP, double rho, double* __restrict__ part) {
+  __shared__ double red[kJudgeSums][4];
+  double acc[kJudgeSums];
+#pragma unroll
+  for (int j = 0; j < kJudgeSums; ++j) acc[j] = 0;
   for (int k = blockIdx.x * 256 + threadIdx.x; k < N; k += gridDim.x * 256) {
     double n1, ux1, uy1, n2, ux2, uy2;
     velocity_at<T, SHIFT>(f1, NP, k, rho, n1, ux1, uy1);
     velocity_at<T, SHIFT>(f2, NP, k, rho, n2, ux2, uy2);
-    const double ex = ux1 + (double)delta[(size_t)k * kFeat] - ux2;
-    const double ey = uy1 + (double)delta[(size_t)k * kFeat + 1] - uy2;
-    e += ex * ex + ey * ey;
-    s += ux2 * ux2 + uy2 * uy2;
+    const float* d = delta + (size_t)k * kFeat;
+    const double ex = ux1 + (double)cal.a[0] * (double)d[0] - ux2;
+    const double ey = uy1 + (double)cal.a[1] * (double)d[1] - uy2;
+    acc[0] += ex * ex + ey * ey;
+    acc[1] += ux2 * ux2 + uy2 * uy2;
+    const double p2 = (double)phi2[k] + (phic2 ? (double)phic2[k] : 0.0);
+    const double p1 = (double)phi1[k] + (dphi1 ? (double)dphi1[k] : 0.0);
+    const double t[5] = {ux2 - ux1, uy2 - uy1, (double)q2xx[k] - (double)q1xx[k],
+                         (double)q2yx[k] - (double)q1yx[k], p2 - p1};
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      const double pc = (double)d[c];
+      acc[2 + 2 * c] += t[c] * pc;
+      acc[3 + 2 * c] += pc * pc;
+    }
   }
-  e = wave_sum(e);
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) {
-    r0[threadIdx.x >> 6] = e;
-    r1[threadIdx.x >> 6] = s;
+#pragma unroll
+  for (int j = 0; j < kJudgeSums; ++j) {
+    const double v = wave_sum(acc[j]);
+    if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = (r0[0] + r0[1]) + (r0[2] + r0[3]);
-    part[2 * blockIdx.x + 1] = (r1[0] + r1[1]) + (r1[2] + r1[3]);
+  if (threadIdx.x < kJudgeSums) {
+    const int j = threadIdx.x;
+    part[kJudgeSums * blockIdx.x + j] = (red[j][0] + red[j][1]) + (red[j][2] + red[j][3]);
   }
 }
 

@@ -13,7 +13,7 @@ hipError_t launch_net_fused(const StatePlanes& in, const StatePlanes& out, int L
                             int xc, const Params<float>& P, double rho, const NetFusedW& w,
                             float* delta, hipStream_t s) {
   if (LX < kNetMinSide || LY < kNetMinSide || xc < 1 || NP < LX * LY || !w.wfrag || !w.bias ||
-      !w.norm || !in.f || !in.qxx || !in.qyx || !in.phi || !in.phic || !out.f || !out.qxx ||
+      !w.norm || !w.out_scale || !in.f || !in.qxx || !in.qyx || !in.phi || !in.phic || !out.f || !out.qxx ||
       !out.qyx || !out.phi || !out.phic)
     return hipErrorInvalidValue;
   Geo g{};
@@ -42,6 +42,7 @@ hipError_t launch_net_fused(const StatePlanes& in, const StatePlanes& out, int L
   a.wfrag = static_cast<const netf::bf16x8*>(w.wfrag);
   a.bias = w.bias;
   a.norm = w.norm;
+  a.out_scale = w.out_scale;
   a.delta = delta;
   a.rho = rho;
   hipError_t e = hipFuncSetAttribute((const void*)netf::net_fused_kernel,

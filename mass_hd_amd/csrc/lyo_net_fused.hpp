@@ -75,7 +75,8 @@ struct Args {
   float* phic_out;
   const bf16x8* wfrag;  // kFrags x 64 fused fragments
   const float* bias;    // kNBias: layer biases in layer order
-  const float* norm;    // mean[8], inv_std[8], out_scale[8]
+  const float* norm;       // mean[8], inv_std[8] of the inputs
+  const float* out_scale;  // [8] physical increment per network output
   float* delta;         // optional [N][8]: the increments applied (tests)
   double rho;
 };
@@ -375,7 +376,7 @@ net_fused_kernel(const Args a, const Params<float> P, const Geo g) {
   float* cl = reinterpret_cast<float*>(lds + kLdsW);
   for (int e = threadIdx.x; e < kFrags * 64; e += blockDim.x) wl[e] = a.wfrag[e];
   for (int e = threadIdx.x; e < kNBias; e += blockDim.x) cl[e] = a.bias[e];
-  if (threadIdx.x < 8) cl[kNBias + threadIdx.x] = a.norm[16 + threadIdx.x];
+  if (threadIdx.x < 8) cl[kNBias + threadIdx.x] = a.out_scale[threadIdx.x];
   __syncthreads();
   int b = blockIdx.x;
   if (g.xcd) {

@@ -13,12 +13,14 @@ constexpr int kNetRepeat = 10;   // rows a chunk computes beyond its own
 constexpr int kNetMinSide = 16;  // smallest LX and LY
 
 // The network's weights in the fused kernel's layout (device memory):
-// fragments (47 x 64 lanes x 8 bf16), layer biases (104) and the
-// normalisation (mean[8], inv_std[8], out_scale[8]).
+// fragments (47 x 64 lanes x 8 bf16), layer biases (104), the input
+// normalisation (mean[8], inv_std[8]) and the output scales (8: the
+// network's own, or a run's calibrated ones).
 struct NetFusedW {
   const void* wfrag = nullptr;
   const float* bias = nullptr;
   const float* norm = nullptr;
+  const float* out_scale = nullptr;
 };
 
 // A state buffer set: f' (9 planes of NP values), Qxx, Qyx, phi and phi's

@@ -14,11 +14,15 @@ hipError_t net_predict(mass_lyo_net_t n, const float* feat, int LX, int LY, floa
 // [N][8], allocated on first use) and its input normalisation (device:
 // mean[8], inv_std[8])
 hipError_t net_input(mass_lyo_net_t n, size_t N, __bf16** in, const float** norm);
-// delta (FP32 [N][8], physical units) = network(the input buffer) on s
-hipError_t net_predict_input(mass_lyo_net_t n, int LX, int LY, float* delta, hipStream_t s);
+// delta (FP32 [N][8], physical units) = network(the input buffer) on s;
+// out_scale (device [8]) replaces the network's own output scales when given
+hipError_t net_predict_input(mass_lyo_net_t n, int LX, int LY, float* delta, hipStream_t s,
+                             const float* out_scale = nullptr);
+// the network's output scales (host copy, 8 values)
+void net_out_scale(mass_lyo_net_t n, float* out8);
 // trained (normalisation known)
 bool net_ready(mass_lyo_net_t n);
 // the weights in the layout of the one-kernel network step (device memory,
 // repacked with the other fragments after every parameter update)
-NetFusedW net_fused_weights(mass_lyo_net_t n);
+NetFusedW net_fused_weights(mass_lyo_net_t n, const float* out_scale = nullptr);
 }  // namespace lyo

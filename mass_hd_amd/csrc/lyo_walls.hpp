@@ -1028,7 +1028,7 @@ class WallEngine final : public Engine {
   }
   hipError_t net_targets(float*, hipStream_t) override { return hipErrorNotSupported; }
   hipError_t net_apply(const float*, hipStream_t) override { return hipErrorNotSupported; }
-  hipError_t net_judge(const float*, double*, int, hipStream_t) override {
+  hipError_t net_judge(const float*, const NetCalib&, double*, int, hipStream_t) override {
     return hipErrorNotSupported;
   }
   hipError_t steps_with(int, int, const LaunchCfg&, hipStream_t) override {

@@ -462,6 +462,8 @@ def _net_api(lib):
     lib.mass_lyo_surrogate_disable.argtypes = [h]
     lib.mass_lyo_surrogate_get_stats.argtypes = [h, P(SurrogateStats)]
     lib.mass_lyo_surrogate_set_kernels.argtypes = [h, i32]
+    lib.mass_lyo_surrogate_set_calibration.argtypes = [h, i32]
+    lib.mass_lyo_surrogate_get_calibration.argtypes = [h, dp]
     lib._net_api = True
     return lib
 
@@ -580,6 +582,20 @@ def set_surrogate_kernels(model, fused):
     default); 0: separate feature, convolution and increment kernels."""
     lib = _net_api(model.lib)
     model._check(lib.mass_lyo_surrogate_set_kernels(model.h, int(fused)))
+
+
+def set_surrogate_calibration(model, on):
+    """Judged steps also fit per-output factors of the network's increments
+    (least squares against the plain corrector's); 1 on, 0 off (resets)."""
+    lib = _net_api(model.lib)
+    model._check(lib.mass_lyo_surrogate_set_calibration(model.h, int(on)))
+
+
+def surrogate_calibration(model):
+    lib = _net_api(model.lib)
+    a = (ctypes.c_double * 5)()
+    model._check(lib.mass_lyo_surrogate_get_calibration(model.h, a))
+    return list(a)
 
 
 def disable_surrogate(model):

@@ -75,10 +75,17 @@ def config2(steps=5000, n=64, sampler_seed=2024, base_seed=1000, chunks=5):
 
 
 def train_surrogate(frames_per_run=8, spacing=50, warm=200, iters=2000, lr=2e-3, n=64,
-                    sampler_seed=2024, base_seed=1000, seed=0):
+                    sampler_seed=2024, base_seed=1000, seed=0, matched=None):
     """Trains the surrogate on frames of the 512^2 members of the config2
-    sweep (their own plain npc=2 steps)."""
-    runs = [r for r in config2_runs(n, sampler_seed, base_seed) if r["LX"] == 512]
+    sweep (their own plain npc=2 steps), or -- matched = (zeta, xi, count) --
+    of `count` 512^2 runs with the physics of the run it will stand in for
+    (other seeds than that run's)."""
+    if matched:
+        zeta, xi, count = matched
+        runs = [dict(CONFIG0_PHYS, LX=512, LY=512, zeta=zeta, xi=xi, seed=base_seed + i)
+                for i in range(count)]
+    else:
+        runs = [r for r in config2_runs(n, sampler_seed, base_seed) if r["LX"] == 512]
     net = lyo.Net(seed=seed)
     b, _ = make_batch(runs, autotune=False)
     b.step(warm)
@@ -98,21 +105,31 @@ def train_surrogate(frames_per_run=8, spacing=50, warm=200, iters=2000, lr=2e-3,
 
 
 def compare(L, net, steps, zeta=None, seed=7, judge_every=50, tol=1e-3, every=None,
-            fp64_steps=0):
+            fp64_steps=0, twin=True, calibrate=False):
     """Plain npc=2 vs surrogate (and FP64 plain over fp64_steps) from one
-    initial state; time series of u_rms and defect density."""
+    initial state; time series of u_rms and defect density.  twin: a second
+    plain run from the same state with phi perturbed by 1e-9 (random signs),
+    whose distance from the plain run is the chaotic spread of two runs that
+    differ by a negligible amount -- the yardstick for the surrogate, whose
+    trajectory decorrelates from the plain one in the same way."""
     phys = dict(CONFIG0_PHYS)
     if zeta is not None:
         phys["zeta"] = zeta
     every = every or max(1, steps // 20)
     series = {}
     timing = {}
-    for mode in (["fp64"] if fp64_steps else []) + ["plain", "surrogate"]:
+    modes = (["fp64"] if fp64_steps else []) + ["plain"] + (["twin"] if twin else []) + ["surrogate"]
+    for mode in modes:
         prec = "fp64" if mode == "fp64" else "fp32"
         m = lyo.Lyotropic(L, L, npc=2, precision=prec, **phys)
         m.configure(**dict(CONFIG0_INIT, seed=seed))
+        if mode == "twin":
+            sgn = np.sign(np.random.default_rng(seed).uniform(-1, 1, L * L))
+            m.set_state(m.get("ff"), m.get("QQxx"), m.get("QQyx"),
+                        m.get("phi") + 1e-9 * sgn.reshape(m.get("phi").shape))
         if mode == "surrogate":
             lyo.enable_surrogate(m, net, judge_every, tol)
+            lyo.set_surrogate_calibration(m, 1 if calibrate else 0)
         n = fp64_steps if mode == "fp64" else steps
         rows, t_ms, done = [], 0.0, 0
         while done < n:
@@ -129,11 +146,44 @@ def compare(L, net, steps, zeta=None, seed=7, judge_every=50, tol=1e-3, every=No
         timing[mode] = {"steps": n, "ms": t_ms, "mlups": L * L * n / (t_ms / 1e3) / 1e6}
         if mode == "surrogate":
             timing[mode]["stats"] = lyo.surrogate_stats(m)
+            timing[mode]["calibration"] = lyo.surrogate_calibration(m)
         series[mode] = rows
         log("%s %d^2: %d steps, %.0f MLUPS, final u_rms %.4g, defect density %.4g" %
             (mode, L, n, timing[mode]["mlups"], rows[-1][1], rows[-1][2]))
         m.close()
-    return {"L": L, "steps": steps, "series": series, "timing": timing}
+    return {"L": L, "steps": steps, "series": series, "timing": timing,
+            "relative_to_plain": relative_stats(series, windows=10)}
+
+
+def relative_stats(series, windows=10):
+    """Relative differences of u_rms and defect density from the plain run:
+    per checkpoint (max and mean of |d|) and over `windows` equal windows of
+    checkpoints (the statistics of a chaotic flow are its time averages)."""
+    base = np.array(series["plain"])
+    out = {}
+    for mode, rows in series.items():
+        if mode == "plain":
+            continue
+        r = np.array(rows)
+        n = min(len(r), len(base))
+        if n == 0:
+            continue
+        r, b = r[:n], base[:n]
+        rec = {"checkpoints": n}
+        for col, name in ((1, "u_rms"), (2, "defect_density")):
+            d = r[:, col] / b[:, col] - 1.0
+            w = max(1, n // windows) if mode != "fp64" else n
+            nw = n // w
+            wm = [float(r[k * w:(k + 1) * w, col].mean() / b[k * w:(k + 1) * w, col].mean() - 1.0)
+                  for k in range(nw)]
+            rec[name] = {"checkpoint_max_abs": float(np.abs(d).max()),
+                         "checkpoint_mean": float(d.mean()),
+                         "checkpoint_rms": float(np.sqrt(np.mean(d * d))),
+                         "window_steps": int(w * (r[1, 0] - r[0, 0]) if n > 1 else r[0, 0]),
+                         "window_means": wm,
+                         "window_max_abs": float(np.abs(wm).max()) if wm else None}
+        out[mode] = rec
+    return out
 
 
 def main(argv=None):
@@ -154,6 +204,13 @@ def main(argv=None):
     ap.add_argument("--spacing", type=int, default=0,
                     help="steps between training frames (0: 50 for config3, 500 for config4)")
     ap.add_argument("--frames", type=int, default=8, help="training frames per run")
+    ap.add_argument("--matched", type=int, default=0,
+                    help="N > 0: train on N 512^2 runs with the run's own zeta and xi instead "
+                         "of the config2 sweep's 512^2 members")
+    ap.add_argument("--calibrate", type=int, default=0,
+                    help="1: the surrogate's judged steps recalibrate its output scales")
+    ap.add_argument("--no-twin", action="store_true",
+                    help="skip the plain run from a state perturbed by 1e-9")
     a = ap.parse_args(argv)
     if a.what == "config1":
         r = precision_report(1024, a.steps or 10000, 2, 10, progress=log)
@@ -161,14 +218,21 @@ def main(argv=None):
         r = config2(a.steps or 5000)
     else:
         late = a.what == "config4"
+        matched = None
+        if a.matched > 0:
+            matched = (0.03 if late else CONFIG0_PHYS["zeta"], CONFIG0_PHYS["xi"], a.matched)
         net, tr = train_surrogate(frames_per_run=a.frames,
                                   spacing=a.spacing or (500 if late else 50),
-                                  warm=a.warm or (1000 if late else 200), iters=a.iters)
+                                  warm=a.warm or (1000 if late else 200), iters=a.iters,
+                                  matched=matched)
+        tr["matched"] = matched
         if a.what == "config3":
-            r = {"training": tr, "run": compare(512, net, a.steps or 2000)}
+            r = {"training": tr, "run": compare(512, net, a.steps or 2000, twin=not a.no_twin,
+                                                calibrate=bool(a.calibrate))}
         else:
             r = {"training": tr, "run": compare(4096, net, a.steps or 100000, zeta=0.03, seed=7,
-                                                every=a.every, fp64_steps=a.fp64_steps)}
+                                                every=a.every, fp64_steps=a.fp64_steps,
+                                                twin=not a.no_twin, calibrate=bool(a.calibrate))}
     txt = json.dumps(r, indent=1)
     if a.out:
         open(a.out, "w").write(txt)

@@ -128,3 +128,59 @@ def test_fused_kernels_switch(trained_net):
     with pytest.raises(lyo.MassLyoError):
         lyo.set_surrogate_kernels(small, 1)
     lyo.set_surrogate_kernels(small, 0)
+
+
+def _layers_scaled(net, factor):
+    """the network's parameters with the last layer (weights and bias)
+    multiplied by factor (a power of two: exact in BF16 and FP32)"""
+    p = net.get_params().copy()
+    n4 = 9 * 32 * 8 + 8
+    p[-n4:] *= factor
+    return p
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_calibration_is_scale_equivariant(trained_net, fused):
+    """Output calibration fits a_c = sum t.p / sum p.p per output over the
+    judged steps.  A network whose last layer is doubled predicts exactly
+    twice the increments, so its factors come out exactly halved and, after
+    the first judgement, the calibrated run applies bit for bit the original
+    network's calibrated increments (same state, judgements and fallbacks);
+    without calibration the two runs differ."""
+    p0 = trained_net.get_params().copy()
+    runs = {}
+    try:
+        for factor in (1.0, 2.0):
+            trained_net.set_params(_layers_scaled(trained_net, factor) if factor != 1.0 else p0)
+            for calib in (1, 0):
+                m = _model(48, 64, 9)
+                m.step(10)
+                lyo.enable_surrogate(m, trained_net, judge_every=4, tol=1.0)
+                lyo.set_surrogate_kernels(m, fused)
+                lyo.set_surrogate_calibration(m, calib)
+                m.step(13)
+                runs[(factor, calib)] = (m.get("ff"), m.get("QQxx"), m.get("phi"),
+                                         lyo.surrogate_stats(m), lyo.surrogate_calibration(m))
+                m.close()
+    finally:
+        trained_net.set_params(p0)
+    a1, a2 = runs[(1.0, 1)][4], runs[(2.0, 1)][4]
+    assert all(0.25 < a < 4.0 for a in a1), a1
+    assert any(a != 1.0 for a in a1), a1
+    assert a2 == [a / 2 for a in a1], (a1, a2)
+    for k in range(3):
+        assert np.array_equal(runs[(1.0, 1)][k], runs[(2.0, 1)][k]), k
+    # the first judgement measures the uncalibrated network (factors one);
+    # from then on the two runs apply the same increments
+    s1, s2 = runs[(1.0, 1)][3], runs[(2.0, 1)][3]
+    for k in ("steps", "surrogate_steps", "plain_steps", "judged", "fallbacks", "last_error"):
+        assert s1[k] == s2[k], k
+    assert runs[(1.0, 0)][4] == [1.0] * 5
+    assert not np.array_equal(runs[(1.0, 0)][1], runs[(2.0, 0)][1])
+    assert not np.array_equal(runs[(1.0, 0)][1], runs[(1.0, 1)][1])
+
+
+def test_calibration_switch_needs_surrogate(require_gpu):
+    m = _model(32, 32, 1)
+    with pytest.raises(lyo.MassLyoError):
+        lyo.set_surrogate_calibration(m, 1)
