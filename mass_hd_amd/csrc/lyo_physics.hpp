@@ -109,6 +109,18 @@ __device__ __forceinline__ T lap5(T c, T xp, T xm, const YNb<T, EXACT>& y) {
   }
 }
 
+// a / b rounded to nearest from y = RN(1/b), for finite a and finite b > 0
+// (Markstein's theorem: with y the correctly rounded reciprocal, q0 =
+// RN(a y) is within one ulp of a/b, the residual r = a - q0 b is exact in one
+// FMA, and RN(q0 + r y) = RN(a/b)).  The sign of a zero quotient is the
+// dividend's (b > 0), which q0 + r y would lose for a = -0.  One IEEE
+// division per divisor instead of one per quotient, the same bits.
+__device__ __forceinline__ double quot_rn(double a, double b, double y) {
+  const double q0 = a * y;
+  const double r = __builtin_fma(-q0, b, a);
+  return __builtin_copysign(__builtin_fma(r, y, q0), a);
+}
+
 template <class T>
 __device__ __forceinline__ T fast_rcp(T x) {
   if constexpr (sizeof(T) == 4) {
@@ -372,9 +384,11 @@ __device__ __forceinline__ void update_fields(const CenterVals<T>& c, T Qxx, T Q
     constexpr int cx[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
     constexpr int cy[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
     constexpr double wf[9] = {0, 1 / 3., 1 / 3., 1 / 3., 1 / 3., 1 / 12., 1 / 12., 1 / 12., 1 / 12.};
+    // (fe_v - f_v) / tau, rounded as the division (tau > 0)
+    const T itau = T(1) / tau;
 #pragma unroll
     for (int v = 0; v < 9; ++v)
-      om[v] = (fe[v] - f[v]) / tau + wf[v] * (Fx * T(cx[v]) + Fy * T(cy[v]));
+      om[v] = quot_rn(fe[v] - f[v], tau, itau) + wf[v] * (Fx * T(cx[v]) + Fy * T(cy[v]));
   } else {
     // Omega_v = g_v - f'_v / tau with g_v = fe'_v / tau + force_v and the
     // shifted equilibrium fe'_v = fe_v - rest_weight(v) rho
