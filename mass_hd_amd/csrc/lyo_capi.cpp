@@ -165,6 +165,9 @@ int mass_lyo_create(const mass_lyo_params* p, mass_lyo_t* out) {
   const int per_simd = p->precision == MASS_LYO_FP64 ? 1 : 2;  // resident waves per SIMD
   h->launch.xc3 = chunk_rows_for(p->LX, p->LY, p->npc == 1 ? lyo::kFull1Strip : lyo::kFullStrip,
                                  p->npc == 1 ? 4 : 8, prop.multiProcessorCount * 4 * per_simd);
+  // the one-corrector form (surrogate steps of npc = 2 runs) has its own
+  h->launch.xc3b = chunk_rows_for(p->LX, p->LY, lyo::kFull1Strip, 4,
+                                  prop.multiProcessorCount * 4 * per_simd);
   // the block-cooperative variant: one block of four waves per CU and wave
   // slot per SIMD (its LDS allows two FP32 blocks per CU, one FP64 block)
   h->launch.xc4 = chunk_rows_for(p->LX, p->LY, lyo::kXStrip, 8,
@@ -688,7 +691,7 @@ int mass_lyo_set_schedule(mass_lyo_t h, int32_t fuse_last_pair, int32_t pair_chu
   if (pair_chunk_rows) h->launch.xc2 = pair_chunk_rows;
   if (alternate >= 0) h->launch.alt = alternate;
   if (one_kernel_step >= 0) h->launch.full = one_kernel_step;
-  if (step_chunk_rows) h->launch.xc3 = h->launch.xc4 = step_chunk_rows;
+  if (step_chunk_rows) h->launch.xc3 = h->launch.xc3b = h->launch.xc4 = step_chunk_rows;
   return MASS_LYO_OK;
 }
 

@@ -81,6 +81,7 @@ struct LaunchCfg {
   int full = 1;  // npc <= 2: the whole step in one kernel (2: the block-cooperative
                  // variant for npc = 2, single runs)
   int xc3 = 32;  // rows per wave chunk of the one-kernel step
+  int xc3b = 0;  // ... of its one-corrector form when the run's npc is 2 (0: xc3)
   int xc4 = 32;  // rows per block chunk of the block-cooperative one-kernel step
   int loops = 0; // FP64: one kernel per loop of the reference's step instead
   int xc5 = 32;  // rows per wave chunk of the one-kernel network step (surrogate)
@@ -108,7 +109,9 @@ inline Geo make_geo(int LX, int LY, int NP, const LaunchCfg& L, int kind = 0) {
                 : (kind == 3 ? kFull1Strip
                              : (kind == 2 ? kFullStrip : (kind == 1 ? kPairStrip : kStrip)));
   g.nstrips = (LY + strip - 1) / strip;
-  g.xc = kind == 4 ? L.xc4 : (kind >= 2 ? L.xc3 : (kind == 1 ? L.xc2 : L.xc));
+  g.xc = kind == 4 ? L.xc4
+       : kind == 3 ? (L.xc3b > 0 ? L.xc3b : L.xc3)
+       : (kind == 2 ? L.xc3 : (kind == 1 ? L.xc2 : L.xc));
   g.nchunks = (LX + g.xc - 1) / g.xc;
   g.nwaves = g.nstrips * g.nchunks;
   g.xcd = L.xcd;
