@@ -726,6 +726,9 @@ class EngineT final : public Engine {
     ptrs_.clear();
     bytes_ = 0;
     B_ = Bufs<T>();
+    if (dphi_part_) (void)hipFree(dphi_part_);
+    dphi_part_ = nullptr;
+    dphi_cap_ = 0;
   }
 
   void set_params(const double* v) override {
@@ -840,9 +843,16 @@ class EngineT final : public Engine {
   hipError_t net_apply(const float* delta, hipStream_t s) override {
     const int N = LX_ * LY_, cur = B_.cur;
     const int bs = 256, nb = (N + bs - 1) / bs;
+    hipError_t e = ensure_dphi_part(kPhiBlocks);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((apply_increment_kernel<T, !EXACT>), dim3(nb), dim3(bs), 0, s, B_.Sf[cur],
                        B_.Sqxx[cur], B_.Sqyx[cur], B_.Sphi[cur], B_.Sphic[cur], delta, N, NP_,
                        rho_);
+    // sum(phi) as the plain corrector keeps it: the mean increment removed
+    hipLaunchKernelGGL(dphi_partial_kernel<kFeat>, dim3(kPhiBlocks), dim3(256), 0, s, delta, N,
+                       dphi_part_);
+    hipLaunchKernelGGL((phi_shift_kernel<T>), dim3(kPhiBlocks), dim3(256), 0, s, B_.Sphi[cur],
+                       B_.Sphic[cur], N, (const double*)dphi_part_, kPhiBlocks);
     return hipGetLastError();
   }
   hipError_t net_judge(const float* delta, const NetCalib& cal, double* part, int nparts,
@@ -874,7 +884,15 @@ class EngineT final : public Engine {
       const int cur = B_.cur;
       const StatePlanes in{B_.Sf[cur], B_.Sqxx[cur], B_.Sqyx[cur], B_.Sphi[cur], B_.Sphic[cur]};
       const StatePlanes out{B_.Xf, B_.Xqxx, B_.Xqyx, B_.Xphi, B_.Xphic};
-      const hipError_t e = launch_net_fused(in, out, LX_, LY_, NP_, xc, P_, rho_, w, delta, s);
+      const int nw = net_fused_waves(LX_, LY_, xc);
+      hipError_t e = ensure_dphi_part(nw);
+      if (e != hipSuccess) return e;
+      e = launch_net_fused(in, out, LX_, LY_, NP_, xc, P_, rho_, w, delta, dphi_part_, s);
+      if (e != hipSuccess) return e;
+      // sum(phi) as the plain corrector keeps it: the mean increment removed
+      hipLaunchKernelGGL((phi_shift_kernel<T>), dim3(kPhiBlocks), dim3(256), 0, s, B_.Xphi,
+                         B_.Xphic, LX_ * LY_, (const double*)dphi_part_, nw);
+      e = hipGetLastError();
       if (e != hipSuccess) return e;
       // the corrected state becomes S[cur]; the old state S[cur ^ 1] and the
       // frame fields' bookkeeping (lazy_cur, uq_*) do not refer to it
@@ -998,7 +1016,20 @@ class EngineT final : public Engine {
     return e;
   }
 
+  // partial sums of the surrogate's phi increments (one per block or wave)
+  hipError_t ensure_dphi_part(int n) {
+    if (n <= dphi_cap_) return hipSuccess;
+    if (dphi_part_) (void)hipFree(dphi_part_);
+    dphi_part_ = nullptr;
+    dphi_cap_ = 0;
+    const hipError_t e = hipMalloc((void**)&dphi_part_, sizeof(double) * (size_t)n);
+    if (e == hipSuccess) dphi_cap_ = n;
+    return e;
+  }
+
   Bufs<T> B_;
+  double* dphi_part_ = nullptr;
+  int dphi_cap_ = 0;
   Params<T> P_{};
   double rho_ = 0;
   int LX_ = 0, LY_ = 0, NP_ = 0, npc_ = 1;

@@ -250,6 +250,57 @@ __global__ void apply_increment_kernel(T* __restrict__ f, T* __restrict__ qxx, T
   if (phic) phic[k] = pc;
 }
 
+// Conservation of sum(phi) in surrogate steps.  The plain corrector's phi
+// increment is a flux divergence, so it keeps sum(phi) (to rounding); the
+// network's is not.  After the increments are applied, every node's phi is
+// shifted by -mean(dphi) (the smallest change that restores the sum), from
+// per-block (or per-wave) partial sums of the applied dphi in double, added in
+// a fixed order.
+constexpr int kPhiBlocks = 1024;  // blocks of the partial-sum and shift kernels
+
+// part[b] = sum of delta[k][4] over the nodes k of block b (grid stride,
+// fixed order); C: channels per node of delta
+template <int C>
+__global__ void __launch_bounds__(256)
+dphi_partial_kernel(const float* __restrict__ delta, int N, double* __restrict__ part) {
+  __shared__ double red[4];
+  double a = 0;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < N; k += gridDim.x * 256)
+    a += (double)delta[(size_t)k * C + 4];
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// phi -= (sum of part[0..nparts)) / N on every node; FP32 (phic given): the
+// shift enters the compensated sum as the increments do
+template <class T>
+__global__ void __launch_bounds__(256)
+phi_shift_kernel(T* __restrict__ phi, T* __restrict__ phic, int N,
+                 const double* __restrict__ part, int nparts) {
+  __shared__ double red[4];
+  double a = 0;
+  for (int i = threadIdx.x; i < nparts; i += 256) a += part[i];
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  const double mean = ((red[0] + red[1]) + (red[2] + red[3])) / (double)N;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < N; k += gridDim.x * 256) {
+    if (phic) {
+      const T yv = (T)(-mean) + phic[k];
+      const T p0 = phi[k];
+      const T t = p0 + yv;
+      const T bp = t - p0;
+      const T ap = t - bp;
+      phic[k] = (p0 - ap) + (yv - bp);
+      phi[k] = t;
+    } else {
+      phi[k] = (T)((double)phi[k] - mean);
+    }
+  }
+}
+
 // Output calibration of the surrogate: per output c (du_x, du_y, dQxx, dQyx,
 // dphi) a factor a_c on the network's increments, least-squares over judged
 // steps.

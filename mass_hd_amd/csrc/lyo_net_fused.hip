@@ -9,10 +9,14 @@ namespace lyo {
 static_assert(netf::kStrip == kNetStrip, "strip width of the launch interface");
 static_assert(netf::kWarm == kNetRepeat, "rows a chunk repeats");
 
+int net_fused_waves(int LX, int LY, int xc) {
+  return ((LY + kNetStrip - 1) / kNetStrip) * ((LX + xc - 1) / xc);
+}
+
 hipError_t launch_net_fused(const StatePlanes& in, const StatePlanes& out, int LX, int LY, int NP,
                             int xc, const Params<float>& P, double rho, const NetFusedW& w,
-                            float* delta, hipStream_t s) {
-  if (LX < kNetMinSide || LY < kNetMinSide || xc < 1 || NP < LX * LY || !w.wfrag || !w.bias ||
+                            float* delta, double* dphi_part, hipStream_t s) {
+  if (LX < kNetMinSide || LY < kNetMinSide || xc < 1 || NP < LX * LY || !dphi_part || !w.wfrag || !w.bias ||
       !w.norm || !w.out_scale || !in.f || !in.qxx || !in.qyx || !in.phi || !in.phic || !out.f || !out.qxx ||
       !out.qyx || !out.phi || !out.phic)
     return hipErrorInvalidValue;
@@ -44,6 +48,7 @@ hipError_t launch_net_fused(const StatePlanes& in, const StatePlanes& out, int L
   a.norm = w.norm;
   a.out_scale = w.out_scale;
   a.delta = delta;
+  a.dphi_part = dphi_part;
   a.rho = rho;
   hipError_t e = hipFuncSetAttribute((const void*)netf::net_fused_kernel,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,

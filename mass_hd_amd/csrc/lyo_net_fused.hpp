@@ -78,6 +78,7 @@ struct Args {
   const float* norm;       // mean[8], inv_std[8] of the inputs
   const float* out_scale;  // [8] physical increment per network output
   float* delta;         // optional [N][8]: the increments applied (tests)
+  double* dphi_part;    // [nwaves]: per-wave sums of the applied phi increments
   double rho;
 };
 
@@ -226,6 +227,7 @@ __device__ __forceinline__ void item(const Args& a, const Params<float>& P, cons
   bf16x8 r2[3][2][2] = {}, r3[3][2][2] = {}, r4[3][2][2] = {};
 
   const int nI = x1 - x0 + kWarm;  // head rows x0-4 .. x1+5
+  double dphi_acc = 0;             // the lane's applied phi increments
 
   // One head row; U = i % 3 (ring slots), i the iteration.
   auto row = [&](const int i, auto uu) {
@@ -337,6 +339,7 @@ __device__ __forceinline__ void item(const Args& a, const Params<float>& P, cons
       // the next row's loads have arrived: no wait after the divergent stores
       wait_vmem<false>();
       if (col_ok) {
+        dphi_acc += (double)d[4];
         const int o = roff(xo);
 #pragma unroll
         for (int v = 0; v < 9; ++v) ly.st(a.f_out + (v * N + o), sf[v]);
@@ -367,6 +370,8 @@ __device__ __forceinline__ void item(const Args& a, const Params<float>& P, cons
     row(i + 1, std::integral_constant<int, 1>{});
     row(i + 2, std::integral_constant<int, 2>{});
   }
+  dphi_acc = wave_sum(dphi_acc);
+  if (lane == 0) a.dphi_part[wid] = dphi_acc;
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))

@@ -35,9 +35,12 @@ struct StatePlanes {
 
 // The network part of a surrogate step: reads the one-corrector state `in`,
 // writes the corrected state to `out` (another buffer set), xc rows per wave
-// chunk.  delta (optional, [LX*LY][8]): the increments applied.
+// chunk.  delta (optional, [LX*LY][8]): the increments applied.  dphi_part
+// receives one sum of the applied phi increments per wave (net_fused_waves).
 hipError_t launch_net_fused(const StatePlanes& in, const StatePlanes& out, int LX, int LY, int NP,
                             int xc, const Params<float>& P, double rho, const NetFusedW& w,
-                            float* delta, hipStream_t s);
+                            float* delta, double* dphi_part, hipStream_t s);
+// waves of that launch
+int net_fused_waves(int LX, int LY, int xc);
 
 }  // namespace lyo

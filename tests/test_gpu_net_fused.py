@@ -184,3 +184,20 @@ def test_calibration_switch_needs_surrogate(require_gpu):
     m = _model(32, 32, 1)
     with pytest.raises(lyo.MassLyoError):
         lyo.set_surrogate_calibration(m, 1)
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_surrogate_conserves_phi(trained_net, fused):
+    """The plain corrector's phi increment is a flux divergence and keeps
+    sum(phi); the network's is not, so surrogate steps remove its mean.  Over
+    thirty steps in surrogate mode sum(phi) stays where the plain steps keep
+    it (FP64 sums of the compensated FP32 phi), as does sum(f)."""
+    m = _model(80, 96, 13)
+    f0, p0 = m.sums()
+    lyo.enable_surrogate(m, trained_net, judge_every=10, tol=1.0)
+    lyo.set_surrogate_kernels(m, fused)
+    m.step(30)
+    assert lyo.surrogate_stats(m)["surrogate_steps"] == 27
+    f1, p1 = m.sums()
+    assert abs(p1 / p0 - 1.0) < 1e-11, p1 / p0 - 1.0
+    assert abs(f1 / f0 - 1.0) < 1e-11, f1 / f0 - 1.0
