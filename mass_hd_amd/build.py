@@ -6,10 +6,18 @@ Everything is compiled in-tree with hipcc for gfx950:
   - lyo_capi.cpp   the C ABI of include/mass_lyo.h
   - lyo_config.cpp host-side initial configuration (std::mt19937)
 The host program (mass_hip) is linked against the library with g++.
+
+A second copy of the library and of the host program, with AddressSanitizer
+in the host code of the ABI units (lib/asan/, bin/asan/, plus the ABI cycle
+program of tests/native/), lets the tests check the host side's memory: no
+access outside what it owns and nothing left allocated once every handle is
+destroyed.  Device code is not instrumented; it is the same as in the
+regular build.
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "mass_hd_amd")
@@ -20,8 +28,15 @@ BINDIR = os.path.join(PKG, "bin")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(LIBDIR, "libmass_lyo.so")
 HOSTBIN = os.path.join(BINDIR, "mass_hip")
+ASAN_LIBDIR = os.path.join(LIBDIR, "asan")
+ASAN_BINDIR = os.path.join(BINDIR, "asan")
+ASAN_LIB = os.path.join(ASAN_LIBDIR, "libmass_lyo.so")
+ASAN_HOSTBIN = os.path.join(ASAN_BINDIR, "mass_hip")
+ASAN_CYCLE = os.path.join(ASAN_BINDIR, "abi_cycle")
+NATIVE_TESTS = os.path.join(ROOT, "tests", "native")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CLANGXX = "/opt/rocm/llvm/bin/clang++"
 ARCH = "gfx950"
 COMMON = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
@@ -49,6 +64,13 @@ UNITS = [
 # can differ from separate cos and sin calls in the last bit).
 GXX_UNITS = ["lyo_config.cpp"]
 GXX_FLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra"]
+# AddressSanitizer for the host code only (each flag after -Xarch_host, so
+# the device code is compiled exactly as in the regular build)
+ASAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer",
+             "-Xarch_host", "-g"]
+# the units whose host code owns memory: handles, batches, the network
+ASAN_UNITS = ("lyo_capi.cpp", "lyo_batch.cpp", "lyo_net.cpp")
+JOBS = max(1, min(8, os.cpu_count() or 1))
 
 
 def _run(cmd, verbose):
@@ -63,21 +85,46 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build_lib(verbose=False):
+def _obj(src, sub=""):
+    return os.path.join(OBJDIR, sub, os.path.splitext(src)[0] + ".o")
+
+
+def _compile_all(cmds, verbose):
+    """Runs independent compile commands, JOBS at a time."""
+    with ThreadPoolExecutor(JOBS) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+
+
+def build_lib(verbose=False, asan=True):
+    """libmass_lyo.so and, with asan, its host-sanitized copy in lib/asan."""
     os.makedirs(LIBDIR, exist_ok=True)
-    os.makedirs(OBJDIR, exist_ok=True)
-    objs = []
+    os.makedirs(os.path.join(OBJDIR, "asan"), exist_ok=True)
+    cmds, objs, asan_objs = [], [], []
     for src, extra in UNITS:
-        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-        _run([HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        cmds.append([HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", _obj(src)])
+        objs.append(_obj(src))
+        if asan and src in ASAN_UNITS:
+            cmds.append([HIPCC] + COMMON + extra + ASAN_HOST +
+                        ["-c", os.path.join(CSRC, src), "-o", _obj(src, "asan")])
+            asan_objs.append(_obj(src, "asan"))
+        else:
+            asan_objs.append(_obj(src))
     for src in GXX_UNITS:
-        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-        _run(["g++"] + GXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        cmds.append(["g++"] + GXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", _obj(src)])
+        objs.append(_obj(src))
+        asan_objs.append(_obj(src))
+    _compile_all(cmds, verbose)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
+    if asan:
+        # the sanitizer runtime is linked into the executables that load this copy
+        os.makedirs(ASAN_LIBDIR, exist_ok=True)
+        tmp = ASAN_LIB + ".tmp"
+        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + asan_objs,
+             verbose)
+        os.replace(tmp, ASAN_LIB)
     return LIB
 
 
@@ -97,6 +144,28 @@ def build_host(verbose=False):
     return HOSTBIN
 
 
+def build_asan_programs(verbose=False):
+    """bin/asan/mass_hip (the host program) and bin/asan/abi_cycle (every
+    handle kind created, used and destroyed), both with AddressSanitizer and
+    linked against lib/asan/libmass_lyo.so."""
+    os.makedirs(ASAN_BINDIR, exist_ok=True)
+    san = ["-fsanitize=address", "-fno-omit-frame-pointer", "-g", "-O1"]
+    link = ["-L" + ASAN_LIBDIR, "-lmass_lyo", "-Wl,-rpath,$ORIGIN/../../lib/asan"]
+    inc = ["-I" + os.path.join(ROOT, "include")]
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+    jobs = [([CLANGXX, "-std=c++17", "-Wall"] + san + inc + ["-I" + HOST, "-o",
+             ASAN_HOSTBIN + ".tmp"] + srcs + link, ASAN_HOSTBIN),
+            # a C program, linked by the C++ driver so that operator new and
+            # delete are the sanitizer's (their callers then appear in the
+            # allocation stacks)
+            ([CLANGXX, "-x", "c", "-std=c11", "-Wall", "-Wextra"] + san + inc +
+             ["-o", ASAN_CYCLE + ".tmp", os.path.join(NATIVE_TESTS, "abi_cycle.c"), "-x", "none"] +
+             link, ASAN_CYCLE)]
+    _compile_all([c for c, _ in jobs], verbose)
+    for _, out in jobs:
+        os.replace(out + ".tmp", out)
+
+
 def build_oracle(verbose=False):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], verbose)
 
@@ -104,6 +173,7 @@ def build_oracle(verbose=False):
 def build_all(verbose=False):
     build_lib(verbose)
     build_host(verbose)
+    build_asan_programs(verbose)
     build_oracle(verbose)
 
 
