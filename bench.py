@@ -144,6 +144,92 @@ def growth_leg(lyo, steps, L=500):
             "phi_sum_after": round(float(phi.sum()), 3)}
 
 
+# the same proliferation runcard as the host program reads it
+RUNCARD = """model = go-or-grow
+config = circle
+nsteps = {steps}
+ninfo = 100
+LX = 500
+LY = 500
+bc = 0
+seed = 1001
+GammaQ = 0.3
+GammaP = 5e-2
+zeta = 0
+zetaI = 0
+friction = 0.0
+tauNem = 1.0
+tauIso = 1.0
+AA = 0.03
+CC = 0.1
+LL = 0.002
+KK = 0.06
+rho = 40
+xi = 0.3
+B = 0.1
+Snem = 1
+level = 250
+radius = 60
+conc = 1
+noise = 1
+angle = 0
+relax-steps = 2000
+relax-dt = 1.0
+relax-phi = 1
+relax-Q = 1
+chi-config = noise
+chi0 = 1
+chi-noise = 0
+chi-length = 0
+Dchi = 0
+Achi = 0
+Ochi = 0
+phiswitch = 0
+growTogether = 0
+beta = 0
+death-rate = 0
+death-time = 1
+death-radius = 0
+division-rate = 0.1
+division-time = 2
+division-radius = 3
+alpha = 0.001
+phi-critical = 1.2
+"""
+
+
+def runcard_leg(steps):
+    """The proliferation runcard end to end through the host program
+    (mass_hd_amd/bin/mass_hip, a child process): configure, `steps` steps and
+    a JSON frame every 100 steps (each ~6.5 M numbers, 70 MB), as a MASS user
+    runs it; the run's own report of its total and output-writing time."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mass_hd_amd", "bin", "mass_hip")
+    tmp = tempfile.mkdtemp(prefix="mass_runcard_")
+    try:
+        card = os.path.join(tmp, "run.dat")
+        with open(card, "w") as f:
+            f.write(RUNCARD.format(steps=steps))
+        t0 = time.time()
+        r = subprocess.run([exe, card, "-o", os.path.join(tmp, "out") + "/", "-f"],
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=600)
+        wall = time.time() - t0
+        if r.returncode != 0:
+            raise RuntimeError("host program failed: %s" % r.stderr[-2000:])
+        frames = len([f for f in os.listdir(os.path.join(tmp, "out")) if f.startswith("frame")])
+        total = float(re.search(r"Total run time :\s*([0-9.]+)", r.stdout).group(1))
+        write = float(re.search(r"Total time spent writing output :\s*([0-9.]+)", r.stdout).group(1))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {"workload": "go-or-grow 500x500 proliferation runcard through mass_hip, npc=1, FP64, "
+                        "a JSON frame every 100 steps",
+            "steps": steps, "frames": frames, "run_s": total, "write_s": write,
+            "process_wall_s": round(wall, 2)}
+
+
 def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warmup=200):
     """Surrogate corrector at LxL against the plain npc=2 step: a network
     trained briefly on frames of a 512^2 plain run (the full training is
@@ -339,6 +425,9 @@ def main():
     ap.add_argument("--skip-growth", action="store_true",
                     help="skip the go-or-grow leg (500^2 proliferation runcard)")
     ap.add_argument("--growth-steps", type=int, default=300)
+    ap.add_argument("--skip-host", action="store_true",
+                    help="skip the proliferation runcard through the host program")
+    ap.add_argument("--runcard-steps", type=int, default=2000)
     ap.add_argument("--fp64-preload-ms", type=float, default=80.0,
                     help="minimum untimed FP64 work before its timed steps (clock settling)")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -379,6 +468,10 @@ def main():
                                                                     args.growth_steps))
         extra["go_or_grow"] = growth_leg(lyo, args.growth_steps)
         log("go-or-grow: %s" % extra["go_or_grow"])
+    if not args.skip_host:
+        log("proliferation runcard through the host program: %d steps" % args.runcard_steps)
+        extra["host_runcard"] = runcard_leg(args.runcard_steps)
+        log("runcard: %s" % extra["host_runcard"])
 
     log("fp32 %dx%d npc=%d: configure" % (L, L, npc))
     m32 = make_run(lyo, L, lyo.FP32, npc, **sched)

@@ -137,11 +137,22 @@ def build_host(verbose=False):
     if not srcs:
         return None
     tmp = HOSTBIN + ".tmp"
-    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+    _run(["g++", "-std=c++17", "-O2", "-pthread", "-Wall", "-Wextra",
+          "-I" + os.path.join(ROOT, "include"),
           "-I" + HOST, "-o", tmp] + srcs +
          ["-L" + LIBDIR, "-lmass_lyo", "-Wl,-rpath,$ORIGIN/../lib"], verbose)
     os.replace(tmp, HOSTBIN)
     return HOSTBIN
+
+
+def build_checks(verbose=False):
+    """bin/format_check: the JSON writer against the number stream it
+    replaces (tests/native/format_check.cpp)."""
+    os.makedirs(BINDIR, exist_ok=True)
+    out = os.path.join(BINDIR, "format_check")
+    _run(["g++", "-std=c++17", "-O2", "-pthread", "-Wall", "-Wextra", "-I" + HOST, "-o",
+          out + ".tmp", os.path.join(NATIVE_TESTS, "format_check.cpp")], verbose)
+    os.replace(out + ".tmp", out)
 
 
 def build_asan_programs(verbose=False):
@@ -153,7 +164,7 @@ def build_asan_programs(verbose=False):
     link = ["-L" + ASAN_LIBDIR, "-lmass_lyo", "-Wl,-rpath,$ORIGIN/../../lib/asan"]
     inc = ["-I" + os.path.join(ROOT, "include")]
     srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
-    jobs = [([CLANGXX, "-std=c++17", "-Wall"] + san + inc + ["-I" + HOST, "-o",
+    jobs = [([CLANGXX, "-std=c++17", "-pthread", "-Wall"] + san + inc + ["-I" + HOST, "-o",
              ASAN_HOSTBIN + ".tmp"] + srcs + link, ASAN_HOSTBIN),
             # a C program, linked by the C++ driver so that operator new and
             # delete are the sanitizer's (their callers then appear in the
@@ -174,6 +185,7 @@ def build_all(verbose=False):
     build_lib(verbose)
     build_host(verbose)
     build_asan_programs(verbose)
+    build_checks(verbose)
     build_oracle(verbose)
 
 

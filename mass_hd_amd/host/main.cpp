@@ -42,14 +42,14 @@ static void zip_file(const std::string& archive, const std::string& file) {
 static void WriteFrame(unsigned t) {
   // src/write.cpp:28-61
   const std::string oname = inline_str(output_dir, "frame", t, ".json");
-  std::stringstream buffer;
+  piece_stream buffer;  // the whole frame first: a NaN leaves no file
   {
     oarchive ar(buffer, "frame", 1);
     model.ser->serialize_frame(ar);
     if (ar.bad_value()) throw error_msg("nan found while writing file.");
   }
   std::ofstream ofs(oname.c_str(), std::ios::out);
-  ofs << buffer.rdbuf();
+  buffer.pieces().write_to(ofs);
   ofs.close();
   if (compress) zip_file(oname + ".zip", oname);
   if (compress_full) zip_file(runname + ".zip", oname);

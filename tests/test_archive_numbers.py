@@ -1,0 +1,23 @@
+"""The host program's JSON writer (mass_hd_amd/host/serialization.hpp)
+formats numbers with std::to_chars and long lists in parallel chunks; the
+reference's archive streams each number through a std::stringstream
+(src/serialization.hpp:187-361).  tests/native/format_check.cpp holds the
+writer to the stream byte for byte: 14.8 million doubles (random bit
+patterns, values on and next to the sixth-digit rounding boundaries, exact
+ties, subnormals, signed zeros, infinities, NaNs) against printf("%.6g") and
+operator<<, integer and bool lists, lists on both sides of the parallel
+split, and a whole frame through the writer's piece stream against the same
+frame through a stringstream."""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CHECK = os.path.join(ROOT, "mass_hd_amd", "bin", "format_check")
+
+
+def test_writer_matches_stream_byte_for_byte():
+    assert os.path.exists(CHECK), "format_check is not built (run __graft_entry__.build())"
+    r = subprocess.run([CHECK], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert r.stdout.startswith("format ok")
