@@ -297,8 +297,10 @@ int mass_lyo_checkpoint_load(mass_lyo_t h, const void* buf, uint64_t bytes);
  * loop kernels move the reference's ~291 values per node and step instead of
  * the fused kernels' 26-108, and exist to measure that layout.  On walled
  * runs (bc 1-4) the fused kernels are the ghost-layer tile sweeps with Move
- * folded into the correctors' reads; go-or-grow runs always use the loops and
- * refuse 0.  The first call with 1 allocates the 41 extra planes.  sweep_ms of mass_lyo_time_steps
+ * folded into the correctors' reads; periodic go-or-grow and
+ * lyotropic-with-division runs fuse each sweep into one tile kernel, walled
+ * ones always use the loops and refuse 0.  The first call with 1 allocates
+ * the 41 extra planes.  sweep_ms of mass_lyo_time_steps
  * gets zeros in this mode.  Reference: Lyotropic::Step,
  * src/models/lyotropic.cpp:343-386,500-525. */
 int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on);

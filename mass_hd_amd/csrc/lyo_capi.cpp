@@ -800,7 +800,8 @@ int mass_lyo_set_growth(mass_lyo_t h, const mass_lyo_growth* g) {
   if (e != hipSuccess) return hip_fail(h, e, "go-or-grow buffers");
   h->growth = true;
   h->has_state = false;
-  h->launch.loops = 1;
+  // periodic runs: fused tile sweeps; walled runs: the node loops
+  h->launch.loops = h->prm.bc != 0 ? 1 : 0;
   return MASS_LYO_OK;
 }
 
@@ -809,8 +810,8 @@ int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on) {
   if (on != 0 && on != 1) return fail(h, MASS_LYO_EINVAL, "on must be 0 or 1");
   if (on && h->prm.precision != MASS_LYO_FP64)
     return fail(h, MASS_LYO_EINVAL, "the loop kernels run the FP64 path only");
-  if (!on && h->growth)
-    return fail(h, MASS_LYO_EINVAL, "go-or-grow runs the loop kernels only");
+  if (!on && h->growth && h->prm.bc != 0)
+    return fail(h, MASS_LYO_EINVAL, "walled go-or-grow runs the loop kernels only");
   if (on) {
     const hipError_t e = h->eng->alloc_loops();
     if (e != hipSuccess) return hip_fail(h, e, "loop kernel buffers");

@@ -27,6 +27,77 @@ struct GrowthDev {
   int growTogether;
 };
 
+// GoOrGrow::UpdateQuantitiesAtNode (go-or-grow.cpp:224-303) for one node:
+// the 23 derived values in the order of the loop planes (LoopPlane), from the
+// node's populations, the five-point values (centre, x+1, x-1, y+1, y-1) of
+// Qxx, Qyx and phi, and chi at the node.
+__device__ __forceinline__ void growth_quantities(const double (&fl)[9], double Qxx, double qxx_xp,
+                                                  double qxx_xm, double qxx_yp, double qxx_ym,
+                                                  double Qyx, double qyx_xp, double qyx_xm,
+                                                  double qyx_yp, double qyx_ym, double p,
+                                                  double p_xp, double p_xm, double p_yp,
+                                                  double p_ym, double chik,
+                                                  const Params<double>& P, const GrowthDev& g,
+                                                  double (&out)[kLoopPlanes]) {
+  using T = double;
+  const T nn = fl[0] + fl[1] + fl[2] + fl[3] + fl[4] + fl[5] + fl[6] + fl[7] + fl[8];
+  const T vx = (fl[1] - fl[2] + fl[5] - fl[6] - fl[7] + fl[8]) / nn;
+  const T vy = (fl[3] - fl[4] + fl[5] - fl[6] + fl[7] - fl[8]) / nn;
+
+  const T del2Qxx = lap5<T, true>(Qxx, qxx_xp, qxx_xm, qxx_yp, qxx_ym);
+  const T dxQxx = dcen<T, true>(qxx_xp, qxx_xm);
+  const T dyQxx = dcen<T, true>(qxx_yp, qxx_ym);
+  const T del2Qyx = lap5<T, true>(Qyx, qyx_xp, qyx_xm, qyx_yp, qyx_ym);
+  const T dxQyx = dcen<T, true>(qyx_xp, qyx_xm);
+  const T dyQyx = dcen<T, true>(qyx_yp, qyx_ym);
+  const T del2p = lap5<T, true>(p, p_xp, p_xm, p_yp, p_ym);
+  const T dxPhi = dcen<T, true>(p_xp, p_xm);
+  const T dyPhi = dcen<T, true>(p_yp, p_ym);
+
+  const T q2 = Qxx * Qxx + Qyx * Qyx;
+  const T term = g.Snem - q2;
+  const T Hxx = T(2) * P.CC * term * Qxx + P.LL * del2Qxx;
+  const T Hyx = T(2) * P.CC * term * Qyx + P.LL * del2Qyx;
+  const T dp_critical = p - g.phi_critical;
+  const T f_compress = dp_critical > 0 ? T(.5) * g.B * dp_critical * dp_critical : T(0.);
+  const T mu_compress = dp_critical > 0 ? g.B * dp_critical : T(0.);
+  const T mu = P.AA * p * (T(1) - p) * (T(1) - T(2) * p) + mu_compress - P.KK * del2p;
+
+  const T sigmaB = T(.5) * P.AA * p * p * (T(1) - p) * (T(1) - p) + f_compress +
+                   T(.5) * P.CC * term * term - mu * p;
+  const T ap = P.zeta * p * (T(1) - chik);
+  const T sigmaF = T(2) * P.xi * ((Qxx * Qxx - T(1)) * Hxx + Qxx * Qyx * Hyx) - ap * Qxx +
+                   T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) +
+                   P.LL * (dyQxx * dyQxx + dyQyx * dyQyx - dxQxx * dxQxx - dxQyx * dxQyx);
+  const T sigmaS = T(2) * P.xi * (Qyx * Qxx * Hxx + (Qyx * Qyx - T(1)) * Hyx) - ap * Qyx -
+                   P.KK * dxPhi * dyPhi - T(2) * P.LL * (dxQxx * dyQxx + dxQyx * dyQyx);
+  const T sigmaA = T(2) * (Qxx * Hyx - Qyx * Hxx);
+
+  out[LP_N] = nn;
+  out[LP_UX] = vx;
+  out[LP_UY] = vy;
+  out[LP_UXP] = vx * p;
+  out[LP_UYP] = vy * p;
+  out[LP_HXX] = Hxx;
+  out[LP_HYX] = Hyx;
+  out[LP_MU] = mu;
+  out[LP_DXQXX] = dxQxx;
+  out[LP_DXQYX] = dxQyx;
+  out[LP_DYQXX] = dyQxx;
+  out[LP_DYQYX] = dyQyx;
+  out[LP_SXX] = sigmaF + sigmaB + P.zetaI * (P.conc - p);
+  out[LP_SYY] = -sigmaF + sigmaB + P.zetaI * (P.conc - p);
+  out[LP_SXY] = sigmaS + sigmaA;
+  out[LP_SYX] = sigmaS - sigmaA;
+  out[LP_BULK] = sigmaB + P.zetaI * (P.conc - p);
+  out[LP_ELXX] = sigmaF - T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) + ap * Qxx;
+  out[LP_ELYX] = sigmaS + P.KK * dxPhi * dyPhi + ap * Qyx;
+  out[LP_PFXX] = T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi);
+  out[LP_PFYX] = -P.KK * dxPhi * dyPhi;
+  out[LP_ACTXX] = -ap * Qxx;
+  out[LP_ACTYX] = -ap * Qyx;
+}
+
 // GoOrGrow::UpdateQuantitiesAtNode (go-or-grow.cpp:224-303): the 23 derived
 // planes of one domain node.
 template <class G = LayerGrid>
@@ -43,66 +114,41 @@ growth_uq_kernel(const double* __restrict__ f, const double* __restrict__ qxx,
   T fl[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) fl[v] = f[v * NP + k];
-  const T Qxx = qxx[k], Qyx = qyx[k], p = phi[k];
-  const T nn = fl[0] + fl[1] + fl[2] + fl[3] + fl[4] + fl[5] + fl[6] + fl[7] + fl[8];
-  const T vx = (fl[1] - fl[2] + fl[5] - fl[6] - fl[7] + fl[8]) / nn;
-  const T vy = (fl[3] - fl[4] + fl[5] - fl[6] + fl[7] - fl[8]) / nn;
-
-  const T del2Qxx = lap5<T, true>(Qxx, qxx[n.kxp], qxx[n.kxm], qxx[n.kyp], qxx[n.kym]);
-  const T dxQxx = dcen<T, true>(qxx[n.kxp], qxx[n.kxm]);
-  const T dyQxx = dcen<T, true>(qxx[n.kyp], qxx[n.kym]);
-  const T del2Qyx = lap5<T, true>(Qyx, qyx[n.kxp], qyx[n.kxm], qyx[n.kyp], qyx[n.kym]);
-  const T dxQyx = dcen<T, true>(qyx[n.kxp], qyx[n.kxm]);
-  const T dyQyx = dcen<T, true>(qyx[n.kyp], qyx[n.kym]);
-  const T del2p = lap5<T, true>(p, phi[n.kxp], phi[n.kxm], phi[n.kyp], phi[n.kym]);
-  const T dxPhi = dcen<T, true>(phi[n.kxp], phi[n.kxm]);
-  const T dyPhi = dcen<T, true>(phi[n.kyp], phi[n.kym]);
-
-  const T q2 = Qxx * Qxx + Qyx * Qyx;
-  const T term = g.Snem - q2;
-  const T Hxx = T(2) * P.CC * term * Qxx + P.LL * del2Qxx;
-  const T Hyx = T(2) * P.CC * term * Qyx + P.LL * del2Qyx;
-  const T dp_critical = p - g.phi_critical;
-  const T f_compress = dp_critical > 0 ? T(.5) * g.B * dp_critical * dp_critical : T(0.);
-  const T mu_compress = dp_critical > 0 ? g.B * dp_critical : T(0.);
-  const T mu = P.AA * p * (T(1) - p) * (T(1) - T(2) * p) + mu_compress - P.KK * del2p;
-
-  const T sigmaB = T(.5) * P.AA * p * p * (T(1) - p) * (T(1) - p) + f_compress +
-                   T(.5) * P.CC * term * term - mu * p;
-  const T ap = P.zeta * p * (T(1) - chi[k]);
-  const T sigmaF = T(2) * P.xi * ((Qxx * Qxx - T(1)) * Hxx + Qxx * Qyx * Hyx) - ap * Qxx +
-                   T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) +
-                   P.LL * (dyQxx * dyQxx + dyQyx * dyQyx - dxQxx * dxQxx - dxQyx * dxQyx);
-  const T sigmaS = T(2) * P.xi * (Qyx * Qxx * Hxx + (Qyx * Qyx - T(1)) * Hyx) - ap * Qyx -
-                   P.KK * dxPhi * dyPhi - T(2) * P.LL * (dxQxx * dyQxx + dxQyx * dyQyx);
-  const T sigmaA = T(2) * (Qxx * Hyx - Qyx * Hxx);
-
-  const T out[kLoopPlanes] = {
-      nn,
-      vx,
-      vy,
-      vx * p,
-      vy * p,
-      Hxx,
-      Hyx,
-      mu,
-      dxQxx,
-      dxQyx,
-      dyQxx,
-      dyQyx,
-      sigmaF + sigmaB + P.zetaI * (P.conc - p),
-      -sigmaF + sigmaB + P.zetaI * (P.conc - p),
-      sigmaS + sigmaA,
-      sigmaS - sigmaA,
-      sigmaB + P.zetaI * (P.conc - p),
-      sigmaF - T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi) + ap * Qxx,
-      sigmaS + P.KK * dxPhi * dyPhi + ap * Qyx,
-      T(.5) * P.KK * (dyPhi * dyPhi - dxPhi * dxPhi),
-      -P.KK * dxPhi * dyPhi,
-      -ap * Qxx,
-      -ap * Qyx};
+  T out[kLoopPlanes];
+  growth_quantities(fl, qxx[k], qxx[n.kxp], qxx[n.kxm], qxx[n.kyp], qxx[n.kym], qyx[k],
+                    qyx[n.kxp], qyx[n.kxm], qyx[n.kyp], qyx[n.kym], phi[k], phi[n.kxp],
+                    phi[n.kxm], phi[n.kyp], phi[n.kym], chi[k], P, g, out);
 #pragma unroll
   for (int i = 0; i < kLoopPlanes; ++i) D[(size_t)i * NP + k] = out[i];
+}
+
+// The phenotype density's increment Dm of GoOrGrow::UpdateFields
+// (go-or-grow.cpp:324-344) at one node: chi, phi and MU at the node and its
+// four neighbours (x+1, x-1, y+1, y-1), m and the velocity component along
+// the axis at the neighbours, and the growth source R.
+__device__ __forceinline__ double growth_dm(double chi_eff, double chi_px, double chi_mx,
+                                            double chi_py, double chi_my, double p, double p_xp,
+                                            double p_xm, double p_yp, double p_ym, double m_xp,
+                                            double m_xm, double m_yp, double m_ym, double ux_xp,
+                                            double ux_xm, double uy_yp, double uy_ym, double mu,
+                                            double mu_xp, double mu_xm, double mu_yp,
+                                            double mu_ym, double R, const Params<double>& P,
+                                            const GrowthDev& g) {
+  using T = double;
+  const T mFlux = T(.5) * (ux_xp * m_xp - ux_xm * m_xm) + T(.5) * (uy_yp * m_yp - uy_ym * m_ym);
+  const T diffusiveMFlux = T(.5) * (chi_px + chi_eff) * (mu_xp - mu) -
+                           T(.5) * (chi_eff + chi_mx) * (mu - mu_xm) +
+                           T(.5) * (chi_py + chi_eff) * (mu_yp - mu) -
+                           T(.5) * (chi_eff + chi_my) * (mu - mu_ym);
+  const T phenotypeDiffusion = T(.5) * (p_xp + p) * (chi_px - chi_eff) -
+                               T(.5) * (p + p_xm) * (chi_eff - chi_mx) +
+                               T(.5) * (p_yp + p) * (chi_py - chi_eff) -
+                               T(.5) * (p + p_ym) * (chi_eff - chi_my);
+  const T dVdChi = T(2) * g.Achi * chi_eff * (T(1) - chi_eff) * (T(1) - T(2) * chi_eff) +
+                   g.Ochi * (p - g.phiswitch);
+  const T Sswitch = -p * dVdChi;
+  const T mGrowth = g.growTogether ? chi_eff * R : R;
+  return P.GammaP * diffusiveMFlux + g.Dchi * phenotypeDiffusion - mFlux + Sswitch + mGrowth;
 }
 
 // Inputs and outputs of one go-or-grow UpdateFields loop: Lyotropic's
@@ -149,24 +195,11 @@ __global__ void __launch_bounds__(256) growth_uf_kernel(GrowthUF a, Params<doubl
   const T chi_eff = DIV ? T(0) : chi[k];
   const T R = DIV ? phi[k] * growth_rate : chi_eff * phi[k] * growth_rate;
   T Dm = T(0);
-  if constexpr (!DIV) {
-    const T mFlux = T(.5) * (ux[n.kxp] * m[n.kxp] - ux[n.kxm] * m[n.kxm]) +
-                    T(.5) * (uy[n.kyp] * m[n.kyp] - uy[n.kym] * m[n.kym]);
-    const T chi_px = chi[n.kxp], chi_mx = chi[n.kxm], chi_py = chi[n.kyp], chi_my = chi[n.kym];
-    const T diffusiveMFlux = T(.5) * (chi_px + chi_eff) * (MU[n.kxp] - MU[k]) -
-                             T(.5) * (chi_eff + chi_mx) * (MU[k] - MU[n.kxm]) +
-                             T(.5) * (chi_py + chi_eff) * (MU[n.kyp] - MU[k]) -
-                             T(.5) * (chi_eff + chi_my) * (MU[k] - MU[n.kym]);
-    const T phenotypeDiffusion = T(.5) * (phi[n.kxp] + phi[k]) * (chi_px - chi_eff) -
-                                 T(.5) * (phi[k] + phi[n.kxm]) * (chi_eff - chi_mx) +
-                                 T(.5) * (phi[n.kyp] + phi[k]) * (chi_py - chi_eff) -
-                                 T(.5) * (phi[k] + phi[n.kym]) * (chi_eff - chi_my);
-    const T dVdChi = T(2) * g.Achi * chi_eff * (T(1) - chi_eff) * (T(1) - T(2) * chi_eff) +
-                     g.Ochi * (phi[k] - g.phiswitch);
-    const T Sswitch = -phi[k] * dVdChi;
-    const T mGrowth = g.growTogether ? chi_eff * R : R;
-    Dm = P.GammaP * diffusiveMFlux + g.Dchi * phenotypeDiffusion - mFlux + Sswitch + mGrowth;
-  }
+  if constexpr (!DIV)
+    Dm = growth_dm(chi_eff, chi[n.kxp], chi[n.kxm], chi[n.kyp], chi[n.kym], phi[k], phi[n.kxp],
+                   phi[n.kxm], phi[n.kyp], phi[n.kym], m[n.kxp], m[n.kxm], m[n.kyp], m[n.kym],
+                   ux[n.kxp], ux[n.kxm], uy[n.kyp], uy[n.kym], MU[k], MU[n.kxp], MU[n.kxm],
+                   MU[n.kyp], MU[n.kym], R, P, g);
 
   // Lyotropic::UpdateFieldsAtNode (lyotropic.cpp:249-341)
   CenterVals<T> c;

@@ -479,6 +479,220 @@ wall_sweep_kernel(WallSweep a, Params<double> P, WallGeo g) {
   }
 }
 
+// Inputs and outputs of one go-or-grow sweep on a periodic grid: the
+// Lyotropic planes of a wall sweep plus the phenotype density and the masks.
+struct GrowthSweep {
+  WallSweep w;
+  const double* m;    // go-or-grow: m of the inputs
+  const double* chi;  // go-or-grow: chi = m/phi of the inputs
+  const double* mn;   // corrector: m + Dm/2 of the predictor
+  const unsigned char* division;
+  const unsigned char* death;
+  double* m_out;
+  double* mn_out;     // predictor
+  double* chi_out;    // chi = m/phi of the outputs
+  const double* countphi;  // conserve_phi: device sum of phi of the inputs
+  double totalphi;
+  int conserve;
+};
+
+// A go-or-grow (DIV: lyotropic-with-division) sweep on a periodic grid as
+// one tile kernel: the tile's Q and phi plus two nodes of halo staged in LDS
+// with periodic wrap on both axes, UpdateQuantities (go-or-grow's, with chi
+// at the node) on the tile plus one node into LDS, UpdateFields with the
+// phenotype update on the tile.  chi and m at the neighbours are read from
+// memory (chi is m/phi of the inputs: UpdatePhenotypeQuantities follows every
+// UpdateQuantities and UpdateFields of the reference, go-or-grow.cpp:201-209,
+// 305-362, so the sweep before wrote it with its outputs).  The node
+// arithmetic is the loop kernels' device functions (growth_quantities,
+// growth_dm, update_fields), so the results are theirs bit for bit.  Modes
+// and pulls as wall_sweep_kernel's; kSweepDerive stores the 23 planes.
+template <int MODE, bool PULL_F = false, bool PULL_FN = false, bool DIV = false>
+__global__ void __launch_bounds__(kWallThreads) __attribute__((amdgpu_waves_per_eu(4)))
+growth_sweep_kernel(GrowthSweep a, Params<double> P, GrowthDev gd, WallGeo g) {
+  using T = double;
+  constexpr int TX = kWallTX, TY = kWallTY;
+  constexpr int PW = TY + 4, PN = (TX + 4) * PW;  // staged Q, phi: tile + 2
+  constexpr int DW = TY + 2, DN = (TX + 2) * DW;  // derived: tile + 1 (= threads)
+  __shared__ T sq[3][PN];
+  __shared__ T sd[9][DN];  // ux uy uxp uyp mu sxx syy sxy syx
+  int x0, y0;
+  {
+    const int nb = gridDim.x * gridDim.y, b = blockIdx.x + gridDim.x * blockIdx.y;
+    const int q = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
+    const int t = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + idx;
+    x0 = (t / gridDim.x) * TX;
+    y0 = (t % gridDim.x) * TY;
+  }
+  const int tid = threadIdx.x;
+  const int NP = g.NP;
+  auto wrap = [](int v, int L) { return v < 0 ? v + L : (v >= L ? v - L : v); };
+  auto node = [&](int x, int y) { return (y + 1) + g.W * (x + 1); };
+  const WallSweep& w = a.w;
+
+  for (int i = tid; i < PN; i += kWallThreads) {
+    const int r = i / PW, c = i - r * PW;
+    const int k = node(wall_axis(x0 - 2 + r, g.LX, true), wall_axis(y0 - 2 + c, g.LY, true));
+    sq[0][i] = w.qxx[k];
+    sq[1][i] = w.qyx[k];
+    sq[2][i] = w.phi[k];
+  }
+  __syncthreads();
+
+  // this thread's node of the tile + 1 (periodic image of the position)
+  const int r = tid / DW, c = tid - r * DW;
+  const bool tile = r >= 1 && r <= TX && c >= 1 && c <= TY;
+  const int xv = x0 - 1 + r, yv = y0 - 1 + c;
+  const int xw = wall_axis(xv, g.LX, true), yw = wall_axis(yv, g.LY, true);
+  const int k = node(xw, yw);
+  const int q = (r + 1) * PW + (c + 1);
+  auto pulled = [&](const T* pl, int v, bool pull) {
+    if (!pull) return pl[v * NP + k];
+    return pl[v * NP + node(wrap(xw - kWallCX[v], g.LX), wrap(yw - kWallCY[v], g.LY))];
+  };
+  T fl[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) fl[v] = pulled(w.f, v, PULL_F);
+  CenterVals<T> cv{};
+  if constexpr (DIV) {
+    NbrVals<T> nv;
+    DiagVals<T> dg;
+    update_quantities<T, true, true>(
+        fl, sq[0][q], sq[0][q + PW], sq[0][q - PW], ynb<T, true>(sq[0][q + 1], sq[0][q - 1]),
+        sq[1][q], sq[1][q + PW], sq[1][q - PW], ynb<T, true>(sq[1][q + 1], sq[1][q - 1]),
+        sq[2][q], sq[2][q + PW], sq[2][q - PW], ynb<T, true>(sq[2][q + 1], sq[2][q - 1]), P, nv,
+        cv, &dg);
+    if constexpr (MODE == kSweepDerive) {
+      if (tile && xv < g.LX && yv < g.LY) {
+        const T out[kLoopPlanes] = {cv.nn,    nv.ux,    nv.uy,     nv.uxp,    nv.uyp,  cv.Hxx,
+                                    cv.Hyx,   nv.mu,    cv.dxQxx,  cv.dxQyx,  cv.dyQxx, cv.dyQyx,
+                                    nv.sxx,   nv.syy,   nv.sxy,    nv.syx,    dg.bulk, dg.el_xx,
+                                    dg.el_yx, dg.pf_xx, dg.pf_yx, dg.act_xx, dg.act_yx};
+#pragma unroll
+        for (int pl = 0; pl < kLoopPlanes; ++pl) w.D[(size_t)pl * NP + k] = out[pl];
+      }
+      return;
+    }
+    const T vals[9] = {nv.ux, nv.uy, nv.uxp, nv.uyp, nv.mu, nv.sxx, nv.syy, nv.sxy, nv.syx};
+#pragma unroll
+    for (int pl = 0; pl < 9; ++pl) sd[pl][tid] = vals[pl];
+  } else {
+    T out[kLoopPlanes];
+    growth_quantities(fl, sq[0][q], sq[0][q + PW], sq[0][q - PW], sq[0][q + 1], sq[0][q - 1],
+                      sq[1][q], sq[1][q + PW], sq[1][q - PW], sq[1][q + 1], sq[1][q - 1],
+                      sq[2][q], sq[2][q + PW], sq[2][q - PW], sq[2][q + 1], sq[2][q - 1],
+                      a.chi[k], P, gd, out);
+    if constexpr (MODE == kSweepDerive) {
+      if (tile && xv < g.LX && yv < g.LY) {
+#pragma unroll
+        for (int pl = 0; pl < kLoopPlanes; ++pl) w.D[(size_t)pl * NP + k] = out[pl];
+      }
+      return;
+    }
+    cv.nn = out[LP_N];
+    cv.vx = out[LP_UX];
+    cv.vy = out[LP_UY];
+    cv.Hxx = out[LP_HXX];
+    cv.Hyx = out[LP_HYX];
+    cv.dxQxx = out[LP_DXQXX];
+    cv.dyQxx = out[LP_DYQXX];
+    cv.dxQyx = out[LP_DXQYX];
+    cv.dyQyx = out[LP_DYQYX];
+    const T vals[9] = {out[LP_UX],  out[LP_UY],  out[LP_UXP], out[LP_UYP], out[LP_MU],
+                       out[LP_SXX], out[LP_SYY], out[LP_SXY], out[LP_SYX]};
+#pragma unroll
+    for (int pl = 0; pl < 9; ++pl) sd[pl][tid] = vals[pl];
+  }
+  __syncthreads();
+
+  // UpdateFields (lyotropic.cpp:249-341, go-or-grow.cpp:322-362) on the tile
+  if (!tile || xv >= g.LX || yv >= g.LY) return;
+  const int p = tid;
+  cv.dn = T(0);
+  UFStencil<T, true> st;
+  st.mu_c = sd[4][p];
+  st.ux_xp = sd[0][p + DW];
+  st.ux_xm = sd[0][p - DW];
+  st.uy_xp = sd[1][p + DW];
+  st.uy_xm = sd[1][p - DW];
+  st.uxp_xp = sd[2][p + DW];
+  st.uxp_xm = sd[2][p - DW];
+  st.sxx_xp = sd[5][p + DW];
+  st.sxx_xm = sd[5][p - DW];
+  st.syx_xp = sd[8][p + DW];
+  st.syx_xm = sd[8][p - DW];
+  st.mu_xp = sd[4][p + DW];
+  st.mu_xm = sd[4][p - DW];
+  st.ux_y = ynb<T, true>(sd[0][p + 1], sd[0][p - 1]);
+  st.uy_y = ynb<T, true>(sd[1][p + 1], sd[1][p - 1]);
+  st.uyp_y = ynb<T, true>(sd[3][p + 1], sd[3][p - 1]);
+  st.sxy_y = ynb<T, true>(sd[7][p + 1], sd[7][p - 1]);
+  st.syy_y = ynb<T, true>(sd[6][p + 1], sd[6][p - 1]);
+  st.mu_y = ynb<T, true>(sd[4][p + 1], sd[4][p - 1]);
+  const T Qxx = sq[0][q], Qyx = sq[1][q], ph = sq[2][q];
+
+  // the growth source and the phenotype increment (go-or-grow.cpp:324-344;
+  // lyotropic-with-division.cpp:110-126 with DIV)
+  const T division_m = a.division[k] ? T(1.) : T(0.);
+  const T death_m = a.death[k] ? T(1.) : T(0.);
+  const T growth_rate = gd.alpha * division_m - gd.beta * death_m;
+  T R, Dm = T(0);
+  if constexpr (DIV) {
+    R = ph * growth_rate;
+  } else {
+    const int kxp = node(wrap(xw + 1, g.LX), yw), kxm = node(wrap(xw - 1, g.LX), yw);
+    const int kyp = node(xw, wrap(yw + 1, g.LY)), kym = node(xw, wrap(yw - 1, g.LY));
+    const T chi_eff = a.chi[k];
+    R = chi_eff * ph * growth_rate;
+    Dm = growth_dm(chi_eff, a.chi[kxp], a.chi[kxm], a.chi[kyp], a.chi[kym], ph, sq[2][q + PW],
+                   sq[2][q - PW], sq[2][q + 1], sq[2][q - 1], a.m[kxp], a.m[kxm], a.m[kyp],
+                   a.m[kym], sd[0][p + DW], sd[0][p - DW], sd[1][p + 1], sd[1][p - 1], sd[4][p],
+                   sd[4][p + DW], sd[4][p - DW], sd[4][p + 1], sd[4][p - 1], R, P, gd);
+  }
+
+  T Dxx, Dyx, Dp, om[9];
+  update_fields<T, true>(cv, Qxx, Qyx, ph, fl, st, P, Dxx, Dyx, Dp, om);
+  Dp = Dp - (a.conserve ? (*a.countphi - a.totalphi) / T(g.LX * g.LY) : T(0));
+  T phi_new;
+  if constexpr (MODE == kSweepPredictor) {
+    w.qnxx_out[k] = Qxx + T(.5) * Dxx;
+    w.qnyx_out[k] = Qyx + T(.5) * Dyx;
+    w.phn_out[k] = ph + T(.5) * Dp;
+    w.qxx_out[k] = Qxx + Dxx;
+    w.qyx_out[k] = Qyx + Dyx;
+    phi_new = ph + Dp;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      w.fn_out[v * NP + k] = fl[v] + T(.5) * om[v];
+      w.f_out[v * NP + k] = fl[v] + om[v];
+    }
+  } else {
+    w.qxx_out[k] = w.qnxx[k] + T(.5) * Dxx;
+    w.qyx_out[k] = w.qnyx[k] + T(.5) * Dyx;
+    phi_new = w.phn[k] + T(.5) * Dp;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) w.f_out[v * NP + k] = pulled(w.fn, v, PULL_FN) + T(.5) * om[v];
+  }
+  phi_new += R;
+  w.phi_out[k] = phi_new;
+  if constexpr (DIV) return;
+  T mt;
+  if constexpr (MODE == kSweepPredictor) {
+    const T mk = a.m[k];
+    a.mn_out[k] = mk + T(.5) * Dm;
+    mt = mk + Dm;
+  } else {
+    mt = a.mn[k] + T(.5) * Dm;
+  }
+  const T upper = phi_new > 0 ? phi_new : T(0.);
+  if (mt < 0)
+    mt = 0;
+  else if (mt > upper)
+    mt = upper;
+  a.m_out[k] = mt;
+  a.chi_out[k] = phi_new > 1e-12 ? mt / phi_new : 0.;
+}
+
 // Move over every node of the torus (lyotropic.cpp:371-386), as a pull:
 // node p takes direction v from p - c_v, the node whose push lands on p.
 template <class T>
@@ -752,7 +966,9 @@ class WallEngine final : public Engine {
   // grids take longer to issue than to run.  The graph holds kernels only.
   hipError_t steps(int nsteps, const LaunchCfg& L, hipStream_t s, SweepTimer* tm) override {
     (void)tm;
-    loops_ = growth_ || L.loops != 0;
+    // periodic go-or-grow runs as fused sweeps (growth_step); on walled
+    // grids it keeps the node loops
+    loops_ = L.loops != 0 || (growth_ && bc_ != 0);
     for (int st = 0; st < nsteps; ++st) {
       if (growth_) {
         const hipError_t e = set_masks(s);
@@ -822,6 +1038,108 @@ class WallEngine final : public Engine {
     d_stale_ = true;  // the frame fields are recomputed from the last corrector's inputs
     return hipGetLastError();
   }
+  // go-or-grow (or lyotropic-with-division) on a periodic grid as fused
+  // sweeps: the predictor writes the post-collision planes, every corrector
+  // pulls the streamed fn from them (the first one f too), so no Move runs;
+  // conserve_phi sums phi before each sweep
+  hipError_t growth_step(hipStream_t s) {
+    const dim3 grid = sweep_grid();
+    const bool div_only = gp_.division_only != 0;
+    auto countphi = [&]() {
+      if (!conserve_) return;
+      const int N = LX_ * LY_;
+      int nparts = (N + 4095) / 4096;
+      if (nparts > kGrowthParts) nparts = kGrowthParts;
+      const int per = (N + nparts - 1) / nparts;
+      hipLaunchKernelGGL((wall_partial_sums_kernel<T>), dim3(nparts), dim3(256), 0, s, geo(),
+                         (const T*)F_[cur_], (const T*)phi_[pc_], per, part_,
+                         part_ + kGrowthParts);
+      hipLaunchKernelGGL(final_sums_kernel, dim3(1), dim3(256), 0, s, part_,
+                         part_ + kGrowthParts, nparts, 0.0, part_ + 2 * kGrowthParts);
+    };
+    auto common = [&](GrowthSweep& a) {
+      a.division = div_;
+      a.death = death_;
+      a.countphi = part_ + 2 * kGrowthParts + 1;
+      a.totalphi = totalphi_;
+      a.conserve = conserve_ ? 1 : 0;
+      if (!div_only) {
+        a.m = m_[mc_];
+        a.chi = chi_;
+        a.m_out = m_[mc_ ^ 1];
+        a.chi_out = chi2_;
+      }
+    };
+    auto advance = [&]() {
+      swap_q();
+      pc_ ^= 1;
+      if (!div_only) {
+        mc_ ^= 1;
+        swap_chi();
+      }
+    };
+    countphi();
+    GrowthSweep a{};
+    common(a);
+    a.w.f = F_[cur_];
+    a.w.qxx = qxx_;
+    a.w.qyx = qyx_;
+    a.w.phi = phi_[pc_];
+    a.w.f_out = F_[cur_ ^ 1];
+    a.w.fn_out = Fn_[cur_ ^ 1];
+    a.w.qxx_out = qxx2_;
+    a.w.qyx_out = qyx2_;
+    a.w.qnxx_out = qnxx_;
+    a.w.qnyx_out = qnyx_;
+    a.w.phi_out = phi_[pc_ ^ 1];
+    a.w.phn_out = phn_;
+    a.mn_out = div_only ? nullptr : mn_;
+    if (div_only)
+      hipLaunchKernelGGL((growth_sweep_kernel<kSweepPredictor, false, false, true>), grid,
+                         dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+    else
+      hipLaunchKernelGGL((growth_sweep_kernel<kSweepPredictor>), grid, dim3(kWallThreads), 0, s,
+                         a, P_, gd_, geo());
+    advance();
+    int fc = cur_ ^ 1;  // the corrector's f input
+    for (int c = 1; c <= npc_; ++c) {
+      countphi();
+      GrowthSweep b{};
+      common(b);
+      b.w.f = F_[fc];
+      b.w.qxx = qxx_;
+      b.w.qyx = qyx_;
+      b.w.phi = phi_[pc_];
+      b.w.fn = Fn_[cur_ ^ 1];
+      b.w.qnxx = qnxx_;
+      b.w.qnyx = qnyx_;
+      b.w.phn = phn_;
+      b.w.f_out = F_[fc ^ 1];
+      b.w.qxx_out = qxx2_;
+      b.w.qyx_out = qyx2_;
+      b.w.phi_out = phi_[pc_ ^ 1];
+      b.mn = div_only ? nullptr : mn_;
+      if (div_only) {
+        if (c == 1)
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepCorrector, true, true, true>), grid,
+                             dim3(kWallThreads), 0, s, b, P_, gd_, geo());
+        else
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepCorrector, false, true, true>), grid,
+                             dim3(kWallThreads), 0, s, b, P_, gd_, geo());
+      } else if (c == 1) {
+        hipLaunchKernelGGL((growth_sweep_kernel<kSweepCorrector, true, true>), grid,
+                           dim3(kWallThreads), 0, s, b, P_, gd_, geo());
+      } else {
+        hipLaunchKernelGGL((growth_sweep_kernel<kSweepCorrector, false, true>), grid,
+                           dim3(kWallThreads), 0, s, b, P_, gd_, geo());
+      }
+      advance();
+      fc ^= 1;
+    }
+    cur_ = fc;
+    d_stale_ = true;  // the frame fields are recomputed from the last corrector's inputs
+    return hipGetLastError();
+  }
   dim3 sweep_grid() const {
     return dim3((unsigned)((LY_ + kWallTY - 1) / kWallTY), (unsigned)((LX_ + kWallTX - 1) / kWallTX));
   }
@@ -829,6 +1147,35 @@ class WallEngine final : public Engine {
   // inputs (f, Q and phi before the last corrector, left intact by it)
   hipError_t materialize(hipStream_t s) {
     if (!d_stale_) return hipSuccess;
+    if (growth_) {
+      // the last corrector's inputs: its f (the streamed post-collision
+      // planes with npc = 1), Q, phi, m and chi, all left intact by it
+      GrowthSweep a{};
+      a.w.f = F_[cur_ ^ 1];
+      a.w.qxx = qxx2_;
+      a.w.qyx = qyx2_;
+      a.w.phi = phi_[pc_ ^ 1];
+      a.w.D = D_;
+      if (gp_.division_only) {
+        if (npc_ == 1)
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, true, false, true>), sweep_grid(),
+                             dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+        else
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, false, false, true>),
+                             sweep_grid(), dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+      } else {
+        a.m = m_[mc_ ^ 1];
+        a.chi = chi2_;
+        if (npc_ == 1)
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, true>), sweep_grid(),
+                             dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+        else
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, false>), sweep_grid(),
+                             dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+      }
+      d_stale_ = false;
+      return hipGetLastError();
+    }
     WallSweep a{};
     a.f = F_[cur_ ^ 1];  // with npc = 1: the post-collision planes, streamed as read
     a.qxx = qxx2_;
@@ -855,7 +1202,7 @@ class WallEngine final : public Engine {
   }
 
   hipError_t step_body(hipStream_t s) {
-    if (!loops_) return fused_step(s);
+    if (!loops_) return growth_ ? growth_step(s) : fused_step(s);
     const int bs = 256, nbt = (TP_ + bs - 1) / bs;
     {
       fields_bc(s);

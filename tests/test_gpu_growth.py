@@ -118,6 +118,31 @@ def test_growth_512_bounds_and_repeats(require_gpu):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("division_only", [0, 1])
+@pytest.mark.parametrize("npc", [1, 2])
+def test_growth_fused_sweeps_equal_loops(require_gpu, npc, division_only):
+    """Periodic go-or-grow (and lyotropic-with-division) as fused tile sweeps
+    against the reference's node loops on the same run: bitwise every state
+    and frame field, m, chi and the masks, on a grid of partial tiles
+    (130 x 70 against 10 x 62 tiles), over steps with mask redraws."""
+    print("go-or-grow fused vs loops npc=%d division_only=%d" % (npc, division_only), flush=True)
+    g = dict(GROWTH, relax_steps=10, division_only=division_only)
+    runs = []
+    for loops in (False, True):
+        a = lyo.Lyotropic(130, 70, npc=npc, precision=lyo.FP64, **PHYS)
+        a.set_growth(**g)
+        a.set_loop_kernels(loops)
+        a.configure(**dict(INIT, radius=25.0, level=35.0))
+        a.step(7)
+        names = (["ff", "QQxx", "QQyx", "phi"] + ([] if division_only else ["m", "chi"]) +
+                 DERIVED + ["division_mask", "death_mask"])
+        runs.append({n: a.get(n) for n in names})
+        a.close()
+    for n, x in runs[0].items():
+        assert np.array_equal(x, runs[1][n]), n
+    assert runs[0]["division_mask"].any()
+
+
 def test_growth_refusals(require_gpu):
     f32 = lyo.Lyotropic(16, 16, npc=1, precision=lyo.FP32, **PHYS)
     with pytest.raises(lyo.MassLyoError):
@@ -131,7 +156,12 @@ def test_growth_refusals(require_gpu):
         a.set_growth(**dict(GROWTH, chi_noise=-1.0))
     a.set_growth(**GROWTH)
     a.configure(**INIT)
-    for call in (a.diagnostics, a.checkpoint, lambda: a.set_loop_kernels(False)):
+    a.set_loop_kernels(False)  # periodic: the fused sweeps (the default)
+    w = lyo.Lyotropic(16, 16, npc=1, precision=lyo.FP64, bc=2, **PHYS)
+    w.set_growth(**GROWTH)
+    with pytest.raises(lyo.MassLyoError):
+        w.set_loop_kernels(False)  # walled go-or-grow runs the node loops only
+    for call in (a.diagnostics, a.checkpoint):
         with pytest.raises(lyo.MassLyoError):
             call()
     a.step(2)
