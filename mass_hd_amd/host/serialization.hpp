@@ -30,9 +30,19 @@
 #include <algorithm>
 #include <array>
 #include <charconv>
+#include <condition_variable>
+#include <functional>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
 #include <ostream>
 #include <sstream>
 #include <string>
+#include <string_view>
 #include <thread>
 #include <type_traits>
 #include <utility>
@@ -74,12 +84,141 @@ namespace archive_detail {
 
 // one value as operator<< of a default std::ostream writes it; returns
 // whether it is a NaN
-inline bool put(std::string& s, double v) {
-  char b[40];
-  const auto r = std::to_chars(b, b + sizeof b, v, std::chars_format::general, 6);
-  s.append(b, r.ptr);
-  return v != v;
+// 10^k for |k| <= 300, each the double nearest the exact power (strtod
+// rounds correctly)
+struct Pow10Table {
+  static constexpr int kSpan = 300;
+  double p[2 * kSpan + 1];
+  Pow10Table() {
+    char b[16];
+    for (int k = -kSpan; k <= kSpan; ++k) {
+      std::snprintf(b, sizeof b, "1e%d", k);
+      p[k + kSpan] = std::strtod(b, nullptr);
+    }
+  }
+  double operator()(int k) const { return p[k + kSpan]; }
+};
+inline const Pow10Table kPow10;
+
+// "%.6g" of v without a library call where that is safe: q = v * 10^(5-e)
+// with e = floor(log10|v|) carries a relative error below 2.3e-16 (one
+// rounding in the power, one in the product), i.e. below 2.3e-10 at q < 1e6,
+// so when the fraction of q is more than 1e-9 away from one half, rounding q
+// to the nearest integer gives the six correctly rounded digits.  Writes at
+// most 13 characters at b and returns their end, or nullptr (having written
+// nothing) for zeros, infinities, NaNs, magnitudes outside [1e-290, 1e290]
+// and near-ties, which take std::to_chars.
+inline char* put_g6(char* b, double v) {
+  const double a = std::fabs(v);
+  if (!(a >= 1e-290 && a <= 1e290)) return nullptr;
+  uint64_t bits;
+  std::memcpy(&bits, &a, sizeof bits);
+  const int e2 = (int)(bits >> 52) - 1023;  // a in [2^e2, 2^(e2+1)): a is normal here
+  // floor(e2 log10 2) = (e2 * 78913) >> 18 for |e2| <= 1100: floor(log10 a)
+  // or one less
+  int e = (e2 * 78913) >> 18;
+  double x = a * kPow10(5 - e);
+  if (x >= 1e6) {
+    ++e;
+    x = a * kPow10(5 - e);
+  }
+  const long t = (long)x;  // x > 0: truncation is floor
+  const double f = x - (double)t;
+  if (std::fabs(f - 0.5) < 1e-9) return nullptr;
+  int q = (int)t + (f > 0.5 ? 1 : 0);
+  if (q == 1000000) {
+    q = 100000;
+    ++e;
+  }
+  if (q < 100000 || q > 999999) return nullptr;
+  // the six digits as three independent pairs
+  static constexpr char kPairs[] =
+      "00010203040506070809101112131415161718192021222324252627282930313233343536373839"
+      "40414243444546474849505152535455565758596061626364656667686970717273747576777879"
+      "8081828384858687888990919293949596979899";
+  const int p0 = q / 10000, p1 = (q / 100) % 100, p2 = q % 100;
+  const char d[6] = {kPairs[2 * p0], kPairs[2 * p0 + 1], kPairs[2 * p1],
+                     kPairs[2 * p1 + 1], kPairs[2 * p2], kPairs[2 * p2 + 1]};
+  int n = 6;  // significant digits once trailing zeros are dropped
+  while (n > 1 && d[n - 1] == '0') --n;
+  char* w = b;
+  if (v < 0) *w++ = '-';
+  if (e < -4 || e >= 6) {
+    *w++ = d[0];
+    if (n > 1) {
+      *w++ = '.';
+      for (int i = 1; i < n; ++i) *w++ = d[i];
+    }
+    *w++ = 'e';
+    int ae = e;
+    if (e < 0) {
+      *w++ = '-';
+      ae = -e;
+    } else {
+      *w++ = '+';
+    }
+    if (ae >= 100) *w++ = (char)('0' + ae / 100);
+    *w++ = (char)('0' + (ae / 10) % 10);
+    *w++ = (char)('0' + ae % 10);
+  } else if (e >= 0) {
+    for (int i = 0; i <= e; ++i) *w++ = d[i];
+    if (n > e + 1) {
+      *w++ = '.';
+      for (int i = e + 1; i < n; ++i) *w++ = d[i];
+    }
+  } else {
+    *w++ = '0';
+    *w++ = '.';
+    for (int i = 0; i < -e - 1; ++i) *w++ = '0';
+    for (int i = 0; i < n; ++i) *w++ = d[i];
+  }
+  return w;
 }
+
+// one double as operator<< writes it, at most 16 characters; sets nan
+inline char* put_raw(char* b, double v, bool& nan) {
+  if (char* w = put_g6(b, v)) return w;
+  nan = nan || v != v;
+  return std::to_chars(b, b + 16, v, std::chars_format::general, 6).ptr;
+}
+
+inline bool put(std::string& s, double v) {
+  char b[16];
+  bool nan = false;
+  s.append(b, put_raw(b, v, nan));
+  return nan;
+}
+
+// "[ v0, ..., v(M-1) ]" of a node's values, at most 20 M + 4 characters
+template <size_t M>
+inline char* put_raw(char* w, const std::array<double, M>& a, bool& nan) {
+  *w++ = '[';
+  *w++ = ' ';
+  for (size_t i = 0; i < M; ++i) {
+    if (i) {
+      *w++ = ',';
+      *w++ = ' ';
+    }
+    w = put_raw(w, a[i], nan);
+  }
+  *w++ = ' ';
+  *w++ = ']';
+  return w;
+}
+
+// value types written straight into a buffer, with a bound on their length
+template <class T>
+struct raw_width {
+  static constexpr size_t value = 0;
+};
+template <>
+struct raw_width<double> {
+  static constexpr size_t value = 16;
+};
+template <size_t M>
+struct raw_width<std::array<double, M>> {
+  static constexpr size_t value = 20 * M + 4;
+};
 // (character types would stream as characters: none is archived)
 template <class T>
 inline typename std::enable_if<std::is_integral<T>::value && !std::is_same<T, bool>::value &&
@@ -115,36 +254,190 @@ inline unsigned list_workers(size_t n) {
   return (unsigned)std::min<size_t>({size_t(16), size_t(hw), n >> 14});
 }
 
+// Buffers of the long lists, kept for the next frame: a frame's lists have
+// the sizes of the last frame's, and memory the process has already touched
+// costs no page faults (at ~70 MB per 500^2 frame those took as long as the
+// formatting).  At most 1 GiB is kept.
+class buffer_pool {
+  std::mutex mu_;
+  std::vector<std::pair<size_t, std::unique_ptr<char[]>>> free_;
+  size_t bytes_ = 0;
+
+ public:
+  // a buffer of at least n bytes and its capacity
+  std::unique_ptr<char[]> get(size_t n, size_t& cap) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      size_t best = free_.size();
+      for (size_t i = 0; i < free_.size(); ++i)
+        if (free_[i].first >= n && (best == free_.size() || free_[i].first < free_[best].first))
+          best = i;
+      if (best < free_.size()) {
+        cap = free_[best].first;
+        std::unique_ptr<char[]> b = std::move(free_[best].second);
+        free_.erase(free_.begin() + (std::ptrdiff_t)best);
+        bytes_ -= cap;
+        return b;
+      }
+    }
+    cap = n;
+    return std::unique_ptr<char[]>(new char[n]);
+  }
+  void put(std::unique_ptr<char[]> b, size_t cap) {
+    if (!b) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    if (bytes_ + cap > (size_t(1) << 30)) return;  // freed here
+    bytes_ += cap;
+    free_.emplace_back(cap, std::move(b));
+  }
+};
+inline buffer_pool& list_buffers() {
+  static buffer_pool pool;
+  return pool;
+}
+
+// A run of formatted text in storage of its own (not zero-filled first:
+// the long lists are written straight into it), handed back to the buffer
+// pool when it is dropped.
+struct piece {
+  std::unique_ptr<char[]> data;
+  size_t size = 0, cap = 0;
+  piece() = default;
+  piece(const char* p, size_t n) : data(new char[n ? n : 1]), size(n), cap(n ? n : 1) {
+    if (n) std::memcpy(data.get(), p, n);
+  }
+  piece(piece&&) = default;
+  piece& operator=(piece&& o) {
+    if (this != &o) {
+      release();
+      data = std::move(o.data);
+      size = o.size;
+      cap = o.cap;
+    }
+    return *this;
+  }
+  ~piece() { release(); }
+  void release() {
+    if (data && cap >= (size_t(1) << 16)) list_buffers().put(std::move(data), cap);
+    data.reset();
+  }
+  std::string_view view() const { return std::string_view(data.get(), size); }
+};
+
+// Worker threads for the long lists, started on first use and kept until
+// the process ends: a frame formats ~20 lists, and starting and joining a
+// thread per chunk of each took longer than the formatting itself.
+class worker_pool {
+  std::mutex mu_;
+  std::condition_variable wake_, done_;
+  std::vector<std::thread> threads_;
+  const std::function<void(unsigned)>* job_ = nullptr;
+  unsigned njobs_ = 0, next_ = 0, busy_ = 0;
+  uint64_t generation_ = 0;
+  bool stop_ = false;
+
+  // claims and runs chunks of the current job until none is left; mu_ held
+  // on entry and exit
+  void drain(std::unique_lock<std::mutex>& lk) {
+    while (next_ < njobs_) {
+      const unsigned i = next_++;
+      ++busy_;
+      const std::function<void(unsigned)>* job = job_;
+      lk.unlock();
+      (*job)(i);
+      lk.lock();
+      --busy_;
+    }
+    if (busy_ == 0) done_.notify_all();
+  }
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu_);
+    uint64_t seen = generation_;
+    for (;;) {
+      wake_.wait(lk, [&] { return stop_ || generation_ != seen; });
+      if (stop_) return;
+      seen = generation_;
+      drain(lk);
+    }
+  }
+
+ public:
+  ~worker_pool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+  // runs f(0) ... f(n-1), the calling thread taking part; returns when all
+  // have finished.  One job at a time (the archive is written by one thread).
+  void run(unsigned n, const std::function<void(unsigned)>& f) {
+    std::unique_lock<std::mutex> lk(mu_);
+    while (threads_.size() + 1 < n) threads_.emplace_back([this] { loop(); });
+    job_ = &f;
+    njobs_ = n;
+    next_ = 0;
+    busy_ = 0;
+    ++generation_;
+    wake_.notify_all();
+    drain(lk);
+    done_.wait(lk, [&] { return next_ >= njobs_ && busy_ == 0; });
+    job_ = nullptr;
+    njobs_ = 0;
+  }
+};
+inline worker_pool& list_workers_pool() {
+  static worker_pool pool;
+  return pool;
+}
+
 // "v0, v1, ..." of a vector (std::vector<bool> included) as consecutive
 // pieces, formatted in parallel for long vectors; sets nan if any value is
 // a NaN
 template <class V>
-std::vector<std::string> format_items(const V& v, bool& nan) {
+std::vector<piece> format_items(const V& v, bool& nan) {
   const size_t n = v.size();
   const unsigned nt = list_workers(n);
-  std::vector<std::string> part(nt);
+  std::vector<piece> part(nt);
   std::vector<char> bad(nt, 0);
   auto work = [&](unsigned t) {
     const size_t a = n * t / nt, b = n * (t + 1) / nt;
-    // built in the thread's own string and moved out at the end: the
-    // headers of neighbouring strings share cache lines
-    std::string s;
-    s.reserve((b - a) * 14 + 8);
     bool any = false;
-    for (size_t i = a; i < b; ++i) {
-      if (i) s += ", ";
-      any |= put(s, static_cast<typename V::value_type>(v[i]));
+    constexpr size_t width = raw_width<typename V::value_type>::value;
+    piece out;
+    if constexpr (width > 0) {
+      // doubles (and nodes of them) straight into the piece: ", " and at
+      // most `width` characters each
+      out.data = list_buffers().get((b - a) * (width + 2) + 2, out.cap);
+      char* w = out.data.get();
+      for (size_t i = a; i < b; ++i) {
+        if (i) {
+          *w++ = ',';
+          *w++ = ' ';
+        }
+        w = put_raw(w, v[i], any);
+      }
+      out.size = (size_t)(w - out.data.get());
+    } else {
+      std::string s;
+      s.reserve((b - a) * 14 + 8);
+      for (size_t i = a; i < b; ++i) {
+        if (i) s += ", ";
+        any |= put(s, static_cast<typename V::value_type>(v[i]));
+      }
+      out = piece(s.data(), s.size());
     }
-    part[t] = std::move(s);
+    // moved out at the end: the headers of neighbouring pieces share cache
+    // lines
+    part[t] = std::move(out);
     bad[t] = any;
   };
   if (nt == 1) {
     work(0);
   } else {
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work, t);
-    work(0);
-    for (auto& th : pool) th.join();
+    const std::function<void(unsigned)> job = work;
+    list_workers_pool().run(nt, job);
   }
   for (unsigned t = 0; t < nt; ++t) nan = nan || bad[t];
   return part;
@@ -154,7 +447,7 @@ std::vector<std::string> format_items(const V& v, bool& nan) {
 template <class V>
 std::string format_list(const V& v, bool& nan) {
   std::string out = "[ ";
-  for (const auto& p : format_items(v, nan)) out += p;
+  for (const auto& p : format_items(v, nan)) out += p.view();
   out += " ]";
   return out;
 }
@@ -165,12 +458,12 @@ std::string format_list(const V& v, bool& nan) {
 // lists of a frame are handed over as they were formatted, without copying
 // them into one growing buffer, and written out piece by piece.
 class piece_buffer : public std::streambuf {
-  std::vector<std::string> pieces_;
+  std::vector<archive_detail::piece> pieces_;
   std::string cur_;
 
   void seal() {
     if (!cur_.empty()) {
-      pieces_.push_back(std::move(cur_));
+      pieces_.emplace_back(cur_.data(), cur_.size());
       cur_.clear();
     }
   }
@@ -186,13 +479,13 @@ class piece_buffer : public std::streambuf {
   }
 
  public:
-  void take(std::string&& piece) {
+  void take(archive_detail::piece&& p) {
     seal();
-    pieces_.push_back(std::move(piece));
+    pieces_.push_back(std::move(p));
   }
   void write_to(std::ostream& o) {
     seal();
-    for (const auto& p : pieces_) o.write(p.data(), (std::streamsize)p.size());
+    for (const auto& p : pieces_) o.write(p.data.get(), (std::streamsize)p.size);
   }
 };
 
@@ -240,12 +533,12 @@ class oarchive {
   void element(const std::string& v) { stream_ << '"' << v << '"'; }
   template <class T>
   void element(const std::vector<T>& v) {
-    std::vector<std::string> items = archive_detail::format_items(v, bad_);
+    std::vector<archive_detail::piece> items = archive_detail::format_items(v, bad_);
     stream_ << "[ ";
     if (auto* ps = dynamic_cast<piece_stream*>(&stream_)) {
       for (auto& p : items) ps->pieces().take(std::move(p));
     } else {
-      for (const auto& p : items) stream_ << p;
+      for (const auto& p : items) stream_ << p.view();
     }
     stream_ << " ]";
   }

@@ -1,5 +1,6 @@
 """The host program's JSON writer (mass_hd_amd/host/serialization.hpp)
-formats numbers with std::to_chars and long lists in parallel chunks; the
+formats doubles by an exact six-digit path (std::to_chars near ties) and
+long lists in parallel chunks; the
 reference's archive streams each number through a std::stringstream
 (src/serialization.hpp:187-361).  tests/native/format_check.cpp holds the
 writer to the stream byte for byte: 14.8 million doubles (random bit
