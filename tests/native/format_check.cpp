@@ -9,7 +9,9 @@
 //    one-element lists; double, unsigned, int, bool and 9-value nodes)
 //    against the same archive written one value at a time through a
 //    stringstream, and the NaN flag.
-// Prints "format ok <values>" and exits 0, or lists the first mismatches.
+// Prints "format ok <values> (short path <n>)" and exits 0 -- the short exact
+// path must have written most of the doubles -- or lists the first
+// mismatches.
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -25,6 +27,7 @@
 
 static int bad = 0;
 static long checked = 0;
+static long doubles = 0, fast = 0;  // doubles checked, and those the short exact path wrote
 
 static std::string streamed(double v) {
   std::stringstream ss;
@@ -33,6 +36,9 @@ static std::string streamed(double v) {
 }
 
 static void check_double(double v) {
+  char probe[16];
+  ++doubles;
+  if (archive_detail::put_g6(probe, v)) ++::fast;
   std::string fast;
   archive_detail::put(fast, v);
   char b[64];
@@ -232,6 +238,10 @@ int main() {
     std::fprintf(stderr, "%d mismatches in %ld values\n", bad, checked);
     return 1;
   }
-  std::printf("format ok %ld\n", checked);
+  if (::fast < doubles / 2) {
+    std::fprintf(stderr, "the short path wrote only %ld of %ld doubles\n", ::fast, doubles);
+    return 1;
+  }
+  std::printf("format ok %ld (short path %ld of %ld doubles)\n", checked, ::fast, doubles);
   return 0;
 }
