@@ -87,7 +87,7 @@ class LyotropicHip : public Model {
     std::vector<std::array<double, 9>> nodes(DomainSize);
     for (size_t k = 0; k < nodes.size(); ++k)
       for (int v = 0; v < 9; ++v) nodes[k][v] = ff[9 * k + v];
-    ar.add("ff", nodes)
+    ar.add("ff", std::move(nodes))
         .add("QQxx", field(MASS_LYO_QQXX))
         .add("QQyx", field(MASS_LYO_QQYX))
         .add("phi", field(MASS_LYO_PHI))

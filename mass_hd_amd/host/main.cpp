@@ -3,7 +3,10 @@
 // loop with its frame and check cadence, and the JSON frame / parameter
 // files.
 #include <chrono>
+#include <exception>
 #include <fstream>
+#include <memory>
+#include <thread>
 #include <iomanip>
 #include <iostream>
 #include <random>
@@ -39,20 +42,46 @@ static void zip_file(const std::string& archive, const std::string& file) {
   if (ret != 0) throw error_msg("zip non-zero return value ", ret, ".");
 }
 
+// Frames are formatted and written by a thread of their own while the GPU
+// runs the next steps (Step only queues work): the values are downloaded --
+// the state at t -- and checked for NaN in the step loop, so a NaN still
+// stops the run before the frame's file exists; one frame is in flight at a
+// time, and an error of the writer surfaces at the next frame or the end.
+static std::thread frame_writer;
+static std::exception_ptr frame_writer_error;
+
+static void JoinFrameWriter() {
+  if (frame_writer.joinable()) frame_writer.join();
+  if (frame_writer_error) {
+    std::exception_ptr e = frame_writer_error;
+    frame_writer_error = nullptr;
+    std::rethrow_exception(e);
+  }
+}
+
 static void WriteFrame(unsigned t) {
   // src/write.cpp:28-61
   const std::string oname = inline_str(output_dir, "frame", t, ".json");
-  piece_stream buffer;  // the whole frame first: a NaN leaves no file
-  {
-    oarchive ar(buffer, "frame", 1);
-    model.ser->serialize_frame(ar);
-    if (ar.bad_value()) throw error_msg("nan found while writing file.");
-  }
-  std::ofstream ofs(oname.c_str(), std::ios::out);
-  buffer.pieces().write_to(ofs);
-  ofs.close();
-  if (compress) zip_file(oname + ".zip", oname);
-  if (compress_full) zip_file(runname + ".zip", oname);
+  auto values = std::make_shared<frame_values>();
+  model.ser->serialize_frame(*values);
+  if (values->has_nan()) throw error_msg("nan found while writing file.");
+  JoinFrameWriter();
+  frame_writer = std::thread([values, oname]() {
+    try {
+      piece_stream buffer;
+      {
+        oarchive ar(buffer, "frame", 1);
+        values->emit(ar);
+      }
+      std::ofstream ofs(oname.c_str(), std::ios::out);
+      buffer.pieces().write_to(ofs);
+      ofs.close();
+      if (compress) zip_file(oname + ".zip", oname);
+      if (compress_full) zip_file(runname + ".zip", oname);
+    } catch (...) {
+      frame_writer_error = std::current_exception();
+    }
+  });
 }
 
 static void WriteParams() {
@@ -244,10 +273,14 @@ static void Algorithm() {
     }
   }
   if (!no_write && nsteps >= nstart) WriteFrame(nsteps);
+  const auto start = std::chrono::steady_clock::now();
+  JoinFrameWriter();  // the last frame is on disk when the run ends
+  write_duration += std::chrono::steady_clock::now() - start;
 }
 
 int main(int argc, char** argv) {
   std::cout << "\n   mass (MI355X)  --  lyotropic active nematic on the GPU\n" << std::endl;
+  int rc = 0;
   try {
     if (argc < 2) throw error_msg("no argument provided. Type -h for help.");
     DeclareModels();
@@ -300,13 +333,15 @@ int main(int argc, char** argv) {
     }
   } catch (const error_msg& e) {
     std::cerr << argv[0] << ": error: " << e.what() << std::endl;
-    return 1;
+    rc = 1;
   } catch (const std::bad_alloc& ba) {
     std::cerr << argv[0] << ": error initializing memory: " << ba.what() << std::endl;
-    return 1;
+    rc = 1;
   } catch (const std::exception& e) {
     std::cerr << argv[0] << ": " << e.what() << std::endl;
-    return 1;
+    rc = 1;
   }
-  return 0;
+  // a frame still being written when an error stopped the run
+  if (frame_writer.joinable()) frame_writer.join();
+  return rc;
 }

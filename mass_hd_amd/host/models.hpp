@@ -50,12 +50,16 @@ struct ModelPtr {
     virtual ~Serializer() {}
     virtual void serialize_params(oarchive&) = 0;
     virtual void serialize_frame(oarchive&) = 0;
+    virtual void serialize_frame(frame_values&) = 0;
   };
   template <class Derived>
   struct SerializerT : Serializer {
     explicit SerializerT(Derived* p) : Serializer(p) {}
     void serialize_params(oarchive& ar) override { static_cast<Derived*>(ptr)->serialize_params(ar); }
     void serialize_frame(oarchive& ar) override { static_cast<Derived*>(ptr)->serialize_frame(ar); }
+    void serialize_frame(frame_values& ar) override {
+      static_cast<Derived*>(ptr)->serialize_frame(ar);
+    }
   };
 
   std::shared_ptr<Serializer> ser;

@@ -575,3 +575,64 @@ class oarchive {
 
   bool bad_value() const { return bad_; }
 };
+
+namespace archive_detail {
+inline bool any_nan(double v) { return v != v; }
+template <size_t M>
+inline bool any_nan(const std::array<double, M>& a) {
+  for (double v : a)
+    if (v != v) return true;
+  return false;
+}
+template <class T>
+inline bool any_nan(const std::vector<T>& v) {
+  if constexpr (std::is_same<T, double>::value || std::is_same<T, std::array<double, 9>>::value) {
+    for (const auto& x : v)
+      if (any_nan(x)) return true;
+  }
+  return false;
+}
+template <class T>
+inline bool any_nan(const T&) {
+  return false;
+}
+}  // namespace archive_detail
+
+// The values of a frame, kept as they are handed over (downloaded fields
+// are moved in) and written out later, in the same order, through an
+// oarchive: the host program formats and writes a frame in a thread of its
+// own while the GPU runs the next steps.
+class frame_values {
+  struct entry_base {
+    std::string name;
+    virtual ~entry_base() {}
+    virtual void emit(oarchive& ar) const = 0;
+    virtual bool has_nan() const = 0;
+  };
+  template <class T>
+  struct entry : entry_base {
+    T value;
+    void emit(oarchive& ar) const override { ar.add(name, value); }
+    bool has_nan() const override { return archive_detail::any_nan(value); }
+  };
+  std::vector<std::unique_ptr<entry_base>> entries_;
+
+ public:
+  template <class T>
+  frame_values& add(const std::string& name, T&& value) {
+    auto e = std::make_unique<entry<typename std::decay<T>::type>>();
+    e->name = name;
+    e->value = std::forward<T>(value);
+    entries_.push_back(std::move(e));
+    return *this;
+  }
+  // what the archive's bad-value flag would say of these values
+  bool has_nan() const {
+    for (const auto& e : entries_)
+      if (e->has_nan()) return true;
+    return false;
+  }
+  void emit(oarchive& ar) const {
+    for (const auto& e : entries_) e->emit(ar);
+  }
+};

@@ -234,6 +234,48 @@ int main() {
       std::fprintf(stderr, "frame through the piece stream differs (%zu vs %zu bytes)\n",
                    bb.str().size(), a.str().size());
   }
+  // a frame collected first and written later (the host program's frame
+  // writer thread): the same bytes as the archive written directly, and the
+  // NaN check of the collected values says what the archive's flag says
+  {
+    std::vector<std::array<double, 9>> ff(20000);
+    for (auto& a : ff)
+      for (auto& x : a) x = 40. / 9. + nd(gen) * 1e-3;
+    std::vector<double> phi(70000);
+    for (auto& x : phi) x = ud(gen);
+    std::vector<bool> mask(70000);
+    for (size_t i = 0; i < mask.size(); ++i) mask[i] = (gen() % 5) == 0;
+    std::stringstream direct;
+    {
+      oarchive ar(direct, "frame", 1);
+      ar.add("ff", ff).add("phi", phi).add("division_mask", mask);
+    }
+    frame_values fv;
+    fv.add("ff", ff).add("phi", std::vector<double>(phi)).add("division_mask", mask);
+    std::stringstream later;
+    {
+      oarchive ar(later, "frame", 1);
+      fv.emit(ar);
+    }
+    if (direct.str() != later.str() && bad++ < 20)
+      std::fprintf(stderr, "a collected frame writes other bytes\n");
+    if (fv.has_nan() && bad++ < 20) std::fprintf(stderr, "NaN reported in a frame without one\n");
+    for (int where = 0; where < 2; ++where) {
+      frame_values f2;
+      auto ff2 = ff;
+      auto phi2 = phi;
+      if (where == 0)
+        ff2[12345][7] = std::numeric_limits<double>::quiet_NaN();
+      else
+        phi2[54321] = -std::numeric_limits<double>::quiet_NaN();
+      f2.add("ff", std::move(ff2)).add("phi", std::move(phi2)).add("division_mask", mask);
+      std::stringstream o;
+      oarchive ar(o, "frame", 1);
+      f2.emit(ar);
+      if ((!f2.has_nan() || !ar.bad_value()) && bad++ < 20)
+        std::fprintf(stderr, "NaN in a collected frame not reported (case %d)\n", where);
+    }
+  }
   if (bad) {
     std::fprintf(stderr, "%d mismatches in %ld values\n", bad, checked);
     return 1;
