@@ -147,12 +147,19 @@ def build_host(verbose=False):
 
 def build_checks(verbose=False):
     """bin/format_check: the JSON writer against the number stream it
-    replaces (tests/native/format_check.cpp)."""
+    replaces (tests/native/format_check.cpp); bin/format_check_tsan: the same
+    under ThreadSanitizer (host code, g++), for its parallel parts."""
     os.makedirs(BINDIR, exist_ok=True)
-    out = os.path.join(BINDIR, "format_check")
-    _run(["g++", "-std=c++17", "-O2", "-pthread", "-Wall", "-Wextra", "-I" + HOST, "-o",
-          out + ".tmp", os.path.join(NATIVE_TESTS, "format_check.cpp")], verbose)
-    os.replace(out + ".tmp", out)
+    src = os.path.join(NATIVE_TESTS, "format_check.cpp")
+    jobs = []
+    for name, flags in (("format_check", ["-O2"]),
+                        ("format_check_tsan", ["-O1", "-g", "-fsanitize=thread"])):
+        out = os.path.join(BINDIR, name)
+        jobs.append((["g++", "-std=c++17", "-pthread", "-Wall", "-Wextra"] + flags +
+                     ["-I" + HOST, "-o", out + ".tmp", src], out))
+    _compile_all([c for c, _ in jobs], verbose)
+    for _, out in jobs:
+        os.replace(out + ".tmp", out)
 
 
 def build_asan_programs(verbose=False):

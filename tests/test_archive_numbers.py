@@ -22,3 +22,15 @@ def test_writer_matches_stream_byte_for_byte():
                        timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     assert r.stdout.startswith("format ok")
+
+
+def test_parallel_writer_has_no_data_races():
+    """The list and frame parts of the same check (pooled worker threads,
+    the buffer pool, the piece stream) under ThreadSanitizer."""
+    tsan = CHECK + "_tsan"
+    assert os.path.exists(tsan), "format_check_tsan is not built (run __graft_entry__.build())"
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66")
+    r = subprocess.run([tsan, "--lists"], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=600, env=env)
+    assert r.returncode == 0 and "ThreadSanitizer" not in r.stderr, r.stderr[-3000:]
+    assert r.stdout.startswith("format ok")
