@@ -932,8 +932,11 @@ class WallEngine final : public Engine {
     if (sel_val_) (void)hipFree(sel_val_);
     if (hsel_idx_) (void)hipHostFree(hsel_idx_);
     if (hsel_val_) (void)hipHostFree(hsel_val_);
+    if (hpend_) (void)hipHostFree(hpend_);
     sel_idx_ = hsel_idx_ = nullptr;
     sel_val_ = hsel_val_ = nullptr;
+    hpend_ = nullptr;
+    npending_ = 0;
     sel_last_ = 0;
     div_ = death_ = nullptr;
     growth_ = false;
@@ -1675,6 +1678,50 @@ class WallEngine final : public Engine {
                        s, (const int*)cent_[slot], nc, R, r2, LX_, LY_, mask);
     return hipGetLastError();
   }
+  // The list length of a step whose due redraws all have rate zero, read
+  // back without waiting; their masks are cleared and their counters reset.
+  hipError_t count_only_redraw(bool div_due, bool death_due, hipStream_t s) {
+    const int N = LX_ * LY_, nsb = (N + kSelBlock - 1) / kSelBlock;
+    hipError_t e = hipSuccess;
+    if (!sel_idx_) {
+      e = hipMalloc((void**)&sel_idx_, sizeof(int) * ((size_t)N + nsb + 1));
+      if (e == hipSuccess) e = hipMalloc((void**)&sel_val_, sizeof(double) * N);
+      if (e == hipSuccess)
+        e = hipHostMalloc((void**)&hsel_idx_, sizeof(int) * ((size_t)N + 1), hipHostMallocDefault);
+      if (e == hipSuccess)
+        e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * N, hipHostMallocDefault);
+      if (e != hipSuccess) return e;
+    }
+    if (!hpend_) {
+      e = hipHostMalloc((void**)&hpend_, sizeof(int) * kMaxPending, hipHostMallocDefault);
+      if (e != hipSuccess) return e;
+    }
+    if (npending_ == kMaxPending) {  // bound the lengths in flight
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return e;
+      consume_pending();
+    }
+    int* counts = sel_idx_ + N;
+    int* total = counts + nsb;
+    hipLaunchKernelGGL(select_count_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
+                       (const T*)phi_[pc_], counts);
+    hipLaunchKernelGGL(select_write_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
+                       (const T*)phi_[pc_], (const int*)counts, sel_idx_, sel_val_, total);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hpend_ + npending_, total, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    pend_times_[npending_++] = (div_due ? 1 : 0) + (death_due ? 1 : 0);
+    if (div_due) {
+      div_count_ = (uint32_t)gp_.division_time;
+      e = hipMemsetAsync(div_, 0, (size_t)TP_, s);
+    }
+    if (e == hipSuccess && death_due) {
+      death_count_ = (uint32_t)gp_.death_time;
+      e = hipMemsetAsync(death_, 0, (size_t)TP_, s);
+    }
+    return e;
+  }
   // The draws' uniform numbers, taken from gen_ ahead of use and consumed in
   // order: the sequence each draw sees is the one drawing on demand gives.
   double next_uniform() {
@@ -1686,11 +1733,33 @@ class WallEngine final : public Engine {
     ufifo_.clear();
     ufront_ = 0;
     uexpect_ = 0;
+    npending_ = 0;
+  }
+  // The draws of redraws that could accept no centre, consumed now: their
+  // list lengths have arrived (the stream was synchronised after they were
+  // queued), and they came before any redraw that is about to draw.
+  void consume_pending() {
+    for (int i = 0; i < npending_; ++i) {
+      const long long n = (long long)hpend_[i] * pend_times_[i];
+      for (long long j = 0; j < n; ++j) (void)next_uniform();
+    }
+    npending_ = 0;
   }
   hipError_t set_masks(hipStream_t s) {
     const uint32_t div_old = div_count_--;
     const uint32_t death_old = death_count_--;
     if (div_old != 0 && death_old != 0) return hipSuccess;
+    // A due redraw with rate zero accepts no centre (random_real() < 0 never
+    // holds): its mask is cleared and it only draws one number per listed
+    // node (lyotropic-with-division.cpp:44-50, 80-86).  When no due redraw
+    // can accept, the step needs the list's length only, not the list: it is
+    // read back without waiting and the numbers are drawn at the next
+    // redraw that accepts (before its own), so the step queues without a
+    // host synchronisation.  (The reference's proliferation runcards have
+    // death-rate = 0: half their redraws are these.)
+    const bool accept = (div_old == 0 && gp_.division_rate != 0) ||
+                        (death_old == 0 && gp_.death_rate != 0);
+    if (!accept) return count_only_redraw(div_old == 0, death_old == 0, s);
     // draw about as many numbers as the last redraw used while the queued
     // steps still run, before waiting for phi
     if (ufront_ > 0) {
@@ -1733,6 +1802,7 @@ class WallEngine final : public Engine {
       e = hipMemcpyAsync(hsel_val_, sel_val_, sizeof(double) * guess, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
+    consume_pending();
     const int nsel = hsel_idx_[N];
     if (nsel < 0 || nsel > N) return hipErrorUnknown;
     if (nsel > guess) {
@@ -1971,6 +2041,12 @@ class WallEngine final : public Engine {
   size_t ufront_ = 0;          // the next unused one
   size_t udraws_ = 0;          // numbers the draws have consumed
   size_t uexpect_ = 0;         // used by the last redraw
+  // redraws whose draws are still to be consumed: their list lengths (pinned,
+  // read back without waiting) and how many redraws drew on each list
+  static constexpr int kMaxPending = 64;
+  int* hpend_ = nullptr;
+  int pend_times_[kMaxPending] = {};
+  int npending_ = 0;
   int* sel_idx_ = nullptr;     // nodes with phi >= 1/2: domain index (N), block
                                // counts, the count
   double* sel_val_ = nullptr;  // and their phi (N)

@@ -143,6 +143,25 @@ def test_growth_fused_sweeps_equal_loops(require_gpu, npc, division_only):
     assert runs[0]["division_mask"].any()
 
 
+def test_growth_count_only_redraws_keep_the_draw_order(require_gpu):
+    """Redraws with rate zero accept nothing and only consume one number per
+    listed node; the engine reads their list lengths back without waiting and
+    consumes the numbers at the next redraw that accepts.  Death redrawn every
+    step at rate zero and division every 101st step: 100 such redraws between
+    two accepting ones (past the 64 whose lengths may be in flight), then the
+    division draws must see the oracle's numbers -- masks and state bit for
+    bit."""
+    print("go-or-grow count-only redraws", flush=True)
+    g = dict(division_time=100.0, division_rate=0.5, death_time=0.0, death_rate=0.0,
+             beta=0.0, relax_steps=5)
+    a, o, ta, to = pair(40, 36, growth=g)
+    for s in range(3):
+        a.step(70)
+        o.step(70)
+        check(a, o, STATE, "after %d steps" % (70 * s + 70))
+    assert a.get("division_mask").any()
+
+
 def test_growth_refusals(require_gpu):
     f32 = lyo.Lyotropic(16, 16, npc=1, precision=lyo.FP32, **PHYS)
     with pytest.raises(lyo.MassLyoError):
