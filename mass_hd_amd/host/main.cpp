@@ -68,13 +68,16 @@ static void WriteFrame(unsigned t) {
   JoinFrameWriter();
   frame_writer = std::thread([values, oname]() {
     try {
+      // (the values hold no NaN, so the file is written as the lists are
+      // formatted)
+      std::ofstream ofs(oname.c_str(), std::ios::out);
       piece_stream buffer;
+      buffer.pieces().stream_to(ofs);
       {
         oarchive ar(buffer, "frame", 1);
         values->emit(ar);
       }
-      std::ofstream ofs(oname.c_str(), std::ios::out);
-      buffer.pieces().write_to(ofs);
+      buffer.pieces().finish();
       ofs.close();
       if (compress) zip_file(oname + ".zip", oname);
       if (compress_full) zip_file(runname + ".zip", oname);

@@ -31,6 +31,7 @@
 #include <array>
 #include <charconv>
 #include <condition_variable>
+#include <fstream>
 #include <functional>
 #include <cmath>
 #include <cstdint>
@@ -460,11 +461,42 @@ std::string format_list(const V& v, bool& nan) {
 class piece_buffer : public std::streambuf {
   std::vector<archive_detail::piece> pieces_;
   std::string cur_;
+  // streaming to a file: an I/O thread writes the pieces in order as they
+  // come, while the lists after them are formatted
+  std::ofstream* file_ = nullptr;
+  std::thread io_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  size_t written_ = 0;  // pieces the I/O thread has taken
+  bool closing_ = false;
 
+  void push(archive_detail::piece&& p) {
+    if (!file_) {
+      pieces_.push_back(std::move(p));
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      pieces_.push_back(std::move(p));
+    }
+    cv_.notify_one();
+  }
   void seal() {
     if (!cur_.empty()) {
-      pieces_.emplace_back(cur_.data(), cur_.size());
+      push(archive_detail::piece(cur_.data(), cur_.size()));
       cur_.clear();
+    }
+  }
+  void io_loop() {
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      cv_.wait(lk, [&] { return closing_ || written_ < pieces_.size(); });
+      if (written_ == pieces_.size()) return;  // closing, all written
+      archive_detail::piece p = std::move(pieces_[written_++]);
+      lk.unlock();
+      file_->write(p.data.get(), (std::streamsize)p.size);
+      p.release();  // back to the buffer pool
+      lk.lock();
     }
   }
 
@@ -479,13 +511,32 @@ class piece_buffer : public std::streambuf {
   }
 
  public:
+  ~piece_buffer() { finish(); }
   void take(archive_detail::piece&& p) {
     seal();
-    pieces_.push_back(std::move(p));
+    push(std::move(p));
   }
   void write_to(std::ostream& o) {
     seal();
     for (const auto& p : pieces_) o.write(p.data.get(), (std::streamsize)p.size);
+  }
+  // from now on the pieces go to the file, in order, from an I/O thread
+  // (call before anything is written)
+  void stream_to(std::ofstream& f) {
+    file_ = &f;
+    io_ = std::thread([this] { io_loop(); });
+  }
+  // everything written: the I/O thread has written every piece
+  void finish() {
+    if (!file_) return;
+    seal();
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      closing_ = true;
+    }
+    cv_.notify_one();
+    if (io_.joinable()) io_.join();
+    file_ = nullptr;
   }
 };
 

@@ -17,11 +17,14 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <random>
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "serialization.hpp"
 
@@ -234,6 +237,26 @@ int main(int argc, char** argv) {
     fill(b);
     std::stringstream bb;
     b.pieces().write_to(bb);
+    // and streamed to a file from the I/O thread as the lists are formatted
+    const std::string path = "/tmp/format_check_" + std::to_string((long)getpid()) + ".json";
+    {
+      std::ofstream f(path);
+      piece_stream c;
+      c.pieces().stream_to(f);
+      fill(c);
+      c.pieces().finish();
+    }
+    std::string from_file;
+    {
+      std::ifstream in(path, std::ios::binary);
+      std::stringstream ss;
+      ss << in.rdbuf();
+      from_file = ss.str();
+    }
+    std::remove(path.c_str());
+    if (from_file != a.str() && bad++ < 20)
+      std::fprintf(stderr, "frame streamed to a file differs (%zu vs %zu bytes)\n",
+                   from_file.size(), a.str().size());
     checked += (long)(9 * ff.size() + phi.size() + mask.size());
     if (a.str() != bb.str() && bad++ < 20)
       std::fprintf(stderr, "frame through the piece stream differs (%zu vs %zu bytes)\n",
