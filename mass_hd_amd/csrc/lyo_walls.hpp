@@ -738,8 +738,12 @@ __global__ void __launch_bounds__(256) wall_move_wrap_kernel(WallGeo g, const T*
 
 // The domain nodes with phi >= 1/2, in node order (the nodes the mask draws
 // visit, lyotropic-with-division.cpp:42-50): select_count_kernel counts each
-// block's; select_write_kernel adds up the earlier blocks' counts and writes
-// its nodes' (domain index, phi) at that offset, in order within the block.
+// block's; select_write_kernel adds up the earlier blocks' counts and writes,
+// at that offset and in order within the block, each node's domain index
+// and the probabilities its draw is compared with: the division rate
+// rate_div*phi*(1 - phi/phi_critical) into adj[pos], the death rate
+// rate_death*phi into adj[n + pos] (:46, :82; FP64 without contraction, as
+// the host would compute them).
 constexpr int kSelBlock = 1024;
 __device__ __forceinline__ bool select_node(const WallGeo& g, const double* phi, int t) {
   return t < g.LX * g.LY && phi[LayerGrid::at(t, g.LX, g.LY).k] >= .5;
@@ -760,7 +764,8 @@ static __global__ void __launch_bounds__(kSelBlock) select_count_kernel(WallGeo 
 }
 static __global__ void __launch_bounds__(kSelBlock) select_write_kernel(
     WallGeo g, const double* __restrict__ phi, const int* __restrict__ counts,
-    int* __restrict__ idx, double* __restrict__ val, int* __restrict__ total) {
+    int* __restrict__ idx, double* __restrict__ adj, double rate_div, double rate_death,
+    double phi_critical, int* __restrict__ total) {
   __shared__ int ws[kSelBlock / 64];
   __shared__ int part[kSelBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -784,7 +789,9 @@ static __global__ void __launch_bounds__(kSelBlock) select_write_kernel(
     const unsigned long long lower = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const int pos = base + before + __popcll(b & lower);
     idx[pos] = t;
-    val[pos] = phi[LayerGrid::at(t, g.LX, g.LY).k];
+    const double p = phi[LayerGrid::at(t, g.LX, g.LY).k];
+    adj[pos] = rate_div * p * (1 - p / phi_critical);
+    adj[g.LX * g.LY + pos] = rate_death * p;
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = base + all;
 }
@@ -1635,16 +1642,14 @@ class WallEngine final : public Engine {
   // disks marked on the device.  The centres go through a pinned buffer per
   // mask, so nothing waits for the copy: the next draw into that buffer comes
   // after set_masks' next synchronous download.
-  hipError_t redraw(unsigned char* mask, const int* sel_idx, const double* sel_phi, int nsel,
-                    double rate, double radius, bool division, hipStream_t s) {
+  hipError_t redraw(unsigned char* mask, const int* sel_idx, const double* adjusted, int nsel,
+                    double radius, bool division, hipStream_t s) {
     const int slot = division ? 0 : 1;
     std::vector<int>& centres = centres_;
     centres.clear();
     for (int j = 0; j < nsel; ++j) {
       const int k = sel_idx[j];
-      const double p = sel_phi[j];
-      const double adjusted = division ? rate * p * (1 - p / gp_.phi_critical) : rate * p;
-      if (next_uniform() < adjusted) {
+      if (next_uniform() < adjusted[j]) {
         centres.push_back(k / LY_);
         centres.push_back(k % LY_);
       }
@@ -1685,11 +1690,12 @@ class WallEngine final : public Engine {
     hipError_t e = hipSuccess;
     if (!sel_idx_) {
       e = hipMalloc((void**)&sel_idx_, sizeof(int) * ((size_t)N + nsb + 1));
-      if (e == hipSuccess) e = hipMalloc((void**)&sel_val_, sizeof(double) * N);
+      if (e == hipSuccess) e = hipMalloc((void**)&sel_val_, sizeof(double) * 2 * (size_t)N);
       if (e == hipSuccess)
         e = hipHostMalloc((void**)&hsel_idx_, sizeof(int) * ((size_t)N + 1), hipHostMallocDefault);
       if (e == hipSuccess)
-        e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * N, hipHostMallocDefault);
+        e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * 2 * (size_t)N,
+                          hipHostMallocDefault);
       if (e != hipSuccess) return e;
     }
     if (!hpend_) {
@@ -1706,7 +1712,8 @@ class WallEngine final : public Engine {
     hipLaunchKernelGGL(select_count_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
                        (const T*)phi_[pc_], counts);
     hipLaunchKernelGGL(select_write_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
-                       (const T*)phi_[pc_], (const int*)counts, sel_idx_, sel_val_, total);
+                       (const T*)phi_[pc_], (const int*)counts, sel_idx_, sel_val_,
+                       gp_.division_rate, gp_.death_rate, gp_.phi_critical, total);
     e = hipGetLastError();
     if (e == hipSuccess)
       e = hipMemcpyAsync(hpend_ + npending_, total, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -1779,11 +1786,12 @@ class WallEngine final : public Engine {
     hipError_t e = hipSuccess;
     if (!sel_idx_) {
       e = hipMalloc((void**)&sel_idx_, sizeof(int) * ((size_t)N + nsb + 1));
-      if (e == hipSuccess) e = hipMalloc((void**)&sel_val_, sizeof(double) * N);
+      if (e == hipSuccess) e = hipMalloc((void**)&sel_val_, sizeof(double) * 2 * (size_t)N);
       if (e == hipSuccess)
         e = hipHostMalloc((void**)&hsel_idx_, sizeof(int) * ((size_t)N + 1), hipHostMallocDefault);
       if (e == hipSuccess)
-        e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * N, hipHostMallocDefault);
+        e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * 2 * (size_t)N,
+                          hipHostMallocDefault);
       if (e != hipSuccess) return e;
     }
     int* counts = sel_idx_ + N;
@@ -1791,15 +1799,26 @@ class WallEngine final : public Engine {
     hipLaunchKernelGGL(select_count_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
                        (const T*)phi_[pc_], counts);
     hipLaunchKernelGGL(select_write_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
-                       (const T*)phi_[pc_], (const int*)counts, sel_idx_, sel_val_, total);
+                       (const T*)phi_[pc_], (const int*)counts, sel_idx_, sel_val_,
+                       gp_.division_rate, gp_.death_rate, gp_.phi_critical, total);
     e = hipGetLastError();
     const int guess = std::min(N, sel_last_ + sel_last_ / 4 + 1024);
     if (e == hipSuccess)
       e = hipMemcpyAsync(hsel_idx_ + N, total, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(hsel_idx_, sel_idx_, sizeof(int) * guess, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(hsel_val_, sel_val_, sizeof(double) * guess, hipMemcpyDeviceToHost, s);
+    // the rates of the due redraws: division at [0, N), death at [N, 2N)
+    auto copy_rates = [&](size_t from, size_t count) {
+      hipError_t r = hipSuccess;
+      if (div_old == 0)
+        r = hipMemcpyAsync(hsel_val_ + from, sel_val_ + from, sizeof(double) * count,
+                           hipMemcpyDeviceToHost, s);
+      if (r == hipSuccess && death_old == 0)
+        r = hipMemcpyAsync(hsel_val_ + N + from, sel_val_ + N + from, sizeof(double) * count,
+                           hipMemcpyDeviceToHost, s);
+      return r;
+    };
+    if (e == hipSuccess) e = copy_rates(0, (size_t)guess);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     consume_pending();
@@ -1809,21 +1828,18 @@ class WallEngine final : public Engine {
       const size_t rest = (size_t)(nsel - guess);
       e = hipMemcpyAsync(hsel_idx_ + guess, sel_idx_ + guess, sizeof(int) * rest,
                          hipMemcpyDeviceToHost, s);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(hsel_val_ + guess, sel_val_ + guess, sizeof(double) * rest,
-                           hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = copy_rates((size_t)guess, rest);
       if (e == hipSuccess) e = hipStreamSynchronize(s);
       if (e != hipSuccess) return e;
     }
     sel_last_ = nsel;
     if (div_old == 0) {
       div_count_ = (uint32_t)gp_.division_time;
-      e = redraw(div_, hsel_idx_, hsel_val_, nsel, gp_.division_rate, gp_.division_radius, true,
-                 s);
+      e = redraw(div_, hsel_idx_, hsel_val_, nsel, gp_.division_radius, true, s);
     }
     if (e == hipSuccess && death_old == 0) {
       death_count_ = (uint32_t)gp_.death_time;
-      e = redraw(death_, hsel_idx_, hsel_val_, nsel, gp_.death_rate, gp_.death_radius, false, s);
+      e = redraw(death_, hsel_idx_, hsel_val_ + N, nsel, gp_.death_radius, false, s);
     }
     uexpect_ = udraws_ - used0;
     return e;
@@ -2049,7 +2065,7 @@ class WallEngine final : public Engine {
   int npending_ = 0;
   int* sel_idx_ = nullptr;     // nodes with phi >= 1/2: domain index (N), block
                                // counts, the count
-  double* sel_val_ = nullptr;  // and their phi (N)
+  double* sel_val_ = nullptr;  // and their division and death rates (2N)
   int* hsel_idx_ = nullptr;    // pinned host copies (the count at [N])
   double* hsel_val_ = nullptr;
   int sel_last_ = 0;           // the last list's length
