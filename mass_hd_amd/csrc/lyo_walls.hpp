@@ -1683,21 +1683,28 @@ class WallEngine final : public Engine {
                        s, (const int*)cent_[slot], nc, R, r2, LX_, LY_, mask);
     return hipGetLastError();
   }
+  // the node list buffers of the mask draws (device and pinned host), kept
+  // for the run
+  hipError_t ensure_node_lists() {
+    const int N = LX_ * LY_, nsb = (N + kSelBlock - 1) / kSelBlock;
+    hipError_t e = hipSuccess;
+    // (each buffer on its own: a failed call leaves the others for the next)
+    if (!sel_idx_) e = hipMalloc((void**)&sel_idx_, sizeof(int) * ((size_t)N + nsb + 1));
+    if (e == hipSuccess && !sel_val_)
+      e = hipMalloc((void**)&sel_val_, sizeof(double) * 2 * (size_t)N);
+    if (e == hipSuccess && !hsel_idx_)
+      e = hipHostMalloc((void**)&hsel_idx_, sizeof(int) * ((size_t)N + 1), hipHostMallocDefault);
+    if (e == hipSuccess && !hsel_val_)
+      e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * 2 * (size_t)N, hipHostMallocDefault);
+    return e;
+  }
   // The list length of a step whose due redraws all have rate zero, read
   // back without waiting; their masks are cleared and their counters reset.
   hipError_t count_only_redraw(bool div_due, bool death_due, hipStream_t s) {
     const int N = LX_ * LY_, nsb = (N + kSelBlock - 1) / kSelBlock;
     hipError_t e = hipSuccess;
-    if (!sel_idx_) {
-      e = hipMalloc((void**)&sel_idx_, sizeof(int) * ((size_t)N + nsb + 1));
-      if (e == hipSuccess) e = hipMalloc((void**)&sel_val_, sizeof(double) * 2 * (size_t)N);
-      if (e == hipSuccess)
-        e = hipHostMalloc((void**)&hsel_idx_, sizeof(int) * ((size_t)N + 1), hipHostMallocDefault);
-      if (e == hipSuccess)
-        e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * 2 * (size_t)N,
-                          hipHostMallocDefault);
-      if (e != hipSuccess) return e;
-    }
+    e = ensure_node_lists();
+    if (e != hipSuccess) return e;
     if (!hpend_) {
       e = hipHostMalloc((void**)&hpend_, sizeof(int) * kMaxPending, hipHostMallocDefault);
       if (e != hipSuccess) return e;
@@ -1784,16 +1791,8 @@ class WallEngine final : public Engine {
     // the last list held plus a margin, the rest only if the list grew more
     const int N = LX_ * LY_, nsb = (N + kSelBlock - 1) / kSelBlock;
     hipError_t e = hipSuccess;
-    if (!sel_idx_) {
-      e = hipMalloc((void**)&sel_idx_, sizeof(int) * ((size_t)N + nsb + 1));
-      if (e == hipSuccess) e = hipMalloc((void**)&sel_val_, sizeof(double) * 2 * (size_t)N);
-      if (e == hipSuccess)
-        e = hipHostMalloc((void**)&hsel_idx_, sizeof(int) * ((size_t)N + 1), hipHostMallocDefault);
-      if (e == hipSuccess)
-        e = hipHostMalloc((void**)&hsel_val_, sizeof(double) * 2 * (size_t)N,
-                          hipHostMallocDefault);
-      if (e != hipSuccess) return e;
-    }
+    e = ensure_node_lists();
+    if (e != hipSuccess) return e;
     int* counts = sel_idx_ + N;
     int* total = counts + nsb;
     hipLaunchKernelGGL(select_count_kernel, dim3(nsb), dim3(kSelBlock), 0, s, geo(),
