@@ -190,7 +190,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["config1", "config2", "config3", "config4"])
     ap.add_argument("--steps", type=int, default=0)
-    ap.add_argument("--iters", type=int, default=2000)
+    ap.add_argument("--iters", type=int, default=0,
+                    help="training iterations (0: 2000 for config3, 8000 for config4)")
     ap.add_argument("--fp64-steps", type=int, default=10000)
     ap.add_argument("--every", type=int, default=1000)
     ap.add_argument("--out", default="")
@@ -203,7 +204,9 @@ def main(argv=None):
                          "(0: 200 for config3, 1000 for config4)")
     ap.add_argument("--spacing", type=int, default=0,
                     help="steps between training frames (0: 50 for config3, 500 for config4)")
-    ap.add_argument("--frames", type=int, default=8, help="training frames per run")
+    ap.add_argument("--frames", type=int, default=0,
+                    help="training frames per run (0: 8 for config3, 24 for config4; more "
+                         "frames and iterations lowered config4's long-run bias)")
     ap.add_argument("--matched", type=int, default=0,
                     help="N > 0: train on N 512^2 runs with the run's own zeta and xi instead "
                          "of the config2 sweep's 512^2 members")
@@ -221,9 +224,10 @@ def main(argv=None):
         matched = None
         if a.matched > 0:
             matched = (0.03 if late else CONFIG0_PHYS["zeta"], CONFIG0_PHYS["xi"], a.matched)
-        net, tr = train_surrogate(frames_per_run=a.frames,
+        net, tr = train_surrogate(frames_per_run=a.frames or (24 if late else 8),
                                   spacing=a.spacing or (500 if late else 50),
-                                  warm=a.warm or (1000 if late else 200), iters=a.iters,
+                                  warm=a.warm or (1000 if late else 200),
+                                  iters=a.iters or (8000 if late else 2000),
                                   matched=matched)
         tr["matched"] = matched
         if a.what == "config3":
