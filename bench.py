@@ -230,6 +230,35 @@ def runcard_leg(steps):
             "process_wall_s": round(wall, 2)}
 
 
+def checkpoint_leg(m, reps=3):
+    """Binary checkpoint of the run after its timed steps, through the C ABI:
+    the image (64-byte header + the raw device planes) saved into a host
+    buffer (pageable, touched once before) and loaded back, best of `reps`;
+    the run continues from the loaded image bitwise (tests)."""
+    import ctypes
+    import numpy as np
+    n = ctypes.c_uint64()
+    m._check(m.lib.mass_lyo_checkpoint_size(m.h, ctypes.byref(n)))
+    size = int(n.value)
+    buf = np.ones(size, dtype=np.uint8)
+    ptr = ctypes.c_void_p(buf.ctypes.data)
+    save, load = [], []
+    for _ in range(reps):
+        m.synchronize()
+        t0 = time.perf_counter()
+        m._check(m.lib.mass_lyo_checkpoint_save(m.h, ptr, n.value))
+        t1 = time.perf_counter()
+        m._check(m.lib.mass_lyo_checkpoint_load(m.h, ptr, n.value))
+        m.synchronize()
+        t2 = time.perf_counter()
+        save.append(t1 - t0)
+        load.append(t2 - t1)
+    return {"workload": "checkpoint of the %dx%d FP32 run" % (m.LX, m.LY), "bytes": size,
+            "save_ms": round(1e3 * min(save), 2), "load_ms": round(1e3 * min(load), 2),
+            "save_GBps": round(size / min(save) / 1e9, 2),
+            "load_GBps": round(size / min(load) / 1e9, 2)}
+
+
 def surrogate_leg(lyo, L, steps, judge_every=50, tol=1e-3, train_iters=500, warmup=200):
     """Surrogate corrector at LxL against the plain npc=2 step: a network
     trained briefly on frames of a 512^2 plain run (the full training is
@@ -428,6 +457,8 @@ def main():
     ap.add_argument("--skip-host", action="store_true",
                     help="skip the proliferation runcard through the host program")
     ap.add_argument("--runcard-steps", type=int, default=2000)
+    ap.add_argument("--skip-checkpoint", action="store_true",
+                    help="skip the checkpoint save/load timing of the FP32 run")
     ap.add_argument("--fp64-preload-ms", type=float, default=80.0,
                     help="minimum untimed FP64 work before its timed steps (clock settling)")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -489,6 +520,9 @@ def main():
         log("fp64: %.3f ms/step, %.0f MLUPS" % (t64 / s64, mlups64))
     log("fp32 %dx%d npc=%d: %d warmup + %d timed steps" % (L, L, npc, args.warmup, args.steps))
     t32, k32 = timed(m32, args.steps, args.warmup)
+    if not args.skip_checkpoint:
+        extra["checkpoint"] = checkpoint_leg(m32)
+        log("checkpoint: %s" % extra["checkpoint"])
     m32.close()
     if m64 is not None:
         m64.close()
