@@ -288,6 +288,10 @@ struct WallSweep {
 // 77.5 KB of LDS per block: two blocks per CU, held at 6 waves per SIMD by
 // an 80-register cap (the correctors spill a few values to scratch).
 constexpr int kWallTX = 10, kWallTY = 62, kWallThreads = (kWallTX + 2) * (kWallTY + 2);
+// go-or-grow sweeps (growth_sweep_kernel): tiles of kGrowTX rows (their
+// kernels hold 128 VGPRs, four waves per SIMD; 6-row tiles, two 512-thread
+// blocks per CU, ran the 500^2 step as fast as 10-row ones, within noise)
+constexpr int kGrowTX = 10, kGrowThreads = (kGrowTX + 2) * (kWallTY + 2);
 
 // Domain coordinate a stencil reads for coordinate v of an axis of length L
 // (periodic: wrapped; walled: the ghost at -1 / L reads its Neumann source
@@ -508,10 +512,10 @@ struct GrowthSweep {
 // growth_dm, update_fields), so the results are theirs bit for bit.  Modes
 // and pulls as wall_sweep_kernel's; kSweepDerive stores the 23 planes.
 template <int MODE, bool PULL_F = false, bool PULL_FN = false, bool DIV = false>
-__global__ void __launch_bounds__(kWallThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kGrowThreads) __attribute__((amdgpu_waves_per_eu(4)))
 growth_sweep_kernel(GrowthSweep a, Params<double> P, GrowthDev gd, WallGeo g) {
   using T = double;
-  constexpr int TX = kWallTX, TY = kWallTY;
+  constexpr int TX = kGrowTX, TY = kWallTY;
   constexpr int PW = TY + 4, PN = (TX + 4) * PW;  // staged Q, phi: tile + 2
   constexpr int DW = TY + 2, DN = (TX + 2) * DW;  // derived: tile + 1 (= threads)
   __shared__ T sq[3][PN];
@@ -530,7 +534,7 @@ growth_sweep_kernel(GrowthSweep a, Params<double> P, GrowthDev gd, WallGeo g) {
   auto node = [&](int x, int y) { return (y + 1) + g.W * (x + 1); };
   const WallSweep& w = a.w;
 
-  for (int i = tid; i < PN; i += kWallThreads) {
+  for (int i = tid; i < PN; i += kGrowThreads) {
     const int r = i / PW, c = i - r * PW;
     const int k = node(wall_axis(x0 - 2 + r, g.LX, true), wall_axis(y0 - 2 + c, g.LY, true));
     sq[0][i] = w.qxx[k];
@@ -1053,7 +1057,7 @@ class WallEngine final : public Engine {
   // pulls the streamed fn from them (the first one f too), so no Move runs;
   // conserve_phi sums phi before each sweep
   hipError_t growth_step(hipStream_t s) {
-    const dim3 grid = sweep_grid();
+    const dim3 grid = growth_grid();
     const bool div_only = gp_.division_only != 0;
     auto countphi = [&]() {
       if (!conserve_) return;
@@ -1106,9 +1110,9 @@ class WallEngine final : public Engine {
     a.mn_out = div_only ? nullptr : mn_;
     if (div_only)
       hipLaunchKernelGGL((growth_sweep_kernel<kSweepPredictor, false, false, true>), grid,
-                         dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+                         dim3(kGrowThreads), 0, s, a, P_, gd_, geo());
     else
-      hipLaunchKernelGGL((growth_sweep_kernel<kSweepPredictor>), grid, dim3(kWallThreads), 0, s,
+      hipLaunchKernelGGL((growth_sweep_kernel<kSweepPredictor>), grid, dim3(kGrowThreads), 0, s,
                          a, P_, gd_, geo());
     advance();
     int fc = cur_ ^ 1;  // the corrector's f input
@@ -1132,16 +1136,16 @@ class WallEngine final : public Engine {
       if (div_only) {
         if (c == 1)
           hipLaunchKernelGGL((growth_sweep_kernel<kSweepCorrector, true, true, true>), grid,
-                             dim3(kWallThreads), 0, s, b, P_, gd_, geo());
+                             dim3(kGrowThreads), 0, s, b, P_, gd_, geo());
         else
           hipLaunchKernelGGL((growth_sweep_kernel<kSweepCorrector, false, true, true>), grid,
-                             dim3(kWallThreads), 0, s, b, P_, gd_, geo());
+                             dim3(kGrowThreads), 0, s, b, P_, gd_, geo());
       } else if (c == 1) {
         hipLaunchKernelGGL((growth_sweep_kernel<kSweepCorrector, true, true>), grid,
-                           dim3(kWallThreads), 0, s, b, P_, gd_, geo());
+                           dim3(kGrowThreads), 0, s, b, P_, gd_, geo());
       } else {
         hipLaunchKernelGGL((growth_sweep_kernel<kSweepCorrector, false, true>), grid,
-                           dim3(kWallThreads), 0, s, b, P_, gd_, geo());
+                           dim3(kGrowThreads), 0, s, b, P_, gd_, geo());
       }
       advance();
       fc ^= 1;
@@ -1152,6 +1156,9 @@ class WallEngine final : public Engine {
   }
   dim3 sweep_grid() const {
     return dim3((unsigned)((LY_ + kWallTY - 1) / kWallTY), (unsigned)((LX_ + kWallTX - 1) / kWallTX));
+  }
+  dim3 growth_grid() const {
+    return dim3((unsigned)((LY_ + kWallTY - 1) / kWallTY), (unsigned)((LX_ + kGrowTX - 1) / kGrowTX));
   }
   // The derived planes of the last UpdateQuantities of a fused step, from its
   // inputs (f, Q and phi before the last corrector, left intact by it)
@@ -1168,20 +1175,20 @@ class WallEngine final : public Engine {
       a.w.D = D_;
       if (gp_.division_only) {
         if (npc_ == 1)
-          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, true, false, true>), sweep_grid(),
-                             dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, true, false, true>), growth_grid(),
+                             dim3(kGrowThreads), 0, s, a, P_, gd_, geo());
         else
           hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, false, false, true>),
-                             sweep_grid(), dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+                             growth_grid(), dim3(kGrowThreads), 0, s, a, P_, gd_, geo());
       } else {
         a.m = m_[mc_ ^ 1];
         a.chi = chi2_;
         if (npc_ == 1)
-          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, true>), sweep_grid(),
-                             dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, true>), growth_grid(),
+                             dim3(kGrowThreads), 0, s, a, P_, gd_, geo());
         else
-          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, false>), sweep_grid(),
-                             dim3(kWallThreads), 0, s, a, P_, gd_, geo());
+          hipLaunchKernelGGL((growth_sweep_kernel<kSweepDerive, false>), growth_grid(),
+                             dim3(kGrowThreads), 0, s, a, P_, gd_, geo());
       }
       d_stale_ = false;
       return hipGetLastError();
