@@ -182,6 +182,28 @@ def test_config4_headline_geometry(require_gpu):
     a.close()
 
 
+@pytest.mark.parametrize("precision,LX,LY", [(lyo.FP32, 4099, 4093), (lyo.FP64, 1027, 1021)])
+def test_ragged_large_grid_one_kernel_step(require_gpu, precision, LX, LY):
+    """The default launch plan on large grids of odd sizes: partial last
+    strip and partial last chunk of the one-kernel step (chunk length from
+    the wave-round model for these sizes), against the separate sweeps bit
+    for bit."""
+    print("one-kernel step %s %dx%d vs separate sweeps" % (precision, LX, LY), flush=True)
+    phys = dict(CONFIG0_PHYS, zeta=0.03)
+    init = dict(CONFIG0_INIT, seed=11)
+    runs = []
+    for sched in ({}, dict(one_kernel_step=0, fuse_last_pair=0)):
+        m = lyo.Lyotropic(LX, LY, npc=2, precision=precision, **phys)
+        if sched:
+            m.set_schedule(**sched)
+        m.configure(**init)
+        m.step(3)
+        runs.append({f: m.get(f) for f in list(STATE) + ["sigmaXY"]})
+        m.close()
+    for f, v in runs[0].items():
+        assert np.array_equal(v, runs[1][f]), f
+
+
 def test_long_run_statistics_fp32_vs_fp64(require_gpu):
     """Defect density, u_rms and the phi histogram of FP32 against FP64 over
     a long run at 512^2: within 2% at every checkpoint (histogram: L1

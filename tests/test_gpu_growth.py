@@ -143,6 +143,26 @@ def test_growth_fused_sweeps_equal_loops(require_gpu, npc, division_only):
     assert runs[0]["division_mask"].any()
 
 
+def test_growth_fused_sweeps_equal_loops_large(require_gpu):
+    """The fused sweeps against the node loops on a large grid of odd sizes
+    (1003 x 997: partial tiles in both directions), with masks drawn."""
+    print("go-or-grow fused vs loops 1003x997", flush=True)
+    g = dict(GROWTH, relax_steps=10)
+    runs = []
+    for loops in (False, True):
+        a = lyo.Lyotropic(1003, 997, npc=1, precision=lyo.FP64, **PHYS)
+        a.set_growth(**g)
+        a.set_loop_kernels(loops)
+        a.configure(**dict(INIT, radius=200.0, level=500.0))
+        a.step(6)
+        runs.append({n: a.get(n) for n in ["ff", "QQxx", "phi", "m", "chi", "sigmaXY",
+                                            "division_mask"]})
+        a.close()
+    for n, x in runs[0].items():
+        assert np.array_equal(x, runs[1][n]), n
+    assert runs[0]["division_mask"].any()
+
+
 def test_growth_count_only_redraws_keep_the_draw_order(require_gpu):
     """Redraws with rate zero accept nothing and only consume one number per
     listed node; the engine reads their list lengths back without waiting and
