@@ -151,6 +151,25 @@ def test_walls_checkpoint_continues_bitwise(require_gpu, bc):
             c.restore(img)
 
 
+@pytest.mark.parametrize("bc", [1, 2, 3, 4])
+def test_walls_fused_sweeps_equal_loops_large(require_gpu, bc):
+    """The fused tile sweeps against the reference's node loops on a large
+    grid of odd sizes (1003 x 517: partial tiles in both directions), npc 2,
+    every state and frame field bit for bit."""
+    print("walls bc=%d fused vs loops 1003x517" % bc, flush=True)
+    runs = []
+    for loops in (False, True):
+        m = lyo.Lyotropic(1003, 517, npc=2, precision=lyo.FP64, bc=bc,
+                          **dict(CONFIG0_PHYS, zeta=0.03))
+        m.set_loop_kernels(loops)
+        m.configure(**CONFIG0_INIT)
+        m.step(4)
+        runs.append({f: m.get(f) for f in STATE + DERIVED})
+        m.close()
+    for f, v in runs[0].items():
+        assert np.array_equal(v, runs[1][f]), f
+
+
 def test_walls_refuse_what_needs_a_periodic_grid(require_gpu):
     with pytest.raises(lyo.MassLyoError):
         lyo.Lyotropic(16, 16, npc=2, precision=lyo.FP32, bc=2, **ACTIVE)
