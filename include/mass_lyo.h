@@ -17,8 +17,9 @@
  *  - One handle is used by one host thread.  Device work is queued on the
  *    handle's own HIP stream; calls that return data synchronise it.
  *  - Boundaries: bc 0 periodic on every path; bc 1-4 (channels and boxes on
- *    a ghost layer) in FP64 on the reference's loop structure.  Node index
- *    and layout at the boundary are those of the domain either way.
+ *    a ghost layer) in FP64, as fused tile sweeps or, on request, the
+ *    reference's loop structure.  Node index and layout at the boundary are
+ *    those of the domain either way.
  */
 #ifndef MASS_LYO_H_
 #define MASS_LYO_H_
@@ -63,8 +64,9 @@ typedef struct mass_lyo_params {
                            1 free-slip channel, 2 no-slip channel (walls at
                            y = -1/2 and LY-1/2, periodic in x), 3 free-slip
                            box, 4 no-slip box.  bc 1-4 keep a ghost layer
-                           (geometry.cpp:140-183), need FP64 and run the
-                           reference's loop structure; batches, the
+                           (geometry.cpp:140-183) and need FP64 (fused tile
+                           sweeps, or the reference's loop structure:
+                           mass_lyo_set_loop_kernels); batches, the
                            surrogate and the diagnostics need bc = 0 */
   double GammaQ, GammaP, xi, tauNem, tauIso, rho, friction;
   double AA, CC, LL, KK, zeta, zetaI, conc;
@@ -319,8 +321,9 @@ int mass_lyo_set_loop_kernels(mass_lyo_t h, int32_t on);
  * (conserve_phi, lyotropic.cpp:297-298) when neither division nor death can
  * happen; its sum is then the device's fixed-tree sum, not the reference's
  * serial one, so such runs match the reference to rounding, not bitwise.
- * The run runs the reference's loop structure on the ghost-layer grid
- * (periodic ghosts for bc 0); the diagnostics, surrogate, batches and
+ * The run uses the ghost-layer grid: on a periodic one each sweep is one
+ * fused tile kernel (the reference's loop structure on request), on a walled
+ * one the reference's loop structure; the diagnostics, surrogate, batches and
  * checkpoints are not offered for it. */
 int mass_lyo_set_growth(mass_lyo_t h, const mass_lyo_growth* g);
 
