@@ -330,7 +330,9 @@ class Lyotropic:
         """FP64: run each step as the reference's node loops, one kernel per
         loop (UpdateQuantities storing its 23 derived fields, UpdateFields,
         Move); False returns to the fused kernels (on walled runs the
-        ghost-layer tile sweeps).  Bitwise the same results."""
+        ghost-layer tile sweeps, on periodic go-or-grow runs their tile
+        sweeps; walled go-or-grow runs refuse False).  Bitwise the same
+        results."""
         self._check(self.lib.mass_lyo_set_loop_kernels(self.h, 1 if on else 0))
 
 
