@@ -105,13 +105,14 @@ def train_surrogate(frames_per_run=8, spacing=50, warm=200, iters=2000, lr=2e-3,
 
 
 def compare(L, net, steps, zeta=None, seed=7, judge_every=50, tol=1e-3, every=None,
-            fp64_steps=0, twin=True, calibrate=False):
+            fp64_steps=0, twin=True, calibrate=False, twin_eps=1e-9):
     """Plain npc=2 vs surrogate (and FP64 plain over fp64_steps) from one
     initial state; time series of u_rms and defect density.  twin: a second
-    plain run from the same state with phi perturbed by 1e-9 (random signs),
-    whose distance from the plain run is the chaotic spread of two runs that
-    differ by a negligible amount -- the yardstick for the surrogate, whose
-    trajectory decorrelates from the plain one in the same way."""
+    plain run from the same state with phi perturbed by twin_eps (random
+    signs), whose distance from the plain run is the spread of two plain runs
+    once their trajectories have parted -- the yardstick for the surrogate,
+    whose trajectory parts from the plain one within its first steps (at
+    4096^2 a 1e-9 perturbation takes ~6e4 steps to show, 1e-3 far fewer)."""
     phys = dict(CONFIG0_PHYS)
     if zeta is not None:
         phys["zeta"] = zeta
@@ -126,7 +127,7 @@ def compare(L, net, steps, zeta=None, seed=7, judge_every=50, tol=1e-3, every=No
         if mode == "twin":
             sgn = np.sign(np.random.default_rng(seed).uniform(-1, 1, L * L))
             m.set_state(m.get("ff"), m.get("QQxx"), m.get("QQyx"),
-                        m.get("phi") + 1e-9 * sgn.reshape(m.get("phi").shape))
+                        m.get("phi") + twin_eps * sgn.reshape(m.get("phi").shape))
         if mode == "surrogate":
             lyo.enable_surrogate(m, net, judge_every, tol)
             lyo.set_surrogate_calibration(m, 1 if calibrate else 0)
@@ -212,6 +213,9 @@ def main(argv=None):
                          "of the config2 sweep's 512^2 members")
     ap.add_argument("--calibrate", type=int, default=0,
                     help="1: the surrogate's judged steps recalibrate its output scales")
+    ap.add_argument("--twin-eps", type=float, default=1e-9,
+                    help="amplitude of the twin run's phi perturbation (1e-9 stays on the "
+                         "plain trajectory for most of config4; 1e-3 decorrelates early)")
     ap.add_argument("--no-twin", action="store_true",
                     help="skip the plain run from a state perturbed by 1e-9")
     a = ap.parse_args(argv)
@@ -232,11 +236,13 @@ def main(argv=None):
         tr["matched"] = matched
         if a.what == "config3":
             r = {"training": tr, "run": compare(512, net, a.steps or 2000, twin=not a.no_twin,
+                                                twin_eps=a.twin_eps,
                                                 calibrate=bool(a.calibrate))}
         else:
             r = {"training": tr, "run": compare(4096, net, a.steps or 100000, zeta=0.03, seed=7,
                                                 every=a.every, fp64_steps=a.fp64_steps,
-                                                twin=not a.no_twin, calibrate=bool(a.calibrate))}
+                                                twin=not a.no_twin, calibrate=bool(a.calibrate),
+                                                twin_eps=a.twin_eps)}
     txt = json.dumps(r, indent=1)
     if a.out:
         open(a.out, "w").write(txt)
