@@ -388,3 +388,37 @@ def test_host_restart_from_checkpoint(card, tmp_path, require_gpu):
     rc, _, err = run([card(RUNCARD + "restart = %s\n" % (tmp_path / "missing")), "--no-write"])
     assert rc == 1 and "restart file" in err
 
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flag", ["-c", "--compress-full"])
+def test_host_compressed_output(card, tmp_path, require_gpu, flag):
+    """The reference's zip output modes (write.cpp:48-59): -c zips every
+    file on its own, --compress-full adds them all to one archive; either
+    way the archives hold exactly the files an uncompressed run writes."""
+    import shutil
+    import zipfile
+    if shutil.which("zip") is None:
+        pytest.skip("zip is not installed")
+    plain = tmp_path / "plain"
+    rc, out, err = run([card(), "-o", str(plain) + "/", "-f"])
+    assert rc == 0, err
+    if flag == "-c":
+        outdir = tmp_path / "zipped"
+        rc, out, err = run([card(), "-o", str(outdir) + "/", "-f", "-c"])
+        assert rc == 0, err
+        names = sorted(os.listdir(outdir))
+        assert names == sorted(f + ".zip" for f in os.listdir(plain))
+        for z in names:
+            with zipfile.ZipFile(outdir / z) as zf:
+                (inner,) = zf.namelist()
+                assert inner == z[:-4]
+                assert zf.read(inner) == (plain / inner).read_bytes(), inner
+    else:
+        archive = tmp_path / "run"
+        rc, out, err = run([card(), "-o", str(archive), "-f", "--compress-full"])
+        assert rc == 0, err
+        with zipfile.ZipFile(str(archive) + ".zip") as zf:
+            assert sorted(zf.namelist()) == sorted(os.listdir(plain))
+            for inner in zf.namelist():
+                assert zf.read(inner) == (plain / inner).read_bytes(), inner
