@@ -422,3 +422,20 @@ def test_host_compressed_output(card, tmp_path, require_gpu, flag):
             assert sorted(zf.namelist()) == sorted(os.listdir(plain))
             for inner in zf.namelist():
                 assert zf.read(inner) == (plain / inner).read_bytes(), inner
+
+
+@pytest.mark.gpu
+def test_host_nstart(card, tmp_path, require_gpu):
+    """Frames from step nstart on, nstart rounded up to a multiple of ninfo
+    (options.cpp:206-216): nstart = 5 with ninfo = 10 writes frames 10 and
+    20 only; their contents are those of a run that writes every frame."""
+    full = tmp_path / "full"
+    rc, out, err = run([card(), "-o", str(full) + "/", "-f"])
+    assert rc == 0, err
+    late = tmp_path / "late"
+    rc, out, err = run([card(RUNCARD.replace("ninfo = 10", "ninfo = 10\nnstart = 5")), "-o",
+                        str(late) + "/", "-f"])
+    assert rc == 0, err
+    assert sorted(os.listdir(late)) == ["frame10.json", "frame20.json", "parameters.json"]
+    for f in ("frame10.json", "frame20.json"):
+        assert (late / f).read_bytes() == (full / f).read_bytes(), f
