@@ -439,3 +439,18 @@ def test_host_nstart(card, tmp_path, require_gpu):
     assert sorted(os.listdir(late)) == ["frame10.json", "frame20.json", "parameters.json"]
     for f in ("frame10.json", "frame20.json"):
         assert (late / f).read_bytes() == (full / f).read_bytes(), f
+
+
+@pytest.mark.gpu
+def test_host_nan_stops_before_the_frame(card, tmp_path, require_gpu):
+    """A NaN in a frame's fields stops the run with the reference's message
+    (write.cpp:41) and exit status 1 before that frame's file exists; the
+    earlier frames are on disk.  tauNem = tauIso = 0 makes the first
+    collision 0/0 (f starts at its equilibrium)."""
+    text = RUNCARD.replace("tauNem = 2.0", "tauNem = 0").replace("tauIso = 2.0", "tauIso = 0")
+    outdir = tmp_path / "out"
+    rc, out, err = run([card(text), "-o", str(outdir) + "/", "-f"])
+    assert rc == 1
+    assert "nan found while writing file." in err
+    files = sorted(os.listdir(outdir))
+    assert "frame0.json" in files and "frame10.json" not in files, files
