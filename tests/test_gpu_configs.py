@@ -204,6 +204,28 @@ def test_ragged_large_grid_one_kernel_step(require_gpu, precision, LX, LY):
         assert np.array_equal(v, runs[1][f]), f
 
 
+def test_fp64_step_chunk_lengths_at_4096(require_gpu):
+    """FP64 one-kernel step at 4096^2, npc = 2, with chunks of 64 and 342
+    rows (many chunks per strip; 12 chunks, the last one partial) and the
+    default length: each equals the separate sweeps bit for bit."""
+    L = 4096
+    phys = dict(CONFIG0_PHYS, zeta=0.03)
+    runs = []
+    for sched in (dict(one_kernel_step=0, fuse_last_pair=0), {}, dict(step_chunk_rows=64),
+                  dict(step_chunk_rows=342)):
+        print("fp64 %dx%d npc=2 schedule %s" % (L, L, sched), flush=True)
+        m = lyo.Lyotropic(L, L, npc=2, precision=lyo.FP64, **phys)
+        if sched:
+            m.set_schedule(**sched)
+        m.configure(**CONFIG0_INIT)
+        m.step(2)
+        runs.append({f: m.get(f) for f in STATE})
+        m.close()
+    for r in runs[1:]:
+        for f, v in runs[0].items():
+            assert np.array_equal(v, r[f]), f
+
+
 def test_long_run_statistics_fp32_vs_fp64(require_gpu):
     """Defect density, u_rms and the phi histogram of FP32 against FP64 over
     a long run at 512^2: within 2% at every checkpoint (histogram: L1
